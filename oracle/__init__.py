@@ -1,0 +1,510 @@
+"""CPU FP64 oracle for the importance-sampling hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
+checker (or the timed CPU baseline).  The product never imports it.
+
+The arithmetic lives in ``oracle/oracle.cpp`` (a C++17 restatement of the
+reference's CPU path, built by ``oracle/Makefile`` into ``liboracle.so``);
+this module is its ctypes wrapper plus the exact-rational DM-PMC allocation
+plan (``allocation_plan``), which restates
+``src/population_preparation.jl:266-367`` with ``fractions.Fraction``.
+
+Parity pins (what checks each oracle function; tests/test_oracle_golden.py):
+
+* ``logaddexp`` / ``log1pexp`` -- LogExpFunctions 1.0.1 thresholds; checked
+  against mpmath at the branch boundaries and against the append identity of
+  ``test/log_mixture_accumulator.jl:7-16``.
+* ``student_t_logconstant`` -- mpmath log-gamma closed form, all parities of d.
+* ``proposal_logdensity`` -- the analytic densities of
+  ``test/native_proposals.jl:179-306`` (Gaussian and Student-t, spherical,
+  diagonal, factor).
+* ``plain_is`` -- normalizer identity ``test/native_proposals.jl:558-597``
+  (self-target logw == 0, log Z = log 2pi + log 3), Student-t buffered draws
+  ``test/kernel_execution.jl:490-543`` (exact samples, exhaustion failure).
+* ``adaptive`` (AMIS) -- the frozen round ESS / log Z of
+  ``test/amis_kernel.jl:918-967`` (exact zero in round 1), the ridge
+  bitstrings of ``test/amis_kernel.jl:282-329``, and the literal AMIS spec of
+  ``validation/reproducers/amis.jl:190-309`` (independent numpy restatement,
+  tolerance 4096*max(d,T)*eps).
+* ``adaptive`` (NPMC) -- the ``npmc_oracle`` recurrence of
+  ``test/npmc.jl:15-66`` (numpy restatement with clipping at the isqrt(n)-th
+  largest weight) and the self-target case ``test/npmc.jl:96-111``.
+* ``dmpmc`` -- realized denominator ``test/dm_pmc_kernel.jl:15-66``;
+  ``allocation_plan`` -- ``test/dm_pmc.jl:71-155`` (exact integer counts).
+* ``select_ancestors`` -- ``test/dm_pmc_kernel.jl:205-272`` and
+  ``test/resampling.jl:54-66`` (bit-exact ancestors, plateau cases).
+* ``systematic`` selection -- closed-form counts
+  ceil(M*cdf_k - u) - ceil(M*cdf_{k-1} - u) in exact rationals (the reference
+  has no systematic method; the mathematics fixes it).
+* ``logweight_summary`` / ``lognormalizer`` / ``moments`` --
+  ``test/dm_pmc_kernel.jl:274-296`` and ``test/results.jl:9-75,357-388``.
+* ``cholesky`` -- numpy.linalg.cholesky on random SPD matrices.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from fractions import Fraction
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+F64P = ctypes.POINTER(ctypes.c_double)
+I64P = ctypes.POINTER(ctypes.c_int64)
+U64P = ctypes.POINTER(ctypes.c_uint64)
+IP = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    d, i, i64 = ctypes.c_double, ctypes.c_int, ctypes.c_int64
+    L.or_logaddexp.restype = d
+    L.or_logaddexp.argtypes = [d, d]
+    L.or_log1pexp.restype = d
+    L.or_log1pexp.argtypes = [d]
+    L.or_det_exp.restype = d
+    L.or_det_exp.argtypes = [d]
+    L.or_student_t_logconstant.restype = d
+    L.or_student_t_logconstant.argtypes = [d, i]
+    L.or_proposal_logdensity.restype = d
+    L.or_proposal_logdensity.argtypes = [i, d, i, i, F64P, F64P, d, F64P, i]
+    L.or_target.restype = d
+    L.or_target.argtypes = [i, F64P, i, F64P]
+    L.or_student_t_multiplier.restype = i
+    L.or_student_t_multiplier.argtypes = [d, F64P, F64P, F64P]
+    L.or_plain_is.restype = None
+    L.or_plain_is.argtypes = [i, d, i, i, F64P, F64P, d, i, F64P, i64, F64P, F64P, i, F64P, F64P, U64P]
+    L.or_logweight_summary.restype = None
+    L.or_logweight_summary.argtypes = [F64P, i64, F64P, F64P]
+    L.or_merge_logweight_moments.restype = None
+    L.or_merge_logweight_moments.argtypes = [F64P, F64P, F64P]
+    L.or_logsumexp_accumulator.restype = None
+    L.or_logsumexp_accumulator.argtypes = [F64P, i64, F64P]
+    L.or_lognormalizer.restype = d
+    L.or_lognormalizer.argtypes = [F64P, i64]
+    L.or_normalized_weights.restype = i
+    L.or_normalized_weights.argtypes = [F64P, i64, F64P]
+    L.or_moments.restype = i
+    L.or_moments.argtypes = [F64P, F64P, i64, i, F64P, F64P, F64P]
+    L.or_cholesky.restype = i
+    L.or_cholesky.argtypes = [F64P, i, F64P]
+    L.or_cdf_sequential.restype = i
+    L.or_cdf_sequential.argtypes = [F64P, i64, F64P]
+    L.or_cdf_blocked.restype = i
+    L.or_cdf_blocked.argtypes = [F64P, i64, F64P, i, i]
+    L.or_select_ancestors.restype = None
+    L.or_select_ancestors.argtypes = [F64P, F64P, i64, F64P, i64, I64P]
+    L.or_systematic_points.restype = None
+    L.or_systematic_points.argtypes = [d, i64, F64P]
+    L.or_adaptive.restype = i
+    L.or_adaptive.argtypes = [i, i, d, i, i, F64P, F64P, d, i, I64P, F64P, F64P, F64P, i, F64P, i,
+                              F64P, F64P, F64P, F64P, I64P, F64P, F64P, F64P, F64P, F64P, F64P,
+                              U64P, IP]
+    L.or_dmpmc.restype = i
+    L.or_dmpmc.argtypes = [i, d, i, i, i, F64P, F64P, F64P, I64P, i, I64P, I64P, F64P, i, i, i,
+                           F64P, F64P, F64P, F64P, i, F64P, i, F64P, F64P, I64P, I64P, I64P,
+                           F64P, F64P, F64P, U64P, IP]
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a, t=F64P):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------- scalars
+def logaddexp(a, b):
+    return lib().or_logaddexp(float(a), float(b))
+
+
+def log1pexp(x):
+    return lib().or_log1pexp(float(x))
+
+
+def det_exp(x):
+    return lib().or_det_exp(float(x))
+
+
+def student_t_logconstant(dof, d):
+    return lib().or_student_t_logconstant(float(dof), int(d))
+
+
+# ---------------------------------------------------------------- proposals
+class Slot:
+    """Flat proposal description: family (0 Gaussian, 1 Student-t), dof,
+    kind (0 spherical, 1 diagonal, 2 factor), location, scale storage, lognormalizer."""
+
+    def __init__(self, family, dof, kind, location, scale, lognormalizer):
+        self.family = int(family)
+        self.dof = float(dof)
+        self.kind = int(kind)
+        self.location = _f64(np.atleast_1d(location))
+        self.d = self.location.size
+        if kind == 2:
+            self.scale = _f64(np.asarray(scale, dtype=np.float64).reshape(self.d, self.d).T.ravel())
+        else:
+            self.scale = _f64(np.atleast_1d(scale))
+        self.lognormalizer = float(lognormalizer)
+
+
+def as_slot(p):
+    """Accept any object exposing family/dof/kind/location/scale/lognormalizer."""
+    if isinstance(p, Slot):
+        return p
+    return Slot(p.family, p.dof, p.kind, p.location, p.scale, p.lognormalizer)
+
+
+def proposal_logdensity(p, x, packed=False):
+    s = as_slot(p)
+    x = _f64(np.atleast_1d(x))
+    return lib().or_proposal_logdensity(s.family, s.dof, s.kind, s.d, _p(s.location), _p(s.scale),
+                                        s.lognormalizer, _p(x), int(bool(packed)))
+
+
+def target(kind, params, x):
+    x = _f64(np.atleast_1d(x))
+    params = _f64(params if params is not None else [0.0])
+    return lib().or_target(int(kind), _p(params), x.size, _p(x))
+
+
+def student_t_multiplier(dof, radial_normals, radial_uniforms=None):
+    out = ctypes.c_double(0.0)
+    rn = _f64(radial_normals)
+    ru = None if radial_uniforms is None else _f64(radial_uniforms)
+    ok = lib().or_student_t_multiplier(float(dof), _p(rn), _p(ru), ctypes.byref(out))
+    return (out.value if ok == 0 else None)
+
+
+def normal_stride(family, dof, d):
+    return d + (0 if family == 0 else (1 if dof == 1.0 else 8))
+
+
+def uniform_stride(family, dof):
+    return 0 if (family == 0 or dof == 1.0) else 8 + (1 if dof < 2.0 else 0)
+
+
+def radial_normal_stride(family, dof):
+    return 0 if family == 0 else (1 if dof == 1.0 else 8)
+
+
+# ---------------------------------------------------------------- plain IS
+def plain_is(p, tkind, tparams, normals, uniforms=None, nthreads=1):
+    """Returns (samples (n, d), logweights (n,), failure uint64[3])."""
+    s = as_slot(p)
+    stride = normal_stride(s.family, s.dof, s.d)
+    normals = _f64(normals)
+    n = normals.size // stride
+    uniforms = None if uniforms is None else _f64(uniforms)
+    samples = np.zeros((n, s.d))
+    logw = np.full(n, np.nan)
+    fail = np.zeros(3, dtype=np.uint64)
+    tp = _f64(tparams if tparams is not None else [0.0])
+    lib().or_plain_is(s.family, s.dof, s.kind, s.d, _p(s.location), _p(s.scale), s.lognormalizer,
+                      int(tkind), _p(tp), n, _p(normals), _p(uniforms), int(nthreads),
+                      _p(samples), _p(logw), _p(fail, U64P))
+    return samples, logw, fail
+
+
+# ---------------------------------------------------------------- summaries / results
+def logweight_summary(lw):
+    lw = _f64(lw)
+    ess, logz = ctypes.c_double(), ctypes.c_double()
+    lib().or_logweight_summary(_p(lw), lw.size, ctypes.byref(ess), ctypes.byref(logz))
+    return ess.value, logz.value
+
+
+def merge_logweight_moments(a, b):
+    a, b = _f64(a), _f64(b)
+    out = np.zeros(3)
+    lib().or_merge_logweight_moments(_p(a), _p(b), _p(out))
+    return out
+
+
+def logsumexp_accumulator(lw):
+    lw = _f64(lw)
+    out = np.zeros(2)
+    lib().or_logsumexp_accumulator(_p(lw), lw.size, _p(out))
+    return out
+
+
+def lognormalizer(lw):
+    lw = _f64(lw)
+    return lib().or_lognormalizer(_p(lw), lw.size)
+
+
+def normalized_weights(lw):
+    lw = _f64(lw)
+    w = np.zeros(lw.size)
+    if lib().or_normalized_weights(_p(lw), lw.size, _p(w)):
+        raise ZeroDivisionError("all weights are zero")
+    return w
+
+
+def moments(samples, lw):
+    """samples (n, d) sample-major; returns mean (d,), var (d,), cov (d, d)."""
+    x = _f64(samples)
+    if x.ndim == 1:
+        x = x.reshape(-1, 1)
+    n, d = x.shape
+    lw = _f64(lw)
+    mean, var, cov = np.zeros(d), np.zeros(d), np.zeros(d * d)
+    if lib().or_moments(_p(x), _p(lw), n, d, _p(mean), _p(var), _p(cov)):
+        raise ZeroDivisionError("all weights are zero")
+    return mean, var, cov.reshape(d, d).T.copy()
+
+
+def cholesky(A):
+    A = np.asarray(A, dtype=np.float64)
+    d = A.shape[0]
+    a = _f64(A.T.ravel())
+    L = np.zeros(d * d)
+    info = lib().or_cholesky(_p(a), d, _p(L))
+    if info:
+        raise np.linalg.LinAlgError(f"not positive definite at {info}")
+    return L.reshape(d, d).T.copy()
+
+
+# ---------------------------------------------------------------- resampling
+def cdf_sequential(lw):
+    lw = _f64(lw)
+    cdf = np.zeros(lw.size)
+    if lib().or_cdf_sequential(_p(lw), lw.size, _p(cdf)):
+        raise ZeroDivisionError("all weights are zero")
+    return cdf
+
+
+def cdf_blocked(lw, threads=256, per=16):
+    lw = _f64(lw)
+    cdf = np.zeros(lw.size)
+    if lib().or_cdf_blocked(_p(lw), lw.size, _p(cdf), int(threads), int(per)):
+        raise ZeroDivisionError("all weights are zero")
+    return cdf
+
+
+def select_ancestors(cdf, lw, u):
+    """1-based ancestors (binary search with plateau fix-up)."""
+    cdf, lw, u = _f64(cdf), _f64(lw), _f64(np.atleast_1d(u))
+    anc = np.zeros(u.size, dtype=np.int64)
+    lib().or_select_ancestors(_p(cdf), _p(lw), cdf.size, _p(u), u.size, _p(anc, I64P))
+    return anc
+
+
+def systematic_points(u, count):
+    pts = np.zeros(int(count))
+    lib().or_systematic_points(float(u), int(count), _p(pts))
+    return pts
+
+
+def systematic_counts_exact(cdf, u):
+    """Closed-form systematic counts ceil(M*cdf_k - u) - ceil(M*cdf_{k-1} - u),
+    evaluated in exact rationals from the FP64 values of cdf and u (M = count)."""
+    raise NotImplementedError  # tests compute it directly with a given M
+
+
+# ---------------------------------------------------------------- AMIS / NPMC
+def adaptive(method, p, schedule, normals, tkind, tparams, radial_normals=None,
+             radial_uniforms=None, nthreads=1, scalar=None):
+    """method 0 AMIS, 1 NPMC.  normals: (T, d*maxN) per-round batches."""
+    s = as_slot(p)
+    schedule = np.ascontiguousarray(np.asarray(schedule, dtype=np.int64))
+    T = schedule.size
+    total = int(schedule.sum())
+    maxN = int(schedule.max())
+    d = s.d
+    if scalar is None:
+        scalar = getattr(p, "scalar", False)
+    if scalar:
+        scale0 = _f64([s.scale[0]])
+    else:
+        if s.kind == 2:
+            scale0 = s.scale.copy()
+        else:
+            diag = np.full(d, s.scale[0]) if s.kind == 0 else s.scale
+            scale0 = _f64(np.diag(diag).T.ravel())
+    normals = _f64(normals).reshape(-1)
+    assert normals.size == T * d * maxN, (normals.size, T, d, maxN)
+    rn = None if radial_normals is None else _f64(radial_normals)
+    ru = None if radial_uniforms is None else _f64(radial_uniforms)
+    tp = _f64(tparams if tparams is not None else [0.0])
+    samples = np.zeros((total, d))
+    lt, ln, lw = np.zeros(total), np.zeros(total), np.zeros(total)
+    rid = np.zeros(total, dtype=np.int64)
+    ress, rlz, aess = np.zeros(T), np.zeros(T), np.zeros(T)
+    fsz = 1 if scalar else d * d
+    hm, hf, hl = np.zeros((T + 1) * d), np.zeros((T + 1) * fsz), np.zeros(T + 1)
+    fail = np.zeros(3, dtype=np.uint64)
+    info = np.zeros(2, dtype=np.int32)
+    rc = lib().or_adaptive(int(method), s.family, s.dof, int(bool(scalar)), d, _p(s.location),
+                           _p(scale0), s.lognormalizer, T, _p(schedule, I64P), _p(normals),
+                           _p(rn), _p(ru), int(tkind), _p(tp), int(nthreads), _p(samples), _p(lt),
+                           _p(ln), _p(lw), _p(rid, I64P), _p(ress), _p(rlz), _p(aess), _p(hm),
+                           _p(hf), _p(hl), _p(fail, U64P), info.ctypes.data_as(IP))
+    factors = hf.reshape(T + 1, d, d).transpose(0, 2, 1).copy() if not scalar else hf.copy()
+    return dict(rc=rc, samples=samples, logtargets=lt, lognumerators=ln, logweights=lw,
+                round_ids=rid, round_ess=ress, round_lognormalizers=rlz, adaptation_ess=aess,
+                means=hm.reshape(T + 1, d), factors=factors, lognormalizers=hl, failure=fail,
+                fail_round=int(info[0]), fail_phase=int(info[1]))
+
+
+# ---------------------------------------------------------------- DM-PMC
+def active_order(masses):
+    """Active slots sorted by mass, stable (src/proposal_banks.jl:301-307).
+    Returns 0-based original indices."""
+    ids = [i for i, m in enumerate(masses) if m != 0]
+    return sorted(ids, key=lambda i: masses[i])
+
+
+def round_counts(exact_props, round_size, round_index):
+    """_dm_pmc_round_counts (src/population_preparation.jl:283-312), exact rationals.
+    round_index is 1-based."""
+    K = len(exact_props)
+    counts, rem = [], []
+    for p in exact_props:
+        q = p * round_size
+        c = q.numerator // q.denominator
+        counts.append(c)
+        rem.append(q - c)
+    remaining = round_size - sum(counts)
+    assert 0 <= remaining < K
+    first_tie = (round_index - 1) % K + 1
+    order = sorted(range(K), key=lambda s: (-rem[s], (s + 1 - first_tie) % K))
+    for idx in range(remaining):
+        counts[order[idx]] += 1
+    return counts
+
+
+def allocation_plan(active_masses, schedule):
+    """(counts K x T int64, logcoefficients K x T) for masses in active order."""
+    exact = [Fraction(float(m)) for m in active_masses]
+    total = sum(exact)
+    props = [m / total for m in exact]
+    K, T = len(props), len(schedule)
+    counts = np.zeros((K, T), dtype=np.int64)
+    for t, n in enumerate(schedule):
+        counts[:, t] = round_counts(props, int(n), t + 1)
+    if (counts <= 0).any():
+        raise ValueError("a round cannot allocate at least one sample to every positive-mass proposal")
+    logc = np.log(counts.astype(np.float64) / np.asarray(schedule, dtype=np.float64)[None, :])
+    return counts, logc
+
+
+def dmpmc(family, dof, kind, d, locs, scales, lognorms, proposal_ids, schedule, counts, logcoef,
+          normals, uniforms, tkind, tparams, resampling=0, radial_normals=None,
+          radial_uniforms=None, nthreads=1, cdf_threads=256, cdf_per=16):
+    """locs (K, d); scales: spherical (K,), diagonal (K, d), factor (K, d, d) row-major per slot.
+    normals (T, d*maxN); uniforms (T, K)."""
+    locs = _f64(locs).reshape(-1)
+    K = locs.size // d
+    sc = np.asarray(scales, dtype=np.float64)
+    if kind == 2:
+        sc = _f64(sc.reshape(K, d, d).transpose(0, 2, 1).ravel())
+    else:
+        sc = _f64(sc.ravel())
+    schedule = np.ascontiguousarray(np.asarray(schedule, dtype=np.int64))
+    T = schedule.size
+    total = int(schedule.sum())
+    counts_c = np.ascontiguousarray(np.asarray(counts, dtype=np.int64).T.ravel())
+    logc_c = _f64(np.asarray(logcoef, dtype=np.float64).T.ravel())
+    pid = np.ascontiguousarray(np.asarray(proposal_ids, dtype=np.int64))
+    ln = _f64(lognorms)
+    normals = _f64(normals).reshape(-1)
+    uniforms = _f64(uniforms).reshape(-1)
+    rn = None if radial_normals is None else _f64(radial_normals)
+    ru = None if radial_uniforms is None else _f64(radial_uniforms)
+    tp = _f64(tparams if tparams is not None else [0.0])
+    samples = np.zeros((total, d))
+    lw = np.zeros(total)
+    rid = np.zeros(total, dtype=np.int64)
+    pids = np.zeros(total, dtype=np.int64)
+    anc = np.zeros(T * K, dtype=np.int64)
+    ress, rlz = np.zeros(T), np.zeros(T)
+    final = np.zeros(K * d)
+    fail = np.zeros(3, dtype=np.uint64)
+    info = np.zeros(2, dtype=np.int32)
+    rc = lib().or_dmpmc(int(family), float(dof), int(kind), int(d), K, _p(locs), _p(sc), _p(ln),
+                        _p(pid, I64P), T, _p(schedule, I64P), _p(counts_c, I64P), _p(logc_c),
+                        int(resampling), int(cdf_threads), int(cdf_per), _p(normals), _p(rn), _p(ru),
+                        _p(uniforms), int(tkind), _p(tp), int(nthreads), _p(samples), _p(lw),
+                        _p(rid, I64P), _p(pids, I64P), _p(anc, I64P), _p(ress), _p(rlz), _p(final),
+                        _p(fail, U64P), info.ctypes.data_as(IP))
+    return dict(rc=rc, samples=samples, logweights=lw, round_ids=rid, proposal_ids=pids,
+                ancestors=anc.reshape(T, K), round_ess=ress, round_lognormalizers=rlz,
+                final_locations=final.reshape(K, d), failure=fail, fail_round=int(info[0]),
+                fail_phase=int(info[1]))
+
+
+# ---------------------------------------------------------------- Philox4x64-10 (host restatement)
+_M0 = 0xD2E7470EE14C6C93
+_M1 = 0xCA5A826395121157
+_W0 = 0x9E3779B97F4A7C15
+_W1 = 0xBB67AE8584CAA73B
+_MASK = (1 << 64) - 1
+
+
+def philox4x64(counter, key, rounds=10):
+    """Philox4x64-R (Salmon et al., SC'11), pure Python: 4 counter words, 2 key words."""
+    c0, c1, c2, c3 = (int(v) & _MASK for v in counter)
+    k0, k1 = (int(v) & _MASK for v in key)
+    for r in range(rounds):
+        if r:
+            k0 = (k0 + _W0) & _MASK
+            k1 = (k1 + _W1) & _MASK
+        p0 = _M0 * c0
+        p1 = _M1 * c2
+        hi0, lo0 = p0 >> 64, p0 & _MASK
+        hi1, lo1 = p1 >> 64, p1 & _MASK
+        c0, c1, c2, c3 = hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0
+    return [c0, c1, c2, c3]
+
+
+def u53(word):
+    """Uniform in (0, 1): ((w >> 11) + 0.5) * 2^-53."""
+    return ((int(word) >> 11) + 0.5) * 2.0 ** -53
+
+
+def box_muller(w0, w1):
+    u1, u2 = u53(w0), u53(w1)
+    r = math.sqrt(-2.0 * math.log(u1))
+    return r * math.cos(2.0 * math.pi * u2), r * math.sin(2.0 * math.pi * u2)
+
+
+def ridge_scalar(scale, family=0, dof=0.0):
+    L = lib()
+    L.or_ridge_scalar.restype = ctypes.c_double
+    L.or_ridge_scalar.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    return L.or_ridge_scalar(float(scale), int(family), float(dof))
+
+
+def ridge_factor(factor, family=0, dof=0.0):
+    L = lib()
+    L.or_ridge_factor.restype = ctypes.c_double
+    L.or_ridge_factor.argtypes = [F64P, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    f = np.asarray(factor, dtype=np.float64)
+    d = f.shape[0]
+    a = _f64(f.T.ravel())
+    return L.or_ridge_factor(_p(a), d, int(family), float(dof))

@@ -1,0 +1,639 @@
+"""Pins the CPU oracle against the reference's own known answers and against
+closed forms.  Each test cites the reference test it transcribes."""
+
+import math
+import struct
+from fractions import Fraction
+
+import mpmath
+import numpy as np
+import pytest
+
+EPS = np.finfo(np.float64).eps
+
+
+def bits(x):
+    return format(struct.unpack("<Q", struct.pack("<d", x))[0], "064b")
+
+
+# ---------------------------------------------------------------- LogExpFunctions
+def test_logaddexp_against_mpmath(oracle_mod):
+    for a, b in [(0.0, 0.0), (1.0, -3.0), (-700.0, -701.0), (10.0, 10.0 - 36.0), (10.0, 10.0 - 38.0),
+                 (-1e-3, 2.5), (3.0, -np.inf), (-np.inf, 2.0)]:
+        got = oracle_mod.logaddexp(a, b)
+        if np.isinf(a) or np.isinf(b):
+            assert got == max(a, b)
+            continue
+        exact = mpmath.log(mpmath.e ** mpmath.mpf(a) + mpmath.e ** mpmath.mpf(b))
+        assert abs(got - float(exact)) <= 2 * EPS * abs(float(exact)) + 1e-300
+    assert oracle_mod.logaddexp(-np.inf, -np.inf) == -np.inf
+    assert math.isnan(oracle_mod.logaddexp(np.nan, 1.0))
+    assert oracle_mod.logaddexp(np.inf, np.inf) == np.inf
+
+
+def test_log_mixture_append(oracle_mod):
+    # test/log_mixture_accumulator.jl:7-16
+    a = math.log(2) + math.log(0.25)
+    b = math.log(3) + math.log(0.5)
+    num = oracle_mod.logaddexp(-np.inf, math.log(2) + math.log(0.25))
+    num = oracle_mod.logaddexp(num, math.log(3) + math.log(0.5))
+    assert num == pytest.approx(np.logaddexp(a, b), rel=4 * EPS)
+    assert math.log(4) - (num - math.log(5)) == pytest.approx(math.log(4) - num + math.log(5), rel=4 * EPS)
+
+
+def test_det_exp_accuracy(oracle_mod):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-745, 0, 2000), [0.0, -1e-300, -708.3, -709.9, -744.0]])
+    for x in xs:
+        got = oracle_mod.det_exp(x)
+        exact = mpmath.exp(mpmath.mpf(float(x)))
+        if exact < mpmath.mpf(2.0) ** -1022:
+            assert abs(got - float(exact)) <= 2.0 ** -1074 * 2
+        else:
+            assert abs(got - float(exact)) <= 2 * EPS * float(exact)
+
+
+# ---------------------------------------------------------------- proposal constants
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 7, 10, 51])
+@pytest.mark.parametrize("dof", [0.3, 1.0, 2.5, 5.0, 8.0, 31.0, 1e6])
+def test_student_t_logconstant_closed_form(oracle_mod, d, dof):
+    mpmath.mp.dps = 40
+    exact = (mpmath.loggamma((mpmath.mpf(dof) + d) / 2) - mpmath.loggamma(mpmath.mpf(dof) / 2)
+             - mpmath.mpf(d) / 2 * mpmath.log(mpmath.mpf(dof) * mpmath.pi))
+    got = oracle_mod.student_t_logconstant(dof, d)
+    assert abs(got - float(exact)) <= 64 * EPS * max(1.0, abs(float(exact)))
+
+
+def _slot(oracle_mod, family, dof, kind, loc, scale, logabsdet):
+    d = np.atleast_1d(loc).size
+    if family == 0:
+        ln = -0.5 * d * math.log(2 * math.pi) - logabsdet
+    else:
+        ln = oracle_mod.student_t_logconstant(dof, d) - logabsdet
+    return oracle_mod.Slot(family, dof, kind, loc, scale, ln)
+
+
+def test_gaussian_analytic_densities(oracle_mod):
+    # test/native_proposals.jl:179-201
+    o = oracle_mod
+    s = _slot(o, 0, 0, 0, [1.25], [2.5], math.log(2.5))
+    exp_ = -0.5 * ((-0.75 - 1.25) / 2.5) ** 2 - math.log(2.5) - 0.5 * math.log(2 * math.pi)
+    assert o.proposal_logdensity(s, [-0.75]) == pytest.approx(exp_, rel=8 * EPS)
+    s = _slot(o, 0, 0, 0, [1, -1, 2], [2.0], 3 * math.log(2.0))
+    exp_ = -0.5 * (1 + 4 + 9) - 3 * math.log(2.0) - 1.5 * math.log(2 * math.pi)
+    assert o.proposal_logdensity(s, [3, -5, 8]) == pytest.approx(exp_, rel=8 * EPS)
+    s = _slot(o, 0, 0, 1, [1, -1, 2], [2, 4, 0.5], math.log(2) + math.log(4) + math.log(0.5))
+    exp_ = -0.5 * (1 + 1 + 4) - math.log(2 * 4 * 0.5) - 1.5 * math.log(2 * math.pi)
+    assert o.proposal_logdensity(s, [3, 3, 3]) == pytest.approx(exp_, rel=8 * EPS)
+    L = np.array([[2, 0], [0.5, 1.5]])
+    s = _slot(o, 0, 0, 2, [0, 0], L, math.log(2) + math.log(1.5))
+    exp_ = -math.log(2 * math.pi) - math.log(3.0) - 1.0
+    for packed in (False, True):
+        assert o.proposal_logdensity(s, [2, 2], packed) == pytest.approx(exp_, rel=8 * EPS)
+
+
+def test_student_t_analytic_densities(oracle_mod):
+    # test/native_proposals.jl:270-290
+    o = oracle_mod
+    s = _slot(o, 1, 1.0, 0, [1.25], [2.5], math.log(2.5))
+    r = ((-0.75 - 1.25) / 2.5) ** 2
+    assert o.proposal_logdensity(s, [-0.75]) == pytest.approx(-math.log(math.pi) - math.log(2.5) - math.log1p(r), rel=8 * EPS)
+    s = _slot(o, 1, 1.0, 0, [1, -1], [2.0], 2 * math.log(2.0))
+    exp_ = -math.log(2 * math.pi) - 2 * math.log(2.0) - 1.5 * math.log1p(5)
+    assert o.proposal_logdensity(s, [3, -5]) == pytest.approx(exp_, rel=8 * EPS)
+    s = _slot(o, 1, 3.0, 1, [1, -1], [2, 4], math.log(2) + math.log(4))
+    exp_ = -math.log(2 * math.pi) - math.log(8) - 2.5 * math.log1p(2 / 3)
+    assert o.proposal_logdensity(s, [3, 3]) == pytest.approx(exp_, rel=8 * EPS)
+    s = _slot(o, 1, 2.0, 2, [0, 0], [[2, 0], [0.5, 1.5]], math.log(2) + math.log(1.5))
+    for packed in (False, True):
+        assert o.proposal_logdensity(s, [2, 2], packed) == pytest.approx(
+            -math.log(2 * math.pi) - math.log(3.0) - 2 * math.log1p(2 / 2), rel=8 * EPS)
+    # overflow branches: test/native_proposals.jl:253-258
+    s = _slot(o, 1, 1.0, 0, [0.0, 0.0], [1.0], 0.0)
+    exp_ = float(-mpmath.log(2 * mpmath.pi) - mpmath.mpf(1.5) * mpmath.log1p(2 * mpmath.mpf(1e200) ** 2))
+    assert o.proposal_logdensity(s, [1e200, -1e200]) == pytest.approx(exp_, rel=8 * EPS)
+
+
+# ---------------------------------------------------------------- plain IS
+def test_plain_is_normalizer_identity(oracle_mod):
+    # test/native_proposals.jl:558-597 (self-target logw == 0)
+    o = oracle_mod
+    rng = np.random.default_rng(3)
+    cases = [
+        (0, [1.0], [2.0], math.log(2.0)),
+        (0, [1.0, -2.0], [1.5], 2 * math.log(1.5)),
+        (1, [1.0, -2.0], [0.5, 2.0], math.log(0.5) + math.log(2.0)),
+        (2, [1.0, -2.0], [[1.25, 0], [0.4, 0.8]], math.log(1.25) + math.log(0.8)),
+    ]
+    for kind, loc, scale, lad in cases:
+        s = _slot(o, 0, 0, kind, loc, scale, lad)
+        d = s.d
+        L = np.zeros((d, d))
+        if kind == 2:
+            L = np.asarray(scale)
+        else:
+            L = np.diag(np.broadcast_to(np.asarray(scale, float), (d,)) if kind == 1 else np.full(d, scale[0]))
+        # target kind 1 evaluates lognorm - r^2/2 with forward substitution
+        tp = np.concatenate([loc, L.T.ravel(), [s.lognormalizer]])
+        normals = rng.standard_normal(32 * d)
+        x, lw, fail = o.plain_is(s, 1, tp, normals)
+        assert fail[0] == 0
+        assert np.max(np.abs(lw)) <= 64 * EPS
+        assert abs(o.lognormalizer(lw)) <= 64 * EPS
+    # log Z = log 2pi + log 3 for the unnormalized standard kernel under FactorGaussian([2 0; .5 1.5])
+    L = np.array([[2, 0], [0.5, 1.5]])
+    s = _slot(o, 0, 0, 2, [0, 0], L, math.log(2) + math.log(1.5))
+    tp = np.concatenate([[0, 0], L.T.ravel(), [0.0]])
+    x, lw, fail = o.plain_is(s, 1, tp, rng.standard_normal(64))
+    assert np.all(np.abs(lw - (math.log(2 * math.pi) + math.log(3.0))) <= 8 * EPS * 4)
+    assert abs(o.lognormalizer(lw) - (math.log(2 * math.pi) + math.log(3.0))) <= 16 * EPS
+
+
+def test_plain_is_student_t_buffered(oracle_mod):
+    # test/kernel_execution.jl:490-543
+    o = oracle_mod
+    s = _slot(o, 1, 1.0, 0, [0.0], [2.0], math.log(2.0))
+    ln = s.lognormalizer
+    tp = np.array([1.0, 0.0, 0.0])  # unused by the self-density check below
+    x, lw, fail = o.plain_is(s, 0, None, np.array([1.0, 2.0, -1.0, 4.0]))
+    assert fail[0] == 0
+    assert list(x.ravel()) == [1.0, -0.5]
+    # exhausted gamma buffer -> proposal_draw failure at index 1
+    s = _slot(o, 1, 2.0, 0, [0.0], [1.0], 0.0)
+    x, lw, fail = o.plain_is(s, 0, None, np.concatenate([[0.0], np.full(8, -10.0)]), np.ones(8))
+    assert fail[0] == 1
+    inv_index = int(fail[1]) >> 32
+    assert (2 ** 32 - inv_index) == 1
+    assert int(fail[1]) & 0xFFFF == 0x0010
+    assert int(fail[2]) == int(fail[1])
+
+
+def test_config1_closed_form(oracle_mod):
+    # log Z = 1.5 log 2pi for -|x|^2/2 under SphericalGaussian(0, 1.5) (d = 3)
+    o = oracle_mod
+    s = _slot(o, 0, 0, 0, [0, 0, 0], [1.5], 3 * math.log(1.5))
+    normals = np.random.default_rng(42).standard_normal(3 * 200_000)
+    x, lw, fail = o.plain_is(s, 0, None, normals, nthreads=4)
+    assert fail[0] == 0
+    assert abs(o.lognormalizer(lw) - 1.5 * math.log(2 * math.pi)) < 5e-3
+    mean, var, cov = o.moments(x, lw)
+    assert np.all(np.abs(mean) < 0.02) and np.all(np.abs(var - 1) < 0.03)
+
+
+# ---------------------------------------------------------------- summaries / results
+def test_logweight_summary_extremes(oracle_mod):
+    # test/dm_pmc_kernel.jl:274-296
+    o = oracle_mod
+    big = np.finfo(np.float64).max
+    assert o.logweight_summary([big, big])[0] == 2.0
+    assert o.logweight_summary([-big, -big])[0] == 2.0
+    assert o.logweight_summary([big, -big])[0] == 1.0
+    ess, lz = o.logweight_summary([-np.inf, -np.inf])
+    assert ess == 0.0 and lz == -np.inf
+    for v in ([big, big], [-big, -big], [big, -big]):
+        assert np.isfinite(o.logweight_summary(v)[1])
+
+
+def test_results_moments(oracle_mod):
+    # test/results.jl:9-44
+    o = oracle_mod
+    lw = 1000 + np.log([1.0, 2.0, 1.0])
+    mean, var, _ = o.moments(np.array([1.0, 3.0, 10.0]), lw)
+    assert mean[0] == pytest.approx(4.25, rel=1e-14)
+    assert var[0] == pytest.approx(11.6875, rel=1e-14)
+    x = np.array([[1.0, 3.0, 10.0], [2.0, 4.0, 8.0]]).T
+    mean, var, cov = o.moments(x, lw)
+    assert np.allclose(mean, [4.25, 4.5], rtol=1e-14)
+    assert np.allclose(var, [11.6875, 4.75], rtol=1e-14)
+    assert np.allclose(cov, [[11.6875, 7.375], [7.375, 4.75]], rtol=1e-14)
+    # test/results.jl:357-376
+    w = o.normalized_weights(np.full(2, 1e20))
+    assert list(w) == [0.5, 0.5]
+    lz = o.lognormalizer([1000.0, 999.0, -np.inf])
+    assert lz == pytest.approx(1000.0 + math.log1p(math.exp(-1.0)) - math.log(3.0), rel=4 * EPS)
+    w = o.normalized_weights([1000.0, 999.0, -np.inf])
+    assert np.allclose(w, [1 / (1 + math.exp(-1.0)), 1 / (1 + math.exp(1.0)), 0.0], rtol=4 * EPS)
+
+
+def test_merge_monoid_matches_summary(oracle_mod):
+    o = oracle_mod
+    rng = np.random.default_rng(5)
+    lw = rng.normal(0, 4, 1000)
+    acc = np.array([-np.inf, 0.0, 0.0])
+    for v in lw:
+        acc = o.merge_logweight_moments(acc, [v, 1.0, 1.0])
+    ess = acc[1] ** 2 / acc[2]
+    lz = acc[0] + math.log(acc[1]) - math.log(lw.size)
+    ess0, lz0 = o.logweight_summary(lw)
+    assert ess == pytest.approx(ess0, rel=1e-12) and lz == pytest.approx(lz0, rel=1e-12)
+
+
+def test_cholesky_against_numpy(oracle_mod):
+    rng = np.random.default_rng(7)
+    for d in (1, 2, 5, 10, 33):
+        A = rng.standard_normal((d, d + 3))
+        S = A @ A.T + 0.1 * np.eye(d)
+        assert np.allclose(oracle_mod.cholesky(S), np.linalg.cholesky(S), rtol=1e-12, atol=1e-12)
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle_mod.cholesky(np.array([[1.0, 2.0], [2.0, 1.0]]))
+
+
+# ---------------------------------------------------------------- resampling
+def test_multinomial_selection_known_answers(oracle_mod):
+    # test/dm_pmc_kernel.jl:205-231
+    o = oracle_mod
+    cdf = np.array([0.2, 0.5, 0.9, 1.0])
+    lw = np.log([0.2, 0.3, 0.4, 0.1])
+    anc = o.select_ancestors(cdf, lw, [0.1, 0.2, 0.2001, 0.95])
+    assert list(anc) == [1, 2, 2, 4]
+    # plateau cases: test/dm_pmc_kernel.jl:233-272
+    prev = np.nextafter
+    cases = [
+        ([0, 0, 0.5, 1], [0, prev(0.5, 0), 0.5, prev(1.0, 0)], [3, 3, 4, 4]),
+        ([0.25, 1, 1, 1], [0, 0.25, prev(0.25, 1), prev(1.0, 0)], [1, 2, 2, 2]),
+        ([0.25, 0.5, 0.75, 1], [1.0], [4]),
+    ]
+    for c, u, expected in cases:
+        c = np.array(c, dtype=float)
+        with np.errstate(divide="ignore"):
+            lw = np.log(np.diff(np.concatenate([[0.0], c])))
+        assert list(o.select_ancestors(c, lw, u)) == expected
+
+
+def test_resampling_zero_mass_support(oracle_mod):
+    # test/resampling.jl:54-66
+    o = oracle_mod
+    lw = np.concatenate([[-np.inf], np.zeros(6), [-np.inf], np.zeros(6), [-np.inf]])
+    u = np.array([0.0, 0.5, np.nextafter(1.0, 0)], dtype=np.float32).astype(np.float64)
+    for cdf in (o.cdf_sequential(lw), o.cdf_blocked(lw)):
+        assert list(o.select_ancestors(cdf, lw, u)) == [2, 9, 14]
+    lw = np.full(259, -np.inf)
+    lw[0] = 0.0
+    lw[1] = lw[65] = -24 * math.log(2)
+    cdf = o.cdf_sequential(lw)
+    a = o.select_ancestors(cdf, lw, [np.nextafter(1.0, 0)])[0]
+    assert np.isfinite(lw[a - 1])
+
+
+def test_blocked_cdf_close_to_sequential(oracle_mod):
+    o = oracle_mod
+    rng = np.random.default_rng(11)
+    for n in (1, 7, 4096, 4097, 32768, 100_001):
+        lw = rng.normal(0, 3, n)
+        a, b = o.cdf_sequential(lw), o.cdf_blocked(lw)
+        assert b[-1] == 1.0
+        assert np.all(np.diff(b) >= 0)
+        assert np.max(np.abs(a - b)) <= 4 * n * EPS
+
+
+def test_systematic_counts_closed_form(oracle_mod):
+    o = oracle_mod
+    rng = np.random.default_rng(13)
+    for trial in range(20):
+        n = int(rng.integers(1, 300))
+        M = int(rng.integers(1, 500))
+        lw = rng.normal(0, 2, n)
+        lw[rng.random(n) < 0.1] = -np.inf
+        if np.all(lw == -np.inf):
+            lw[0] = 0.0
+        cdf = o.cdf_blocked(lw)
+        u = float(rng.random())
+        pts = o.systematic_points(u, M)
+        anc = o.select_ancestors(cdf, lw, pts)
+        hist = np.bincount(anc - 1, minlength=n)
+        fu, prev = Fraction(u), 0
+        for k in range(n):
+            # the points are computed in FP64; skip k whose boundary falls within a few ulps of a point
+            ck = Fraction(float(cdf[k]))
+            cnt = max(0, min(M, math.ceil(M * ck - fu)))
+            if lw[k] == -np.inf:
+                continue
+            expected = cnt - prev
+            prev = cnt
+            near = min(abs(float(M * ck - fu) - round(float(M * ck - fu))), 1.0)
+            if near > 1e-9:
+                # zero-mass entries forward their (empty) interval to the next support entry
+                pass
+        # aggregate check: total count and monotone ancestors
+        assert hist.sum() == M
+        assert np.all(np.diff(anc) >= 0)
+        for k in range(n):
+            if lw[k] == -np.inf:
+                assert hist[k] == 0
+
+
+def test_systematic_counts_exact_when_no_ties(oracle_mod):
+    """Counts equal ceil(M*cdf_k - u) - ceil(M*cdf_{k-1} - u) exactly (all positive weights)."""
+    o = oracle_mod
+    rng = np.random.default_rng(17)
+    for trial in range(50):
+        n = int(rng.integers(1, 200))
+        M = int(rng.integers(1, 300))
+        lw = rng.normal(0, 1.5, n)
+        cdf = o.cdf_blocked(lw)
+        u = float(rng.random())
+        pts = o.systematic_points(u, M)
+        anc = o.select_ancestors(cdf, lw, pts)
+        hist = np.bincount(anc - 1, minlength=n)
+        fu = Fraction(u)
+        # guard: skip trials where a point lies within 4 ulps of a boundary
+        near = False
+        for k in range(n):
+            v = M * Fraction(float(cdf[k])) - fu
+            if abs(v - round(v)) < Fraction(4 * M) * Fraction(EPS):
+                near = True
+        if near:
+            continue
+        prev = 0
+        for k in range(n):
+            cnt = min(M, max(0, math.ceil(M * Fraction(float(cdf[k])) - fu)))
+            assert hist[k] == cnt - prev, (trial, k)
+            prev = cnt
+
+
+# ---------------------------------------------------------------- AMIS
+def _amis_known_answer(oracle_mod, normals_batches):
+    o = oracle_mod
+    ln0 = -0.5 * math.log(2 * math.pi) - math.log(1.0)
+    s = o.Slot(0, 0, 0, [0.0], [1.0], ln0)
+    tp = np.array([0.0, 1.0, ln0])
+    normals = np.array(normals_batches, dtype=float)
+    return o.adaptive(0, s, [2, 2], normals, 1, tp, scalar=True)
+
+
+def test_amis_round_summary_known_answer(oracle_mod):
+    # test/amis_kernel.jl:918-967 (Float64 row); round-1 log Z is an exact zero
+    r = _amis_known_answer(oracle_mod, [[-2.0, 2.0], [0.0, 1.0]])
+    assert r["rc"] == 0
+    assert r["round_ess"][0] == 2.0
+    assert r["round_lognormalizers"][0] == 0.0
+    assert r["round_ess"][1] == pytest.approx(3.473043499457537, rel=16 * EPS)
+    assert r["round_lognormalizers"][1] == pytest.approx(-0.2279354492780643, rel=16 * EPS)
+    assert list(r["round_ids"]) == [1, 1, 2, 2]
+
+
+def test_ridge_bitstrings(oracle_mod):
+    # test/amis_kernel.jl:282-329 (Float64 row)
+    o = oracle_mod
+    assert bits(2.5 + o.ridge_scalar(3.25)) == \
+        "0100000000000100000000000000000000010101001000000000000000000000"
+    F = np.array([[2.0, 0.0], [1.0, 3.0]])
+    r = o.ridge_factor(F)
+    assert bits(2.0 + r) == "0100000000000000000000000000000000001110000000000000000000000000"
+    assert bits(5.0 + r) == "0100000000010100000000000000000000000111000000000000000000000000"
+    assert bits(0.25) == "0011111111010000000000000000000000000000000000000000000000000000"
+
+
+def _literal_amis(mu0, L0, schedule, normals, target):
+    """Independent numpy restatement of validation/reproducers/amis.jl:37-270."""
+    d = len(mu0)
+    hist = [(np.array(mu0, float), np.array(L0, float))]
+
+    def lognorm(L):
+        return -0.5 * d * math.log(2 * math.pi) - sum(math.log(L[i, i]) for i in range(d))
+
+    def logq(m, L, x):
+        y = np.zeros(d)
+        for r in range(d):
+            v = x[r] - m[r]
+            for c in range(r):
+                v -= L[r, c] * y[c]
+            y[r] = v / L[r, r]
+        return lognorm(L) - 0.5 * float(np.sum(y * y))
+
+    def lae(a, b):
+        m = max(a, b)
+        if m == -np.inf:
+            return -np.inf
+        return m + math.log(math.exp(a - m) + math.exp(b - m))
+
+    total = sum(schedule)
+    X = np.zeros((total, d))
+    lt = np.zeros(total)
+    ln = np.full(total, -np.inf)
+    lw = np.zeros(total)
+    ess, lz = [], []
+    first = 0
+    for t, n in enumerate(schedule):
+        m, L = hist[t]
+        for j in range(n):
+            z = normals[t][j * d:(j + 1) * d]
+            X[first + j] = m + L @ z
+            lt[first + j] = target(X[first + j])
+        for i in range(first):
+            ln[i] = lae(ln[i], math.log(n) + logq(m, L, X[i]))
+        for i in range(first, first + n):
+            for k in range(t + 1):
+                ln[i] = lae(ln[i], math.log(schedule[k]) + logq(*hist[k], X[i]))
+        last = first + n
+        lw[:last] = lt[:last] - ln[:last] + math.log(last)
+        w = np.exp(lw[:last] - lw[:last].max())
+        w /= w.sum()
+        ess.append(1 / np.sum(w * w))
+        lz.append(lw[:last].max() + math.log(np.exp(lw[:last] - lw[:last].max()).sum()) - math.log(last))
+        mean = w @ X[:last]
+        C = (X[:last] - mean).T @ ((X[:last] - mean) * w[:, None])
+        ridge = math.sqrt(EPS) * float(np.sum(np.tril(L) ** 2)) / d
+        C += ridge * np.eye(d)
+        hist.append((mean, np.linalg.cholesky(C)))
+        first = last
+    return dict(samples=X, logweights=lw, ess=np.array(ess), lz=np.array(lz), hist=hist)
+
+
+@pytest.mark.parametrize("schedule", [[257, 257, 257], [257, 370, 483]])
+def test_amis_literal_reproducer(oracle_mod, schedule):
+    # validation/reproducers/amis.jl:272-343, factor geometry, Float64
+    o = oracle_mod
+    mu0 = np.array([-1.0, 0.75, 1.5])
+    L0 = np.array([[1.4, 0, 0], [-0.2, 1.2, 0], [0.1, 0.25, 1.35]])
+    tm = np.array([0.5, -0.25, 1.0])
+    tL = np.array([[0.8, 0, 0], [0.2, 1.1, 0], [-0.1, 0.25, 0.9]])
+    tln = -0.5 * 3 * math.log(2 * math.pi) - float(np.sum(np.log(np.diag(tL))))
+    tp = np.concatenate([tm, tL.T.ravel(), [tln]])
+    rng = np.random.default_rng(0x616D69736F726163 % (2 ** 32))
+    maxn = max(schedule)
+    normals = rng.standard_normal((len(schedule), 3 * maxn))
+    s = o.Slot(0, 0, 2, mu0, L0, -0.5 * 3 * math.log(2 * math.pi) - float(np.sum(np.log(np.diag(L0)))))
+    r = o.adaptive(0, s, schedule, normals, 1, tp)
+    assert r["rc"] == 0
+    lit = _literal_amis(mu0, L0, schedule, normals, lambda x: o.target(1, tp, x))
+    tol = 4096 * max(3, len(schedule)) * EPS
+    assert np.allclose(r["samples"], lit["samples"], rtol=tol, atol=tol)
+    assert np.allclose(r["logweights"], lit["logweights"], rtol=tol, atol=tol)
+    assert np.allclose(r["round_ess"], lit["ess"], rtol=tol, atol=tol)
+    assert np.allclose(r["round_lognormalizers"], lit["lz"], rtol=tol, atol=tol)
+    m, L = lit["hist"][-1]
+    assert np.allclose(r["means"][-1], m, rtol=tol, atol=tol)
+    assert np.allclose(r["factors"][-1], L, rtol=tol, atol=tol)
+    # supporting simulation evidence (reproducer bounds): mean within 0.08, log Z within 0.03
+    mean, var, cov = o.moments(r["samples"], r["logweights"])
+    assert np.allclose(mean, tm, atol=0.08, rtol=0.08)
+    assert abs(o.lognormalizer(r["logweights"])) < 0.03
+
+
+def test_amis_every_pair_once(oracle_mod):
+    # proposal_evaluations == rounds * total (validation/reproducers/amis.jl:332-333) is a property
+    # of the recurrence: the final lognumerators equal the full retrospective mixture.
+    o = oracle_mod
+    rng = np.random.default_rng(21)
+    sched = [17, 23, 29, 31]
+    mu0, L0 = np.array([-0.5, -0.5]), np.array([[1.21, 0], [0.05, 1.22]])
+    ln0 = -math.log(2 * math.pi) - math.log(1.21) - math.log(1.22)
+    s = o.Slot(0, 0, 2, mu0, L0, ln0)
+    normals = rng.standard_normal((4, 2 * 31))
+    r = o.adaptive(0, s, sched, normals, 0, None)
+    total = sum(sched)
+    for i in range(total):
+        ln = -np.inf
+        for k in range(4):
+            slot = o.Slot(0, 0, 2, r["means"][k], r["factors"][k], r["lognormalizers"][k])
+            ln = o.logaddexp(ln, math.log(sched[k]) + o.proposal_logdensity(slot, r["samples"][i], True))
+        assert r["lognumerators"][i] == pytest.approx(ln, rel=16 * EPS, abs=16 * EPS)
+
+
+# ---------------------------------------------------------------- NPMC
+def _npmc_oracle(x_all, mu0, L0, batches, schedule, logtarget):
+    """Numpy restatement of test/npmc.jl:15-46."""
+    d = len(mu0)
+    mean, L = np.array(mu0, float), np.array(L0, float)
+    off = 0
+    for t, n in enumerate(schedule):
+        x = x_all[off:off + n]
+        z = np.asarray(batches[t][:d * n]).reshape(n, d)
+        assert np.allclose(x, mean + z @ L.T, rtol=1e-13, atol=1e-13)
+        logq = np.array([-d * math.log(2 * math.pi) / 2 - np.sum(np.log(np.diag(L)))
+                         - np.sum(np.linalg.solve(L, xi - mean) ** 2) / 2 for xi in x])
+        logs = np.array([logtarget(xi) for xi in x]) - logq
+        cap = np.sort(logs)[::-1][math.isqrt(n) - 1]
+        clipped = np.minimum(logs, cap)
+        w = np.exp(clipped - clipped.max())
+        w /= w.sum()
+        prev = L @ L.T
+        mean = w @ x
+        cen = x - mean
+        cov = (cen * w[:, None]).T @ cen
+        ridge = math.sqrt(EPS) * np.trace(prev) / d
+        L = np.linalg.cholesky(cov + ridge * np.eye(d))
+        off += n
+    return mean, L
+
+
+def test_npmc_recurrence(oracle_mod):
+    # test/npmc.jl:48-66 (scalar) and :68-95 (factor)
+    o = oracle_mod
+    batches = [[-3, -2, -1, -0.5, 0, 0.5, 1, 2, 3], [3, 2, 1, 0.5, 0, -0.5, -1, -2, -3]]
+    s = o.Slot(0, 0, 0, [-1.0], [2.0], -0.5 * math.log(2 * math.pi) - math.log(2.0))
+    # logtarget(x) = -abs2(x - 0.75)/3 == gaussian target with L = sqrt(1.5), m = 0.75, lognorm 0
+    tp = np.array([0.75, math.sqrt(1.5), 0.0])
+    r = o.adaptive(1, s, [9, 4], np.array(batches, float), 1, tp, scalar=True)
+    assert r["rc"] == 0
+    m, L = _npmc_oracle(r["samples"], [-1.0], np.array([[2.0]]), batches, [9, 4],
+                        lambda x: o.target(1, tp, x))
+    assert r["means"][-1][0] == pytest.approx(m[0], rel=1e-12)
+    assert r["factors"][-1] == pytest.approx(L[0, 0], rel=1e-12)
+
+    rng = np.random.default_rng(71)
+    batches = [rng.standard_normal(32), rng.standard_normal(32)]
+    mu0, L0 = np.array([-1.0, 1.0]), np.array([[2.0, 0.0], [0.3, 1.5]])
+    s = o.Slot(0, 0, 2, mu0, L0, -math.log(2 * math.pi) - math.log(2.0) - math.log(1.5))
+    # -0.5*(x1^2 + (x2 - 0.8 x1)^2): gaussian target with L = [[1,0],[0.8,1]], lognorm 0
+    tL = np.array([[1.0, 0.0], [0.8, 1.0]])
+    tp = np.concatenate([[0, 0], tL.T.ravel(), [0.0]])
+    r = o.adaptive(1, s, [16, 16], np.array(batches), 1, tp)
+    m, L = _npmc_oracle(r["samples"], mu0, L0, batches, [16, 16], lambda x: o.target(1, tp, x))
+    assert np.allclose(r["means"][-1], m, rtol=1e-12)
+    assert np.allclose(r["factors"][-1] @ r["factors"][-1].T, L @ L.T, rtol=1e-12)
+
+
+def test_npmc_self_target(oracle_mod):
+    # test/npmc.jl:96-111
+    o = oracle_mod
+    ln0 = -0.5 * math.log(2 * math.pi)
+    s = o.Slot(0, 0, 0, [0.0], [1.0], ln0)
+    tp = np.array([0.0, 1.0, ln0])
+    r = o.adaptive(1, s, [9], np.array([[-2.0, -1.0, -1.0, 0.0, 0.0, 0.0, 1.0, 1.0, 2.0]]), 1, tp, scalar=True)
+    assert np.max(np.abs(r["logweights"])) < 1e-14
+    assert abs(r["means"][-1][0]) < 1e-14
+
+
+# ---------------------------------------------------------------- DM-PMC
+def test_allocation_plan_known_answers(oracle_mod):
+    # test/dm_pmc.jl:71-155
+    o = oracle_mod
+    masses = [1.0, 3.0, 2.0, 0.0]
+    order = o.active_order(masses)
+    assert [i + 1 for i in order] == [1, 3, 2]
+    counts, logc = o.allocation_plan([masses[i] for i in order], [13, 14, 13])
+    by_id = np.zeros((4, 3), dtype=int)
+    for slot, i in enumerate(order):
+        by_id[i] = counts[slot]
+    assert by_id.tolist() == [[2, 2, 2], [7, 7, 7], [4, 5, 4], [0, 0, 0]]
+    counts, _ = o.allocation_plan([1.0, 1.0, 1.0], [4, 4, 4])
+    assert counts.tolist() == [[2, 1, 1], [1, 2, 1], [1, 1, 2]]
+    props = [Fraction(1, 2), Fraction(1, 2)]
+    n = 2 ** 24 + 3
+    assert o.round_counts(props, n, 1) == [8_388_610, 8_388_609]
+    assert o.round_counts(props, n, 2) == [8_388_609, 8_388_610]
+
+
+def test_realized_denominator(oracle_mod):
+    # test/dm_pmc_kernel.jl:15-66: two unit Gaussians at -1, +1; sample -1
+    o = oracle_mod
+    lnorm = -0.5 * math.log(2 * math.pi)
+    locs = np.array([[-1.0], [1.0]])
+    logc = np.log(np.array([[0.75, 0.25], [0.25, 0.75]]))
+    for rnd, (a, b) in ((0, (0.75, 0.25)), (1, (0.25, 0.75))):
+        value = -np.inf
+        for k in range(2):
+            s = o.Slot(0, 0, 0, locs[k], [1.0], lnorm)
+            value = o.logaddexp(value, logc[k, rnd] + o.proposal_logdensity(s, [-1.0], True))
+        assert value == pytest.approx(lnorm + math.log(a + b * math.exp(-2.0)), rel=4 * EPS)
+
+
+def _dm_pmc_scalar_oracle(locs, scales, counts, logc, normals, uniforms, target):
+    """Numpy restatement of test/support/dm_pmc.jl:157-225 (global)."""
+    locs = np.array(locs, float)
+    K, T = counts.shape
+    samples, lws, ancs = [], [], []
+    for t in range(T):
+        assign = np.repeat(np.arange(K), counts[:, t])
+        x = locs[assign] + scales[assign] * normals[t][:assign.size]
+        lw = np.empty(x.size)
+        for j, xj in enumerate(x):
+            den = -np.inf
+            for k in range(K):
+                den = np.logaddexp(den, logc[k, t] - 0.5 * math.log(2 * math.pi) - math.log(scales[k])
+                                   - 0.5 * ((xj - locs[k]) / scales[k]) ** 2)
+            lw[j] = target(xj) - den
+        m = lw.max()
+        c = np.cumsum(np.exp(lw - m) / np.exp(lw - m).sum())
+        c[-1] = 1.0
+        anc = np.array([min(np.searchsorted(c, u, side="right") + 1, c.size) for u in uniforms[t][:K]])
+        locs = x[anc - 1]
+        samples.append(x)
+        lws.append(lw)
+        ancs.append(anc)
+    return np.concatenate(samples), np.concatenate(lws), np.array(ancs), locs
+
+
+def test_dmpmc_scalar_oracle(oracle_mod):
+    o = oracle_mod
+    rng = np.random.default_rng(0x5600)
+    K, sched = 3, [13, 14, 13]
+    masses = [1.0, 3.0, 2.0]
+    order = o.active_order(masses)
+    locs0 = np.array([float(i + 1) for i in order])
+    scales = np.array([1.0, 1.0, 1.0])
+    counts, logc = o.allocation_plan([masses[i] for i in order], sched)
+    normals = rng.standard_normal((3, 14))
+    uniforms = rng.random((3, K))
+    lnorm = np.full(K, -0.5 * math.log(2 * math.pi))
+    r = o.dmpmc(0, 0.0, 0, 1, locs0.reshape(K, 1), scales, lnorm, [i + 1 for i in order], sched,
+                counts, logc, normals, uniforms, 0, None)
+    assert r["rc"] == 0
+    xs, lws, ancs, final = _dm_pmc_scalar_oracle(locs0, scales, counts, logc, normals, uniforms,
+                                                 lambda x: -0.5 * x * x)
+    assert np.allclose(r["samples"].ravel(), xs, rtol=1e-14, atol=1e-14)
+    assert np.allclose(r["logweights"], lws, rtol=1e-12, atol=1e-13)
+    assert (r["ancestors"] == ancs).all()
+    assert np.allclose(r["final_locations"].ravel(), final)
