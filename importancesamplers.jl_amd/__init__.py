@@ -1,0 +1,47 @@
+"""MI355X (gfx950) engine for the ImportanceSamplers.jl sampling hot path.
+
+The compute lives in ``libissampler.so`` (hand-written HIP kernels behind the
+C ABI of ``include/issampler.h``).  This package is the host-side mirror of the
+reference API used by the tests and the benchmark.
+"""
+
+from .api import (  # noqa: F401
+    AMIS,
+    NPMC,
+    AllZeroWeightsError,
+    AMISRoundError,
+    DeterministicMixturePMC,
+    DiagonalGaussian,
+    DiagonalStudentT,
+    DMPMCRoundError,
+    FactorGaussian,
+    FactorStudentT,
+    GaussianTarget,
+    GlobalResampling,
+    ImportanceSampling,
+    LinearRegressionTarget,
+    LogisticRegressionTarget,
+    MultinomialResampling,
+    NPMCRoundError,
+    PrefilledRNG,
+    ProposalBank,
+    SamplerBusyError,
+    SamplerExecutionError,
+    SphericalGaussian,
+    SphericalStudentT,
+    StandardGaussianTarget,
+    SystematicResampling,
+    WeightedSamples,
+    context,
+    cov,
+    current_proposal,
+    importance_sample,
+    importance_sample_,
+    lognormalizer,
+    mean,
+    prepare_sampler,
+    resample,
+    student_t_logconstant,
+    var,
+)
+from . import _lib  # noqa: F401

@@ -1,0 +1,745 @@
+"""Host mirror of the ImportanceSamplers.jl user API over the gfx950 engine.
+
+Names follow the reference (src/ImportanceSamplers.jl:19-91); Julia's
+mutating `importance_sample!` is `importance_sample_` here.  Everything that
+computes runs in libissampler.so on the GPU; this module only builds job
+descriptors, moves host buffers across the C ABI and raises the reference's
+error types.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import F64P, I64P, VP, JobDesc
+
+GAUSSIAN, STUDENT_T = 0, 1
+SPHERICAL, DIAGONAL, FACTOR = 0, 1, 2
+PLAIN, AMIS_M, NPMC_M, DMPMC_M = 0, 1, 2, 3
+FP64, FP32 = 0, 1
+
+PHASES = {0: None, 1: "sampling", 2: "target", 3: "denominator", 4: "weight", 5: "moment",
+          6: "factorization", 7: "resampling"}
+PLAIN_PHASES = {1: "proposal_draw", 2: "target", 3: "proposal_logdensity", 4: "logweight"}
+
+
+# ---------------------------------------------------------------- errors (reference types)
+class SamplerExecutionError(RuntimeError):
+    """src/methods/importance_sampling.jl:210-225: phase and first logical index."""
+
+    def __init__(self, phase, index, reason_bits=0, message=""):
+        super().__init__(f"sampler failed during {phase} at sample {index}: {message}")
+        self.phase = phase
+        self.index = index
+        self.reason_bits = reason_bits
+
+
+class _RoundError(RuntimeError):
+    method = "sampler"
+
+    def __init__(self, round, phase, cause=None, diagnostics=None):
+        super().__init__(f"{self.method} failed in round {round} during {phase}: {cause}")
+        self.round = round
+        self.phase = phase
+        self.cause = cause
+        self.diagnostics = diagnostics or {}
+
+
+class AMISRoundError(_RoundError):
+    method = "AMIS"
+
+
+class NPMCRoundError(_RoundError):
+    method = "NPMC"
+
+
+class DMPMCRoundError(_RoundError):
+    method = "DM-PMC"
+
+
+class AllZeroWeightsError(ArithmeticError):
+    """src/results.jl: every raw log weight is -Inf."""
+
+
+class SamplerBusyError(RuntimeError):
+    """src/methods/importance_sampling.jl:105-116."""
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc == _lib.IS_OK:
+        return
+    msg = _lib.last_error()
+    if rc == _lib.IS_EBUSY:
+        raise SamplerBusyError(msg)
+    if rc == _lib.IS_EINVAL:
+        raise ValueError(msg)
+    if rc == _lib.IS_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise EngineError(f"engine error {rc}: {msg}")
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a, t=F64P):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------- proposals (src/proposals.jl)
+def _gaussian_lognormalizer(d, logabsdet):
+    return -0.5 * d * math.log(2.0 * math.pi) - logabsdet
+
+
+def student_t_logconstant(dof, d):
+    """src/proposals.jl:410-437 (host evaluation for proposal constructors)."""
+    value = -d * math.log(2.0 * math.pi) / 2.0
+    if d % 2 == 1:
+        shifted, correction = dof, 0.0
+        while shifted < 32.0:
+            if shifted < 1.0:
+                correction += (math.log(shifted) + math.log(shifted + 2.0)) / 2.0 - math.log1p(shifted)
+            else:
+                correction += math.log1p(-((1.0 / (shifted + 1.0)) ** 2)) / 2.0
+            shifted += 2.0
+        r = 1.0 / shifted
+        r2 = r * r
+        poly = 691.0 / 88.0
+        for coef in (-31.0 / 36.0, 17.0 / 112.0, -1.0 / 20.0, 1.0 / 24.0, -1.0 / 4.0):
+            poly = poly * r2 + coef
+        value += correction + r * poly
+    acc = 0.0
+    for offset in range(1 if d % 2 == 1 else 0, d - 1, 2):
+        x = float(offset)
+        acc += math.log1p(x / dof) if x <= dof else math.log(x) + math.log1p(dof / x) - math.log(dof)
+    return value + acc
+
+
+@dataclass
+class NativeProposal:
+    """A native radial proposal (`_GaussianProposal` / `_StudentTProposal`)."""
+
+    family: int
+    kind: int
+    location: np.ndarray
+    scale: np.ndarray
+    lognormalizer: float
+    dof: float = 0.0
+    scalar: bool = False
+
+    @property
+    def dim(self):
+        return self.location.size
+
+    def factor(self):
+        d = self.dim
+        if self.kind == FACTOR:
+            return self.scale.reshape(d, d).copy()
+        if self.kind == DIAGONAL:
+            return np.diag(self.scale)
+        return np.eye(d) * self.scale[0]
+
+
+def _loc(location):
+    scalar = np.ndim(location) == 0
+    loc = np.atleast_1d(np.asarray(location, dtype=np.float64)).copy()
+    if loc.size == 0 or not np.all(np.isfinite(loc)):
+        raise ValueError("native parameter must contain only finite values")
+    return loc, scalar
+
+
+def SphericalGaussian(location, scale):
+    loc, scalar = _loc(location)
+    s = float(scale)
+    if not (math.isfinite(s) and s > 0):
+        raise ValueError("scale must be finite and positive")
+    ln = _gaussian_lognormalizer(loc.size, loc.size * math.log(s))
+    return NativeProposal(GAUSSIAN, SPHERICAL, loc, np.array([s]), ln, scalar=scalar)
+
+
+def DiagonalGaussian(location, scales):
+    loc, scalar = _loc(location)
+    if scalar:
+        raise ValueError("DiagonalGaussian requires a vector location")
+    sc = _f64(scales).ravel()
+    if sc.size != loc.size or not np.all(np.isfinite(sc) & (sc > 0)):
+        raise ValueError("diagonal scales must be finite, positive and match the location length")
+    ln = _gaussian_lognormalizer(loc.size, float(np.sum(np.log(sc))))
+    return NativeProposal(GAUSSIAN, DIAGONAL, loc, sc, ln)
+
+
+def _factor(loc, factor):
+    L = np.array(factor, dtype=np.float64)
+    d = loc.size
+    if L.shape != (d, d):
+        raise ValueError("factor must be square and match the location length")
+    if not np.all(np.isfinite(L)):
+        raise ValueError("factor must contain only finite values")
+    if np.any(np.triu(L, 1) != 0):
+        raise ValueError("factor must be lower triangular")
+    if np.any(np.diag(L) <= 0):
+        raise ValueError("factor diagonal must be positive")
+    return L
+
+
+def FactorGaussian(location, factor):
+    loc, scalar = _loc(location)
+    if scalar:
+        raise ValueError("FactorGaussian requires a vector location")
+    L = _factor(loc, factor)
+    ln = _gaussian_lognormalizer(loc.size, float(np.sum(np.log(np.diag(L)))))
+    return NativeProposal(GAUSSIAN, FACTOR, loc, L.ravel(), ln)
+
+
+def _dof(dof):
+    v = float(dof)
+    if not (math.isfinite(v) and v > 0):
+        raise ValueError("degrees of freedom must be finite and positive")
+    return v
+
+
+def SphericalStudentT(dof, location, scale):
+    nu = _dof(dof)
+    p = SphericalGaussian(location, scale)
+    p.family, p.dof = STUDENT_T, nu
+    p.lognormalizer = student_t_logconstant(nu, p.dim) - p.dim * math.log(float(scale))
+    return p
+
+
+def DiagonalStudentT(dof, location, scales):
+    nu = _dof(dof)
+    p = DiagonalGaussian(location, scales)
+    p.family, p.dof = STUDENT_T, nu
+    p.lognormalizer = student_t_logconstant(nu, p.dim) - float(np.sum(np.log(p.scale)))
+    return p
+
+
+def FactorStudentT(dof, location, factor):
+    nu = _dof(dof)
+    p = FactorGaussian(location, factor)
+    p.family, p.dof = STUDENT_T, nu
+    p.lognormalizer = student_t_logconstant(nu, p.dim) - float(np.sum(np.log(np.diag(p.factor()))))
+    return p
+
+
+@dataclass
+class ProposalBank:
+    """src/proposal_banks.jl:12-97: proposals with nonnegative masses (zero = inert)."""
+
+    proposals: list
+    masses: np.ndarray = None
+
+    def __post_init__(self):
+        if self.masses is None:
+            self.masses = np.ones(len(self.proposals))
+        self.masses = _f64(self.masses).ravel()
+        if len(self.proposals) == 0 or self.masses.size != len(self.proposals):
+            raise ValueError("bank needs one mass per proposal")
+        p0 = self.proposals[0]
+        for p in self.proposals:
+            if (p.family, p.kind, p.dim, p.dof) != (p0.family, p0.kind, p0.dim, p0.dof):
+                raise ValueError("bank proposals must share family, scale kind, dimension and dof")
+
+
+# ---------------------------------------------------------------- targets (engine built-ins)
+@dataclass
+class StandardGaussianTarget:
+    """log pi(x) = -sum(abs2, x)/2 (the plain-IS known-answer target)."""
+
+    kind: int = 0
+
+    def params(self, d):
+        return np.zeros(1)
+
+
+@dataclass
+class GaussianTarget:
+    """Normalized Gaussian log density N(mean, L L^T) (or with `lognormalizer` overridden)."""
+
+    mean: np.ndarray
+    factor: np.ndarray
+    lognormalizer: float = None
+    kind: int = 1
+
+    def __post_init__(self):
+        self.mean = np.atleast_1d(_f64(self.mean))
+        self.factor = np.atleast_2d(_f64(self.factor))
+        if self.lognormalizer is None:
+            self.lognormalizer = _gaussian_lognormalizer(self.mean.size, float(np.sum(np.log(np.diag(self.factor)))))
+
+    def params(self, d):
+        return np.concatenate([self.mean, self.factor.T.ravel(), [self.lognormalizer]])
+
+
+@dataclass
+class LogisticRegressionTarget:
+    """Bayesian logistic regression with prior N(0, prior_var I)."""
+
+    X: np.ndarray
+    y: np.ndarray
+    prior_var: float = 10.0
+    kind: int = 2
+
+    def params(self, d):
+        X = _f64(self.X)
+        return np.concatenate([[X.shape[0], X.shape[1], self.prior_var], X.ravel(), _f64(self.y)])
+
+
+@dataclass
+class LinearRegressionTarget:
+    """Bayesian linear regression y ~ N(X beta, sigma^2), prior N(0, tau^2 I)."""
+
+    X: np.ndarray
+    y: np.ndarray
+    sigma: float = 0.5
+    tau: float = 2.0
+    kind: int = 3
+
+    def params(self, d):
+        X = _f64(self.X)
+        return np.concatenate([[X.shape[0], X.shape[1], self.sigma, self.tau], X.ravel(), _f64(self.y)])
+
+
+# ---------------------------------------------------------------- algorithms (src/methods/*.jl)
+def _schedule(rounds, round_size):
+    if int(rounds) < 1:
+        raise ValueError("rounds must be positive")
+    if np.ndim(round_size) == 0:
+        sched = [int(round_size)] * int(rounds)
+    else:
+        sched = [int(v) for v in round_size]
+        if len(sched) != int(rounds):
+            raise ValueError("round_size must have one entry per round")
+    if any(v <= 0 for v in sched):
+        raise ValueError("round sizes must be positive")
+    return sched
+
+
+@dataclass
+class ImportanceSampling:
+    proposal: NativeProposal
+    nsamples: int
+
+    def __post_init__(self):
+        if int(self.nsamples) <= 0:
+            raise ValueError("nsamples must be positive")
+
+
+@dataclass
+class AMIS:
+    proposal: NativeProposal
+    rounds: int
+    round_size: object
+
+    def __post_init__(self):
+        self.schedule = _schedule(self.rounds, self.round_size)
+        if self.proposal.family == STUDENT_T and not self.proposal.dof > 2:
+            raise ValueError("covariance adaptation requires Student-t degrees of freedom > 2")
+
+
+@dataclass
+class NPMC(AMIS):
+    pass
+
+
+class MultinomialResampling:
+    code = 0
+
+
+class SystematicResampling:
+    """New method (the reference has only multinomial): one uniform, points (j + u)/M."""
+
+    code = 1
+
+
+class GlobalResampling:
+    pass
+
+
+@dataclass
+class DeterministicMixturePMC:
+    bank: ProposalBank
+    rounds: int
+    round_size: object
+    resampling: object = None
+    method: object = None  # MultinomialResampling (default) or SystematicResampling
+
+    def __post_init__(self):
+        self.schedule = _schedule(self.rounds, self.round_size)
+        if self.resampling is not None and not isinstance(self.resampling, GlobalResampling):
+            raise NotImplementedError("only GlobalResampling is provided by this engine")
+        if self.method is None:
+            self.method = MultinomialResampling()
+
+
+# ---------------------------------------------------------------- RNG seams
+@dataclass
+class PrefilledRNG:
+    """Feeds exact normals/uniforms per round (the reference tests' prefilled RNG seam).
+
+    normals[t]: plain IS -> stride*n (per sample d normals then radial normals);
+    adaptive -> d*N_t.  radial_normals/radial_uniforms[t]: Student-t gamma buffers.
+    uniforms[t]: DM-PMC resampling uniforms (K per round)."""
+
+    normals: list
+    radial_normals: list = None
+    radial_uniforms: list = None
+    uniforms: list = None
+    seed: int = 0
+
+
+# ---------------------------------------------------------------- context
+_ctx = None
+_ctx_lock = threading.Lock()
+
+
+def context(device=0):
+    global _ctx
+    with _ctx_lock:
+        if _ctx is None:
+            lib = _lib.load()
+            h = VP()
+            _check(lib.is_ctx_create(device, ctypes.byref(h)))
+            _ctx = h
+    return _ctx
+
+
+# ---------------------------------------------------------------- prepared sampler
+class PreparedSampler:
+    """`_PreparedImportanceSampler` on the MI355X device (one engine job)."""
+
+    def __init__(self, rng, target, algorithm, precision=FP64):
+        self.algorithm = algorithm
+        self.target = target
+        self.precision = precision
+        lib = _lib.load()
+        ctx = context()
+        desc = JobDesc()
+        keep = []  # keep numpy buffers alive across the call
+
+        def arr(a):
+            a = _f64(a)
+            keep.append(a)
+            return _p(a)
+
+        seed = rng.seed if isinstance(rng, PrefilledRNG) else int(rng)
+        desc.seed = seed & 0xFFFFFFFFFFFFFFFF
+        desc.stream = 0x6D693335357873  # fixed second key word
+        desc.precision = precision
+        if isinstance(algorithm, ImportanceSampling):
+            p, desc.method, sched = algorithm.proposal, PLAIN, [int(algorithm.nsamples)]
+        elif isinstance(algorithm, NPMC):
+            p, desc.method, sched = algorithm.proposal, NPMC_M, algorithm.schedule
+        elif isinstance(algorithm, AMIS):
+            p, desc.method, sched = algorithm.proposal, AMIS_M, algorithm.schedule
+        elif isinstance(algorithm, DeterministicMixturePMC):
+            p, desc.method, sched = algorithm.bank.proposals[0], DMPMC_M, algorithm.schedule
+        else:
+            raise TypeError("unsupported algorithm")
+        self.proposal0 = p
+        self.dim = p.dim
+        self.scalar = bool(p.scalar)
+        self.schedule = list(sched)
+        desc.family, desc.scale_kind, desc.scalar, desc.dim, desc.dof = p.family, p.kind, int(p.scalar), p.dim, p.dof
+        desc.rounds = len(sched)
+        rs = np.asarray(sched, dtype=np.int64)
+        keep.append(rs)
+        desc.round_size = rs.ctypes.data_as(I64P)
+        if desc.method == DMPMC_M:
+            bank = algorithm.bank
+            desc.bank_size = len(bank.proposals)
+            desc.bank_locations = arr(np.concatenate([q.location for q in bank.proposals]))
+            desc.bank_scales = arr(np.concatenate([q.scale if q.kind != FACTOR else q.factor().T.ravel()
+                                                   for q in bank.proposals]))
+            desc.bank_masses = arr(bank.masses)
+            desc.resampling = algorithm.method.code
+        else:
+            desc.location = arr(p.location)
+            desc.scale = arr(p.scale if p.kind != FACTOR else p.factor().T.ravel())
+        desc.target_kind = target.kind
+        tp = _f64(target.params(p.dim))
+        keep.append(tp)
+        desc.target_params = _p(tp)
+        desc.target_params_len = tp.size
+        h = VP()
+        _check(lib.is_job_create(ctx, ctypes.byref(desc), ctypes.byref(h)))
+        self._job = h
+        self._lib = lib
+        self.total = int(lib.is_job_total(h))
+        if desc.method == DMPMC_M:
+            order = sorted([i for i, m in enumerate(algorithm.bank.masses) if m != 0],
+                           key=lambda i: algorithm.bank.masses[i])
+            self.active_ids = [i + 1 for i in order]
+        if isinstance(rng, PrefilledRNG):
+            self.set_prefilled(rng)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_job", None):
+                self._lib.is_job_destroy(self._job)
+                self._job = None
+        except Exception:
+            pass
+
+    @property
+    def method(self):
+        a = self.algorithm
+        if isinstance(a, ImportanceSampling):
+            return PLAIN
+        if isinstance(a, NPMC):
+            return NPMC_M
+        if isinstance(a, AMIS):
+            return AMIS_M
+        return DMPMC_M
+
+    # ---- RNG seams
+    def set_prefilled(self, rng: PrefilledRNG):
+        lib = self._lib
+        for t, batch in enumerate(rng.normals or []):
+            if batch is None:
+                continue
+            b = _f64(batch).ravel()
+            _check(lib.is_job_set_normals(self._job, t, _p(b), b.size))
+        n_rad = max(len(rng.radial_normals or []), len(rng.radial_uniforms or []))
+        for t in range(n_rad):
+            rn = _f64(rng.radial_normals[t]).ravel() if rng.radial_normals and rng.radial_normals[t] is not None else None
+            ru = _f64(rng.radial_uniforms[t]).ravel() if rng.radial_uniforms and rng.radial_uniforms[t] is not None else None
+            _check(lib.is_job_set_radial(self._job, t, _p(rn), 0 if rn is None else rn.size, _p(ru),
+                                         0 if ru is None else ru.size))
+        for t, batch in enumerate(rng.uniforms or []):
+            if batch is None:
+                continue
+            b = _f64(batch).ravel()
+            _check(lib.is_job_set_uniforms(self._job, t, _p(b), b.size))
+
+    def export_randomness(self, round):
+        """The counter-RNG values the next run consumes in `round` (for the oracle)."""
+        lib, d, n = self._lib, self.dim, self.schedule[round]
+        p = self.proposal0
+        rs = 0 if p.family == GAUSSIAN else (1 if p.dof == 1.0 else 8)
+        us = 0 if (p.family == GAUSSIAN or p.dof == 1.0) else 8 + (1 if p.dof < 2 else 0)
+        out = {}
+        if self.method == PLAIN:
+            z = np.zeros((d + rs) * n)
+            _check(lib.is_job_export_normals(self._job, round, _p(z), z.size))
+            out["normals"] = z
+        else:
+            z = np.zeros(d * n)
+            _check(lib.is_job_export_normals(self._job, round, _p(z), z.size))
+            out["normals"] = z
+            if rs:
+                rn = np.zeros(rs * n)
+                _check(lib.is_job_export_radial(self._job, round, _p(rn), rn.size, None, 0))
+                out["radial_normals"] = rn
+        if us:
+            ru = np.zeros(us * n)
+            _check(lib.is_job_export_radial(self._job, round, None, 0, _p(ru), ru.size))
+            out["radial_uniforms"] = ru
+        if self.method == DMPMC_M:
+            K = len(self.active_ids)
+            u = np.zeros(K)
+            _check(lib.is_job_export_uniforms(self._job, round, _p(u), K))
+            out["uniforms"] = u
+        return out
+
+    def clear_prefilled(self):
+        _check(self._lib.is_job_clear_prefilled(self._job))
+
+    def reset(self):
+        _check(self._lib.is_job_reset(self._job))
+
+    # ---- execution
+    def run(self):
+        rc = self._lib.is_job_run(self._job)
+        if rc == _lib.IS_EFAILED:
+            self._raise_failure()
+        _check(rc)
+
+    def failure(self):
+        f = _lib.Failure()
+        _check(self._lib.is_job_failure(self._job, ctypes.byref(f)))
+        return f
+
+    def _raise_failure(self):
+        f = self.failure()
+        if f.draw_first_logical_index:
+            index, bits = f.draw_first_logical_index, f.draw_reason_bits
+        else:
+            index, bits = f.first_logical_index, f.reason_bits
+        phase = PHASES.get(f.phase)
+        if self.method == PLAIN:
+            raise SamplerExecutionError(PLAIN_PHASES.get(f.phase, phase), index, bits)
+        if f.phase in (5, 7):
+            cause = AllZeroWeightsError()
+        elif f.phase == 6:
+            cause = np.linalg.LinAlgError("fitted covariance is not positive definite")
+        else:
+            cause = SamplerExecutionError(phase, index, bits)
+        err = {AMIS_M: AMISRoundError, NPMC_M: NPMCRoundError, DMPMC_M: DMPMCRoundError}[self.method]
+        raise err(f.round, phase, cause, {"failure_count": int(f.count)})
+
+    # ---- results
+    def _get(self, name, n, dtype=np.float64):
+        out = np.zeros(n, dtype=dtype)
+        _check(getattr(self._lib, name)(self._job, _p(out, I64P if dtype == np.int64 else F64P)))
+        return out
+
+    def collect(self):
+        d, n, T = self.dim, self.total, len(self.schedule)
+        samples = self._get("is_job_get_samples", n * d).reshape(n, d)
+        logw = self._get("is_job_get_logweights", n)
+        prov = {"round": self._get("is_job_get_round_ids", n, np.int64)}
+        ess, lz, aess = np.zeros(T), np.zeros(T), np.zeros(T)
+        _check(self._lib.is_job_get_round_summary(self._job, _p(ess), _p(lz), _p(aess)))
+        method = self.method
+        diag = {"method": {PLAIN: "importance_sampling", AMIS_M: "amis", NPMC_M: "npmc",
+                           DMPMC_M: "deterministic_mixture_pmc"}[method],
+                "execution": "mi355x", "rounds": T, "round_sizes": list(self.schedule),
+                "round_ess": ess, "round_lognormalizers": lz,
+                "target_evaluations": n,
+                "proposal_evaluations": (T * n if method == AMIS_M else n),
+                "failures": 0, "precision": "fp32" if self.precision == FP32 else "fp64"}
+        if method == NPMC_M:
+            diag["adaptation_ess"] = aess
+            diag["clipped_counts"] = [math.isqrt(v) for v in self.schedule]
+        if method == DMPMC_M:
+            prov["proposal_id"] = self._get("is_job_get_proposal_ids", n, np.int64)
+            diag["resampling"] = "global"
+        if self.scalar:
+            samples = samples[:, 0].copy()
+        return WeightedSamples(samples, logw, prov, diag, _sampler=self)
+
+    def history(self):
+        T, d = len(self.schedule), self.dim
+        means, facs, lns = np.zeros((T + 1) * d), np.zeros((T + 1) * d * d), np.zeros(T + 1)
+        _check(self._lib.is_job_get_history(self._job, _p(means), _p(facs), _p(lns)))
+        return means.reshape(T + 1, d), facs.reshape(T + 1, d, d).transpose(0, 2, 1).copy(), lns
+
+    def ancestors(self):
+        K, T = len(self.active_ids), len(self.schedule)
+        a = self._get("is_job_get_ancestors", K * T, np.int64)
+        return a.reshape(T, K)
+
+
+def prepare_sampler(rng, target, algorithm, precision="fp64"):
+    """src/methods/importance_sampling.jl:357-404 (device-prepared)."""
+    prec = {"fp64": FP64, "fp32": FP32, FP64: FP64, FP32: FP32}[precision]
+    return PreparedSampler(rng, target, algorithm, prec)
+
+
+def importance_sample_(sampler: PreparedSampler):
+    """`importance_sample!` (src/methods/importance_sampling.jl:912-930)."""
+    sampler.run()
+    return sampler.collect()
+
+
+def importance_sample(rng, target, algorithm, precision="fp64"):
+    """`importance_sample` (src/methods/importance_sampling.jl:857-874)."""
+    return importance_sample_(prepare_sampler(rng, target, algorithm, precision))
+
+
+def current_proposal(sampler: PreparedSampler):
+    """src/adaptive_gaussian.jl:382-412 (and DM-PMC current locations)."""
+    lib, d = sampler._lib, sampler.dim
+    if sampler.method == DMPMC_M:
+        K = len(sampler.active_ids)
+        out = np.zeros(K * d)
+        _check(lib.is_job_get_locations(sampler._job, _p(out)))
+        locs = out.reshape(K, d)
+        bank = sampler.algorithm.bank
+        props = [p for p in bank.proposals]
+        for slot, pid in enumerate(sampler.active_ids):
+            q = props[pid - 1]
+            props[pid - 1] = NativeProposal(q.family, q.kind, locs[slot].copy(), q.scale.copy(),
+                                            q.lognormalizer, q.dof, q.scalar)
+        return ProposalBank(props, bank.masses.copy())
+    loc, fac, ln = np.zeros(d), np.zeros(d * d), np.zeros(1)
+    _check(lib.is_job_get_proposal(sampler._job, _p(loc), _p(fac), _p(ln)))
+    p0 = sampler.proposal0
+    if sampler.scalar:
+        return NativeProposal(p0.family, SPHERICAL, loc, fac[:1].copy(), float(ln[0]), p0.dof, True)
+    L = fac.reshape(d, d).T.copy()
+    return NativeProposal(p0.family, FACTOR, loc, L.ravel(), float(ln[0]), p0.dof, False)
+
+
+# ---------------------------------------------------------------- results (src/results.jl, src/statistics.jl)
+@dataclass
+class WeightedSamples:
+    samples: np.ndarray
+    logweights: np.ndarray
+    provenance: dict = field(default_factory=dict)
+    diagnostics: dict = field(default_factory=dict)
+    _sampler: object = None
+
+    def __len__(self):
+        return self.logweights.size
+
+
+def lognormalizer(result: WeightedSamples):
+    """LSE(logw) - log N (src/results.jl:560-567), reduced on the device when the
+    result still owns its device buffers."""
+    s = result._sampler
+    if s is not None and s._job is not None:
+        out = np.zeros(3)
+        _check(s._lib.is_job_lognormalizer(s._job, _p(out)))
+        return float(out[0])
+    raise ValueError("result is detached from its device sampler")
+
+
+def _device_moments(result):
+    s = result._sampler
+    d = s.dim
+    mean, var, cov = np.zeros(d), np.zeros(d), np.zeros(d * d)
+    rc = s._lib.is_job_moments(s._job, _p(mean), _p(var), _p(cov))
+    if rc == _lib.IS_EFAILED:
+        raise AllZeroWeightsError()
+    _check(rc)
+    return mean, var, cov.reshape(d, d).T.copy()
+
+
+def mean(result: WeightedSamples):
+    """Self-normalized weighted mean (src/statistics.jl:11-15)."""
+    m, _, _ = _device_moments(result)
+    return float(m[0]) if result.samples.ndim == 1 else m
+
+
+def var(result: WeightedSamples, corrected=False):
+    """Uncorrected weighted variance (src/statistics.jl:39-46)."""
+    if corrected:
+        raise ValueError("corrected=true is unsupported")
+    _, v, _ = _device_moments(result)
+    return float(v[0]) if result.samples.ndim == 1 else v
+
+
+def cov(result: WeightedSamples, corrected=False):
+    """Uncorrected weighted covariance (src/statistics.jl:96-103)."""
+    if corrected:
+        raise ValueError("corrected=true is unsupported")
+    _, _, c = _device_moments(result)
+    return c
+
+
+def resample(uniforms, result: WeightedSamples, count=None, method=None):
+    """src/resampling.jl:245-283: multinomial (count uniforms) or systematic (one uniform)."""
+    method = method or MultinomialResampling()
+    s = result._sampler
+    count = len(result) if count is None else int(count)
+    if count <= 0:
+        raise ValueError("resampling count must be positive")
+    u = _f64(np.atleast_1d(uniforms))
+    anc = np.zeros(count, dtype=np.int64)
+    rc = s._lib.is_job_resample(s._job, _p(u), count, method.code, _p(anc, I64P))
+    if rc == _lib.IS_EFAILED:
+        raise AllZeroWeightsError()
+    _check(rc)
+    return anc

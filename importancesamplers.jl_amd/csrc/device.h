@@ -1,0 +1,413 @@
+// Device-side building blocks for the gfx950 importance-sampling engine:
+// counter-based RNG, LogExpFunctions-compatible log-sum-exp helpers, packed
+// proposal slots and their log densities, the Student-t radial multiplier,
+// built-in target log densities and the packed failure record.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace issampler {
+
+// ---------------------------------------------------------------- constants
+// Failure reason bits (src/execution.jl:758-767).
+constexpr uint16_t kTargetNaN = 0x0100;
+constexpr uint16_t kTargetPosInf = 0x0200;
+constexpr uint16_t kProposalInvalid = 0x0400;
+constexpr uint16_t kGeneratedNonfinite = 0x0800;
+constexpr uint16_t kLogweightInvalid = 0x1000;
+constexpr uint16_t kDrawExhausted = 0x0010;
+constexpr uint16_t kMomentFitInvalid = 0x2000;  // src/adaptive_gaussian_execution.jl:9
+constexpr uint16_t kDrawReasons = 0x000f | kGeneratedNonfinite | kDrawExhausted;
+constexpr int kGammaAttempts = 8;  // src/execution.jl:557
+
+// RNG stream purposes (third counter word).
+constexpr uint64_t kPurposeNormals = 0;
+constexpr uint64_t kPurposeRadialNormals = 1;
+constexpr uint64_t kPurposeRadialUniforms = 2;
+constexpr uint64_t kPurposeResampling = 3;
+
+// ---------------------------------------------------------------- Philox4x64-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11).
+struct U64x4 {
+  uint64_t v[4];
+};
+
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+__host__ __device__ inline U64x4 philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                                            uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += W0;
+      k1 += W1;
+    }
+    const uint64_t hi0 = mulhi64(M0, c0), lo0 = M0 * c0;
+    const uint64_t hi1 = mulhi64(M1, c2), lo1 = M1 * c2;
+    const uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  U64x4 out;
+  out.v[0] = c0;
+  out.v[1] = c1;
+  out.v[2] = c2;
+  out.v[3] = c3;
+  return out;
+}
+
+// Open-interval uniform (0, 1) from the top 53 bits (Box-Muller input).
+__host__ __device__ inline double u53_open(uint64_t w) {
+  return ((double)(w >> 11) + 0.5) * 0x1.0p-53;
+}
+// Half-open uniform [0, 1) from the top 53 bits.
+__host__ __device__ inline double u53(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
+
+struct RngKey {
+  uint64_t k0, k1;
+};
+
+// Normal number `index` of (sample, round, purpose): Box-Muller on word pairs
+// (0,1) and (2,3) of the Philox block index/4.
+__device__ inline double rng_normal(const RngKey& key, uint64_t sample, uint64_t round,
+                                    uint64_t purpose, int index) {
+  const U64x4 w = philox4x64(sample, round, purpose, (uint64_t)(index >> 2), key.k0, key.k1);
+  const int lane = index & 3;
+  const uint64_t a = (lane < 2) ? w.v[0] : w.v[2];
+  const uint64_t b = (lane < 2) ? w.v[1] : w.v[3];
+  const double r = sqrt(-2.0 * log(u53_open(a)));
+  double s, c;
+  sincospi(2.0 * u53_open(b), &s, &c);
+  return (lane & 1) ? r * s : r * c;
+}
+
+// Fills out[0..count) with the normals of (sample, round, purpose), four per Philox call.
+template <int MAXN>
+__device__ inline void rng_normals(const RngKey& key, uint64_t sample, uint64_t round,
+                                   uint64_t purpose, int count, double* out) {
+#pragma unroll
+  for (int blk = 0; blk < (MAXN + 3) / 4; ++blk) {
+    if (blk * 4 < count) {
+      const U64x4 w = philox4x64(sample, round, purpose, (uint64_t)blk, key.k0, key.k1);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
+        double s, c;
+        sincospi(2.0 * u53_open(w.v[2 * p + 1]), &s, &c);
+        const int i0 = blk * 4 + 2 * p;
+        if (i0 < MAXN) out[i0] = r * c;
+        if (i0 + 1 < MAXN) out[i0 + 1] = r * s;
+      }
+    }
+  }
+}
+
+__device__ inline double rng_uniform(const RngKey& key, uint64_t sample, uint64_t round,
+                                     uint64_t purpose, int index) {
+  const U64x4 w = philox4x64(sample, round, purpose, (uint64_t)(index >> 2), key.k0, key.k1);
+  return u53(w.v[index & 3]);
+}
+
+// ---------------------------------------------------------------- log-sum-exp helpers
+// LogExpFunctions 1.0.1 log1pexp with the Maechler thresholds for the
+// working precision (53 or 24 mantissa bits).
+__device__ inline double log1pexp(double x) {
+  const double x0 = -36.7368005696771;    // -53 log 2
+  const double x1 = 18.021826694558577;   // 52 log 2 / 2
+  const double x2 = 33.23111882352963;    // -x0 - log(-x0) (1 + 1/x0)
+  if (x < x0) return exp(x);
+  if (x < x1) return log1p(exp(x));
+  if (x < x2) return x + exp(-x);
+  return x;
+}
+__device__ inline float log1pexp(float x) {
+  const float x0 = -16.635532f;   // -24 log 2
+  const float x1 = 7.9711638f;    // 23 log 2 / 2
+  const float x2 = 13.8989f;
+  if (x < x0) return __expf(x);
+  if (x < x1) return log1pf(__expf(x));
+  if (x < x2) return x + __expf(-x);
+  return x;
+}
+
+template <class T>
+__device__ inline T logaddexp(T a, T b) {
+  T diff, mx;
+  if (a < b) {
+    diff = a - b;
+    mx = b;
+  } else {
+    diff = (a == b) ? T(0) : b - a;
+    mx = (b == b) ? a : b;
+  }
+  return mx + log1pexp(diff);
+}
+
+// Online (running max, scaled sum) log-sum-exp accumulator.
+template <class T>
+struct LseAcc {
+  T m, s;
+  __device__ inline void init() {
+    m = -__builtin_huge_val();
+    s = T(0);
+  }
+  __device__ inline void add(T a) {
+    if (a > m) {
+      s = s * fast_exp(m - a) + T(1);
+      m = a;
+    } else {
+      s += fast_exp(a - m);
+    }
+  }
+  __device__ static inline T fast_exp(T v);
+  __device__ inline T value() const { return m + log(s); }
+};
+template <>
+__device__ inline double LseAcc<double>::fast_exp(double v) {
+  return exp(v);
+}
+template <>
+__device__ inline float LseAcc<float>::fast_exp(float v) {
+  return __expf(v);
+}
+
+// Deterministic exponential for the resampling CDF: Cody-Waite reduction with
+// a two-part ln 2, degree-13 Taylor polynomial by fused multiply-add Horner,
+// scaling by 2^k.  The CPU oracle evaluates the same operation sequence.
+__host__ __device__ inline double det_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -745.2) return 0.0;
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  const double inv_ln2 = 1.44269504088896338700e+00;
+  const double k = rint(x * inv_ln2);
+  double r = fma(-k, ln2_hi, x);
+  r = fma(-k, ln2_lo, r);
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  const int ki = (int)k;
+  if (ki < -1000) return ldexp(ldexp(p, ki + 600), -600);
+  return ldexp(p, ki);
+}
+
+// ---------------------------------------------------------------- failure record
+// src/execution.jl:669-699: count, atomic max of (inverted index, inverted
+// block, reason), and the same maximum restricted to draw failures.
+__host__ __device__ inline uint64_t pack_failure(int64_t logical_index, int block, uint16_t reason) {
+  const uint32_t inv_index = (uint32_t)(0xffffffffu - (uint32_t)logical_index + 1u);
+  const uint16_t inv_block = (uint16_t)(0xffffu - (uint16_t)block + 1u);
+  return ((uint64_t)inv_index << 32) | ((uint64_t)inv_block << 16) | (uint64_t)reason;
+}
+
+__device__ inline void record_failure(unsigned long long* rec, int64_t logical_index, int block,
+                                      uint16_t reason) {
+  const unsigned long long packed = pack_failure(logical_index, block, reason);
+  atomicAdd(rec, 1ull);
+  atomicMax(rec + 1, packed);
+  if (reason & kDrawReasons) atomicMax(rec + 2, packed);
+}
+
+// ---------------------------------------------------------------- proposal slots
+// A slot packs one proposal for the density kernels:
+//   [ mu(d) | inv_diag(d) | L lower triangle incl. diagonal, row-major packed | lognorm | logmass ]
+// inv_diag holds 1/L_rr so the forward substitution multiplies instead of divides.
+__host__ __device__ inline int slot_stride(int d) {
+  const int s = 2 * d + d * (d + 1) / 2 + 2;
+  return (s + 1) & ~1;
+}
+__host__ __device__ inline int slot_lognorm_off(int d) { return 2 * d + d * (d + 1) / 2; }
+__host__ __device__ inline int slot_logmass_off(int d) { return 2 * d + d * (d + 1) / 2 + 1; }
+
+template <class T>
+struct Family {
+  int student;  // 0 Gaussian, 1 Student-t
+  T dof;
+  T half_dof_plus_d;  // (dof + d) / 2
+};
+
+// _radial_logdensity (src/proposals.jl:24-55) for the finite-range paths.
+template <class T, int D>
+__device__ inline T radial_logdensity(const Family<T>& fam, T lognorm, T r2, const T (&y)[D], int d) {
+  if (!fam.student) {
+    T half = T(0.5) * r2;
+    if (!isfinite(r2)) {
+      half = T(0);
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        if (c < d) half += (T(0.5) * y[c]) * y[c];
+    }
+    return lognorm - half;
+  }
+  const T ratio = r2 / fam.dof;
+  T radial;
+  const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+  if (r2 < tiny) {
+    const T s = sqrt(fam.dof);
+    T scaled = T(0);
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      if (c < d) {
+        const T v = y[c] / s;
+        scaled += v * v;
+      }
+    radial = log1p(scaled);
+  } else if (isfinite(ratio)) {
+    radial = log1p(ratio);
+  } else if (isfinite(r2)) {
+    radial = log(r2) - log(fam.dof);
+  } else {
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      if (c < d) s = fmax(s, fabs(y[c]));
+    if (!isfinite(s)) return T(__builtin_nan(""));
+    T scaled = T(0);
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      if (c < d) {
+        const T v = y[c] / s;
+        scaled += v * v;
+      }
+    radial = log1pexp(T(2) * log(s) + log(scaled) - log(fam.dof));
+  }
+  return lognorm - fam.half_dof_plus_d * radial;
+}
+
+// Whitened squared radius r^2 = |L^{-1}(x - mu)|^2 by forward substitution
+// (src/native_execution.jl:346-384), then the radial log density.
+template <class T, int D>
+__device__ inline T slot_logdensity(const T* __restrict__ s, int d, const Family<T>& fam,
+                                    const T (&x)[D], T (&y)[D]) {
+  const T* mu = s;
+  const T* inv = s + d;
+  const T* L = s + 2 * d;
+  T r2 = T(0);
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    if (r < d) {
+      T v = x[r] - mu[r];
+      const T* row = L + r * (r + 1) / 2;
+#pragma unroll
+      for (int c = 0; c < r; ++c) v -= row[c] * y[c];
+      v *= inv[r];
+      y[r] = v;
+      r2 += v * v;
+    }
+  }
+  return radial_logdensity<T, D>(fam, s[slot_lognorm_off(d)], r2, y, d);
+}
+
+// Packed affine draw x = mu + L z (src/adaptive_gaussian_execution.jl:49-62).
+template <class T, int D>
+__device__ inline bool slot_draw(const T* __restrict__ s, int d, const T (&z)[D], T (&x)[D]) {
+  const T* mu = s;
+  const T* L = s + 2 * d;
+  bool valid = true;
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    if (r < d) {
+      T v = mu[r];
+      const T* row = L + r * (r + 1) / 2;
+#pragma unroll
+      for (int c = 0; c <= r; ++c) v += row[c] * z[c];
+      x[r] = v;
+      valid &= isfinite(v);
+    }
+  }
+  return valid;
+}
+
+// ---------------------------------------------------------------- Student-t radial multiplier
+// _buffered_unit_gamma / _student_t_radial_multiplier (src/native_execution.jl:147-180)
+// with the gamma candidate of src/proposals.jl:738-751.
+__device__ inline bool student_t_multiplier(double dof, const double* rn, const double* ru,
+                                            bool have_uniforms, double* out) {
+  if (!have_uniforms || dof == 1.0) {
+    const double den = fabs(rn[0]);
+    if (!(isfinite(den) && den > 0.0)) return false;
+    *out = 1.0 / den;
+    return true;
+  }
+  const double shape = dof / 2.0;
+  const double adjusted = shape < 1.0 ? shape + 1.0 : shape;
+  const double offset = adjusted - 1.0 / 3.0;
+  const double coefficient = (1.0 / sqrt(offset)) / 3.0;
+  for (int a = 0; a < kGammaAttempts; ++a) {
+    const double normal = rn[a];
+    const double uniform = ru[a];
+    const double root = 1.0 + coefficient * normal;
+    if (!(root > 0.0)) continue;
+    const double cube = root * root * root;
+    const double n2 = normal * normal;
+    const bool accepted = uniform < 1.0 - 0.0331 * (n2 * n2) ||
+                          log(uniform) < 0.5 * n2 + offset * (1.0 - cube + log(cube));
+    if (!accepted) continue;
+    double gamma = offset * cube;
+    if (shape < 1.0) gamma *= pow(ru[kGammaAttempts], 1.0 / shape);
+    if (!(isfinite(gamma) && gamma > 0.0)) break;
+    *out = sqrt(dof / (2.0 * gamma));
+    return true;
+  }
+  return false;
+}
+
+__host__ __device__ inline int radial_normal_stride(int student, double dof) {
+  return student ? (dof == 1.0 ? 1 : kGammaAttempts) : 0;
+}
+__host__ __device__ inline int radial_uniform_stride(int student, double dof) {
+  return student ? (dof == 1.0 ? 0 : kGammaAttempts + (dof < 2.0 ? 1 : 0)) : 0;
+}
+
+// ---------------------------------------------------------------- reasons
+template <class T>
+__device__ inline uint16_t target_reason(T v) {
+  if (v != v) return kTargetNaN;
+  if (v == T(__builtin_huge_val())) return kTargetPosInf;
+  return 0;
+}
+template <class T>
+__device__ inline uint16_t proposal_reason(T v) {
+  return (v != v || v == -T(__builtin_huge_val())) ? kProposalInvalid : 0;
+}
+template <class T>
+__device__ inline uint16_t logweight_reason(T v) {
+  return (v != v || v == T(__builtin_huge_val())) ? kLogweightInvalid : 0;
+}
+
+// ---------------------------------------------------------------- wavefront reductions
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace issampler

@@ -1,0 +1,1562 @@
+// Host side of the gfx950 importance-sampling engine: job preparation, the
+// per-round launch sequences of plain IS / AMIS / NPMC / DM-PMC, failure
+// snapshots and the C ABI declared in include/issampler.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <memory>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/issampler.h"
+#include "kernels.h"
+
+using namespace issampler;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess) return fail(IS_EDEVICE, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+// ---------------------------------------------------------------- host math
+// Student-t log normalizing constant: src/proposals.jl:410-437 (gamma
+// recurrence for odd d shifted to nu >= 32, six Stirling terms; pairwise
+// log1p terms for the remaining gamma-ratio factors).
+double student_t_logconstant(double dof, int dimension) {
+  double value = -(double)dimension * std::log(2.0 * M_PI) / 2.0;
+  if (dimension % 2 == 1) {
+    double shifted = dof, correction = 0.0;
+    while (shifted < 32.0) {
+      correction += shifted < 1.0
+                        ? (std::log(shifted) + std::log(shifted + 2.0)) / 2.0 - std::log1p(shifted)
+                        : std::log1p(-((1.0 / (shifted + 1.0)) * (1.0 / (shifted + 1.0)))) / 2.0;
+      shifted += 2.0;
+    }
+    const double r = 1.0 / shifted, r2 = r * r;
+    double poly = 691.0 / 88.0;
+    poly = poly * r2 + (-31.0 / 36.0);
+    poly = poly * r2 + (17.0 / 112.0);
+    poly = poly * r2 + (-1.0 / 20.0);
+    poly = poly * r2 + (1.0 / 24.0);
+    poly = poly * r2 + (-1.0 / 4.0);
+    correction += r * poly;
+    value += correction;
+  }
+  double acc = 0.0;
+  for (int offset = (dimension % 2 == 1) ? 1 : 0; offset <= dimension - 2; offset += 2) {
+    const double x = (double)offset;
+    acc += x <= dof ? std::log1p(x / dof) : std::log(x) + std::log1p(dof / x) - std::log(dof);
+  }
+  return value + acc;
+}
+
+double gaussian_lognormalizer(int d, double logabsdet) {  // src/proposals.jl:237-240
+  return -0.5 * (double)d * std::log(2.0 * M_PI) - logabsdet;
+}
+
+// _factor_exponent_bound (src/student_t_range.jl:21-31).
+int factor_exponent_bound(const std::vector<double>& Lcol, int d) {
+  int bound = 0;
+  for (int c = 0; c < d; ++c)
+    for (int r = c; r < d; ++r) {
+      const double v = Lcol[r + (size_t)c * d];
+      if (v == 0.0) continue;
+      if (!std::isfinite(v)) return 1 << 30;
+      int e;
+      std::frexp(v, &e);
+      bound = std::max(bound, std::max(std::abs(e), std::abs(e - 1)));
+    }
+  return bound;
+}
+
+// A proposal in host form: location and a full lower factor (column-major).
+struct HostProp {
+  std::vector<double> mu;
+  std::vector<double> L;  // d*d column-major lower
+  double lognorm = 0.0;
+};
+
+void pack_slot(const HostProp& p, int d, double logmass, double* slot) {
+  const int stride = slot_stride(d);
+  std::fill(slot, slot + stride, 0.0);
+  for (int r = 0; r < d; ++r) {
+    slot[r] = p.mu[r];
+    slot[d + r] = 1.0 / p.L[r + (size_t)r * d];
+    for (int c = 0; c <= r; ++c) slot[2 * d + r * (r + 1) / 2 + c] = p.L[r + (size_t)c * d];
+  }
+  slot[slot_lognorm_off(d)] = p.lognorm;
+  slot[slot_logmass_off(d)] = logmass;
+}
+
+void unpack_slot(const double* slot, int d, HostProp& p) {
+  p.mu.assign(slot, slot + d);
+  p.L.assign((size_t)d * d, 0.0);
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c <= r; ++c) p.L[r + (size_t)c * d] = slot[2 * d + r * (r + 1) / 2 + c];
+  p.lognorm = slot[slot_lognorm_off(d)];
+}
+
+// Builds one proposal from (kind, location, scale storage); returns an error string or "".
+std::string make_prop(int family, double dof, int kind, int d, const double* loc, const double* scale,
+                      HostProp& out) {
+  out.mu.assign(loc, loc + d);
+  out.L.assign((size_t)d * d, 0.0);
+  double logabsdet = 0.0;
+  for (int c = 0; c < d; ++c)
+    if (!std::isfinite(loc[c])) return "native parameter must contain only finite values";
+  if (kind == IS_SPHERICAL) {
+    const double s = scale[0];
+    if (!(std::isfinite(s) && s > 0.0)) return "scale must be finite and positive";
+    for (int c = 0; c < d; ++c) out.L[c + (size_t)c * d] = s;
+    logabsdet = (double)d * std::log(s);
+  } else if (kind == IS_DIAGONAL) {
+    for (int c = 0; c < d; ++c) {
+      const double s = scale[c];
+      if (!(std::isfinite(s) && s > 0.0)) return "diagonal scales must be finite and positive";
+      out.L[c + (size_t)c * d] = s;
+      logabsdet += std::log(s);
+    }
+  } else if (kind == IS_FACTOR) {
+    for (int c = 0; c < d; ++c)
+      for (int r = 0; r < d; ++r) {
+        const double v = scale[r + (size_t)c * d];
+        if (!std::isfinite(v)) return "factor must contain only finite values";
+        if (r < c && v != 0.0) return "factor must be lower triangular";
+        out.L[r + (size_t)c * d] = v;
+      }
+    for (int c = 0; c < d; ++c) {
+      const double v = out.L[c + (size_t)c * d];
+      if (!(v > 0.0)) return "factor diagonal must be positive";
+      logabsdet += std::log(v);
+    }
+  } else {
+    return "unknown scale kind";
+  }
+  if (family == IS_STUDENT_T) {
+    if (!(std::isfinite(dof) && dof > 0.0)) return "degrees of freedom must be finite and positive";
+    if (factor_exponent_bound(out.L, d) >= 511)
+      return "factor magnitudes need the exponent-range repair path, which this engine does not provide";
+    out.lognorm = student_t_logconstant(dof, d) - logabsdet;
+  } else {
+    out.lognorm = gaussian_lognormalizer(d, logabsdet);
+  }
+  return "";
+}
+
+// ---------------------------------------------------------------- exact allocation plan
+// _dm_pmc_round_counts / _deterministic_allocation_plan
+// (src/population_preparation.jl:266-367): exact largest-remainder counts with
+// ties rotated by round.  Masses are converted to integers over a common
+// power of two so every comparison is exact.
+typedef unsigned __int128 u128;
+
+bool exact_integer_masses(const std::vector<double>& m, std::vector<u128>& out) {
+  int emin = 1 << 30;
+  std::vector<uint64_t> mant(m.size());
+  std::vector<int> ex(m.size());
+  for (size_t k = 0; k < m.size(); ++k) {
+    int e;
+    const double f = std::frexp(m[k], &e);
+    uint64_t M = (uint64_t)std::ldexp(f, 53);
+    e -= 53;
+    while (M && !(M & 1)) {
+      M >>= 1;
+      ++e;
+    }
+    mant[k] = M;
+    ex[k] = e;
+    emin = std::min(emin, e);
+  }
+  out.resize(m.size());
+  for (size_t k = 0; k < m.size(); ++k) {
+    const int shift = ex[k] - emin;
+    if (shift + 53 > 90) return false;
+    out[k] = (u128)mant[k] << shift;
+  }
+  return true;
+}
+
+bool round_counts(const std::vector<u128>& I, int64_t n, int round1, std::vector<int64_t>& counts) {
+  const int K = (int)I.size();
+  u128 S = 0;
+  for (auto v : I) S += v;
+  counts.assign(K, 0);
+  std::vector<u128> rem(K);
+  int64_t used = 0;
+  for (int k = 0; k < K; ++k) {
+    const u128 q = I[k] * (u128)n;
+    counts[k] = (int64_t)(q / S);
+    rem[k] = q % S;
+    used += counts[k];
+  }
+  const int64_t remaining = n - used;
+  if (remaining < 0 || remaining >= K) return false;
+  const int first_tie = (round1 - 1) % K + 1;
+  std::vector<int> order(K);
+  for (int k = 0; k < K; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (rem[a] != rem[b]) return rem[a] > rem[b];
+    const int ra = ((a + 1 - first_tie) % K + K) % K, rb = ((b + 1 - first_tie) % K + K) % K;
+    return ra < rb;
+  });
+  for (int64_t i = 0; i < remaining; ++i) counts[order[i]] += 1;
+  return true;
+}
+
+// ---------------------------------------------------------------- device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = n;
+    if (n == 0) return IS_OK;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      p = nullptr;
+      return fail(IS_ENOMEM, "hipMalloc(%zu) failed", n);
+    }
+    return IS_OK;
+  }
+  template <class U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+}  // namespace
+
+struct is_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+};
+
+struct is_job {
+  is_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  std::atomic<bool> busy{false};
+  // configuration
+  int method = 0, family = 0, kind = 0, scalar = 0, d = 0, T = 1, K = 0, resampling = 0;
+  int tkind = 0, precision = 0;
+  double dof = 0.0, logconstant = 0.0;
+  std::vector<int64_t> schedule, offsets;
+  int64_t total = 0, maxN = 0;
+  RngKey key{0, 0};
+  uint64_t run_index = 0;
+  int stride = 0;   // slot stride
+  int nslots = 0;   // slot table entries
+  int D = 4;        // compile-time dimension bucket
+  // committed state
+  HostProp initial, committed;
+  std::vector<double> bank_slots_init;   // K slots (FP64)
+  std::vector<double> committed_locs;    // K*d
+  std::vector<int32_t> bank_ids;         // 1-based configured ids of active slots
+  std::vector<int64_t> plan_counts;      // K*T
+  std::vector<double> plan_logcoef;      // K*T
+  std::vector<double> tslot;             // Gaussian target slot
+  // device buffers
+  DevBuf X, logtargets, lognums, logweights, round_ids, proposal_ids;
+  DevBuf slots64, slotsT, logmassT, zerosT, tslotT, assign, bank_ids_d;
+  DevBuf partials, partials2, reduced, reduced2, summary, counter, fail_rec, fit_status, cdf_status;
+  DevBuf round_ess, round_logz, adapt_ess, clip, cdf, tile_total, uniforms, ancestors, scratch;
+  DevBuf pre_normals, pre_rn, pre_ru, pre_uniforms;
+  std::vector<int> prefilled, prefilled_radial, prefilled_uniforms;
+  // last run
+  bool have_results = false;
+  int last_status = 0;
+  is_failure failure{};
+  std::vector<int64_t> ancestors_host;
+  // optional per-round event timing of the round kernel
+  int timing = 0;
+  std::vector<hipEvent_t> events;  // 2 per round
+  std::vector<double> round_ms;
+};
+
+namespace {
+
+struct BusyGuard {
+  is_job* job;
+  bool ok;
+  explicit BusyGuard(is_job* j) : job(j) {
+    bool expected = false;
+    ok = job->busy.compare_exchange_strong(expected, true);
+  }
+  ~BusyGuard() {
+    if (ok) job->busy.store(false);
+  }
+};
+
+void tick(is_job* j, int t, int which, hipStream_t s) {
+  if (j->timing && (size_t)(2 * t + which) < j->events.size()) (void)hipEventRecord(j->events[2 * t + which], s);
+}
+
+int dim_bucket(int d) {
+  if (d <= 4) return 4;
+  if (d <= 8) return 8;
+  if (d <= 16) return 16;
+  if (d <= 32) return 32;
+  return 64;
+}
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------- launch dispatch
+template <class T, int D, int MODE>
+void launch_round_typed(const RoundArgs<T>& a, int blocks, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_round<T, D, MODE>), dim3(blocks), dim3(kBlock), lds, s, a);
+}
+
+template <class T, int MODE>
+void launch_round_mode(int D, const RoundArgs<T>& a, int blocks, size_t lds, hipStream_t s) {
+  switch (D) {
+    case 4: launch_round_typed<T, 4, MODE>(a, blocks, lds, s); break;
+    case 8: launch_round_typed<T, 8, MODE>(a, blocks, lds, s); break;
+    case 16: launch_round_typed<T, 16, MODE>(a, blocks, lds, s); break;
+    case 32: launch_round_typed<T, 32, MODE>(a, blocks, lds, s); break;
+    default: launch_round_typed<T, 64, MODE>(a, blocks, lds, s); break;
+  }
+}
+
+template <class T>
+void launch_round(int D, int mode, const RoundArgs<T>& a, int blocks, size_t lds, hipStream_t s) {
+  if (mode == kDrawFused)
+    launch_round_mode<T, kDrawFused>(D, a, blocks, lds, s);
+  else if (mode == kDrawOnly)
+    launch_round_mode<T, kDrawOnly>(D, a, blocks, lds, s);
+  else
+    launch_round_mode<T, kPreTarget>(D, a, blocks, lds, s);
+}
+
+template <class T>
+void launch_moments(int D, int blocks, size_t lds, hipStream_t s, const T* X, int64_t cap, int64_t xs,
+                    const T* lw, int64_t first, int64_t n, int d, const double* clip, const double* center,
+                    int moments, int nv, double* partials) {
+#define IS_MOM(DD)                                                                                   \
+  hipLaunchKernelGGL((k_moment_partials<T, DD>), dim3(blocks), dim3(kBlock), lds, s, X, cap, xs, lw, \
+                     first, n, d, clip, center, moments, nv, partials)
+  switch (D) {
+    case 4: IS_MOM(4); break;
+    case 8: IS_MOM(8); break;
+    case 16: IS_MOM(16); break;
+    case 32: IS_MOM(32); break;
+    default: IS_MOM(64); break;
+  }
+#undef IS_MOM
+}
+
+size_t tile_lds_bytes(int d) { return (size_t)kTile * (d + 1) * sizeof(double); }
+constexpr size_t kLdsBudget = 64 * 1024;
+
+// ---------------------------------------------------------------- job helpers
+int64_t isqrt64(int64_t n) {
+  int64_t r = (int64_t)std::sqrt((double)n);
+  while (r * r > n) --r;
+  while ((r + 1) * (r + 1) <= n) ++r;
+  return r;
+}
+
+uint64_t round_word(const is_job* j, int t) { return ((uint64_t)j->run_index << 32) | (uint64_t)t; }
+
+template <class T>
+int upload_slots(is_job* j) {
+  // FP64 master -> T copy
+  std::vector<double> h((size_t)j->nslots * j->stride);
+  HIP_TRY(hipMemcpy(h.data(), j->slots64.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<T> t(h.begin(), h.end());
+  HIP_TRY(hipMemcpy(j->slotsT.p, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  return IS_OK;
+}
+
+template <class T>
+int upload_T(DevBuf& buf, const std::vector<double>& v) {
+  std::vector<T> t(v.begin(), v.end());
+  if (buf.alloc(std::max<size_t>(1, t.size()) * sizeof(T))) return IS_ENOMEM;
+  if (!t.empty()) HIP_TRY(hipMemcpy(buf.p, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  return IS_OK;
+}
+
+template <class T>
+RoundArgs<T> base_args(is_job* j, int t) {
+  RoundArgs<T> a{};
+  a.d = j->d;
+  a.round = t;
+  a.rng_word = round_word(j, t);
+  a.slots = j->slotsT.as<T>();
+  a.stride = j->stride;
+  a.fam.student = j->family;
+  a.fam.dof = (T)j->dof;
+  a.fam.half_dof_plus_d = (T)((j->dof + (double)j->d) / 2.0);
+  a.dof = j->dof;
+  a.key = j->key;
+  a.rs = radial_normal_stride(j->family, j->dof);
+  a.us = radial_uniform_stride(j->family, j->dof);
+  a.tkind = j->tkind;
+  a.tslot = j->tslotT.as<T>();
+  a.X = j->X.as<T>();
+  a.cap = j->total;
+  a.logtargets = j->logtargets.as<T>();
+  a.lognums = j->lognums.as<T>();
+  a.logweights = j->logweights.as<T>();
+  a.round_ids = j->round_ids.as<int32_t>();
+  a.proposal_ids = j->method == IS_DMPMC ? j->proposal_ids.as<int32_t>() : nullptr;
+  a.fail = j->fail_rec.as<unsigned long long>() + 3 * t;
+  a.partials = j->partials.as<double>();
+  return a;
+}
+
+// prefilled buffers for round t
+template <class T>
+void set_prefilled(is_job* j, int t, RoundArgs<T>& a) {
+  const int d = j->d;
+  const int rs = a.rs, us = a.us;
+  if (j->method == IS_PLAIN) {
+    const int stride = d + rs;
+    if (j->prefilled[0]) {
+      a.pre_normals = j->pre_normals.as<double>();
+      a.nstride = stride;
+      a.pre_rn = rs ? a.pre_normals + d : nullptr;
+      a.rn_stride = stride;
+    }
+    if (us && j->prefilled_radial[0] && j->pre_ru.p) {
+      a.pre_ru = j->pre_ru.as<double>();
+      a.ru_stride = us;
+    }
+    return;
+  }
+  if (j->prefilled[t]) {
+    a.pre_normals = j->pre_normals.as<double>() + (size_t)t * d * j->maxN;
+    a.nstride = d;
+  }
+  if (j->family == IS_STUDENT_T && j->prefilled_radial[t]) {
+    // a part that was never uploaded is drawn from the counter RNG
+    a.pre_rn = (rs && j->pre_rn.p) ? j->pre_rn.as<double>() + (size_t)t * rs * j->maxN : nullptr;
+    a.rn_stride = rs;
+    a.pre_ru = (us && j->pre_ru.p) ? j->pre_ru.as<double>() + (size_t)t * us * j->maxN : nullptr;
+    a.ru_stride = us;
+  }
+}
+
+FinishArgs base_finish(is_job* j, int t, unsigned int* counter) {
+  FinishArgs f{};
+  f.d = j->d;
+  f.round = t;
+  f.student = j->family;
+  f.dof = j->dof;
+  f.logconstant = j->logconstant;
+  f.round_ess = j->round_ess.as<double>();
+  f.round_logz = j->round_logz.as<double>();
+  f.adapt_ess = j->adapt_ess.as<double>();
+  f.fit_status = j->fit_status.as<int>();
+  f.counter = counter;
+  f.t_is_float = j->precision == IS_FP32;
+  return f;
+}
+
+size_t finish_lds(int d) { return std::max<size_t>(64, (size_t)(d * d + d) * sizeof(double)); }
+
+template <class T>
+int run_rounds(is_job* j) {
+  hipStream_t s = j->stream;
+  const int d = j->d;
+  const int D = j->D;
+  const int nv_mom = 2 + d + d * (d + 1) / 2;
+  const int pairs_nv_summary = 2;
+  unsigned int* counter = j->counter.as<unsigned int>();
+  const size_t tslot_bytes = (j->tkind == kTGaussian) ? (size_t)j->stride * sizeof(T) : 0;
+  const size_t slot_bytes = (size_t)j->stride * sizeof(T);
+  auto chunk_for = [&](int ncomp) {
+    const size_t avail = kLdsBudget - tile_lds_bytes(d) - tslot_bytes;
+    int c = (int)std::max<size_t>(1, avail / slot_bytes);
+    return std::min(c, std::max(1, ncomp));
+  };
+
+  if (j->method == IS_PLAIN) {
+    const int64_t N = j->schedule[0];
+    RoundArgs<T> a = base_args<T>(j, 0);
+    set_prefilled(j, 0, a);
+    a.first = 0;
+    a.n_new = N;
+    a.n_old = 0;
+    a.old_blocks = 0;
+    a.k0 = a.k1 = 0;
+    a.slot_chunk = 1;
+    a.gen_slot = 0;
+    a.logmass = j->zerosT.as<T>();
+    a.logtotal = T(0);
+    a.record_nonfinite_after_exhaust = 0;
+    a.nv = pairs_nv_summary;
+    a.moments = 0;
+    const int blocks = (int)cdiv(N, kBlock);
+    const size_t lds = tile_lds_bytes(d) + tslot_bytes + slot_bytes;
+    tick(j, 0, 0, s);
+    launch_round<T>(D, kDrawFused, a, blocks, lds, s);
+    tick(j, 0, 1, s);
+    FinishArgs f = base_finish(j, 0, counter);
+    f.flags = kFinishSummary;
+    f.summary_count = N;
+    f.summary_out = j->summary.as<double>();
+    hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s, j->partials.as<double>(),
+                       (int64_t)blocks, a.nv, j->reduced.as<double>(), f);
+    HIP_TRY(hipGetLastError());
+    return IS_OK;
+  }
+
+  for (int t = 0; t < j->T; ++t) {
+    const int64_t N = j->schedule[t];
+    const int64_t first = j->offsets[t];
+    const int64_t last = j->offsets[t + 1];
+    RoundArgs<T> a = base_args<T>(j, t);
+    set_prefilled(j, t, a);
+    a.first = first;
+    a.n_new = N;
+    a.record_nonfinite_after_exhaust = 1;
+    const int new_blocks = (int)cdiv(N, kBlock);
+    double* slot64_t = j->slots64.as<double>() + (size_t)t * j->stride;
+    if (j->method == IS_AMIS) {
+      a.n_old = first;
+      a.old_blocks = (int)cdiv(first, kBlock);
+      a.append_slot = t;
+      a.k0 = 0;
+      a.k1 = t;
+      a.slot_chunk = chunk_for(t + 1);
+      a.gen_slot = t;
+      a.logmass = j->logmassT.as<T>();
+      a.logtotal = (T)std::log((double)last);
+      a.nv = nv_mom;
+      a.moments = 1;
+      a.center = slot64_t;
+      const int blocks = a.old_blocks + new_blocks;
+      const size_t lds = tile_lds_bytes(d) + tslot_bytes + (size_t)std::max(1, a.slot_chunk) * slot_bytes;
+      tick(j, t, 0, s);
+      launch_round<T>(D, kDrawFused, a, blocks, lds, s);
+      tick(j, t, 1, s);
+      FinishArgs f = base_finish(j, t, counter);
+      f.flags = kFinishSummary | kFinishFit;
+      f.summary_count = last;
+      f.center = slot64_t;
+      f.prev_slot = slot64_t;
+      f.next_slot64 = j->slots64.as<double>() + (size_t)(t + 1) * j->stride;
+      f.next_slot_t = j->slotsT.as<T>() + (size_t)(t + 1) * j->stride;
+      hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s,
+                         j->partials.as<double>(), (int64_t)blocks, a.nv, j->reduced.as<double>(), f);
+    } else if (j->method == IS_NPMC) {
+      a.n_old = 0;
+      a.old_blocks = 0;
+      a.k0 = a.k1 = t;
+      a.slot_chunk = 1;
+      a.gen_slot = t;
+      a.logmass = j->zerosT.as<T>();
+      a.logtotal = T(0);
+      a.nv = pairs_nv_summary;
+      a.moments = 0;
+      const size_t lds = tile_lds_bytes(d) + tslot_bytes + slot_bytes;
+      tick(j, t, 0, s);
+      launch_round<T>(D, kDrawFused, a, new_blocks, lds, s);
+      tick(j, t, 1, s);
+      FinishArgs f = base_finish(j, t, counter);
+      f.flags = kFinishSummary;
+      f.summary_count = N;
+      hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s,
+                         j->partials.as<double>(), (int64_t)new_blocks, a.nv, j->reduced.as<double>(), f);
+      // clip at the isqrt(n)-th largest raw weight (src/npmc_execution.jl:21-33)
+      const int64_t kth = N - isqrt64(N) + 1;
+      hipLaunchKernelGGL((k_select_kth<T>), dim3(1), dim3(1024), 0, s, j->logweights.as<T>(), first, N, kth,
+                         j->clip.as<double>());
+      launch_moments<T>(D, new_blocks, tile_lds_bytes(d), s, j->X.as<T>(), j->total, 1, j->logweights.as<T>(),
+                        first, N, d, j->clip.as<double>(), slot64_t, 1, nv_mom, j->partials2.as<double>());
+      FinishArgs g = base_finish(j, t, counter);
+      g.flags = kFinishFit | kFinishAdaptEss;
+      g.summary_count = N;
+      g.center = slot64_t;
+      g.prev_slot = slot64_t;
+      g.next_slot64 = j->slots64.as<double>() + (size_t)(t + 1) * j->stride;
+      g.next_slot_t = j->slotsT.as<T>() + (size_t)(t + 1) * j->stride;
+      hipLaunchKernelGGL(k_reduce_finish, dim3(nv_mom), dim3(kBlock), finish_lds(d), s,
+                         j->partials2.as<double>(), (int64_t)new_blocks, nv_mom, j->reduced2.as<double>(), g);
+    } else {  // DM-PMC
+      a.n_old = 0;
+      a.old_blocks = 0;
+      a.k0 = 0;
+      a.k1 = j->K - 1;
+      a.slot_chunk = chunk_for(j->K);
+      a.gen_slot = 0;
+      a.assign = j->assign.as<int32_t>() + (size_t)t * j->maxN;
+      a.bank_ids = j->bank_ids_d.as<int32_t>();
+      a.logmass = j->logmassT.as<T>() + (size_t)t * j->K;
+      a.logtotal = T(0);
+      a.nv = pairs_nv_summary;
+      a.moments = 0;
+      const size_t lds = tile_lds_bytes(d) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
+      tick(j, t, 0, s);
+      launch_round<T>(D, kDrawFused, a, new_blocks, lds, s);
+      tick(j, t, 1, s);
+      double* summ = j->summary.as<double>() + (size_t)t * 5;
+      FinishArgs f = base_finish(j, t, counter);
+      f.flags = kFinishSummary;
+      f.summary_count = N;
+      f.summary_out = summ;
+      hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s,
+                         j->partials.as<double>(), (int64_t)new_blocks, a.nv, j->reduced.as<double>(), f);
+      // global resampling (src/dm_pmc_execution.jl:267-295)
+      const T* lw = j->logweights.as<T>() + first;
+      const int64_t ntiles = cdiv(N, (int64_t)kCdfThreads * kCdfPer);
+      hipLaunchKernelGGL((k_cdf_tiles<T>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, N, summ,
+                         j->tile_total.as<double>());
+      hipLaunchKernelGGL((k_cdf_write<T>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, N, summ,
+                         j->tile_total.as<double>(), ntiles, j->cdf.as<double>(), j->cdf_status.as<int>() + t);
+      double* u = j->uniforms.as<double>() + (size_t)t * j->K;
+      if (j->prefilled_uniforms[t]) {
+        u = j->pre_uniforms.as<double>() + (size_t)t * j->K;
+      } else {
+        hipLaunchKernelGGL(k_resampling_uniforms, dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->key,
+                           round_word(j, t), (int64_t)j->K, u);
+      }
+      int32_t* anc = j->ancestors.as<int32_t>() + (size_t)t * j->K;
+      hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->cdf.as<double>(), lw, N,
+                         u, (int64_t)j->K, j->resampling == IS_SYSTEMATIC ? 1 : 0, anc);
+      hipLaunchKernelGGL((k_gather_locations<T>), dim3(cdiv((int64_t)j->K * d, 64)), dim3(64), 0, s,
+                         j->X.as<T>(), j->total, first, d, anc, j->K, j->slots64.as<double>(), j->slotsT.as<T>(),
+                         j->stride);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  return IS_OK;
+}
+
+int phase_from_bits(uint64_t w1, uint64_t w2) {
+  if (w2 != 0) return IS_PHASE_SAMPLING;
+  const uint16_t bits = (uint16_t)(w1 & 0xffff);
+  if (bits & kDrawReasons) return IS_PHASE_SAMPLING;
+  if (bits & (kTargetNaN | kTargetPosInf)) return IS_PHASE_TARGET;
+  if (bits & kLogweightInvalid) return IS_PHASE_WEIGHT;
+  return IS_PHASE_DENOMINATOR;
+}
+
+void decode(uint64_t packed, int64_t& index, int32_t& block, uint16_t& bits) {
+  const uint32_t inv_index = (uint32_t)(packed >> 32);
+  const uint16_t inv_block = (uint16_t)((packed >> 16) & 0xffff);
+  index = (int64_t)(uint32_t)(0xffffffffu - inv_index + 1u);
+  block = (int32_t)(uint16_t)(0xffffu - inv_block + 1u);
+  bits = (uint16_t)(packed & 0xffff);
+}
+
+template <class T>
+int job_run_typed(is_job* j) {
+  hipStream_t s = j->stream;
+  const int T_ = j->T;
+  HIP_TRY(hipMemsetAsync(j->fail_rec.p, 0, j->fail_rec.bytes, s));
+  HIP_TRY(hipMemsetAsync(j->fit_status.p, 0, j->fit_status.bytes, s));
+  HIP_TRY(hipMemsetAsync(j->cdf_status.p, 0, j->cdf_status.bytes, s));
+  HIP_TRY(hipMemsetAsync(j->counter.p, 0, j->counter.bytes, s));
+  // committed state into slot 0 (AMIS / NPMC) or the bank slots (DM-PMC)
+  const int d = j->d;
+  if (j->method == IS_DMPMC) {
+    std::vector<double> slots(j->bank_slots_init);
+    for (int k = 0; k < j->K; ++k)
+      for (int c = 0; c < d; ++c) slots[(size_t)k * j->stride + c] = j->committed_locs[(size_t)k * d + c];
+    HIP_TRY(hipMemcpyAsync(j->slots64.p, slots.data(), slots.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    std::vector<T> st(slots.begin(), slots.end());
+    HIP_TRY(hipMemcpyAsync(j->slotsT.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  } else {
+    std::vector<double> slot(j->stride);
+    pack_slot(j->committed, d, 0.0, slot.data());
+    std::vector<T> st(slot.begin(), slot.end());
+    HIP_TRY(hipMemcpyAsync(j->slots64.p, slot.data(), slot.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(j->slotsT.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  int rc = run_rounds<T>(j);
+  if (rc) return rc;
+  // failure snapshot (one transfer per run)
+  std::vector<uint64_t> rec((size_t)3 * T_);
+  std::vector<int> fit(T_), cdfs(T_);
+  HIP_TRY(hipMemcpyAsync(rec.data(), j->fail_rec.p, rec.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(fit.data(), j->fit_status.p, fit.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(cdfs.data(), j->cdf_status.p, cdfs.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  j->run_index += 1;
+  if (j->timing) {
+    j->round_ms.assign(T_, 0.0);
+    for (int t = 0; t < T_; ++t) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, j->events[2 * t], j->events[2 * t + 1]) == hipSuccess) j->round_ms[t] = ms;
+    }
+  }
+  is_failure f{};
+  for (int t = 0; t < T_; ++t) {
+    const uint64_t* w = rec.data() + 3 * t;
+    if (w[0] != 0) {
+      f.round = t + 1;
+      f.count = w[0];
+      f.words[0] = w[0];
+      f.words[1] = w[1];
+      f.words[2] = w[2];
+      decode(w[1], f.first_logical_index, f.first_block, f.reason_bits);
+      if (w[2]) {
+        int32_t blk;
+        decode(w[2], f.draw_first_logical_index, blk, f.draw_reason_bits);
+      }
+      f.phase = phase_from_bits(w[1], w[2]);
+      break;
+    }
+    if (fit[t] != 0) {
+      f.round = t + 1;
+      f.phase = fit[t] == 1 ? IS_PHASE_MOMENT : IS_PHASE_FACTORIZATION;
+      f.reason_bits = kMomentFitInvalid;
+      break;
+    }
+    if (cdfs[t] != 0) {
+      f.round = t + 1;
+      f.phase = IS_PHASE_RESAMPLING;
+      break;
+    }
+  }
+  j->failure = f;
+  if (f.round != 0) {
+    j->have_results = false;
+    j->last_status = IS_EFAILED;
+    return fail(IS_EFAILED, "sampler failed in round %d (phase %d, first index %lld)", f.round, f.phase,
+                (long long)f.first_logical_index);
+  }
+  // transactional commit (src/adaptive_gaussian_execution.jl:1101-1111)
+  if (j->method == IS_AMIS || j->method == IS_NPMC) {
+    std::vector<double> slot(j->stride);
+    HIP_TRY(hipMemcpy(slot.data(), j->slots64.as<double>() + (size_t)T_ * j->stride,
+                      slot.size() * sizeof(double), hipMemcpyDeviceToHost));
+    unpack_slot(slot.data(), d, j->committed);
+  } else if (j->method == IS_DMPMC) {
+    std::vector<double> slots((size_t)j->K * j->stride);
+    HIP_TRY(hipMemcpy(slots.data(), j->slots64.p, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < j->K; ++k)
+      for (int c = 0; c < d; ++c) j->committed_locs[(size_t)k * d + c] = slots[(size_t)k * j->stride + c];
+  }
+  j->have_results = true;
+  j->last_status = IS_OK;
+  return IS_OK;
+}
+
+template <class T>
+int alloc_job_buffers(is_job* j) {
+  const size_t cap = (size_t)j->total;
+  const int d = j->d;
+  int rc;
+  if ((rc = j->X.alloc(cap * d * sizeof(T)))) return rc;
+  if ((rc = j->logtargets.alloc(cap * sizeof(T)))) return rc;
+  if ((rc = j->lognums.alloc(cap * sizeof(T)))) return rc;
+  if ((rc = j->logweights.alloc(cap * sizeof(T)))) return rc;
+  if ((rc = j->round_ids.alloc(cap * sizeof(int32_t)))) return rc;
+  if ((rc = j->proposal_ids.alloc(cap * sizeof(int32_t)))) return rc;
+  HIP_TRY(hipMemset(j->round_ids.p, 0, cap * sizeof(int32_t)));
+  if ((rc = j->slots64.alloc((size_t)j->nslots * j->stride * sizeof(double)))) return rc;
+  if ((rc = j->slotsT.alloc((size_t)j->nslots * j->stride * sizeof(T)))) return rc;
+  HIP_TRY(hipMemset(j->slots64.p, 0, j->slots64.bytes));
+  HIP_TRY(hipMemset(j->slotsT.p, 0, j->slotsT.bytes));
+  // partial records: worst case blocks of the largest round launch
+  const int64_t blocks = cdiv((int64_t)cap, kBlock) + 2;
+  const int nv_mom = 2 + d + d * (d + 1) / 2;
+  if ((rc = j->partials.alloc((size_t)blocks * (1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->partials2.alloc((size_t)blocks * (1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->reduced.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->reduced2.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->summary.alloc((size_t)5 * std::max(1, j->T) * sizeof(double)))) return rc;
+  if ((rc = j->counter.alloc(64))) return rc;
+  if ((rc = j->fail_rec.alloc((size_t)3 * j->T * sizeof(uint64_t)))) return rc;
+  if ((rc = j->fit_status.alloc((size_t)j->T * sizeof(int)))) return rc;
+  if ((rc = j->cdf_status.alloc((size_t)j->T * sizeof(int)))) return rc;
+  if ((rc = j->round_ess.alloc((size_t)j->T * sizeof(double)))) return rc;
+  if ((rc = j->round_logz.alloc((size_t)j->T * sizeof(double)))) return rc;
+  if ((rc = j->adapt_ess.alloc((size_t)j->T * sizeof(double)))) return rc;
+  if ((rc = j->clip.alloc(64))) return rc;
+  HIP_TRY(hipMemset(j->counter.p, 0, 64));
+  std::vector<double> zeros(std::max(j->nslots, 1), 0.0);
+  if ((rc = upload_T<T>(j->zerosT, zeros))) return rc;
+  if (j->method == IS_DMPMC) {
+    if ((rc = j->cdf.alloc((size_t)j->maxN * sizeof(double)))) return rc;
+    if ((rc = j->tile_total.alloc((size_t)(cdiv(j->maxN, kCdfThreads * kCdfPer) + 1) * sizeof(double)))) return rc;
+    if ((rc = j->uniforms.alloc((size_t)j->K * j->T * sizeof(double)))) return rc;
+    if ((rc = j->ancestors.alloc((size_t)j->K * j->T * sizeof(int32_t)))) return rc;
+  }
+  return IS_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* is_last_error(void) { return g_last_error.c_str(); }
+
+int is_ctx_create(int device, is_ctx** out) {
+  if (!out) return fail(IS_EINVAL, "null output pointer");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(IS_EDEVICE, "no HIP device is available");
+  if (device < 0 || device >= count) return fail(IS_EINVAL, "device %d out of range", device);
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(IS_EUNSUPPORTED, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  is_ctx* c = new is_ctx();
+  c->device = device;
+  *out = c;
+  return IS_OK;
+}
+
+int is_ctx_destroy(is_ctx* ctx) {
+  delete ctx;
+  return IS_OK;
+}
+
+int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
+  if (!ctx || !desc || !out) return fail(IS_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int d = desc->dim;
+  if (d < 1 || d > kMaxDim) return fail(IS_EUNSUPPORTED, "dimension %d outside [1, %d]", d, kMaxDim);
+  if (desc->scalar && d != 1) return fail(IS_EINVAL, "scalar proposals have dimension 1");
+  if (desc->method < IS_PLAIN || desc->method > IS_DMPMC) return fail(IS_EINVAL, "unknown method");
+  if (desc->family != IS_GAUSSIAN && desc->family != IS_STUDENT_T) return fail(IS_EINVAL, "unknown family");
+  if (desc->precision != IS_FP64 && desc->precision != IS_FP32) return fail(IS_EINVAL, "unknown precision");
+  if (desc->target_kind != IS_TARGET_STANDARD && desc->target_kind != IS_TARGET_GAUSSIAN)
+    return fail(IS_EUNSUPPORTED, "target kind %d needs the batch-target path", desc->target_kind);
+  const int T = desc->method == IS_PLAIN ? 1 : desc->rounds;
+  if (T < 1 || !desc->round_size) return fail(IS_EINVAL, "rounds must be positive");
+  auto j = new is_job();
+  std::unique_ptr<is_job> guard(j);
+  j->ctx = ctx;
+  j->method = desc->method;
+  j->family = desc->family;
+  j->kind = desc->scale_kind;
+  j->scalar = desc->scalar;
+  j->d = d;
+  j->D = dim_bucket(d);
+  j->T = T;
+  j->dof = desc->dof;
+  j->resampling = desc->resampling;
+  j->tkind = desc->target_kind;
+  j->precision = desc->precision;
+  j->key = RngKey{desc->seed, desc->stream};
+  j->stride = slot_stride(d);
+  if (j->family == IS_STUDENT_T) {
+    if (!(std::isfinite(j->dof) && j->dof > 0.0)) return fail(IS_EINVAL, "degrees of freedom must be finite and positive");
+    j->logconstant = student_t_logconstant(j->dof, d);
+    if ((j->method == IS_AMIS || j->method == IS_NPMC) && !(j->dof > 2.0))
+      return fail(IS_EINVAL, "covariance adaptation requires Student-t degrees of freedom > 2");
+  }
+  j->schedule.assign(desc->round_size, desc->round_size + T);
+  j->offsets.assign(T + 1, 0);
+  for (int t = 0; t < T; ++t) {
+    if (j->schedule[t] <= 0) return fail(IS_EINVAL, "round sizes must be positive");
+    j->offsets[t + 1] = j->offsets[t] + j->schedule[t];
+    j->maxN = std::max(j->maxN, j->schedule[t]);
+  }
+  j->total = j->offsets[T];
+  if (j->total >= ((int64_t)1 << 31)) return fail(IS_EUNSUPPORTED, "sample budget exceeds 2^31 - 1");
+  // proposals
+  if (j->method == IS_DMPMC) {
+    const int Kc = desc->bank_size;
+    if (Kc < 1 || !desc->bank_locations || !desc->bank_scales || !desc->bank_masses)
+      return fail(IS_EINVAL, "DM-PMC needs a proposal bank");
+    std::vector<int> active;
+    for (int k = 0; k < Kc; ++k) {
+      const double m = desc->bank_masses[k];
+      if (!(std::isfinite(m) && m >= 0.0)) return fail(IS_EINVAL, "masses must be finite and nonnegative");
+      if (m != 0.0) active.push_back(k);
+    }
+    if (active.empty()) return fail(IS_EINVAL, "bank has no positive mass");
+    std::stable_sort(active.begin(), active.end(),
+                     [&](int a, int b) { return desc->bank_masses[a] < desc->bank_masses[b]; });
+    const int K = (int)active.size();
+    j->K = K;
+    j->nslots = K;
+    j->bank_slots_init.assign((size_t)K * j->stride, 0.0);
+    j->committed_locs.assign((size_t)K * d, 0.0);
+    std::vector<double> masses(K);
+    const int scale_len = desc->scale_kind == IS_SPHERICAL ? 1 : desc->scale_kind == IS_DIAGONAL ? d : d * d;
+    for (int s = 0; s < K; ++s) {
+      const int k = active[s];
+      HostProp p;
+      std::string err = make_prop(j->family, j->dof, j->kind, d, desc->bank_locations + (size_t)k * d,
+                                  desc->bank_scales + (size_t)k * scale_len, p);
+      if (!err.empty()) return fail(IS_EINVAL, "%s", err.c_str());
+      pack_slot(p, d, 0.0, j->bank_slots_init.data() + (size_t)s * j->stride);
+      for (int c = 0; c < d; ++c) j->committed_locs[(size_t)s * d + c] = p.mu[c];
+      j->bank_ids.push_back(k + 1);
+      masses[s] = desc->bank_masses[k];
+    }
+    std::vector<u128> I;
+    if (!exact_integer_masses(masses, I))
+      return fail(IS_EUNSUPPORTED, "mass ratios too wide for the exact allocation plan");
+    j->plan_counts.assign((size_t)K * T, 0);
+    j->plan_logcoef.assign((size_t)K * T, 0.0);
+    std::vector<int32_t> assign((size_t)T * j->maxN, 0);
+    for (int t = 0; t < T; ++t) {
+      std::vector<int64_t> cnt;
+      if (!round_counts(I, j->schedule[t], t + 1, cnt)) return fail(IS_EINVAL, "invalid DM-PMC allocation remainder");
+      int64_t pos = 0;
+      for (int s = 0; s < K; ++s) {
+        if (cnt[s] <= 0)
+          return fail(IS_EINVAL, "round %d of size %lld cannot allocate at least one sample to every positive-mass proposal",
+                      t + 1, (long long)j->schedule[t]);
+        j->plan_counts[(size_t)t * K + s] = cnt[s];
+        j->plan_logcoef[(size_t)t * K + s] = std::log((double)cnt[s] / (double)j->schedule[t]);
+        for (int64_t q = 0; q < cnt[s]; ++q) assign[(size_t)t * j->maxN + pos++] = s;
+      }
+    }
+    int rc;
+    if ((rc = j->assign.alloc(assign.size() * sizeof(int32_t)))) return rc;
+    HIP_TRY(hipMemcpy(j->assign.p, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if ((rc = j->bank_ids_d.alloc(j->bank_ids.size() * sizeof(int32_t)))) return rc;
+    HIP_TRY(hipMemcpy(j->bank_ids_d.p, j->bank_ids.data(), j->bank_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  } else {
+    if (!desc->location || !desc->scale) return fail(IS_EINVAL, "proposal location and scale are required");
+    std::string err = make_prop(j->family, j->dof, j->kind, d, desc->location, desc->scale, j->initial);
+    if (!err.empty()) return fail(IS_EINVAL, "%s", err.c_str());
+    j->committed = j->initial;
+    j->nslots = j->method == IS_PLAIN ? 1 : T + 1;
+  }
+  // target
+  if (j->tkind == IS_TARGET_GAUSSIAN) {
+    if (!desc->target_params || desc->target_params_len < (int64_t)d + d * d + 1)
+      return fail(IS_EINVAL, "Gaussian target needs mean, factor and lognormalizer");
+    HostProp tp;
+    tp.mu.assign(desc->target_params, desc->target_params + d);
+    tp.L.assign(desc->target_params + d, desc->target_params + d + d * d);
+    tp.lognorm = desc->target_params[d + d * d];
+    for (int c = 0; c < d; ++c)
+      if (!(tp.L[c + (size_t)c * d] != 0.0)) return fail(IS_EINVAL, "target factor diagonal must be nonzero");
+    j->tslot.assign(j->stride, 0.0);
+    pack_slot(tp, d, 0.0, j->tslot.data());
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+  int rc = j->precision == IS_FP32 ? alloc_job_buffers<float>(j) : alloc_job_buffers<double>(j);
+  if (rc) return rc;
+  // logmass tables: AMIS log N_t per slot; DM-PMC per-round log coefficients
+  std::vector<double> logmass;
+  if (j->method == IS_AMIS) {
+    logmass.assign(j->nslots, 0.0);
+    for (int t = 0; t < T; ++t) logmass[t] = std::log((double)j->schedule[t]);
+  } else if (j->method == IS_DMPMC) {
+    logmass = j->plan_logcoef;
+  } else {
+    logmass.assign(std::max(j->nslots, 1), 0.0);
+  }
+  rc = j->precision == IS_FP32 ? upload_T<float>(j->logmassT, logmass) : upload_T<double>(j->logmassT, logmass);
+  if (rc) return rc;
+  if (!j->tslot.empty()) {
+    rc = j->precision == IS_FP32 ? upload_T<float>(j->tslotT, j->tslot) : upload_T<double>(j->tslotT, j->tslot);
+    if (rc) return rc;
+  }
+  j->prefilled.assign(T, 0);
+  j->prefilled_radial.assign(T, 0);
+  j->prefilled_uniforms.assign(T, 0);
+  *out = guard.release();
+  return IS_OK;
+}
+
+int is_job_destroy(is_job* job) {
+  if (!job) return IS_OK;
+  if (job->stream) (void)hipStreamSynchronize(job->stream);
+  for (auto& e : job->events) (void)hipEventDestroy(e);
+  if (job->stream) (void)hipStreamDestroy(job->stream);
+  delete job;
+  return IS_OK;
+}
+
+int is_job_run(is_job* job) {
+  if (!job) return fail(IS_EINVAL, "null job");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  HIP_TRY(hipSetDevice(job->ctx->device));
+  return job->precision == IS_FP32 ? job_run_typed<float>(job) : job_run_typed<double>(job);
+}
+
+int is_job_reset(is_job* job) {
+  if (!job) return fail(IS_EINVAL, "null job");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  job->committed = job->initial;
+  if (job->method == IS_DMPMC)
+    for (int k = 0; k < job->K; ++k)
+      for (int c = 0; c < job->d; ++c)
+        job->committed_locs[(size_t)k * job->d + c] = job->bank_slots_init[(size_t)k * job->stride + c];
+  return IS_OK;
+}
+
+static int upload_prefill(DevBuf& buf, size_t total_doubles, size_t offset, const double* src, size_t n) {
+  if (!buf.p) {
+    int rc = buf.alloc(total_doubles * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(hipMemset(buf.p, 0, buf.bytes));
+  }
+  if (offset + n > total_doubles) return fail(IS_EINVAL, "prefilled buffer too large");
+  HIP_TRY(hipMemcpy(buf.as<double>() + offset, src, n * sizeof(double), hipMemcpyHostToDevice));
+  return IS_OK;
+}
+
+int is_job_set_normals(is_job* job, int32_t round, const double* normals, int64_t count) {
+  if (!job || !normals || round < 0 || round >= job->T) return fail(IS_EINVAL, "bad arguments");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  const int rs = radial_normal_stride(job->family, job->dof);
+  if (job->method == IS_PLAIN) {
+    const size_t need = (size_t)(job->d + rs) * job->maxN;
+    if ((size_t)count != need) return fail(IS_EINVAL, "plain IS normals need %zu values", need);
+    int rc = upload_prefill(job->pre_normals, need, 0, normals, need);
+    if (rc) return rc;
+  } else {
+    const size_t per = (size_t)job->d * job->maxN;
+    if (count < (int64_t)job->d * job->schedule[round] || (size_t)count > per)
+      return fail(IS_EINVAL, "round normals need d * round_size values");
+    int rc = upload_prefill(job->pre_normals, per * job->T, per * round, normals, count);
+    if (rc) return rc;
+  }
+  job->prefilled[round] = 1;
+  return IS_OK;
+}
+
+int is_job_set_radial(is_job* job, int32_t round, const double* normals, int64_t ncount, const double* uniforms,
+                      int64_t ucount) {
+  if (!job || round < 0 || round >= job->T) return fail(IS_EINVAL, "bad arguments");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  const int rs = radial_normal_stride(job->family, job->dof);
+  const int us = radial_uniform_stride(job->family, job->dof);
+  if (job->method == IS_PLAIN) {
+    // plain IS interleaves radial normals into the normal buffer; only uniforms are separate
+    if (us) {
+      const size_t need = (size_t)us * job->maxN;
+      if ((size_t)ucount != need || !uniforms) return fail(IS_EINVAL, "plain IS uniforms need %zu values", need);
+      int rc = upload_prefill(job->pre_ru, need, 0, uniforms, need);
+      if (rc) return rc;
+    }
+    job->prefilled_radial[round] = 1;
+    return IS_OK;
+  }
+  if (rs) {
+    const size_t per = (size_t)rs * job->maxN;
+    if (!normals || ncount > (int64_t)per) return fail(IS_EINVAL, "radial normals too long");
+    int rc = upload_prefill(job->pre_rn, per * job->T, per * round, normals, ncount);
+    if (rc) return rc;
+  }
+  if (us) {
+    const size_t per = (size_t)us * job->maxN;
+    if (!uniforms || ucount > (int64_t)per) return fail(IS_EINVAL, "radial uniforms too long");
+    int rc = upload_prefill(job->pre_ru, per * job->T, per * round, uniforms, ucount);
+    if (rc) return rc;
+  }
+  job->prefilled_radial[round] = 1;
+  return IS_OK;
+}
+
+int is_job_set_uniforms(is_job* job, int32_t round, const double* uniforms, int64_t count) {
+  if (!job || !uniforms || round < 0 || round >= job->T || job->method != IS_DMPMC)
+    return fail(IS_EINVAL, "bad arguments");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  if (count < 1 || count > job->K) return fail(IS_EINVAL, "uniform count must be in [1, K]");
+  int rc = upload_prefill(job->pre_uniforms, (size_t)job->K * job->T, (size_t)job->K * round, uniforms, count);
+  if (rc) return rc;
+  job->prefilled_uniforms[round] = 1;
+  return IS_OK;
+}
+
+int is_job_clear_prefilled(is_job* job) {
+  if (!job) return fail(IS_EINVAL, "null job");
+  std::fill(job->prefilled.begin(), job->prefilled.end(), 0);
+  std::fill(job->prefilled_radial.begin(), job->prefilled_radial.end(), 0);
+  std::fill(job->prefilled_uniforms.begin(), job->prefilled_uniforms.end(), 0);
+  return IS_OK;
+}
+
+static int export_purpose(is_job* job, int32_t round, int64_t n, int count, uint64_t purpose, bool uniform,
+                          std::vector<double>& host) {
+  host.assign((size_t)n * count, 0.0);
+  if (n * count == 0) return IS_OK;
+  DevBuf tmp;
+  int rc = tmp.alloc(host.size() * sizeof(double));
+  if (rc) return rc;
+  const uint64_t word = round_word(job, round);
+  if (uniform)
+    hipLaunchKernelGGL(k_export_uniforms, dim3(cdiv(n, 256)), dim3(256), 0, job->stream, job->key, word, n, count,
+                       purpose, tmp.as<double>());
+  else
+    hipLaunchKernelGGL(k_export_normals, dim3(cdiv(n, 256)), dim3(256), 0, job->stream, job->key, word, n, count,
+                       purpose, tmp.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(host.data(), tmp.p, host.size() * sizeof(double), hipMemcpyDeviceToHost, job->stream));
+  HIP_TRY(hipStreamSynchronize(job->stream));
+  return IS_OK;
+}
+
+int is_job_export_normals(is_job* job, int32_t round, double* out, int64_t count) {
+  if (!job || !out || round < 0 || round >= job->T) return fail(IS_EINVAL, "bad arguments");
+  const int d = job->d;
+  const int rs = radial_normal_stride(job->family, job->dof);
+  const int64_t N = job->schedule[round];
+  std::vector<double> z, r;
+  int rc = export_purpose(job, round, N, d, kPurposeNormals, false, z);
+  if (rc) return rc;
+  if (job->method == IS_PLAIN) {
+    const int stride = d + rs;
+    if (count != (int64_t)stride * N) return fail(IS_EINVAL, "plain IS export needs stride * n values");
+    if ((rc = export_purpose(job, round, N, rs, kPurposeRadialNormals, false, r))) return rc;
+    for (int64_t i = 0; i < N; ++i) {
+      for (int c = 0; c < d; ++c) out[i * stride + c] = z[(size_t)i * d + c];
+      for (int q = 0; q < rs; ++q) out[i * stride + d + q] = r[(size_t)i * rs + q];
+    }
+    return IS_OK;
+  }
+  if (count != (int64_t)d * N) return fail(IS_EINVAL, "round export needs d * round_size values");
+  std::memcpy(out, z.data(), z.size() * sizeof(double));
+  return IS_OK;
+}
+
+int is_job_export_radial(is_job* job, int32_t round, double* normals, int64_t ncount, double* uniforms,
+                         int64_t ucount) {
+  if (!job || round < 0 || round >= job->T) return fail(IS_EINVAL, "bad arguments");
+  const int rs = radial_normal_stride(job->family, job->dof);
+  const int us = radial_uniform_stride(job->family, job->dof);
+  const int64_t N = job->schedule[round];
+  std::vector<double> r, u;
+  int rc;
+  if (normals && rs) {
+    if (ncount != (int64_t)rs * N) return fail(IS_EINVAL, "radial normal export needs rs * n values");
+    if ((rc = export_purpose(job, round, N, rs, kPurposeRadialNormals, false, r))) return rc;
+    std::memcpy(normals, r.data(), r.size() * sizeof(double));
+  }
+  if (uniforms && us) {
+    if (ucount != (int64_t)us * N) return fail(IS_EINVAL, "radial uniform export needs us * n values");
+    if ((rc = export_purpose(job, round, N, us, kPurposeRadialUniforms, true, u))) return rc;
+    std::memcpy(uniforms, u.data(), u.size() * sizeof(double));
+  }
+  return IS_OK;
+}
+
+int is_job_export_uniforms(is_job* job, int32_t round, double* out, int64_t count) {
+  if (!job || !out || round < 0 || round >= job->T || job->method != IS_DMPMC) return fail(IS_EINVAL, "bad arguments");
+  if (count != job->K) return fail(IS_EINVAL, "resampling uniforms: K values");
+  std::vector<double> u;
+  int rc = export_purpose(job, round, job->K, 1, kPurposeResampling, true, u);
+  if (rc) return rc;
+  std::memcpy(out, u.data(), u.size() * sizeof(double));
+  return IS_OK;
+}
+
+int is_job_set_timing(is_job* job, int32_t enable) {
+  if (!job) return fail(IS_EINVAL, "null job");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  job->timing = enable ? 1 : 0;
+  if (job->timing && job->events.empty()) {
+    job->events.resize((size_t)2 * job->T);
+    for (auto& e : job->events) HIP_TRY(hipEventCreate(&e));
+  }
+  return IS_OK;
+}
+
+int is_job_round_kernel_ms(is_job* job, double* out) {
+  if (!job || !out) return fail(IS_EINVAL, "null argument");
+  if (!job->timing || job->round_ms.size() != (size_t)job->T) return fail(IS_ESTATE, "timing was not enabled for the last run");
+  std::memcpy(out, job->round_ms.data(), job->round_ms.size() * sizeof(double));
+  return IS_OK;
+}
+
+int64_t is_job_total(const is_job* job) { return job ? job->total : 0; }
+
+int is_job_failure(const is_job* job, is_failure* out) {
+  if (!job || !out) return fail(IS_EINVAL, "bad arguments");
+  *out = job->failure;
+  return IS_OK;
+}
+
+#define REQUIRE_RESULTS(job)                                                   \
+  do {                                                                         \
+    if (!(job)) return fail(IS_EINVAL, "null job");                            \
+    if (!(job)->have_results) return fail(IS_ESTATE, "no successful run yet"); \
+  } while (0)
+
+int is_job_get_samples(is_job* job, double* out) {
+  REQUIRE_RESULTS(job);
+  const int64_t n = job->total;
+  DevBuf tmp;
+  int rc = tmp.alloc((size_t)n * job->d * sizeof(double));
+  if (rc) return rc;
+  if (job->precision == IS_FP32)
+    hipLaunchKernelGGL((k_soa_to_aos<float>), dim3(cdiv(n * job->d, 256)), dim3(256), 0, job->stream,
+                       job->X.as<float>(), n, n, job->d, tmp.as<double>());
+  else
+    hipLaunchKernelGGL((k_soa_to_aos<double>), dim3(cdiv(n * job->d, 256)), dim3(256), 0, job->stream,
+                       job->X.as<double>(), n, n, job->d, tmp.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost, job->stream));
+  HIP_TRY(hipStreamSynchronize(job->stream));
+  return IS_OK;
+}
+
+static int copy_as_double(is_job* job, const DevBuf& src, int64_t n, double* out) {
+  if (job->precision == IS_FP64) {
+    HIP_TRY(hipMemcpyAsync(out, src.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, job->stream));
+  } else {
+    DevBuf tmp;
+    int rc = tmp.alloc((size_t)n * sizeof(double));
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_to_double<float>), dim3(cdiv(n, 256)), dim3(256), 0, job->stream, src.as<float>(), n,
+                       tmp.as<double>());
+    HIP_TRY(hipMemcpyAsync(out, tmp.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, job->stream));
+    HIP_TRY(hipStreamSynchronize(job->stream));
+    return IS_OK;
+  }
+  HIP_TRY(hipStreamSynchronize(job->stream));
+  return IS_OK;
+}
+
+int is_job_get_logweights(is_job* job, double* out) {
+  REQUIRE_RESULTS(job);
+  return copy_as_double(job, job->logweights, job->total, out);
+}
+
+static int copy_ids(is_job* job, const DevBuf& src, int64_t* out) {
+  std::vector<int32_t> h((size_t)job->total);
+  HIP_TRY(hipMemcpy(h.data(), src.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+  return IS_OK;
+}
+
+int is_job_get_round_ids(is_job* job, int64_t* out) {
+  REQUIRE_RESULTS(job);
+  return copy_ids(job, job->round_ids, out);
+}
+
+int is_job_get_proposal_ids(is_job* job, int64_t* out) {
+  REQUIRE_RESULTS(job);
+  if (job->method != IS_DMPMC) return fail(IS_ESTATE, "proposal ids exist for DM-PMC only");
+  return copy_ids(job, job->proposal_ids, out);
+}
+
+int is_job_get_round_summary(is_job* job, double* ess, double* lognormalizers, double* adaptation_ess) {
+  REQUIRE_RESULTS(job);
+  if (ess) HIP_TRY(hipMemcpy(ess, job->round_ess.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
+  if (lognormalizers)
+    HIP_TRY(hipMemcpy(lognormalizers, job->round_logz.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
+  if (adaptation_ess)
+    HIP_TRY(hipMemcpy(adaptation_ess, job->adapt_ess.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
+  return IS_OK;
+}
+
+int is_job_get_proposal(is_job* job, double* location, double* factor, double* lognormalizer) {
+  if (!job) return fail(IS_EINVAL, "null job");
+  if (job->method == IS_PLAIN || job->method == IS_DMPMC) return fail(IS_ESTATE, "no adaptive proposal");
+  const int d = job->d;
+  if (location) std::memcpy(location, job->committed.mu.data(), d * sizeof(double));
+  if (factor) {
+    if (job->scalar)
+      factor[0] = job->committed.L[0];
+    else
+      std::memcpy(factor, job->committed.L.data(), (size_t)d * d * sizeof(double));
+  }
+  if (lognormalizer) *lognormalizer = job->committed.lognorm;
+  return IS_OK;
+}
+
+int is_job_get_history(is_job* job, double* means, double* factors, double* lognormalizers) {
+  REQUIRE_RESULTS(job);
+  if (job->method != IS_AMIS && job->method != IS_NPMC) return fail(IS_ESTATE, "no proposal history");
+  const int d = job->d;
+  std::vector<double> slots((size_t)job->nslots * job->stride);
+  HIP_TRY(hipMemcpy(slots.data(), job->slots64.p, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < job->nslots; ++s) {
+    HostProp p;
+    unpack_slot(slots.data() + (size_t)s * job->stride, d, p);
+    if (means) std::memcpy(means + (size_t)s * d, p.mu.data(), d * sizeof(double));
+    if (factors) std::memcpy(factors + (size_t)s * d * d, p.L.data(), (size_t)d * d * sizeof(double));
+    if (lognormalizers) lognormalizers[s] = p.lognorm;
+  }
+  return IS_OK;
+}
+
+int is_job_get_locations(is_job* job, double* out) {
+  if (!job || !out) return fail(IS_EINVAL, "bad arguments");
+  if (job->method != IS_DMPMC) return fail(IS_ESTATE, "locations exist for DM-PMC only");
+  std::memcpy(out, job->committed_locs.data(), job->committed_locs.size() * sizeof(double));
+  return IS_OK;
+}
+
+int is_job_get_ancestors(is_job* job, int64_t* out) {
+  REQUIRE_RESULTS(job);
+  if (job->method != IS_DMPMC) return fail(IS_ESTATE, "ancestors exist for DM-PMC only");
+  std::vector<int32_t> h((size_t)job->K * job->T);
+  HIP_TRY(hipMemcpy(h.data(), job->ancestors.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+  return IS_OK;
+}
+
+int is_job_device_results(is_job* job, const void** samples, int64_t* coord_stride, const void** logweights,
+                          int32_t* is_float) {
+  REQUIRE_RESULTS(job);
+  if (samples) *samples = job->X.p;
+  if (coord_stride) *coord_stride = job->total;
+  if (logweights) *logweights = job->logweights.p;
+  if (is_float) *is_float = job->precision == IS_FP32;
+  return IS_OK;
+}
+
+extern "C++" {
+template <class T>
+static int job_lognormalizer_typed(is_job* job, double* out3) {
+  const int64_t n = job->total;
+  const int blocks = (int)cdiv(n, kBlock);
+  launch_moments<T>(4, blocks, tile_lds_bytes(job->d), job->stream, job->X.as<T>(), n, 1, job->logweights.as<T>(), 0,
+                    n, job->d, nullptr, nullptr, 0, 2, job->partials2.as<double>());
+  FinishArgs f{};
+  f.d = job->d;
+  f.flags = kFinishSummary;
+  f.summary_count = n;
+  f.summary_out = job->summary.as<double>();
+  f.counter = job->counter.as<unsigned int>() + 1;
+  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), finish_lds(job->d), job->stream,
+                     job->partials2.as<double>(), (int64_t)blocks, 2, job->reduced2.as<double>(), f);
+  HIP_TRY(hipGetLastError());
+  double s[5];
+  HIP_TRY(hipMemcpyAsync(s, job->summary.p, sizeof(s), hipMemcpyDeviceToHost, job->stream));
+  HIP_TRY(hipStreamSynchronize(job->stream));
+  out3[0] = s[4];
+  out3[1] = s[0];
+  out3[2] = s[1];
+  return IS_OK;
+}
+}  // extern "C++"
+
+int is_job_lognormalizer(is_job* job, double* out3) {
+  REQUIRE_RESULTS(job);
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  return job->precision == IS_FP32 ? job_lognormalizer_typed<float>(job, out3)
+                                   : job_lognormalizer_typed<double>(job, out3);
+}
+
+extern "C++" {
+template <class T>
+static int job_moments_typed(is_job* job, double* mean, double* var, double* cov) {
+  const int64_t n = job->total;
+  const int d = job->d;
+  const int blocks = (int)cdiv(n, kBlock);
+  const int nv = 2 + d + d * (d + 1) / 2;
+  // centre at the FP64 proposal mean of the first slot to keep the single pass well conditioned
+  DevBuf center;
+  int rc = center.alloc((size_t)d * sizeof(double));
+  if (rc) return rc;
+  std::vector<double> c0(d, 0.0);
+  if (job->method == IS_AMIS || job->method == IS_NPMC || job->method == IS_PLAIN)
+    c0 = job->committed.mu;
+  else
+    for (int c = 0; c < d; ++c) c0[c] = job->committed_locs[c];
+  HIP_TRY(hipMemcpy(center.p, c0.data(), d * sizeof(double), hipMemcpyHostToDevice));
+  launch_moments<T>(job->D, blocks, tile_lds_bytes(d), job->stream, job->X.as<T>(), n, 1, job->logweights.as<T>(), 0, n,
+                    d, nullptr, center.as<double>(), 1, nv, job->partials2.as<double>());
+  FinishArgs f{};
+  f.d = d;
+  f.flags = 0;
+  f.counter = job->counter.as<unsigned int>() + 2;
+  hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), finish_lds(d), job->stream,
+                     job->partials2.as<double>(), (int64_t)blocks, nv, job->reduced2.as<double>(), f);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> red(1 + nv);
+  HIP_TRY(hipMemcpyAsync(red.data(), job->reduced2.p, red.size() * sizeof(double), hipMemcpyDeviceToHost, job->stream));
+  HIP_TRY(hipStreamSynchronize(job->stream));
+  const double s0 = red[1];
+  if (!(s0 > 0.0)) return fail(IS_EFAILED, "all weights are zero");
+  std::vector<double> delta(d);
+  for (int c = 0; c < d; ++c) {
+    delta[c] = red[3 + c] / s0;
+    if (mean) mean[c] = c0[c] + delta[c];
+  }
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c <= r; ++c) {
+      const double v = red[3 + d + r * (r + 1) / 2 + c] / s0 - delta[r] * delta[c];
+      if (cov) {
+        cov[r + (size_t)c * d] = v;
+        cov[c + (size_t)r * d] = v;
+      }
+      if (var && r == c) var[r] = v;
+    }
+  return IS_OK;
+}
+}  // extern "C++"
+
+int is_job_moments(is_job* job, double* mean, double* var, double* cov) {
+  REQUIRE_RESULTS(job);
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  return job->precision == IS_FP32 ? job_moments_typed<float>(job, mean, var, cov)
+                                   : job_moments_typed<double>(job, mean, var, cov);
+}
+
+extern "C++" {
+template <class T>
+static int job_resample_typed(is_job* job, const double* uniforms, int64_t count, int32_t method, int64_t* ancestors) {
+  const int64_t n = job->total;
+  double lz[3];
+  int rc = job_lognormalizer_typed<T>(job, lz);
+  if (rc) return rc;
+  if (lz[1] == -__builtin_huge_val()) return fail(IS_EFAILED, "all weights are zero");
+  DevBuf cdf, tiles, u, anc, st;
+  const int64_t ntiles = cdiv(n, (int64_t)kCdfThreads * kCdfPer);
+  const int64_t ucount = method == IS_SYSTEMATIC ? 1 : count;
+  if ((rc = cdf.alloc((size_t)n * sizeof(double))) || (rc = tiles.alloc((size_t)ntiles * sizeof(double))) ||
+      (rc = u.alloc((size_t)ucount * sizeof(double))) || (rc = anc.alloc((size_t)count * sizeof(int32_t))) ||
+      (rc = st.alloc(sizeof(int))))
+    return rc;
+  HIP_TRY(hipMemcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL((k_cdf_tiles<T>), dim3(ntiles), dim3(kCdfThreads), 0, job->stream, job->logweights.as<T>(), n,
+                     job->summary.as<double>(), tiles.as<double>());
+  hipLaunchKernelGGL((k_cdf_write<T>), dim3(ntiles), dim3(kCdfThreads), 0, job->stream, job->logweights.as<T>(), n,
+                     job->summary.as<double>(), tiles.as<double>(), ntiles, cdf.as<double>(), st.as<int>());
+  hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(count, 256)), dim3(256), 0, job->stream, cdf.as<double>(),
+                     job->logweights.as<T>(), n, u.as<double>(), count, method == IS_SYSTEMATIC ? 1 : 0,
+                     anc.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  std::vector<int32_t> h((size_t)count);
+  HIP_TRY(hipMemcpyAsync(h.data(), anc.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, job->stream));
+  HIP_TRY(hipStreamSynchronize(job->stream));
+  for (int64_t i = 0; i < count; ++i) ancestors[i] = h[i];
+  return IS_OK;
+}
+}  // extern "C++"
+
+int is_job_resample(is_job* job, const double* uniforms, int64_t count, int32_t method, int64_t* ancestors) {
+  REQUIRE_RESULTS(job);
+  if (!uniforms || !ancestors || count < 1) return fail(IS_EINVAL, "resampling count must be positive");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  return job->precision == IS_FP32 ? job_resample_typed<float>(job, uniforms, count, method, ancestors)
+                                   : job_resample_typed<double>(job, uniforms, count, method, ancestors);
+}
+
+int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* stream) {
+  if (!ctx || !lw || !cdf || n < 1) return fail(IS_EINVAL, "bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipSetDevice(ctx->device));
+  // max via the partial machinery
+  const int blocks = (int)cdiv(n, kBlock);
+  DevBuf partials, reduced, summary, counter, tiles, st;
+  int rc;
+  const int64_t ntiles = cdiv(n, (int64_t)kCdfThreads * kCdfPer);
+  if ((rc = partials.alloc((size_t)blocks * 3 * sizeof(double))) || (rc = reduced.alloc(4 * sizeof(double))) ||
+      (rc = summary.alloc(5 * sizeof(double))) || (rc = counter.alloc(64)) ||
+      (rc = tiles.alloc((size_t)ntiles * sizeof(double))) || (rc = st.alloc(sizeof(int))))
+    return rc;
+  HIP_TRY(hipMemsetAsync(counter.p, 0, 64, s));
+  launch_moments<double>(4, blocks, tile_lds_bytes(1), s, lw, n, 1, lw, 0, n, 1, nullptr, nullptr, 0, 2,
+                         partials.as<double>());
+  FinishArgs f{};
+  f.d = 1;
+  f.flags = kFinishSummary;
+  f.summary_count = n;
+  f.summary_out = summary.as<double>();
+  f.counter = counter.as<unsigned int>();
+  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), finish_lds(1), s, partials.as<double>(),
+                     (int64_t)blocks, 2, reduced.as<double>(), f);
+  hipLaunchKernelGGL((k_cdf_tiles<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),
+                     tiles.as<double>());
+  hipLaunchKernelGGL((k_cdf_write<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),
+                     tiles.as<double>(), ntiles, cdf, st.as<int>());
+  HIP_TRY(hipGetLastError());
+  int status = 0;
+  HIP_TRY(hipMemcpyAsync(&status, st.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (status) return fail(IS_EFAILED, "all weights are zero");
+  return IS_OK;
+}
+
+int is_select_device(is_ctx* ctx, const double* cdf, const double* lw, int64_t n, const double* uniforms, int64_t count,
+                     int32_t method, int32_t* ancestors, void* stream) {
+  if (!ctx || !cdf || !lw || !uniforms || !ancestors || n < 1 || count < 1) return fail(IS_EINVAL, "bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL((k_select_ancestors<double>), dim3(cdiv(count, 256)), dim3(256), 0, s, cdf, lw, n, uniforms,
+                     count, method == IS_SYSTEMATIC ? 1 : 0, ancestors);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return IS_OK;
+}
+
+int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf) {
+  if (!ctx || !logweights || !cdf || n < 1) return fail(IS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevBuf lw, out;
+  int rc;
+  if ((rc = lw.alloc((size_t)n * sizeof(double))) || (rc = out.alloc((size_t)n * sizeof(double)))) return rc;
+  HIP_TRY(hipMemcpy(lw.p, logweights, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  rc = is_cdf_device(ctx, lw.as<double>(), n, out.as<double>(), nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(cdf, out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  return IS_OK;
+}
+
+int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t n, const double* uniforms,
+              int64_t count, int32_t method, int64_t* ancestors) {
+  if (!ctx || !cdf || !logweights || !uniforms || !ancestors || n < 1 || count < 1)
+    return fail(IS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t ucount = method == IS_SYSTEMATIC ? 1 : count;
+  DevBuf c, lw, u, a;
+  int rc;
+  if ((rc = c.alloc((size_t)n * sizeof(double))) || (rc = lw.alloc((size_t)n * sizeof(double))) ||
+      (rc = u.alloc((size_t)ucount * sizeof(double))) || (rc = a.alloc((size_t)count * sizeof(int32_t))))
+    return rc;
+  HIP_TRY(hipMemcpy(c.p, cdf, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(lw.p, logweights, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
+  rc = is_select_device(ctx, c.as<double>(), lw.as<double>(), n, u.as<double>(), count, method, a.as<int32_t>(),
+                        nullptr);
+  if (rc) return rc;
+  std::vector<int32_t> h((size_t)count);
+  HIP_TRY(hipMemcpy(h.data(), a.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < count; ++i) ancestors[i] = h[i];
+  return IS_OK;
+}
+
+int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3, int64_t n,
+               uint64_t* out) {
+  if (!ctx || !out || n < 1) return fail(IS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevBuf tmp;
+  int rc = tmp.alloc((size_t)n * 4 * sizeof(uint64_t));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rng_raw, dim3(cdiv(n, 256)), dim3(256), 0, 0, key0, key1, c1, c2, c3, n,
+                     tmp.as<unsigned long long>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost));
+  return IS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+/*
+ * issampler.h -- C ABI of the MI355X (gfx950) importance-sampling engine.
+ *
+ * The engine replaces the device-side execution of ImportanceSamplers.jl's
+ * sampling hot path.  A host binding (Julia `ccall`, Python ctypes, ...) maps
+ * the reference's backend hooks onto these entry points; every declaration
+ * names the reference interface it stands in for (paths relative to the
+ * reference repository root).
+ *
+ * Conventions
+ *   - Plain C types only; host pointers unless a name says `device`.
+ *   - Samples cross the boundary in the reference's layout: a d x n column-major
+ *     matrix (each sample's d coordinates contiguous), src/storage.jl:74-85.
+ *   - Return value: 0 = IS_OK, negative = API misuse or device error,
+ *     positive = the run failed inside the sampler (IS_EFAILED); details via
+ *     is_job_failure(), which mirrors the packed failure record of
+ *     src/execution.jl:669-767.
+ *   - A job handle is non-reentrant: a concurrent call on a busy handle
+ *     returns IS_EBUSY (SamplerBusyError, src/methods/importance_sampling.jl:105-116).
+ *     Distinct handles may be used from distinct host threads.
+ */
+#ifndef ISSAMPLER_H
+#define ISSAMPLER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define IS_OK 0
+#define IS_EINVAL (-1)        /* ArgumentError / DimensionMismatch at preparation */
+#define IS_ENOMEM (-2)
+#define IS_EBUSY (-3)         /* SamplerBusyError */
+#define IS_EDEVICE (-4)       /* HIP runtime error */
+#define IS_EUNSUPPORTED (-5)  /* configuration outside the engine's scope */
+#define IS_ESTATE (-6)        /* call order violated (e.g. results before a run) */
+#define IS_EFAILED 1          /* sampler execution failed: see is_job_failure */
+
+/* ---- enumerations (reference types in parentheses) ------------------------------- */
+enum is_method {
+  IS_PLAIN = 0,   /* ImportanceSampling(proposal; nsamples), src/methods/importance_sampling.jl */
+  IS_AMIS = 1,    /* AMIS(proposal; rounds, round_size), src/methods/amis.jl:42-47 */
+  IS_NPMC = 2,    /* NPMC(proposal; rounds, round_size), src/methods/npmc.jl:21-32 */
+  IS_DMPMC = 3    /* DeterministicMixturePMC(bank; rounds, round_size), src/methods/dm_pmc.jl:72-120 */
+};
+enum is_family { IS_GAUSSIAN = 0, IS_STUDENT_T = 1 };             /* GaussianFamily / StudentTFamily */
+enum is_scale_kind { IS_SPHERICAL = 0, IS_DIAGONAL = 1, IS_FACTOR = 2 }; /* src/proposals.jl:158-169 */
+enum is_precision { IS_FP64 = 0, IS_FP32 = 1 };
+enum is_resampling { IS_MULTINOMIAL = 0, IS_SYSTEMATIC = 1 };     /* MultinomialResampling; systematic is new */
+enum is_target_kind {
+  IS_TARGET_STANDARD = 0,  /* -sum(abs2, x)/2                                              */
+  IS_TARGET_GAUSSIAN = 1,  /* params: mean[d], L[d*d] column-major lower, lognormalizer     */
+  IS_TARGET_LOGISTIC = 2,  /* params: n, p, prior_var, X[n*p] row-major, y[n]               */
+  IS_TARGET_LINREG = 3     /* params: n, p, sigma, tau, X[n*p] row-major, y[n]              */
+};
+enum is_phase {            /* AMISRoundError / NPMCRoundError / DMPMCRoundError phases       */
+  IS_PHASE_NONE = 0,
+  IS_PHASE_SAMPLING = 1,   /* :sampling / :proposal_draw                                     */
+  IS_PHASE_TARGET = 2,     /* :target                                                        */
+  IS_PHASE_DENOMINATOR = 3,/* :denominator / :proposal_logdensity                            */
+  IS_PHASE_WEIGHT = 4,     /* :weight / :logweight                                           */
+  IS_PHASE_MOMENT = 5,     /* :moment (AllZeroWeightsError)                                  */
+  IS_PHASE_FACTORIZATION = 6, /* :factorization (PosDefException / invalid candidate)        */
+  IS_PHASE_RESAMPLING = 7  /* :resampling (AllZeroWeightsError)                              */
+};
+
+typedef struct is_ctx is_ctx;
+typedef struct is_job is_job;
+
+/* Job descriptor: the prepared-sampler state of _prepare_importance_sampler
+ * (src/methods/importance_sampling.jl:357-440) and the method states
+ * (src/adaptive_gaussian.jl:208-234, src/population_preparation.jl:370-400). */
+typedef struct is_job_desc {
+  int32_t method;          /* enum is_method */
+  int32_t family;          /* enum is_family */
+  int32_t scale_kind;      /* enum is_scale_kind */
+  int32_t scalar;          /* 1: scalar location (d = 1, scalar history) */
+  int32_t dim;             /* d */
+  double dof;              /* Student-t degrees of freedom */
+  int32_t rounds;          /* T (plain IS: 1) */
+  const int64_t* round_size; /* T entries (plain IS: nsamples) */
+  /* single proposal (plain, AMIS, NPMC) */
+  const double* location;  /* d */
+  const double* scale;     /* spherical: 1, diagonal: d, factor: d*d column-major lower */
+  /* proposal bank (DM-PMC): configured order, zero masses are inert */
+  int32_t bank_size;       /* K */
+  const double* bank_locations; /* d*K column-major */
+  const double* bank_scales;    /* spherical: K, diagonal: d*K, factor: d*d*K */
+  const double* bank_masses;    /* K */
+  int32_t resampling;      /* enum is_resampling */
+  /* target */
+  int32_t target_kind;     /* enum is_target_kind */
+  const double* target_params;
+  int64_t target_params_len;
+  /* execution */
+  int32_t precision;       /* enum is_precision */
+  uint64_t seed;           /* counter-RNG key word 0 */
+  uint64_t stream;         /* counter-RNG key word 1 */
+} is_job_desc;
+
+/* Failure snapshot: _device_failure_snapshot (src/execution.jl:719-734) plus
+ * the round / phase of the adaptive error types. */
+typedef struct is_failure {
+  int32_t round;           /* 1-based failing round, 0 when none */
+  int32_t phase;           /* enum is_phase */
+  uint64_t count;
+  int64_t first_logical_index;
+  int32_t first_block;
+  uint16_t reason_bits;
+  int64_t draw_first_logical_index; /* 0 when no draw failure */
+  uint16_t draw_reason_bits;
+  uint64_t words[3];       /* raw packed record */
+} is_failure;
+
+/* ---- context ---------------------------------------------------------------------
+ * _with_backend_device / _backend_functional (src/methods/importance_sampling.jl:507-508,
+ * ext/ImportanceSamplersCUDAExt.jl:9-19). */
+int is_ctx_create(int device, is_ctx** out);
+int is_ctx_destroy(is_ctx* ctx);
+const char* is_last_error(void);
+
+/* ---- jobs ------------------------------------------------------------------------
+ * is_job_create: prepare_sampler + device transfer + _prepare_backend_execution
+ * (src/methods/importance_sampling.jl:357-404, 599-719). */
+int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out);
+int is_job_destroy(is_job* job);
+
+/* importance_sample! -> _execute_prepared_sampler! (src/methods/importance_sampling.jl:912-953);
+ * adaptive loops src/adaptive_gaussian_execution.jl:897-1112 and src/population_execution.jl:407-567.
+ * Asynchronous on the job's stream up to the final failure snapshot. */
+int is_job_run(is_job* job);
+
+/* RNG seams mirroring the prefilled-RNG test pattern (test/amis_kernel.jl:15-34,
+ * test/npmc.jl:5-13, test/support/dm_pmc.jl:10-42): the given round uses these
+ * normals / uniforms instead of the counter RNG until is_job_clear_prefilled.
+ * normals: plain IS stride*n per src/execution.jl:560-583; adaptive rounds d*N_t;
+ * radial: Student-t gamma buffers (8 normals, 8 + (nu < 2) uniforms per sample; 1 normal for nu == 1). */
+int is_job_set_normals(is_job* job, int32_t round, const double* normals, int64_t count);
+int is_job_set_radial(is_job* job, int32_t round, const double* normals, int64_t ncount,
+                      const double* uniforms, int64_t ucount);
+int is_job_set_uniforms(is_job* job, int32_t round, const double* uniforms, int64_t count);
+int is_job_clear_prefilled(is_job* job);
+/* The counter-RNG values the job consumes in `round` (for feeding the oracle). */
+int is_job_export_normals(is_job* job, int32_t round, double* out, int64_t count);
+int is_job_export_radial(is_job* job, int32_t round, double* normals, int64_t ncount,
+                         double* uniforms, int64_t ucount);
+int is_job_export_uniforms(is_job* job, int32_t round, double* out, int64_t count);
+
+/* Event timing of the round kernel (draw / append / mixture / weight pass) of
+ * each round, on the job's stream; no reference counterpart (measurement seam). */
+int is_job_set_timing(is_job* job, int32_t enable);
+int is_job_round_kernel_ms(is_job* job, double* out); /* T entries, last run */
+
+/* ---- results (WeightedSamples, src/results.jl:79-98) -------------------------------- */
+int64_t is_job_total(const is_job* job);
+int is_job_failure(const is_job* job, is_failure* out);
+int is_job_get_samples(is_job* job, double* out);           /* d x total */
+int is_job_get_logweights(is_job* job, double* out);        /* total */
+int is_job_get_round_ids(is_job* job, int64_t* out);        /* provenance.round */
+int is_job_get_proposal_ids(is_job* job, int64_t* out);     /* provenance.proposal_id (DM-PMC) */
+/* diagnostics.round_ess / round_lognormalizers / adaptation_ess (src/adaptive_gaussian_execution.jl:1068-1089) */
+int is_job_get_round_summary(is_job* job, double* ess, double* lognormalizers, double* adaptation_ess);
+/* current_proposal (src/adaptive_gaussian.jl:382-412): committed mean, factor (d*d column-major
+ * lower; scalar: 1) and lognormalizer. */
+int is_job_get_proposal(is_job* job, double* location, double* factor, double* lognormalizer);
+/* Fitted proposal history slots 1..T+1 (means (T+1)*d, factors (T+1)*d*d, lognormalizers T+1). */
+int is_job_get_history(is_job* job, double* means, double* factors, double* lognormalizers);
+/* DM-PMC: current_proposal locations (d x K, active order) and per-round ancestors (K x T, 1-based). */
+int is_job_get_locations(is_job* job, double* out);
+int is_job_get_ancestors(is_job* job, int64_t* out);
+/* Device-resident logweights / samples (SoA, coordinate stride = capacity) of the last run. */
+int is_job_device_results(is_job* job, const void** samples, int64_t* coord_stride,
+                          const void** logweights, int32_t* is_float);
+/* Reset the committed proposal (AMIS/NPMC) or locations (DM-PMC) to the descriptor's. */
+int is_job_reset(is_job* job);
+
+/* ---- results API on the last run --------------------------------------------------
+ * lognormalizer (src/results.jl:560-567): out = {log Z, max, scaled_sum};
+ * mean / var / cov (src/statistics.jl:11-103): FP64 device reductions. */
+int is_job_lognormalizer(is_job* job, double* out3);
+int is_job_moments(is_job* job, double* mean, double* var, double* cov);
+/* resample (src/resampling.jl:245-283): multinomial or systematic ancestors of the
+ * last run's logweights for host uniforms (count, or 1 for systematic). */
+int is_job_resample(is_job* job, const double* uniforms, int64_t count, int32_t method,
+                    int64_t* ancestors);
+
+/* ---- primitives on device buffers (stream = hipStream_t or NULL) -------------------
+ * _resampling_cdf! (src/resampling.jl:199-225) with the engine's blocked order,
+ * and ancestor selection (src/resampling.jl:92-118). */
+int is_cdf_device(is_ctx* ctx, const double* logweights_device, int64_t n, double* cdf_device,
+                  void* stream);
+int is_select_device(is_ctx* ctx, const double* cdf_device, const double* logweights_device, int64_t n,
+                     const double* uniforms_device, int64_t count, int32_t method,
+                     int32_t* ancestors_device, void* stream);
+/* Host-buffer forms of the two primitives above (copy in, run, copy out); ancestors 1-based. */
+int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf);
+int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t n, const double* uniforms,
+              int64_t count, int32_t method, int64_t* ancestors);
+/* Raw Philox4x64-10 words for counters (i, c1, c2, c3), i = 0..n-1 (RNG known-answer seam). */
+int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3,
+               int64_t n, uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISSAMPLER_H */

@@ -120,7 +120,7 @@ def _declare(L):
     L.or_adaptive.restype = i
     L.or_adaptive.argtypes = [i, i, d, i, i, F64P, F64P, d, i, I64P, F64P, F64P, F64P, i, F64P, i,
                               F64P, F64P, F64P, F64P, I64P, F64P, F64P, F64P, F64P, F64P, F64P,
-                              U64P, IP]
+                              U64P, IP, i, F64P, F64P, F64P]
     L.or_dmpmc.restype = i
     L.or_dmpmc.argtypes = [i, d, i, i, i, F64P, F64P, F64P, I64P, i, I64P, I64P, F64P, i, i, i,
                            F64P, F64P, F64P, F64P, i, F64P, i, F64P, F64P, I64P, I64P, I64P,
@@ -328,8 +328,12 @@ def systematic_counts_exact(cdf, u):
 
 # ---------------------------------------------------------------- AMIS / NPMC
 def adaptive(method, p, schedule, normals, tkind, tparams, radial_normals=None,
-             radial_uniforms=None, nthreads=1, scalar=None):
-    """method 0 AMIS, 1 NPMC.  normals: (T, d*maxN) per-round batches."""
+             radial_uniforms=None, nthreads=1, scalar=None, replay=None):
+    """method 0 AMIS, 1 NPMC.  normals: (T, d*maxN) per-round batches.
+
+    replay=(means (T+1, d), factors (T+1, d, d) or (T+1,), lognormalizers (T+1,)):
+    every round samples from the given slots; each round's own fit is returned
+    in fit_means / fit_factors / fit_lognormalizers."""
     s = as_slot(p)
     schedule = np.ascontiguousarray(np.asarray(schedule, dtype=np.int64))
     T = schedule.size
@@ -357,18 +361,28 @@ def adaptive(method, p, schedule, normals, tkind, tparams, radial_normals=None,
     ress, rlz, aess = np.zeros(T), np.zeros(T), np.zeros(T)
     fsz = 1 if scalar else d * d
     hm, hf, hl = np.zeros((T + 1) * d), np.zeros((T + 1) * fsz), np.zeros(T + 1)
+    fm, ff, fl = np.zeros((T + 1) * d), np.zeros((T + 1) * fsz), np.zeros(T + 1)
+    if replay is not None:
+        rm, rf, rl = replay
+        hm[:] = np.asarray(rm, float).ravel()
+        rf = np.asarray(rf, float)
+        hf[:] = rf.ravel() if scalar else rf.reshape(T + 1, d, d).transpose(0, 2, 1).ravel()
+        hl[:] = np.asarray(rl, float).ravel()
     fail = np.zeros(3, dtype=np.uint64)
     info = np.zeros(2, dtype=np.int32)
     rc = lib().or_adaptive(int(method), s.family, s.dof, int(bool(scalar)), d, _p(s.location),
                            _p(scale0), s.lognormalizer, T, _p(schedule, I64P), _p(normals),
                            _p(rn), _p(ru), int(tkind), _p(tp), int(nthreads), _p(samples), _p(lt),
                            _p(ln), _p(lw), _p(rid, I64P), _p(ress), _p(rlz), _p(aess), _p(hm),
-                           _p(hf), _p(hl), _p(fail, U64P), info.ctypes.data_as(IP))
+                           _p(hf), _p(hl), _p(fail, U64P), info.ctypes.data_as(IP),
+                           int(replay is not None), _p(fm), _p(ff), _p(fl))
     factors = hf.reshape(T + 1, d, d).transpose(0, 2, 1).copy() if not scalar else hf.copy()
+    fit_factors = ff.reshape(T + 1, d, d).transpose(0, 2, 1).copy() if not scalar else ff.copy()
     return dict(rc=rc, samples=samples, logtargets=lt, lognumerators=ln, logweights=lw,
                 round_ids=rid, round_ess=ress, round_lognormalizers=rlz, adaptation_ess=aess,
                 means=hm.reshape(T + 1, d), factors=factors, lognormalizers=hl, failure=fail,
-                fail_round=int(info[0]), fail_phase=int(info[1]))
+                fail_round=int(info[0]), fail_phase=int(info[1]),
+                fit_means=fm.reshape(T + 1, d), fit_factors=fit_factors, fit_lognormalizers=fl)
 
 
 # ---------------------------------------------------------------- DM-PMC

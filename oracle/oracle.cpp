@@ -773,7 +773,11 @@ int or_adaptive(int method, int family, double dof, int scalar, int d, const dou
                 int tkind, const double* tp, int nthreads, double* samples, double* logtargets,
                 double* lognums, double* logweights, int64_t* round_ids, double* round_ess,
                 double* round_logz, double* adaptation_ess, double* hist_means,
-                double* hist_factors, double* hist_lognorm, uint64_t* failure, int* fail_info) {
+                double* hist_factors, double* hist_lognorm, uint64_t* failure, int* fail_info,
+                int replay, double* fit_means, double* fit_factors, double* fit_lognorm) {
+  // replay = 1: hist_* are inputs holding every slot 0..T (for example a device
+  // run's fitted history); each round's fit is written to fit_* instead, so a
+  // long adaptive trajectory is checked one round at a time.
   const double logconstant = family == 1 ? or_student_t_logconstant(dof, d) : 0.0;
   const double multiplier = family == 1 ? dof / (dof - 2.0) : 1.0;  // src/proposals.jl:73-75
   int64_t maxN = 0, total = 0;
@@ -788,13 +792,18 @@ int or_adaptive(int method, int family, double dof, int scalar, int d, const dou
   for (int t = 0; t < T; ++t) logcounts[t] = std::log((double)schedule[t]);
   const int64_t dd = (int64_t)d * d;
   // history slot 0 = initial proposal
-  for (int c = 0; c < d; ++c) hist_means[c] = mu0[c];
-  if (scalar) {
-    hist_factors[0] = scale0[0];
-  } else {
-    for (int64_t i = 0; i < dd; ++i) hist_factors[i] = scale0[i];
+  if (!replay) {
+    for (int c = 0; c < d; ++c) hist_means[c] = mu0[c];
+    if (scalar) {
+      hist_factors[0] = scale0[0];
+    } else {
+      for (int64_t i = 0; i < dd; ++i) hist_factors[i] = scale0[i];
+    }
+    hist_lognorm[0] = lognorm0;
   }
-  hist_lognorm[0] = lognorm0;
+  double* out_means = replay ? fit_means : hist_means;
+  double* out_factors = replay ? fit_factors : hist_factors;
+  double* out_lognorm = replay ? fit_lognorm : hist_lognorm;
   auto slot_prop = [&](int s) {
     return Prop{family, dof, scalar ? (int)kSpherical : (int)kFactor, d, hist_means + (int64_t)s * d,
                 hist_factors + (int64_t)s * (scalar ? 1 : dd), hist_lognorm[s]};
@@ -962,8 +971,8 @@ int or_adaptive(int method, int family, double dof, int scalar, int d, const dou
     }
     double inv_total = 1.0 / tot;
     for (auto& v : w) v *= inv_total;
-    double* mean_out = hist_means + (int64_t)(t + 1) * d;
-    double* fac_out = hist_factors + (int64_t)(t + 1) * (scalar ? 1 : dd);
+    double* mean_out = out_means + (int64_t)(t + 1) * d;
+    double* fac_out = out_factors + (int64_t)(t + 1) * (scalar ? 1 : dd);
     const double* prev_fac = hist_factors + (int64_t)t * (scalar ? 1 : dd);
     if (scalar) {
       double mean = 0.0;
@@ -996,7 +1005,7 @@ int or_adaptive(int method, int family, double dof, int scalar, int d, const dou
       }
       mean_out[0] = mean;
       fac_out[0] = sc;
-      hist_lognorm[t + 1] = ln;
+      out_lognorm[t + 1] = ln;
     } else {
       std::vector<double> mean(d, 0.0), cov(dd, 0.0), cen((size_t)d * fit_n);
       for (int64_t i = 0; i < fit_n; ++i)
@@ -1056,7 +1065,7 @@ int or_adaptive(int method, int family, double dof, int scalar, int d, const dou
       }
       for (int c = 0; c < d; ++c) mean_out[c] = mean[c];
       for (int64_t i = 0; i < dd; ++i) fac_out[i] = L[i];
-      hist_lognorm[t + 1] = ln;
+      out_lognorm[t + 1] = ln;
     }
     // round summary (AMIS: all retained; NPMC: current round raw weights).
     const int64_t s_first = method == 0 ? 0 : first;
