@@ -83,6 +83,7 @@ SIGNATURES = {
     "is_job_export_radial": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64, F64P, ctypes.c_int64]),
     "is_job_export_uniforms": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64]),
     "is_job_set_timing": (ctypes.c_int, [VP, ctypes.c_int32]),
+    "is_job_set_graph": (ctypes.c_int, [VP, ctypes.c_int32]),
     "is_job_round_kernel_ms": (ctypes.c_int, [VP, F64P]),
     "is_job_total": (ctypes.c_int64, [VP]),
     "is_job_failure": (ctypes.c_int, [VP, ctypes.POINTER(Failure)]),

@@ -231,6 +231,15 @@ __device__ inline void record_failure(unsigned long long* rec, int64_t logical_i
   if (reason & kDrawReasons) atomicMax(rec + 2, packed);
 }
 
+// Pointer into the constant address space: loads through it whose address is
+// uniform across the wave are issued as scalar loads.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ inline cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+
 // ---------------------------------------------------------------- proposal slots
 // A slot packs one proposal for the density kernels:
 //   [ mu(d) | inv_diag(d) | L lower triangle incl. diagonal, row-major packed | lognorm | logmass ]
@@ -299,21 +308,20 @@ __device__ inline T radial_logdensity(const Family<T>& fam, T lognorm, T r2, con
 
 // Whitened squared radius r^2 = |L^{-1}(x - mu)|^2 by forward substitution
 // (src/native_execution.jl:346-384), then the radial log density.
-template <class T, int D>
-__device__ inline T slot_logdensity(const T* __restrict__ s, int d, const Family<T>& fam,
-                                    const T (&x)[D], T (&y)[D]) {
-  const T* mu = s;
-  const T* inv = s + d;
-  const T* L = s + 2 * d;
+// P is `const T*` for per-lane slots, or a constant-address-space pointer
+// (cptr<T>) for a slot shared by the whole wave, whose values then arrive by
+// scalar loads and feed the FMAs as SGPR operands.
+template <class T, int D, class P>
+__device__ inline T slot_logdensity(P s, int d, const Family<T>& fam, const T (&x)[D], T (&y)[D]) {
   T r2 = T(0);
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     if (r < d) {
-      T v = x[r] - mu[r];
-      const T* row = L + r * (r + 1) / 2;
+      T v = x[r] - s[r];
+      const int row = 2 * d + r * (r + 1) / 2;
 #pragma unroll
-      for (int c = 0; c < r; ++c) v -= row[c] * y[c];
-      v *= inv[r];
+      for (int c = 0; c < r; ++c) v -= s[row + c] * y[c];
+      v *= s[d + r];
       y[r] = v;
       r2 += v * v;
     }
@@ -322,18 +330,16 @@ __device__ inline T slot_logdensity(const T* __restrict__ s, int d, const Family
 }
 
 // Packed affine draw x = mu + L z (src/adaptive_gaussian_execution.jl:49-62).
-template <class T, int D>
-__device__ inline bool slot_draw(const T* __restrict__ s, int d, const T (&z)[D], T (&x)[D]) {
-  const T* mu = s;
-  const T* L = s + 2 * d;
+template <class T, int D, class P>
+__device__ inline bool slot_draw(P s, int d, const T (&z)[D], T (&x)[D]) {
   bool valid = true;
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     if (r < d) {
-      T v = mu[r];
-      const T* row = L + r * (r + 1) / 2;
+      T v = s[r];
+      const int row = 2 * d + r * (r + 1) / 2;
 #pragma unroll
-      for (int c = 0; c <= r; ++c) v += row[c] * z[c];
+      for (int c = 0; c <= r; ++c) v += s[row + c] * z[c];
       x[r] = v;
       valid &= isfinite(v);
     }
@@ -402,6 +408,11 @@ __device__ inline uint16_t logweight_reason(T v) {
 __device__ inline double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
 }
 __device__ inline double wave_sum(double v) {

@@ -273,6 +273,7 @@ struct is_job {
   int stride = 0;   // slot stride
   int nslots = 0;   // slot table entries
   int D = 4;        // compile-time dimension bucket
+  int64_t pcap = 1; // partial-record column length
   // committed state
   HostProp initial, committed;
   std::vector<double> bank_slots_init;   // K slots (FP64)
@@ -293,6 +294,15 @@ struct is_job {
   int last_status = 0;
   is_failure failure{};
   std::vector<int64_t> ancestors_host;
+  // run counter word on the device (run index << 32): RNG counters read it, so the
+  // captured launch sequence stays valid across runs
+  DevBuf run_dev;
+  uint64_t run_word_host = 0;
+  // the run's launch sequence captured as a HIP graph (replayed while valid)
+  int use_graph = 1;
+  bool graph_dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   // optional per-round event timing of the round kernel
   int timing = 0;
   std::vector<hipEvent_t> events;  // 2 per round
@@ -328,51 +338,93 @@ int dim_bucket(int d) {
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------- launch dispatch
-template <class T, int D, int MODE>
-void launch_round_typed(const RoundArgs<T>& a, int blocks, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_round<T, D, MODE>), dim3(blocks), dim3(kBlock), lds, s, a);
-}
+constexpr int64_t kSweepPerBlock = 512;  // samples per sweep block (one partial record)
 
-template <class T, int MODE>
-void launch_round_mode(int D, const RoundArgs<T>& a, int blocks, size_t lds, hipStream_t s) {
-  switch (D) {
-    case 4: launch_round_typed<T, 4, MODE>(a, blocks, lds, s); break;
-    case 8: launch_round_typed<T, 8, MODE>(a, blocks, lds, s); break;
-    case 16: launch_round_typed<T, 16, MODE>(a, blocks, lds, s); break;
-    case 32: launch_round_typed<T, 32, MODE>(a, blocks, lds, s); break;
-    default: launch_round_typed<T, 64, MODE>(a, blocks, lds, s); break;
-  }
-}
+constexpr size_t kNewSlotBudget = 48 * 1024;  // LDS bytes for staged slots per new-sample block
 
-template <class T>
-void launch_round(int D, int mode, const RoundArgs<T>& a, int blocks, size_t lds, hipStream_t s) {
-  if (mode == kDrawFused)
-    launch_round_mode<T, kDrawFused>(D, a, blocks, lds, s);
-  else if (mode == kDrawOnly)
-    launch_round_mode<T, kDrawOnly>(D, a, blocks, lds, s);
+template <class T, int D>
+void launch_new_typed(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s) {
+  const bool st = a.fam.student != 0;
+  W = std::min(W, new_block_limit(D, (int)sizeof(T), st) / 64);
+  const size_t slot_bytes = (size_t)a.stride * sizeof(T);
+  const size_t tslot_bytes = a.tkind == kTGaussian ? slot_bytes : 0;
+  const size_t avail = kNewSlotBudget > tslot_bytes ? kNewSlotBudget - tslot_bytes : 0;
+  a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / slot_bytes));
+  const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
+  if (st)
+    hipLaunchKernelGGL((k_new<T, D, kDrawFused, true>), dim3(blocks), dim3(64 * W), lds, s, a);
   else
-    launch_round_mode<T, kPreTarget>(D, a, blocks, lds, s);
+    hipLaunchKernelGGL((k_new<T, D, kDrawFused, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+}
+
+// Waves per new-sample block: enough that each wave evaluates a few mixture
+// components and the Box-Muller pairs of the block's 64 samples spread over
+// its threads (capped by the kernel's register-bound thread limit).
+int new_waves(int D, int d, int ncomp) {
+  const int np = (d + 1) / 2;
+  int W = 4;
+  if (ncomp >= 16 || np > 4) W = 8;
+  if (ncomp >= 40) W = 16;
+  return W;
 }
 
 template <class T>
-void launch_moments(int D, int blocks, size_t lds, hipStream_t s, const T* X, int64_t cap, int64_t xs,
-                    const T* lw, int64_t first, int64_t n, int d, const double* clip, const double* center,
-                    int moments, int nv, double* partials) {
-#define IS_MOM(DD)                                                                                   \
-  hipLaunchKernelGGL((k_moment_partials<T, DD>), dim3(blocks), dim3(kBlock), lds, s, X, cap, xs, lw, \
-                     first, n, d, clip, center, moments, nv, partials)
+void launch_mix(int D, int ncomp, const RoundArgs<T>& a, hipStream_t s) {
+  const int W = new_waves(D, a.d, ncomp);
+  const int blocks = (int)std::max<int64_t>(1, cdiv(a.n_new, 64));
   switch (D) {
-    case 4: IS_MOM(4); break;
-    case 8: IS_MOM(8); break;
-    case 16: IS_MOM(16); break;
-    case 32: IS_MOM(32); break;
-    default: IS_MOM(64); break;
+    case 4: launch_new_typed<T, 4>(a, ncomp, blocks, W, s); break;
+    case 8: launch_new_typed<T, 8>(a, ncomp, blocks, W, s); break;
+    case 16: launch_new_typed<T, 16>(a, ncomp, blocks, W, s); break;
+    case 32: launch_new_typed<T, 32>(a, ncomp, blocks, W, s); break;
+    default: launch_new_typed<T, 64>(a, ncomp, blocks, W, s); break;
   }
-#undef IS_MOM
 }
 
-size_t tile_lds_bytes(int d) { return (size_t)kTile * (d + 1) * sizeof(double); }
-constexpr size_t kLdsBudget = 64 * 1024;
+// Moment tiles per side for the augmented vector (x - c, 1) of length d + 1.
+int sweep_nt(int d, bool moments) { return moments ? (d + 1 + 15) / 16 : 0; }
+int sweep_waves(int nt) { return nt <= 1 ? 4 : nt == 2 ? 3 : nt == 3 ? 2 : 1; }
+size_t sweep_lds(int nt, int nw) {
+  const size_t per_wave = nt > 0 ? (size_t)64 * sweep_row_stride(nt) * sizeof(double) : 0;
+  const size_t staging = nw * per_wave;
+  const size_t P = (size_t)nt * (nt + 1) / 2;
+  const size_t reduce = (16 + (size_t)nw * P * 256 + P * 256) * sizeof(double);
+  return std::max(staging, reduce);
+}
+
+template <class T, int D, int NT>
+void launch_sweep_typed(const SweepArgs<T>& a, int blocks, int nw, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_sweep<T, D, NT>), dim3(blocks), dim3(64 * nw), lds, s, a);
+}
+
+// Launches one sweep over [a.lo, a.hi); returns the number of partial records.
+template <class T>
+int launch_sweep(int D, int nt, SweepArgs<T> a, hipStream_t s) {
+  const int nw = sweep_waves(nt);
+  const int blocks = (int)std::max<int64_t>(1, cdiv(a.hi - a.lo, kSweepPerBlock));
+  a.per_block = kSweepPerBlock;
+  const size_t lds = sweep_lds(nt, nw);
+#define IS_SW(DD, NN) launch_sweep_typed<T, DD, NN>(a, blocks, nw, lds, s)
+  switch (D) {
+    case 4:
+      if (nt == 0) IS_SW(4, 0); else IS_SW(4, 1);
+      break;
+    case 8:
+      if (nt == 0) IS_SW(8, 0); else IS_SW(8, 1);
+      break;
+    case 16:
+      if (nt == 0) IS_SW(16, 0); else if (nt == 1) IS_SW(16, 1); else IS_SW(16, 2);
+      break;
+    case 32:
+      if (nt == 0) IS_SW(32, 0); else if (nt == 2) IS_SW(32, 2); else IS_SW(32, 3);
+      break;
+    default:
+      if (nt == 0) IS_SW(64, 0); else if (nt == 3) IS_SW(64, 3); else if (nt == 4) IS_SW(64, 4); else IS_SW(64, 5);
+      break;
+  }
+#undef IS_SW
+  return blocks;
+}
 
 // ---------------------------------------------------------------- job helpers
 int64_t isqrt64(int64_t n) {
@@ -407,7 +459,7 @@ RoundArgs<T> base_args(is_job* j, int t) {
   RoundArgs<T> a{};
   a.d = j->d;
   a.round = t;
-  a.rng_word = round_word(j, t);
+  a.run_hi = j->run_dev.as<unsigned long long>();
   a.slots = j->slotsT.as<T>();
   a.stride = j->stride;
   a.fam.student = j->family;
@@ -479,7 +531,27 @@ FinishArgs base_finish(is_job* j, int t, unsigned int* counter) {
   return f;
 }
 
-size_t finish_lds(int d) { return std::max<size_t>(64, (size_t)(d * d + d) * sizeof(double)); }
+size_t finish_lds(int d) { return (size_t)finish_lds_doubles(d) * sizeof(double); }
+
+template <class T>
+SweepArgs<T> sweep_args(is_job* j, int t) {
+  SweepArgs<T> w{};
+  w.d = j->d;
+  w.stride = j->stride;
+  w.fam.student = j->family;
+  w.fam.dof = (T)j->dof;
+  w.fam.half_dof_plus_d = (T)((j->dof + (double)j->d) / 2.0);
+  w.X = j->X.as<T>();
+  w.cap = j->total;
+  w.logtargets = j->logtargets.as<T>();
+  w.lognums = j->lognums.as<T>();
+  w.logweights = j->logweights.as<T>();
+  w.round_ids = j->round_ids.as<int32_t>();
+  w.fail = j->fail_rec.as<unsigned long long>() + 3 * t;
+  w.partials = j->partials.as<double>();
+  w.pcap = j->pcap;
+  return w;
+}
 
 template <class T>
 int run_rounds(is_job* j) {
@@ -487,14 +559,15 @@ int run_rounds(is_job* j) {
   const int d = j->d;
   const int D = j->D;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
-  const int pairs_nv_summary = 2;
+  const int nt_mom = sweep_nt(d, true);
   unsigned int* counter = j->counter.as<unsigned int>();
-  const size_t tslot_bytes = (j->tkind == kTGaussian) ? (size_t)j->stride * sizeof(T) : 0;
-  const size_t slot_bytes = (size_t)j->stride * sizeof(T);
-  auto chunk_for = [&](int ncomp) {
-    const size_t avail = kLdsBudget - tile_lds_bytes(d) - tslot_bytes;
-    int c = (int)std::max<size_t>(1, avail / slot_bytes);
-    return std::min(c, std::max(1, ncomp));
+  auto mix = [&](RoundArgs<T>& a, int ncomp) {
+    launch_mix<T>(D, ncomp, a, s);
+  };
+  auto reduce = [&](const DevBuf& partials, int nrec, int nv, DevBuf& reduced, const FinishArgs& f) {
+    hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), 0, s, partials.as<double>(),
+                       (int64_t)nrec, j->pcap, nv, reduced.as<double>(), f);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), finish_lds(d), s, reduced.as<double>(), f);
   };
 
   if (j->method == IS_PLAIN) {
@@ -503,27 +576,24 @@ int run_rounds(is_job* j) {
     set_prefilled(j, 0, a);
     a.first = 0;
     a.n_new = N;
-    a.n_old = 0;
-    a.old_blocks = 0;
     a.k0 = a.k1 = 0;
-    a.slot_chunk = 1;
     a.gen_slot = 0;
     a.logmass = j->zerosT.as<T>();
     a.logtotal = T(0);
     a.record_nonfinite_after_exhaust = 0;
-    a.nv = pairs_nv_summary;
-    a.moments = 0;
-    const int blocks = (int)cdiv(N, kBlock);
-    const size_t lds = tile_lds_bytes(d) + tslot_bytes + slot_bytes;
     tick(j, 0, 0, s);
-    launch_round<T>(D, kDrawFused, a, blocks, lds, s);
+    mix(a, 1);
     tick(j, 0, 1, s);
+    SweepArgs<T> w = sweep_args<T>(j, 0);
+    w.lo = 0;
+    w.hi = N;
+    w.nv = 2;
+    const int nrec = launch_sweep<T>(D, 0, w, s);
     FinishArgs f = base_finish(j, 0, counter);
     f.flags = kFinishSummary;
     f.summary_count = N;
     f.summary_out = j->summary.as<double>();
-    hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s, j->partials.as<double>(),
-                       (int64_t)blocks, a.nv, j->reduced.as<double>(), f);
+    reduce(j->partials, nrec, 2, j->reduced, f);
     HIP_TRY(hipGetLastError());
     return IS_OK;
   }
@@ -537,25 +607,28 @@ int run_rounds(is_job* j) {
     a.first = first;
     a.n_new = N;
     a.record_nonfinite_after_exhaust = 1;
-    const int new_blocks = (int)cdiv(N, kBlock);
     double* slot64_t = j->slots64.as<double>() + (size_t)t * j->stride;
     if (j->method == IS_AMIS) {
-      a.n_old = first;
-      a.old_blocks = (int)cdiv(first, kBlock);
-      a.append_slot = t;
+      // new samples against the t-term temporal mixture, then one sweep over every
+      // retained sample: append of the newest component, weights, summary and moments
       a.k0 = 0;
       a.k1 = t;
-      a.slot_chunk = chunk_for(t + 1);
       a.gen_slot = t;
       a.logmass = j->logmassT.as<T>();
       a.logtotal = (T)std::log((double)last);
-      a.nv = nv_mom;
-      a.moments = 1;
-      a.center = slot64_t;
-      const int blocks = a.old_blocks + new_blocks;
-      const size_t lds = tile_lds_bytes(d) + tslot_bytes + (size_t)std::max(1, a.slot_chunk) * slot_bytes;
+      mix(a, t + 1);
+      SweepArgs<T> w = sweep_args<T>(j, t);
+      w.lo = 0;
+      w.hi = last;
+      w.append = first > 0;
+      w.append_hi = first;
+      w.slot = j->slotsT.as<T>() + (size_t)t * j->stride;
+      w.logmass = (T)std::log((double)N);
+      w.logtotal = a.logtotal;
+      w.center = slot64_t;
+      w.nv = nv_mom;
       tick(j, t, 0, s);
-      launch_round<T>(D, kDrawFused, a, blocks, lds, s);
+      const int nrec = launch_sweep<T>(D, nt_mom, w, s);
       tick(j, t, 1, s);
       FinishArgs f = base_finish(j, t, counter);
       f.flags = kFinishSummary | kFinishFit;
@@ -564,33 +637,36 @@ int run_rounds(is_job* j) {
       f.prev_slot = slot64_t;
       f.next_slot64 = j->slots64.as<double>() + (size_t)(t + 1) * j->stride;
       f.next_slot_t = j->slotsT.as<T>() + (size_t)(t + 1) * j->stride;
-      hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s,
-                         j->partials.as<double>(), (int64_t)blocks, a.nv, j->reduced.as<double>(), f);
+      reduce(j->partials, nrec, nv_mom, j->reduced, f);
     } else if (j->method == IS_NPMC) {
-      a.n_old = 0;
-      a.old_blocks = 0;
       a.k0 = a.k1 = t;
-      a.slot_chunk = 1;
       a.gen_slot = t;
       a.logmass = j->zerosT.as<T>();
       a.logtotal = T(0);
-      a.nv = pairs_nv_summary;
-      a.moments = 0;
-      const size_t lds = tile_lds_bytes(d) + tslot_bytes + slot_bytes;
       tick(j, t, 0, s);
-      launch_round<T>(D, kDrawFused, a, new_blocks, lds, s);
+      mix(a, 1);
       tick(j, t, 1, s);
+      SweepArgs<T> w = sweep_args<T>(j, t);
+      w.lo = first;
+      w.hi = last;
+      w.nv = 2;
+      const int nrec = launch_sweep<T>(D, 0, w, s);
       FinishArgs f = base_finish(j, t, counter);
       f.flags = kFinishSummary;
       f.summary_count = N;
-      hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s,
-                         j->partials.as<double>(), (int64_t)new_blocks, a.nv, j->reduced.as<double>(), f);
+      reduce(j->partials, nrec, 2, j->reduced, f);
       // clip at the isqrt(n)-th largest raw weight (src/npmc_execution.jl:21-33)
       const int64_t kth = N - isqrt64(N) + 1;
       hipLaunchKernelGGL((k_select_kth<T>), dim3(1), dim3(1024), 0, s, j->logweights.as<T>(), first, N, kth,
                          j->clip.as<double>());
-      launch_moments<T>(D, new_blocks, tile_lds_bytes(d), s, j->X.as<T>(), j->total, 1, j->logweights.as<T>(),
-                        first, N, d, j->clip.as<double>(), slot64_t, 1, nv_mom, j->partials2.as<double>());
+      SweepArgs<T> w2 = sweep_args<T>(j, t);
+      w2.lo = first;
+      w2.hi = last;
+      w2.clip = j->clip.as<double>();
+      w2.center = slot64_t;
+      w2.nv = nv_mom;
+      w2.partials = j->partials2.as<double>();
+      const int nrec2 = launch_sweep<T>(D, nt_mom, w2, s);
       FinishArgs g = base_finish(j, t, counter);
       g.flags = kFinishFit | kFinishAdaptEss;
       g.summary_count = N;
@@ -598,32 +674,29 @@ int run_rounds(is_job* j) {
       g.prev_slot = slot64_t;
       g.next_slot64 = j->slots64.as<double>() + (size_t)(t + 1) * j->stride;
       g.next_slot_t = j->slotsT.as<T>() + (size_t)(t + 1) * j->stride;
-      hipLaunchKernelGGL(k_reduce_finish, dim3(nv_mom), dim3(kBlock), finish_lds(d), s,
-                         j->partials2.as<double>(), (int64_t)new_blocks, nv_mom, j->reduced2.as<double>(), g);
+      reduce(j->partials2, nrec2, nv_mom, j->reduced2, g);
     } else {  // DM-PMC
-      a.n_old = 0;
-      a.old_blocks = 0;
       a.k0 = 0;
       a.k1 = j->K - 1;
-      a.slot_chunk = chunk_for(j->K);
       a.gen_slot = 0;
       a.assign = j->assign.as<int32_t>() + (size_t)t * j->maxN;
       a.bank_ids = j->bank_ids_d.as<int32_t>();
       a.logmass = j->logmassT.as<T>() + (size_t)t * j->K;
       a.logtotal = T(0);
-      a.nv = pairs_nv_summary;
-      a.moments = 0;
-      const size_t lds = tile_lds_bytes(d) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
       tick(j, t, 0, s);
-      launch_round<T>(D, kDrawFused, a, new_blocks, lds, s);
+      mix(a, j->K);
       tick(j, t, 1, s);
+      SweepArgs<T> w = sweep_args<T>(j, t);
+      w.lo = first;
+      w.hi = last;
+      w.nv = 2;
+      const int nrec = launch_sweep<T>(D, 0, w, s);
       double* summ = j->summary.as<double>() + (size_t)t * 5;
       FinishArgs f = base_finish(j, t, counter);
       f.flags = kFinishSummary;
       f.summary_count = N;
       f.summary_out = summ;
-      hipLaunchKernelGGL(k_reduce_finish, dim3(a.nv), dim3(kBlock), finish_lds(d), s,
-                         j->partials.as<double>(), (int64_t)new_blocks, a.nv, j->reduced.as<double>(), f);
+      reduce(j->partials, nrec, 2, j->reduced, f);
       // global resampling (src/dm_pmc_execution.jl:267-295)
       const T* lw = j->logweights.as<T>() + first;
       const int64_t ntiles = cdiv(N, (int64_t)kCdfThreads * kCdfPer);
@@ -636,7 +709,7 @@ int run_rounds(is_job* j) {
         u = j->pre_uniforms.as<double>() + (size_t)t * j->K;
       } else {
         hipLaunchKernelGGL(k_resampling_uniforms, dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->key,
-                           round_word(j, t), (int64_t)j->K, u);
+                           j->run_dev.as<unsigned long long>(), t, (int64_t)j->K, u);
       }
       int32_t* anc = j->ancestors.as<int32_t>() + (size_t)t * j->K;
       hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->cdf.as<double>(), lw, N,
@@ -693,8 +766,35 @@ int job_run_typed(is_job* j) {
     HIP_TRY(hipMemcpyAsync(j->slotsT.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
   }
-  int rc = run_rounds<T>(j);
-  if (rc) return rc;
+  j->run_word_host = (uint64_t)j->run_index << 32;
+  HIP_TRY(hipMemcpyAsync(j->run_dev.p, &j->run_word_host, sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  int rc = IS_OK;
+  // per-round event timing records events between launches, so a timed run is issued
+  // eagerly; otherwise the whole launch sequence is replayed from one captured graph
+  if (j->use_graph && !j->timing) {
+    if (j->graph_dirty || !j->graph_exec) {
+      if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
+      if (j->graph) (void)hipGraphDestroy(j->graph);
+      j->graph_exec = nullptr;
+      j->graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      rc = run_rounds<T>(j);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(IS_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      j->graph = g;
+      HIP_TRY(hipGraphInstantiate(&j->graph_exec, g, nullptr, nullptr, 0));
+      j->graph_dirty = false;
+    }
+    HIP_TRY(hipGraphLaunch(j->graph_exec, s));
+  } else {
+    rc = run_rounds<T>(j);
+    if (rc) return rc;
+  }
   // failure snapshot (one transfer per run)
   std::vector<uint64_t> rec((size_t)3 * T_);
   std::vector<int> fit(T_), cdfs(T_);
@@ -779,11 +879,11 @@ int alloc_job_buffers(is_job* j) {
   if ((rc = j->slotsT.alloc((size_t)j->nslots * j->stride * sizeof(T)))) return rc;
   HIP_TRY(hipMemset(j->slots64.p, 0, j->slots64.bytes));
   HIP_TRY(hipMemset(j->slotsT.p, 0, j->slotsT.bytes));
-  // partial records: worst case blocks of the largest round launch
-  const int64_t blocks = cdiv((int64_t)cap, kBlock) + 2;
+  // partial records (column-major): one per sweep block over at most `cap` samples
+  j->pcap = cdiv((int64_t)cap, kSweepPerBlock) + 1;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
-  if ((rc = j->partials.alloc((size_t)blocks * (1 + nv_mom) * sizeof(double)))) return rc;
-  if ((rc = j->partials2.alloc((size_t)blocks * (1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->partials.alloc((size_t)j->pcap * (1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->partials2.alloc((size_t)j->pcap * (1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->reduced.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->reduced2.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->summary.alloc((size_t)5 * std::max(1, j->T) * sizeof(double)))) return rc;
@@ -795,6 +895,8 @@ int alloc_job_buffers(is_job* j) {
   if ((rc = j->round_logz.alloc((size_t)j->T * sizeof(double)))) return rc;
   if ((rc = j->adapt_ess.alloc((size_t)j->T * sizeof(double)))) return rc;
   if ((rc = j->clip.alloc(64))) return rc;
+  if ((rc = j->run_dev.alloc(64))) return rc;
+  HIP_TRY(hipMemset(j->run_dev.p, 0, 64));
   HIP_TRY(hipMemset(j->counter.p, 0, 64));
   std::vector<double> zeros(std::max(j->nslots, 1), 0.0);
   if ((rc = upload_T<T>(j->zerosT, zeros))) return rc;
@@ -985,6 +1087,8 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
 int is_job_destroy(is_job* job) {
   if (!job) return IS_OK;
   if (job->stream) (void)hipStreamSynchronize(job->stream);
+  if (job->graph_exec) (void)hipGraphExecDestroy(job->graph_exec);
+  if (job->graph) (void)hipGraphDestroy(job->graph);
   for (auto& e : job->events) (void)hipEventDestroy(e);
   if (job->stream) (void)hipStreamDestroy(job->stream);
   delete job;
@@ -1040,6 +1144,7 @@ int is_job_set_normals(is_job* job, int32_t round, const double* normals, int64_
     if (rc) return rc;
   }
   job->prefilled[round] = 1;
+  job->graph_dirty = true;
   return IS_OK;
 }
 
@@ -1059,6 +1164,7 @@ int is_job_set_radial(is_job* job, int32_t round, const double* normals, int64_t
       if (rc) return rc;
     }
     job->prefilled_radial[round] = 1;
+  job->graph_dirty = true;
     return IS_OK;
   }
   if (rs) {
@@ -1074,6 +1180,7 @@ int is_job_set_radial(is_job* job, int32_t round, const double* normals, int64_t
     if (rc) return rc;
   }
   job->prefilled_radial[round] = 1;
+  job->graph_dirty = true;
   return IS_OK;
 }
 
@@ -1086,6 +1193,7 @@ int is_job_set_uniforms(is_job* job, int32_t round, const double* uniforms, int6
   int rc = upload_prefill(job->pre_uniforms, (size_t)job->K * job->T, (size_t)job->K * round, uniforms, count);
   if (rc) return rc;
   job->prefilled_uniforms[round] = 1;
+  job->graph_dirty = true;
   return IS_OK;
 }
 
@@ -1094,6 +1202,7 @@ int is_job_clear_prefilled(is_job* job) {
   std::fill(job->prefilled.begin(), job->prefilled.end(), 0);
   std::fill(job->prefilled_radial.begin(), job->prefilled_radial.end(), 0);
   std::fill(job->prefilled_uniforms.begin(), job->prefilled_uniforms.end(), 0);
+  job->graph_dirty = true;
   return IS_OK;
 }
 
@@ -1180,6 +1289,16 @@ int is_job_set_timing(is_job* job, int32_t enable) {
     job->events.resize((size_t)2 * job->T);
     for (auto& e : job->events) HIP_TRY(hipEventCreate(&e));
   }
+  job->graph_dirty = true;
+  return IS_OK;
+}
+
+int is_job_set_graph(is_job* job, int32_t enable) {
+  if (!job) return fail(IS_EINVAL, "null job");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  job->use_graph = enable ? 1 : 0;
+  job->graph_dirty = true;
   return IS_OK;
 }
 
@@ -1333,9 +1452,13 @@ extern "C++" {
 template <class T>
 static int job_lognormalizer_typed(is_job* job, double* out3) {
   const int64_t n = job->total;
-  const int blocks = (int)cdiv(n, kBlock);
-  launch_moments<T>(4, blocks, tile_lds_bytes(job->d), job->stream, job->X.as<T>(), n, 1, job->logweights.as<T>(), 0,
-                    n, job->d, nullptr, nullptr, 0, 2, job->partials2.as<double>());
+  SweepArgs<T> w = sweep_args<T>(job, 0);
+  w.lo = 0;
+  w.hi = n;
+  w.nv = 2;
+  w.fail = nullptr;
+  w.partials = job->partials2.as<double>();
+  const int blocks = launch_sweep<T>(job->D, 0, w, job->stream);
   FinishArgs f{};
   f.d = job->d;
   f.flags = kFinishSummary;
@@ -1343,7 +1466,8 @@ static int job_lognormalizer_typed(is_job* job, double* out3) {
   f.summary_out = job->summary.as<double>();
   f.counter = job->counter.as<unsigned int>() + 1;
   hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), finish_lds(job->d), job->stream,
-                     job->partials2.as<double>(), (int64_t)blocks, 2, job->reduced2.as<double>(), f);
+                     job->partials2.as<double>(), (int64_t)blocks, job->pcap, 2, job->reduced2.as<double>(), f);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), finish_lds(job->d), job->stream, job->reduced2.as<double>(), f);
   HIP_TRY(hipGetLastError());
   double s[5];
   HIP_TRY(hipMemcpyAsync(s, job->summary.p, sizeof(s), hipMemcpyDeviceToHost, job->stream));
@@ -1368,7 +1492,6 @@ template <class T>
 static int job_moments_typed(is_job* job, double* mean, double* var, double* cov) {
   const int64_t n = job->total;
   const int d = job->d;
-  const int blocks = (int)cdiv(n, kBlock);
   const int nv = 2 + d + d * (d + 1) / 2;
   // centre at the FP64 proposal mean of the first slot to keep the single pass well conditioned
   DevBuf center;
@@ -1380,14 +1503,20 @@ static int job_moments_typed(is_job* job, double* mean, double* var, double* cov
   else
     for (int c = 0; c < d; ++c) c0[c] = job->committed_locs[c];
   HIP_TRY(hipMemcpy(center.p, c0.data(), d * sizeof(double), hipMemcpyHostToDevice));
-  launch_moments<T>(job->D, blocks, tile_lds_bytes(d), job->stream, job->X.as<T>(), n, 1, job->logweights.as<T>(), 0, n,
-                    d, nullptr, center.as<double>(), 1, nv, job->partials2.as<double>());
+  SweepArgs<T> w = sweep_args<T>(job, 0);
+  w.lo = 0;
+  w.hi = n;
+  w.nv = nv;
+  w.fail = nullptr;
+  w.center = center.as<double>();
+  w.partials = job->partials2.as<double>();
+  const int blocks = launch_sweep<T>(job->D, sweep_nt(d, true), w, job->stream);
   FinishArgs f{};
   f.d = d;
   f.flags = 0;
   f.counter = job->counter.as<unsigned int>() + 2;
-  hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), finish_lds(d), job->stream,
-                     job->partials2.as<double>(), (int64_t)blocks, nv, job->reduced2.as<double>(), f);
+  hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), 0, job->stream,
+                     job->partials2.as<double>(), (int64_t)blocks, job->pcap, nv, job->reduced2.as<double>(), f);
   HIP_TRY(hipGetLastError());
   std::vector<double> red(1 + nv);
   HIP_TRY(hipMemcpyAsync(red.data(), job->reduced2.p, red.size() * sizeof(double), hipMemcpyDeviceToHost, job->stream));
@@ -1465,18 +1594,25 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   if (!ctx || !lw || !cdf || n < 1) return fail(IS_EINVAL, "bad arguments");
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipSetDevice(ctx->device));
-  // max via the partial machinery
-  const int blocks = (int)cdiv(n, kBlock);
+  // maximum via the summary sweep
+  const int64_t pcap = cdiv(n, kSweepPerBlock) + 1;
   DevBuf partials, reduced, summary, counter, tiles, st;
   int rc;
   const int64_t ntiles = cdiv(n, (int64_t)kCdfThreads * kCdfPer);
-  if ((rc = partials.alloc((size_t)blocks * 3 * sizeof(double))) || (rc = reduced.alloc(4 * sizeof(double))) ||
+  if ((rc = partials.alloc((size_t)pcap * 3 * sizeof(double))) || (rc = reduced.alloc(4 * sizeof(double))) ||
       (rc = summary.alloc(5 * sizeof(double))) || (rc = counter.alloc(64)) ||
       (rc = tiles.alloc((size_t)ntiles * sizeof(double))) || (rc = st.alloc(sizeof(int))))
     return rc;
   HIP_TRY(hipMemsetAsync(counter.p, 0, 64, s));
-  launch_moments<double>(4, blocks, tile_lds_bytes(1), s, lw, n, 1, lw, 0, n, 1, nullptr, nullptr, 0, 2,
-                         partials.as<double>());
+  SweepArgs<double> w{};
+  w.d = 1;
+  w.lo = 0;
+  w.hi = n;
+  w.logweights = const_cast<double*>(lw);
+  w.partials = partials.as<double>();
+  w.pcap = pcap;
+  w.nv = 2;
+  const int blocks = launch_sweep<double>(4, 0, w, s);
   FinishArgs f{};
   f.d = 1;
   f.flags = kFinishSummary;
@@ -1484,7 +1620,8 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   f.summary_out = summary.as<double>();
   f.counter = counter.as<unsigned int>();
   hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), finish_lds(1), s, partials.as<double>(),
-                     (int64_t)blocks, 2, reduced.as<double>(), f);
+                     (int64_t)blocks, pcap, 2, reduced.as<double>(), f);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), finish_lds(1), s, reduced.as<double>(), f);
   hipLaunchKernelGGL((k_cdf_tiles<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),
                      tiles.as<double>());
   hipLaunchKernelGGL((k_cdf_write<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),

@@ -24,7 +24,7 @@ template <class T>
 struct RoundArgs {
   int d;
   int round;          // 0-based round index (provenance)
-  uint64_t rng_word;  // RNG counter word: (run << 32) | round
+  const unsigned long long* run_hi;  // device word holding (run index << 32); RNG counter = run_hi | round
   int64_t first;      // global index of the first new sample
   int64_t n_new;
   int64_t n_old;      // retained samples [0, n_old) updated by the append path
@@ -33,7 +33,7 @@ struct RoundArgs {
   const T* slots;     // slot table (T copy)
   int stride;         // slot stride in elements
   int k0, k1;         // mixture components [k0, k1] for new samples
-  int slot_chunk;     // components staged in LDS per chunk
+  int slot_chunk;     // components staged in LDS at a time
   int append_slot;    // component appended to retained samples
   const T* logmass;   // per-slot log mixture mass (indexed by slot)
   int gen_slot;       // generating slot when assign == nullptr
@@ -80,126 +80,8 @@ __device__ inline void pair_index(int p, int& r, int& c) {
   c = p - r * (r + 1) / 2;
 }
 
-// ---------------------------------------------------------------- block partials
-// Reduces (lw, y = x - center) of the block's samples into one partial record
-// [M, s0 = sum e, s2 = sum e^2, s1 = sum e y, S = sum e y y^T] with e = exp(lw - M)
-// and M the block maximum.  lw = -inf marks an empty lane.  Deterministic:
-// fixed sub-tile order and fixed per-value accumulation order.
-template <int D>
-__device__ inline void block_partials(double lw, const double (&y)[D], int d, int nv, int moments,
-                                      double* __restrict__ rec, double* lds) {
-  __shared__ double red[kBlock / 64];
-  __shared__ double bmax_s;
-  const int tid = threadIdx.x;
-  double m = wave_max(lw);
-  if ((tid & 63) == 0) red[tid >> 6] = m;
-  __syncthreads();
-  if (tid == 0) {
-    double mm = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) mm = fmax(mm, red[w]);
-    bmax_s = mm;
-  }
-  __syncthreads();
-  const double M = bmax_s;
-  const double e = (M == -__builtin_huge_val() || lw == -__builtin_huge_val()) ? 0.0 : exp(lw - M);
-  constexpr int kMaxPer = (2 + D + D * (D + 1) / 2 + kBlock - 1) / kBlock;
-  double acc[kMaxPer];
-  int pr[kMaxPer], pc[kMaxPer];
-#pragma unroll
-  for (int q = 0; q < kMaxPer; ++q) {
-    acc[q] = 0.0;
-    const int v = tid + q * kBlock;
-    pr[q] = pc[q] = 0;
-    if (v >= 2 + d && v < nv) pair_index(v - 2 - d, pr[q], pc[q]);
-  }
-  const int rowlen = moments ? d + 1 : 1;
-  for (int sub = 0; sub < kBlock / kTile; ++sub) {
-    __syncthreads();
-    if ((tid / kTile) == sub) {
-      double* row = lds + (tid % kTile) * rowlen;
-      row[0] = e;
-      if (moments) {
-#pragma unroll
-        for (int c = 0; c < D; ++c)
-          if (c < d) row[1 + c] = y[c];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kMaxPer; ++q) {
-      const int v = tid + q * kBlock;
-      if (v < nv) {
-        double s = acc[q];
-        if (v == 0) {
-          for (int i = 0; i < kTile; ++i) s += lds[i * rowlen];
-        } else if (v == 1) {
-          for (int i = 0; i < kTile; ++i) {
-            const double ei = lds[i * rowlen];
-            s += ei * ei;
-          }
-        } else if (v < 2 + d) {
-          const int r = v - 2;
-          for (int i = 0; i < kTile; ++i) s += lds[i * rowlen] * lds[i * rowlen + 1 + r];
-        } else {
-          const int r = pr[q], c = pc[q];
-          for (int i = 0; i < kTile; ++i) {
-            const double* row = lds + i * rowlen;
-            s += (row[0] * row[1 + r]) * row[1 + c];
-          }
-        }
-        acc[q] = s;
-      }
-    }
-  }
-  if (tid == 0) rec[0] = M;
-#pragma unroll
-  for (int q = 0; q < kMaxPer; ++q) {
-    const int v = tid + q * kBlock;
-    if (v < nv) rec[1 + v] = acc[q];
-  }
-}
-
-// ---------------------------------------------------------------- normals for one sample
-template <class T, int D>
-__device__ inline bool sample_normals(const RoundArgs<T>& a, int64_t j, T (&z)[D], int64_t logical) {
-  double zz[D];
-  const int d = a.d;
-#pragma unroll
-  for (int c = 0; c < D; ++c) zz[c] = 0.0;
-  if (a.pre_normals) {
-    const double* src = a.pre_normals + j * a.nstride;
-#pragma unroll
-    for (int c = 0; c < D; ++c)
-      if (c < d) zz[c] = src[c];
-  } else {
-    rng_normals<D>(a.key, (uint64_t)j, a.rng_word, kPurposeNormals, d, zz);
-  }
-  bool ok = true;
-  if (a.fam.student) {
-    double rn[kGammaAttempts], ru[kGammaAttempts + 1];
-    for (int q = 0; q < a.rs; ++q)
-      rn[q] = a.pre_rn ? a.pre_rn[j * a.rn_stride + q]
-                       : rng_normal(a.key, (uint64_t)j, a.rng_word, kPurposeRadialNormals, q);
-    for (int q = 0; q < a.us; ++q)
-      ru[q] = a.pre_ru ? a.pre_ru[j * a.ru_stride + q]
-                       : rng_uniform(a.key, (uint64_t)j, a.rng_word, kPurposeRadialUniforms, q);
-    double mult;
-    if (!student_t_multiplier(a.dof, rn, ru, a.us > 0, &mult)) {
-      record_failure(a.fail, logical, 0, kDrawExhausted);
-      mult = __builtin_nan("");
-      ok = false;
-    }
-#pragma unroll
-    for (int c = 0; c < D; ++c)
-      if (c < d) zz[c] *= mult;
-  }
-#pragma unroll
-  for (int c = 0; c < D; ++c) z[c] = T(zz[c]);
-  return ok;
-}
-
-template <class T, int D>
-__device__ inline T inline_target(int tkind, int d, const T* tslot_lds, const T (&x)[D], T (&y)[D]) {
+template <class T, int D, class P>
+__device__ inline T inline_target(int tkind, int d, P tslot, const T (&x)[D], T (&y)[D]) {
   if (tkind == kTStandard) {
     T s = T(0);
 #pragma unroll
@@ -212,216 +94,10 @@ __device__ inline T inline_target(int tkind, int d, const T* tslot_lds, const T 
   g.student = 0;
   g.dof = T(0);
   g.half_dof_plus_d = T(0);
-  return slot_logdensity<T, D>(tslot_lds, d, g, x, y);
+  return slot_logdensity<T, D>(tslot, d, g, x, y);
 }
 
-// ---------------------------------------------------------------- the round kernel
-// Blocks [0, old_blocks): AMIS append over retained samples
-//   (src/amis_execution.jl:37-65 with the weight formation of :107-132).
-// Blocks [old_blocks, ...): new samples: draw, target, mixture log-sum-exp and
-//   weight (src/mis_execution.jl:601-771; fused plain IS src/native_execution.jl:741-812).
-// LDS: [moment tile: kTile*(d+1) doubles][target slot][component chunk].
-template <class T, int D, int MODE>
-__global__ __launch_bounds__(kBlock) void k_round(RoundArgs<T> a) {
-  extern __shared__ double lds_raw[];
-  const int d = a.d;
-  const int tid = threadIdx.x;
-  double* tile_lds = lds_raw;
-  T* slot_lds = reinterpret_cast<T*>(lds_raw + kTile * (d + 1));
-  double lw_acc = -__builtin_huge_val();
-  double yc[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) yc[c] = 0.0;
-
-  if ((int)blockIdx.x < a.old_blocks) {
-    // ------------------------------------------------ append path
-    const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
-    const T* s = a.slots + (int64_t)a.append_slot * a.stride;
-    for (int q = tid; q < a.stride; q += kBlock) slot_lds[q] = s[q];
-    __syncthreads();
-    if (i < a.n_old) {
-      T x[D], y[D];
-#pragma unroll
-      for (int c = 0; c < D; ++c) x[c] = (c < d) ? a.X[(int64_t)c * a.cap + i] : T(0);
-      const T q = slot_logdensity<T, D>(slot_lds, d, a.fam, x, y);
-      uint16_t reason = (q != q) ? kProposalInvalid : 0;
-      T num = a.lognums[i];
-      if (!reason) {
-        const T value = logaddexp<T>(num, a.logmass[a.append_slot] + q);
-        reason = proposal_reason(value);
-        if (!reason) {
-          num = value;
-          a.lognums[i] = value;
-        }
-      }
-      if (reason) record_failure(a.fail, i + 1, 0, reason);
-      if (a.round_ids[i] != 0) {
-        const T lw = a.logtargets[i] - (num - a.logtotal);
-        const uint16_t wr = logweight_reason(lw);
-        if (wr) {
-          record_failure(a.fail, i + 1, 0, wr);
-        } else {
-          a.logweights[i] = lw;
-          lw_acc = (double)lw;
-        }
-      }
-      if (a.moments) {
-#pragma unroll
-        for (int c = 0; c < D; ++c)
-          if (c < d) yc[c] = (double)x[c] - a.center[c];
-      }
-    }
-  } else {
-    // ------------------------------------------------ new-sample path
-    const int64_t j = (int64_t)(blockIdx.x - a.old_blocks) * kBlock + tid;
-    const bool active = j < a.n_new;
-    const int64_t gi = a.first + j;
-    T x[D], y[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      x[c] = T(0);
-      y[c] = T(0);
-    }
-    bool ok = active;
-    int gen = a.gen_slot;
-    T tv = T(0);
-    const bool staged_target = (a.tkind == kTGaussian && MODE == kDrawFused);
-    T* tslot_lds = slot_lds;
-    T* comp_lds = slot_lds + (staged_target ? a.stride : 0);
-    if (staged_target) {
-      for (int q = tid; q < a.stride; q += kBlock) tslot_lds[q] = a.tslot[q];
-    }
-    __syncthreads();
-    if (active) {
-      if (a.assign) gen = a.assign[j];
-      if (MODE != kDrawOnly) {
-        a.logweights[gi] = -T(__builtin_huge_val());
-        a.logtargets[gi] = -T(__builtin_huge_val());
-        a.lognums[gi] = T(0);
-        a.round_ids[gi] = 0;
-        if (a.proposal_ids) a.proposal_ids[gi] = 0;
-      }
-    }
-    if (MODE == kPreTarget) {
-      if (active) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) x[c] = (c < d) ? a.X[(int64_t)c * a.cap + gi] : T(0);
-        tv = a.pre_logtarget[j];
-        const uint16_t tr = target_reason(tv);
-        if (tr) {
-          record_failure(a.fail, j + 1, 0, tr);
-          ok = false;
-        }
-      }
-    } else if (active) {
-      T z[D];
-      const bool drawn = sample_normals<T, D>(a, j, z, j + 1);
-      if (!drawn && !a.record_nonfinite_after_exhaust) {
-        ok = false;
-      } else {
-        const T* gs = a.slots + (int64_t)gen * a.stride;
-        const bool valid = slot_draw<T, D>(gs, d, z, x);
-#pragma unroll
-        for (int c = 0; c < D; ++c)
-          if (c < d) a.X[(int64_t)c * a.cap + gi] = x[c];
-        if (!valid) {
-          record_failure(a.fail, j + 1, 0, kGeneratedNonfinite);
-          ok = false;
-        } else if (MODE == kDrawFused) {
-          tv = inline_target<T, D>(a.tkind, d, tslot_lds, x, y);
-          const uint16_t tr = target_reason(tv);
-          if (tr) {
-            record_failure(a.fail, j + 1, 0, tr);
-            ok = false;
-          }
-        }
-      }
-    }
-    if (MODE != kDrawOnly) {
-      // mixture denominator over components [k0, k1], staged through LDS in chunks
-      LseAcc<T> acc;
-      acc.init();
-      T generating = T(0);
-      const int ncomp = a.k1 - a.k0 + 1;
-      for (int base = 0; base < ncomp; base += a.slot_chunk) {
-        const int cnt = min(a.slot_chunk, ncomp - base);
-        __syncthreads();
-        const T* src = a.slots + (int64_t)(a.k0 + base) * a.stride;
-        for (int q = tid; q < cnt * a.stride; q += kBlock) comp_lds[q] = src[q];
-        __syncthreads();
-        if (ok) {
-          for (int kk = 0; kk < cnt; ++kk) {
-            const int k = a.k0 + base + kk;
-            const T q = slot_logdensity<T, D>(comp_lds + kk * a.stride, d, a.fam, x, y);
-            if (k == gen) generating = q;
-            acc.add(a.logmass[k] + q);
-          }
-        }
-      }
-      if (ok) {
-        const T value = acc.value();
-        if (generating != generating || generating == -T(__builtin_huge_val()) || value != value ||
-            value == -T(__builtin_huge_val())) {
-          record_failure(a.fail, j + 1, 0, kProposalInvalid);
-        } else {
-          const T lw = tv - (value - a.logtotal);
-          const uint16_t wr = logweight_reason(lw);
-          if (wr) {
-            record_failure(a.fail, j + 1, 0, wr);
-          } else {
-            a.logtargets[gi] = tv;
-            a.lognums[gi] = value;
-            a.logweights[gi] = lw;
-            a.round_ids[gi] = a.round + 1;
-            if (a.proposal_ids) a.proposal_ids[gi] = a.bank_ids ? a.bank_ids[gen] : 0;
-            lw_acc = (double)lw;
-            if (a.moments) {
-#pragma unroll
-              for (int c = 0; c < D; ++c)
-                if (c < d) yc[c] = (double)x[c] - a.center[c];
-            }
-          }
-        }
-      }
-    }
-  }
-  if (MODE != kDrawOnly && a.partials) {
-    block_partials<D>(lw_acc, yc, d, a.nv, a.moments, a.partials + (int64_t)blockIdx.x * (1 + a.nv),
-                      tile_lds);
-  }
-}
-
-// ---------------------------------------------------------------- moment partials over a range
-// NPMC clipped fit and the results API: lw_i = min(lw_i, clip) over [first, first + n).
-template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_moment_partials(const T* __restrict__ X, int64_t cap,
-                                                            int64_t x_stride_sample, const T* __restrict__ lw,
-                                                            int64_t first, int64_t n, int d,
-                                                            const double* __restrict__ clip,
-                                                            const double* __restrict__ center, int moments,
-                                                            int nv, double* __restrict__ partials) {
-  extern __shared__ double lds_raw[];
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  double l = -__builtin_huge_val();
-  double y[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) y[c] = 0.0;
-  if (j < n) {
-    const int64_t i = first + j;
-    l = (double)lw[i];
-    if (clip) l = fmin(l, *clip);
-    if (moments) {
-#pragma unroll
-      for (int c = 0; c < D; ++c)
-        if (c < d) {
-          const double xv = x_stride_sample == 1 ? (double)X[(int64_t)c * cap + i]
-                                                 : (double)X[i * x_stride_sample + c];
-          y[c] = xv - center[c];
-        }
-    }
-  }
-  block_partials<D>(l, y, d, nv, moments, partials + (int64_t)blockIdx.x * (1 + nv), lds_raw);
-}
+#include "mix.h"
 
 // ---------------------------------------------------------------- partial reduction + finish
 struct FinishArgs {
@@ -451,68 +127,118 @@ constexpr int kFinishAdaptEss = 4;
 
 __device__ void finish_block(const double* red, const FinishArgs& f, double* lds);
 
-// grid = nv blocks; block k reduces value k over all records, then the last
-// block to finish runs the summary / fit on the reduced values.
+// grid = nv blocks; block k reduces value k over all partial records.  Partial
+// records are stored column-major: column 0 holds the block maxima, column
+// 1 + v value v, each column `pcap` long, so every read here is coalesced.
+// Each thread keeps its own running maximum and rescaled sum over batches of
+// kReduceUnroll records whose loads are all issued before any is used, so up
+// to kBlock * kReduceUnroll records cost one memory round trip; the threads
+// are then merged in a fixed order (deterministic).
+constexpr int kReduceUnroll = 8;
 __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
-                                                          int64_t nrec, int nv,
+                                                          int64_t nrec, int64_t pcap, int nv,
                                                           double* __restrict__ reduced, FinishArgs f) {
-  __shared__ double red[kBlock / 64];
-  __shared__ double Ms;
-  __shared__ bool last_s;
-  extern __shared__ double lds_raw[];
+  __shared__ double wm[kBlock / 64], ws[kBlock / 64];
   const int tid = threadIdx.x;
   const int k = blockIdx.x;
-  const int rs = 1 + nv;
-  double m = -__builtin_huge_val();
-  for (int64_t r = tid; r < nrec; r += kBlock) m = fmax(m, partials[r * rs]);
-  m = wave_max(m);
-  if ((tid & 63) == 0) red[tid >> 6] = m;
-  __syncthreads();
-  if (tid == 0) {
-    double mm = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) mm = fmax(mm, red[w]);
-    Ms = mm;
-  }
-  __syncthreads();
-  const double M = Ms;
-  double s = 0.0;
-  if (M != -__builtin_huge_val()) {
-    for (int64_t r = tid; r < nrec; r += kBlock) {
-      const double mr = partials[r * rs];
-      if (mr == -__builtin_huge_val()) continue;
-      double sc = exp(mr - M);
-      if (k == 1) sc *= sc;
-      s += partials[r * rs + 1 + k] * sc;
+  const bool squared = (k == 1);  // value 1 is the sum of squared weights
+  const double ninf = -__builtin_huge_val();
+  const double* maxima = partials;
+  const double* column = partials + (int64_t)(1 + k) * pcap;
+  double m = ninf, s = 0.0;
+  for (int64_t base = 0; base < nrec; base += (int64_t)kBlock * kReduceUnroll) {
+    double mv[kReduceUnroll], cv[kReduceUnroll];
+#pragma unroll
+    for (int u = 0; u < kReduceUnroll; ++u) {
+      const int64_t r = base + (int64_t)u * kBlock + tid;
+      mv[u] = r < nrec ? maxima[r] : ninf;
+      cv[u] = r < nrec ? column[r] : 0.0;
     }
+    double mb = m;
+#pragma unroll
+    for (int u = 0; u < kReduceUnroll; ++u) mb = fmax(mb, mv[u]);
+    if (mb == ninf) continue;
+    double sc = (m == ninf) ? 0.0 : exp(m - mb);
+    s *= squared ? sc * sc : sc;
+#pragma unroll
+    for (int u = 0; u < kReduceUnroll; ++u) {
+      if (mv[u] == ninf) continue;
+      double e = exp(mv[u] - mb);
+      if (squared) e *= e;
+      s += cv[u] * e;
+    }
+    m = mb;
   }
-  s = wave_sum(s);
+  // merge the threads: block maximum, then rescaled sums in thread order
+  double M = wave_max(m);
+  if ((tid & 63) == 0) wm[tid >> 6] = M;
   __syncthreads();
-  if ((tid & 63) == 0) red[tid >> 6] = s;
+  M = wm[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) M = fmax(M, wm[w]);
+  double v = 0.0;
+  if (m != ninf) {
+    const double sc = exp(m - M);
+    v = s * (squared ? sc * sc : sc);
+  }
+  v = wave_sum(v);
+  if ((tid & 63) == 0) ws[tid >> 6] = v;
   __syncthreads();
   if (tid == 0) {
     double tot = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) tot += red[w];
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) tot += ws[w];
     reduced[1 + k] = tot;
     if (k == 0) reduced[0] = M;
-    __threadfence();
-    const unsigned prev = atomicAdd(f.counter, 1u);
-    last_s = (prev == (unsigned)(gridDim.x - 1));
   }
-  __syncthreads();
-  if (!last_s) return;
-  __threadfence();
-  if (tid == 0) *f.counter = 0u;
+  (void)f;
+}
+
+// One workgroup: summary and fit on the reduced values (launched after
+// k_reduce_finish; the kernel boundary publishes the reduced column sums).
+__global__ __launch_bounds__(kBlock) void k_finish(const double* __restrict__ reduced, FinishArgs f) {
+  extern __shared__ double lds_raw[];
   finish_block(reduced, f, lds_raw);
 }
 
 // Summary (src/execution.jl:176-186) and FP64 moment fit
 // (src/adaptive_gaussian_execution.jl:303-425): mean = c + s1/s0,
 // cov = S/s0 - (s1/s0)(s1/s0)^T + ridge I, Cholesky, Student-t rescale, log-det.
-__device__ void finish_block(const double* red, const FinishArgs& f, double* lds) {
-  const int tid = threadIdx.x;
+// Every global input (reduced sums, previous factor, centre) is fetched in one
+// batched round trip into LDS; the rest runs out of LDS with one barrier per
+// Cholesky column, so the kernel costs one memory latency plus O(d) barriers.
+__host__ __device__ constexpr int finish_lds_doubles(int d) {
+  return (3 + d + d * (d + 1) / 2) + d * (d + 1) / 2 + d + d * d + 3 * d + 8;
+}
+
+__device__ void finish_block(const double* __restrict__ red, const FinishArgs& f, double* lds) {
+  const int tid = threadIdx.x, nt = blockDim.x;
   const int d = f.d;
-  volatile const double* rv = red;
-  const double M = rv[0], s0 = rv[1], s2 = rv[2];
+  const bool fit = (f.flags & kFinishFit) != 0;
+  const int npk = d * (d + 1) / 2;
+  const int nv = fit ? 3 + d + npk : 3;
+  double* R = lds;                 // reduced values [M, s0, s2, s1(d), S pairs]
+  double* Lp = R + 3 + d + npk;    // previous factor, packed lower
+  double* C = Lp + npk;            // centre
+  double* A = C + d;               // d x d, row-major
+  double* mean = A + d * d;
+  double* dg = mean + d;           // Cholesky diagonal
+  double* lg = dg + d;             // log of the final diagonal
+  double* wred = lg + d;           // per-wave partials (4)
+  __shared__ int status_s;
+  const int nload = fit ? nv + npk + d : nv;
+  for (int q = tid; q < nload; q += nt) {
+    double v;
+    if (q < nv) v = red[q];
+    else if (q < nv + npk) v = f.prev_slot[2 * d + (q - nv)];
+    else v = f.center[q - nv - npk];
+    if (q < nv) R[q] = v;
+    else if (q < nv + npk) Lp[q - nv] = v;
+    else C[q - nv - npk] = v;
+  }
+  if (tid == 0) status_s = 0;
+  __syncthreads();
+  const double M = R[0], s0 = R[1], s2 = R[2];
   if (tid == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) {
     double ess, logz;
     if (s0 == 0.0 || M == -__builtin_huge_val()) {
@@ -535,103 +261,112 @@ __device__ void finish_block(const double* red, const FinishArgs& f, double* lds
     }
     if ((f.flags & kFinishAdaptEss) && f.adapt_ess) f.adapt_ess[f.round] = ess;
   }
-  if (!(f.flags & kFinishFit)) return;
-  __shared__ int status_s;
-  if (tid == 0) status_s = 0;
-  __syncthreads();
+  if (!fit) return;
   if (!(s0 > 0.0) || !isfinite(s0) || M == -__builtin_huge_val()) {
     if (tid == 0) f.fit_status[f.round] = 1;
     return;
   }
-  double* A = lds;           // d*d, row-major full
-  double* mean = lds + d * d;
-  // mean and centred covariance
-  for (int c = tid; c < d; c += kBlock) mean[c] = rv[3 + c] / s0;
+  const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
+  // scale-aware ridge from the previous proposal's factor
+  // (src/adaptive_gaussian_execution.jl:283-301): sqrt(eps) * tr(L L^T) * mult / d
+  double tr = 0.0;
+  for (int q = tid; q < npk; q += nt) tr += Lp[q] * Lp[q];
+  tr = wave_sum(tr);
+  if ((tid & 63) == 0) wred[tid >> 6] = tr;
+  for (int c = tid; c < d; c += nt) mean[c] = R[3 + c] / s0;
   __syncthreads();
-  for (int p = tid; p < d * (d + 1) / 2; p += kBlock) {
+  const double reg = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+  double trace = 0.0;
+  for (int w = 0; w < (nt >> 6); ++w) trace += wred[w];
+  double ridge = reg * (trace * mult) / (double)d;
+  if (!isfinite(ridge)) {
+    // overflow-safe restatement: sum of (L_ij * sqrt(reg) * sqrt(mult / d))^2
+    const double sc = sqrt(reg) * sqrt(mult / (double)d);
+    ridge = 0.0;
+    for (int q = 0; q < npk; ++q) {
+      const double v = Lp[q] * sc;
+      ridge += v * v;
+    }
+  }
+  // centred covariance plus ridge
+  for (int p = tid; p < npk; p += nt) {
     int r, c;
     pair_index(p, r, c);
-    const double v = rv[3 + d + p] / s0 - mean[r] * mean[c];
+    double v = R[3 + d + p] / s0 - mean[r] * mean[c];
+    if (r == c) v += ridge;
     A[r * d + c] = v;
     A[c * d + r] = v;
   }
   __syncthreads();
-  // scale-aware ridge from the previous proposal's factor (src/adaptive_gaussian_execution.jl:283-301)
-  if (tid == 0) {
-    const double reg = 1.4901161193847656e-08;  // sqrt(eps(Float64))
-    const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
-    const double* L = f.prev_slot + 2 * d;
-    double trace = 0.0;
-    for (int r = 0; r < d; ++r)
-      for (int c = 0; c <= r; ++c) trace += L[r * (r + 1) / 2 + c] * L[r * (r + 1) / 2 + c];
-    double ridge = reg * (trace * mult) / (double)d;
-    if (!isfinite(ridge)) {
-      const double sc = sqrt(reg) * sqrt(mult / (double)d);
-      ridge = 0.0;
-      for (int r = 0; r < d; ++r)
-        for (int c = 0; c <= r; ++c) {
-          const double v = L[r * (r + 1) / 2 + c] * sc;
-          ridge += v * v;
-        }
-    }
-    for (int c = 0; c < d; ++c) A[c * d + c] += ridge;
-  }
-  __syncthreads();
-  // left-looking Cholesky, lower factor stored in A's lower triangle
+  // left-looking Cholesky (LAPACK potrf order): each thread forms the column's
+  // diagonal itself from the finished columns, then the rows below it
   for (int c = 0; c < d; ++c) {
+    double v = A[c * d + c];
+    for (int k = 0; k < c; ++k) v -= A[c * d + k] * A[c * d + k];
+    const double diag = sqrt(v);
     if (tid == 0) {
-      double v = A[c * d + c];
-      for (int k = 0; k < c; ++k) v -= A[c * d + k] * A[c * d + k];
       if (!(v > 0.0) || !isfinite(v)) status_s = 2;
-      A[c * d + c] = sqrt(v);
+      dg[c] = diag;
     }
-    __syncthreads();
-    const double diag = A[c * d + c];
-    for (int r = c + 1 + tid; r < d; r += kBlock) {
-      double v = A[r * d + c];
-      for (int k = 0; k < c; ++k) v -= A[r * d + k] * A[c * d + k];
-      A[r * d + c] = v / diag;
+    for (int r = c + 1 + tid; r < d; r += nt) {
+      double w = A[r * d + c];
+      for (int k = 0; k < c; ++k) w -= A[r * d + k] * A[c * d + k];
+      A[r * d + c] = w / diag;
     }
     __syncthreads();
   }
+  const double scale = f.student ? sqrt(1.0 / mult) : 1.0;
+  double* out = f.next_slot64;
+  for (int r = tid; r < d; r += nt) {
+    const double dr = dg[r] * scale;
+    out[r] = C[r] + mean[r];
+    out[d + r] = 1.0 / dr;
+    lg[r] = log(fabs(dr));
+  }
+  bool ok = true;
+  for (int p = tid; p < npk; p += nt) {
+    int r, c;
+    pair_index(p, r, c);
+    const double v = (r == c ? dg[r] : A[r * d + c]) * scale;
+    out[2 * d + p] = v;
+    ok &= isfinite(v);
+    if (r == c) ok &= v > 0.0;
+  }
+  for (int r = tid; r < d; r += nt) ok &= isfinite(C[r] + mean[r]);
+  if (!ok) status_s = status_s == 0 ? 3 : status_s;
+  __syncthreads();
   if (tid == 0) {
     int status = status_s;
-    const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
-    const double scale = f.student ? sqrt(1.0 / mult) : 1.0;
     double logabsdet = 0.0;
-    bool ok = status == 0;
-    double* out = f.next_slot64;
-    for (int r = 0; r < d; ++r) {
-      const double mu = f.center[r] + mean[r];
-      out[r] = mu;
-      ok &= isfinite(mu);
-    }
-    for (int r = 0; r < d; ++r) {
-      for (int c = 0; c <= r; ++c) {
-        const double v = A[r * d + c] * scale;
-        out[2 * d + r * (r + 1) / 2 + c] = v;
-        ok &= isfinite(v);
-      }
-      const double diag = out[2 * d + r * (r + 1) / 2 + r];
-      ok &= diag > 0.0;
-      out[d + r] = 1.0 / diag;
-      logabsdet += log(fabs(diag));
-    }
+    for (int r = 0; r < d; ++r) logabsdet += lg[r];
     const double lognorm = f.student ? f.logconstant - logabsdet
                                      : -0.5 * (double)d * log(2.0 * 3.141592653589793) - logabsdet;
-    ok &= isfinite(lognorm);
+    if (status == 0 && !isfinite(lognorm)) status = 3;
     out[slot_lognorm_off(d)] = lognorm;
-    if (status == 0 && !ok) status = 3;
     f.fit_status[f.round] = status;
   }
-  __syncthreads();
-  // T copy of the new slot (logmass is owned by the host schedule)
+  // T copy of the new slot (logmass is owned by the host schedule); the values
+  // are recomputed rather than re-read so no barrier on the global writes is needed
   const int nfill = slot_lognorm_off(d) + 1;
-  for (int q = tid; q < nfill; q += kBlock) {
+  for (int q = tid; q < nfill; q += nt) {
+    double v;
+    if (q < d) {
+      v = C[q] + mean[q];
+    } else if (q < 2 * d) {
+      v = 1.0 / (dg[q - d] * scale);
+    } else if (q < 2 * d + npk) {
+      int r, c;
+      pair_index(q - 2 * d, r, c);
+      v = (r == c ? dg[r] : A[r * d + c]) * scale;
+    } else {
+      double la = 0.0;
+      for (int r = 0; r < d; ++r) la += lg[r];
+      v = f.student ? f.logconstant - la : -0.5 * (double)d * log(2.0 * 3.141592653589793) - la;
+    }
     if (f.t_is_float)
-      reinterpret_cast<float*>(f.next_slot_t)[q] = (float)f.next_slot64[q];
+      reinterpret_cast<float*>(f.next_slot_t)[q] = (float)v;
     else
-      reinterpret_cast<double*>(f.next_slot_t)[q] = f.next_slot64[q];
+      reinterpret_cast<double*>(f.next_slot_t)[q] = v;
   }
 }
 
@@ -825,10 +560,11 @@ __global__ void k_rng_raw(uint64_t k0, uint64_t k1, uint64_t c1, uint64_t c2, ui
 }
 
 // Resampling uniforms of a round: counter (k, round, purpose 3, 0), word 0.
-__global__ void k_resampling_uniforms(RngKey key, uint64_t round, int64_t count, double* __restrict__ out) {
+__global__ void k_resampling_uniforms(RngKey key, const unsigned long long* run_hi, int round, int64_t count,
+                                      double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
-  out[k] = rng_uniform(key, (uint64_t)k, round, kPurposeResampling, 0);
+  out[k] = rng_uniform(key, (uint64_t)k, *run_hi | (uint64_t)round, kPurposeResampling, 0);
 }
 
 // ---------------------------------------------------------------- layout helpers

@@ -1,0 +1,515 @@
+// New-sample kernel and the streaming sweep with FP64 MFMA moments.
+// Included from kernels.h (uses RoundArgs, sample helpers and device.h).
+#pragma once
+
+// (included inside namespace issampler)
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Merge two running (max, scaled-sum) log-sum-exp states.
+template <class T>
+__device__ inline void lse_merge(T& m, T& s, T m2, T s2) {
+  if (m2 > m) {
+    s = s * LseAcc<T>::fast_exp(m - m2) + s2;
+    m = m2;
+  } else if (m > m2) {
+    s = s + s2 * LseAcc<T>::fast_exp(m2 - m);
+  } else {
+    s = s + s2;
+  }
+}
+
+// ---------------------------------------------------------------- new samples
+// One round's new samples: draw from the generating slot, evaluate the target,
+// the deterministic-mixture log-sum-exp over slots [k0, k1] and the weight
+// (src/mis_execution.jl:601-771; plain IS src/native_execution.jl:741-812).
+// A block of W waves owns 64 consecutive samples, one per lane:
+//   A  the ceil(d/2) Box-Muller pairs of the 64 samples (and their Student-t
+//      radial multipliers) are spread over all 64 W threads and staged in LDS,
+//      while the target slot and the first chunk of mixture slots are copied
+//      into LDS (one memory round trip, overlapped with the Philox work);
+//   B  wave 0 forms x = mu + L z, stores X and shares x through LDS;
+//   C  wave w evaluates the components kk with (kk + 1) mod W == w, so every
+//      slot read is a wave-uniform LDS broadcast; wave 0 also evaluates the
+//      target;
+//   D  wave 0 merges the W partial log-sum-exp states in wave order and writes
+//      the sample's outputs.
+// LDS: normals 64 x (D + 1) doubles | multipliers 64 | x 64 x (D + 1) T |
+//      W x 64 partial (m, s, flag) | 64 draw flags | target slot | slot chunk.
+// Threads per block the register budget allows: x, y and z of D values each.
+__host__ __device__ constexpr int new_block_limit(int D, int tsize, bool st) {
+  return (!st && D * tsize <= 64) ? 1024 : (D * tsize <= 128 ? 512 : 256);
+}
+
+// Byte offset of the slot area (target slot, then the component chunk).
+__host__ __device__ constexpr size_t new_lds_fixed(int D, int W, int tsize) {
+  return (((size_t)(64 * (D + 1) + 64) * 8 + (size_t)64 * (D + 1) * tsize + (size_t)W * 64 * (2 * tsize + 4) +
+           64 * 4) + 15) & ~(size_t)15;
+}
+
+template <class T, int D, int MODE, bool ST>
+__global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(RoundArgs<T> a) {
+  extern __shared__ double lds[];
+  constexpr int ZS = D + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int W = blockDim.x >> 6;
+  const int d = a.d;
+  const int64_t j0 = (int64_t)blockIdx.x * 64;
+  const int64_t j = j0 + lane;
+  const bool active = j < a.n_new;
+  const int64_t gi = a.first + j;
+  double* zs = lds;
+  double* mults = zs + 64 * ZS;
+  T* xs = reinterpret_cast<T*>(mults + 64);
+  T* part_m = xs + 64 * ZS;
+  T* part_s = part_m + W * 64;
+  int* part_f = reinterpret_cast<int*>(part_s + W * 64);
+  int* drawn_f = part_f + W * 64;
+  T* tslot = reinterpret_cast<T*>(reinterpret_cast<char*>(lds) + new_lds_fixed(D, W, sizeof(T)));
+  const bool staged_target = (MODE == kDrawFused && a.tkind == kTGaussian);
+  T* comp = tslot + (staged_target ? a.stride : 0);
+  const int ncomp = (MODE == kDrawOnly) ? 0 : a.k1 - a.k0 + 1;
+  const int chunk = min(ncomp, a.slot_chunk);
+
+  // ---- A: slot staging, normals and radial multipliers
+  if (staged_target)
+    for (int q = tid; q < a.stride; q += blockDim.x) tslot[q] = a.tslot[q];
+  {
+    const T* src = a.slots + (int64_t)a.k0 * a.stride;
+    for (int q = tid; q < chunk * a.stride; q += blockDim.x) comp[q] = src[q];
+  }
+  if (MODE != kPreTarget) {
+    const uint64_t rng_word = *a.run_hi | (uint64_t)a.round;
+    const int np = (d + 1) >> 1;
+    const int njobs = 64 * np + (ST ? 64 : 0);
+    for (int q = tid; q < njobs; q += blockDim.x) {
+      if (q < 64 * np) {
+        const int s = q / np, p = q - s * np;
+        const int64_t js = j0 + s;
+        double z0 = 0.0, z1 = 0.0;
+        if (js < a.n_new) {
+          if (a.pre_normals) {
+            const double* src = a.pre_normals + js * a.nstride;
+            z0 = src[2 * p];
+            if (2 * p + 1 < d) z1 = src[2 * p + 1];
+          } else {
+            // pair p uses words 2(p mod 2), 2(p mod 2) + 1 of Philox block p / 2
+            const U64x4 w = philox4x64((uint64_t)js, rng_word, kPurposeNormals, (uint64_t)(p >> 1), a.key.k0,
+                                       a.key.k1);
+            const uint64_t wa = (p & 1) ? w.v[2] : w.v[0];
+            const uint64_t wb = (p & 1) ? w.v[3] : w.v[1];
+            const double r = sqrt(-2.0 * log(u53_open(wa)));
+            double sn, cs;
+            sincospi(2.0 * u53_open(wb), &sn, &cs);
+            z0 = r * cs;
+            z1 = r * sn;
+          }
+        }
+        zs[s * ZS + 2 * p] = z0;
+        zs[s * ZS + 2 * p + 1] = z1;
+      } else if (ST) {
+        const int s = q - 64 * np;
+        const int64_t js = j0 + s;
+        double mult = 1.0;
+        int drawn = 1;
+        if (js < a.n_new) {
+          double rn[kGammaAttempts], ru[kGammaAttempts + 1];
+          for (int t = 0; t < a.rs; ++t)
+            rn[t] = a.pre_rn ? a.pre_rn[js * a.rn_stride + t]
+                             : rng_normal(a.key, (uint64_t)js, rng_word, kPurposeRadialNormals, t);
+          for (int t = 0; t < a.us; ++t)
+            ru[t] = a.pre_ru ? a.pre_ru[js * a.ru_stride + t]
+                             : rng_uniform(a.key, (uint64_t)js, rng_word, kPurposeRadialUniforms, t);
+          if (!student_t_multiplier(a.dof, rn, ru, a.us > 0, &mult)) {
+            record_failure(a.fail, js + 1, 0, kDrawExhausted);
+            mult = __builtin_nan("");
+            drawn = 0;
+          }
+        }
+        mults[s] = mult;
+        drawn_f[s] = drawn;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- B: affine draw and X store (wave 0), shared through LDS
+  T x[D], y[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    x[c] = T(0);
+    y[c] = T(0);
+  }
+  bool ok = active;
+  int gen = a.gen_slot;
+  if (active && a.assign) gen = a.assign[j];
+  if (MODE == kPreTarget) {
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        if (c < d) x[c] = a.X[(int64_t)c * a.cap + gi];
+    }
+  } else {
+    int good = 0;
+    if (wave == 0 && active) {
+      double mult = 1.0;
+      bool drawn = true;
+      if (ST) {
+        mult = mults[lane];
+        drawn = drawn_f[lane] != 0;
+      }
+      if (drawn || a.record_nonfinite_after_exhaust) {
+        T z[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) z[c] = (c < d) ? T(zs[lane * ZS + c] * mult) : T(0);
+        const int64_t goff = (int64_t)gen * a.stride;
+        const bool valid = a.assign ? slot_draw<T, D>(a.slots + goff, d, z, x)
+                                    : slot_draw<T, D>(as_const(a.slots + goff), d, z, x);
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+          if (c < d) {
+            a.X[(int64_t)c * a.cap + gi] = x[c];
+            xs[lane * ZS + c] = x[c];
+          }
+        if (!valid) record_failure(a.fail, j + 1, 0, kGeneratedNonfinite);
+        good = valid ? 1 : 0;
+      }
+      drawn_f[lane] = good;
+    }
+    if (MODE == kDrawOnly) return;
+    __syncthreads();
+    if (active) {
+      ok = drawn_f[lane] != 0;
+      if (wave != 0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+          if (c < d) x[c] = xs[lane * ZS + c];
+      }
+    }
+  }
+
+  // ---- C: target (wave 0) and this wave's share of the mixture components
+  T tv = T(0);
+  if (wave == 0 && ok) {
+    if (MODE == kPreTarget) {
+      tv = a.pre_logtarget[j];
+    } else {
+      tv = inline_target<T, D>(a.tkind, d, tslot, x, y);
+    }
+    const uint16_t tr = target_reason(tv);
+    if (tr) {
+      record_failure(a.fail, j + 1, 0, tr);
+      ok = false;
+    }
+  }
+  T m = -T(__builtin_huge_val()), sum = T(0);
+  int genbad = 0;
+  for (int base = 0; base < ncomp; base += chunk) {
+    const int cnt = min(chunk, ncomp - base);
+    if (base > 0) {
+      __syncthreads();
+      const T* src = a.slots + (int64_t)(a.k0 + base) * a.stride;
+      for (int q = tid; q < cnt * a.stride; q += blockDim.x) comp[q] = src[q];
+      __syncthreads();
+    }
+    if (ok) {
+      // first kk >= base with (kk + 1) mod W == wave
+      int kk = base + ((wave + W - 1 - base % W) % W + W) % W;
+      for (; kk < base + cnt; kk += W) {
+        const int k = a.k0 + kk;
+        const T q = slot_logdensity<T, D>(comp + (kk - base) * a.stride, d, a.fam, x, y);
+        if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
+        const T v = a.logmass[k] + q;
+        if (v > m) {
+          sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
+          m = v;
+        } else {
+          sum += LseAcc<T>::fast_exp(v - m);
+        }
+      }
+    }
+  }
+  if (W > 1) {
+    part_m[wave * 64 + lane] = m;
+    part_s[wave * 64 + lane] = sum;
+    part_f[wave * 64 + lane] = genbad;
+    __syncthreads();
+  }
+  if (wave != 0 || !active) return;
+
+  // ---- D: merge in wave order and write the sample
+  for (int w = 1; w < W; ++w) {
+    lse_merge(m, sum, part_m[w * 64 + lane], part_s[w * 64 + lane]);
+    genbad |= part_f[w * 64 + lane];
+  }
+  T lt = -T(__builtin_huge_val()), ln = T(0), lwv = -T(__builtin_huge_val());
+  int32_t rid = 0, pid = 0;
+  if (ok) {
+    const T value = m + log(sum);
+    if (genbad || value != value || value == -T(__builtin_huge_val())) {
+      record_failure(a.fail, j + 1, 0, kProposalInvalid);
+    } else {
+      const T lw = tv - (value - a.logtotal);
+      const uint16_t wr = logweight_reason(lw);
+      if (wr) {
+        record_failure(a.fail, j + 1, 0, wr);
+      } else {
+        lt = tv;
+        ln = value;
+        lwv = lw;
+        rid = a.round + 1;
+        pid = a.bank_ids ? a.bank_ids[gen] : 0;
+      }
+    }
+  }
+  a.logtargets[gi] = lt;
+  a.lognums[gi] = ln;
+  a.logweights[gi] = lwv;
+  a.round_ids[gi] = rid;
+  if (a.proposal_ids) a.proposal_ids[gi] = pid;
+}
+
+// ---------------------------------------------------------------- streaming sweep
+// Over samples [lo, hi): samples below append_hi receive the AMIS append of one
+// new mixture component (src/amis_execution.jl:37-65) and their weights are
+// re-formed (src/amis_execution.jl:107-132); every sample then contributes its
+// weight e = exp(lw - m) (running wave maximum m) to the summary monoid
+// (src/execution.jl:159-186) and, when NT > 0, to the weighted moments about
+// `center`.  For the moments each lane stages w = sqrt(e) (x - center, 1) in
+// LDS; sum_s w_s w_s^T then accumulates on FP64 MFMA (16x16x4: the A and B
+// operands of a 4-sample step are the same register), NT tiles per side,
+// lower tile pairs only.  The appended slot and the centre are uniform, so
+// they arrive by scalar loads.  The next 64 samples' loads are issued before
+// the current ones are processed.  One column-major partial record per block.
+template <class T>
+struct SweepArgs {
+  int d;
+  int64_t lo, hi, append_hi;
+  int append;
+  const T* slot;
+  int stride;
+  T logmass, logtotal;
+  Family<T> fam;
+  const T* X;
+  int64_t cap;
+  const T* logtargets;
+  T* lognums;
+  T* logweights;
+  const int32_t* round_ids;
+  unsigned long long* fail;
+  const double* clip;
+  const double* center;
+  double* partials;
+  int64_t pcap;
+  int nv;
+  int64_t per_block;
+};
+
+__host__ __device__ constexpr int sweep_row_stride(int nt) { return 16 * nt + 1; }
+
+template <class T, int D>
+struct SweepLoad {
+  T x[D];
+  T num, tgt, lw;
+  int32_t rid;
+};
+
+template <class T, int D, int NT>
+__device__ inline void sweep_load(const SweepArgs<T>& a, int64_t i, int64_t bend, SweepLoad<T, D>& L) {
+  const int d = a.d;
+  const bool valid = i < bend;
+  const bool app = a.append && i < a.append_hi;
+#pragma unroll
+  for (int c = 0; c < D; ++c) L.x[c] = T(0);
+  L.num = T(0);
+  L.tgt = T(0);
+  L.lw = -T(__builtin_huge_val());
+  L.rid = 0;
+  if (valid) {
+    if (NT > 0 || app) {
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        if (c < d) L.x[c] = a.X[(int64_t)c * a.cap + i];
+    }
+    if (app) {
+      L.num = a.lognums[i];
+      L.tgt = a.logtargets[i];
+      L.rid = a.round_ids[i];
+    } else {
+      L.lw = a.logweights[i];
+    }
+  }
+}
+
+template <class T, int D, int NT>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
+  extern __shared__ double lds[];
+  constexpr int RS = sweep_row_stride(NT);
+  constexpr int P = NT * (NT + 1) / 2;
+  const T ninf = -T(__builtin_huge_val());
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  const int d = a.d;
+  double* rows = lds + wave * (64 * RS);
+  if (NT > 0) {
+    // columns above d stay zero for the whole kernel
+    for (int q = lane; q < 64 * RS; q += 64) rows[q] = 0.0;
+  }
+  double cen[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) cen[c] = (NT > 0 && c < d) ? as_const(a.center)[c] : 0.0;
+  const cptr<T> slot = as_const(a.slot);
+  const double clip = a.clip ? *a.clip : 0.0;
+  T mw = ninf;
+  double s0 = 0.0, s2 = 0.0;
+  double4_t acc[P > 0 ? P : 1];
+#pragma unroll
+  for (int p = 0; p < (P > 0 ? P : 1); ++p) acc[p] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const int64_t bstart = a.lo + (int64_t)blockIdx.x * a.per_block;
+  const int64_t bend = min(a.hi, bstart + a.per_block);
+  const int64_t step = (int64_t)nw * 64;
+  SweepLoad<T, D> cur, nxt;
+  int64_t base = bstart + (int64_t)wave * 64;
+  if (base < bend) sweep_load<T, D, NT>(a, base + lane, bend, cur);
+  for (; base < bend; base += step) {
+    if (base + step < bend) sweep_load<T, D, NT>(a, base + step + lane, bend, nxt);
+    const int64_t i = base + lane;
+    T lw = ninf;
+    if (i < bend) {
+      if (a.append && i < a.append_hi) {
+        T yv[D];
+        const T q = slot_logdensity<T, D>(slot, d, a.fam, cur.x, yv);
+        uint16_t reason = (q != q) ? kProposalInvalid : 0;
+        T num = cur.num;
+        if (!reason) {
+          const T value = logaddexp<T>(num, a.logmass + q);
+          reason = proposal_reason(value);
+          if (!reason) {
+            num = value;
+            a.lognums[i] = value;
+          }
+        }
+        if (reason) record_failure(a.fail, i + 1, 0, reason);
+        if (cur.rid != 0) {
+          const T w = cur.tgt - (num - a.logtotal);
+          const uint16_t wr = logweight_reason(w);
+          if (wr) {
+            record_failure(a.fail, i + 1, 0, wr);
+          } else {
+            a.logweights[i] = w;
+            lw = w;
+          }
+        }
+      } else {
+        lw = cur.lw;
+      }
+      if (a.clip) lw = fmin(lw, (T)clip);
+    }
+    // running wave maximum, recomputed only when some lane exceeds it
+    if (__any(lw > mw)) {
+      const T wm = wave_max(lw);
+      const double f = (mw == ninf) ? 0.0 : exp((double)mw - (double)wm);
+      s0 *= f;
+      s2 *= f * f;
+#pragma unroll
+      for (int p = 0; p < (P > 0 ? P : 1); ++p) acc[p] *= f;
+      mw = wm;
+    }
+    const T se = (lw == ninf) ? T(0) : exp(T(0.5) * (lw - mw));
+    const double sd = (double)se;
+    const double e = sd * sd;
+    s0 += e;
+    s2 += e * e;
+    if (NT > 0) {
+      double* row = rows + lane * RS;
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        if (c < d) row[c] = sd * ((double)cur.x[c] - cen[c]);
+      row[d] = sd;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int st = 0; st < 16; ++st) {
+        const double* r = rows + (4 * st + (lane >> 4)) * RS + (lane & 15);
+        double av[NT > 0 ? NT : 1];
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti) av[ti] = r[ti * 16];
+        int p = 0;
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+          for (int tj = 0; tj <= ti; ++tj) {
+            acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], av[tj], acc[p], 0, 0, 0);
+            ++p;
+          }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    cur = nxt;
+  }
+  // ---- block reduction: rescale the waves to the block maximum, sum in wave order
+  s0 = wave_sum(s0);
+  s2 = wave_sum(s2);
+  __syncthreads();
+  double* wmax = lds;                    // nw
+  double* ws0 = lds + 4;                 // nw
+  double* ws2 = lds + 8;                 // nw
+  double* tiles = lds + 16;              // nw * P * 256
+  if (lane == 0) {
+    wmax[wave] = (double)mw;
+    ws0[wave] = s0;
+    ws2[wave] = s2;
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tiles[((size_t)wave * P + p) * 256 + lane * 4 + r] = acc[p][r];
+  __syncthreads();
+  double M = wmax[0];
+  for (int w = 1; w < nw; ++w) M = fmax(M, wmax[w]);
+  double fw[4];
+  for (int w = 0; w < 4; ++w)
+    fw[w] = (w < nw && wmax[w] != -__builtin_huge_val()) ? exp(wmax[w] - M) : 0.0;
+  // summed tiles go after the per-wave tiles
+  double* sumt = tiles + (size_t)nw * P * 256;
+  for (int q = tid; q < P * 256; q += blockDim.x) {
+    double v = 0.0;
+    for (int w = 0; w < nw; ++w) v += fw[w] * tiles[((size_t)w * P) * 256 + q];
+    sumt[q] = v;
+  }
+  __syncthreads();
+  double* out = a.partials + blockIdx.x;
+  if (tid == 0) out[0] = M;
+  for (int v = tid; v < a.nv; v += blockDim.x) {
+    double val;
+    if (v == 1) {
+      val = 0.0;
+      for (int w = 0; w < nw; ++w) val += fw[w] * fw[w] * ws2[w];
+    } else if (NT == 0) {
+      val = 0.0;
+      for (int w = 0; w < nw; ++w) val += fw[w] * ws0[w];
+    } else {
+      // moment entry C[r][c], r >= c, of the augmented matrix: s0 = C[d][d],
+      // s1[r] = C[d][r], S pairs C[r][c]
+      int r, c;
+      if (v == 0) {
+        r = c = d;
+      } else if (v < 2 + d) {
+        r = d;
+        c = v - 2;
+      } else {
+        pair_index(v - 2 - d, r, c);
+      }
+      // v_mfma_f64_16x16x4f64 result layout: lane l, register i holds tile
+      // entry (row 4 i + l / 16, column l % 16)
+      const int ti = r >> 4, tj = c >> 4, ra = r & 15, cb = c & 15;
+      const int p = ti * (ti + 1) / 2 + tj;
+      const int ln = cb + 16 * (ra & 3), reg = ra >> 2;
+      val = sumt[(size_t)p * 256 + ln * 4 + reg];
+    }
+    out[(int64_t)(1 + v) * a.pcap] = val;
+  }
+}

@@ -152,6 +152,9 @@ int is_job_export_uniforms(is_job* job, int32_t round, double* out, int64_t coun
  * each round, on the job's stream; no reference counterpart (measurement seam). */
 int is_job_set_timing(is_job* job, int32_t enable);
 int is_job_round_kernel_ms(is_job* job, double* out); /* T entries, last run */
+/* Replay a run's launch sequence from one captured HIP graph (default on); a timed run
+ * is always issued eagerly.  No reference counterpart (launch-overhead control). */
+int is_job_set_graph(is_job* job, int32_t enable);
 
 /* ---- results (WeightedSamples, src/results.jl:79-98) -------------------------------- */
 int64_t is_job_total(const is_job* job);

@@ -98,6 +98,25 @@ def test_plain_is_parity(eng, oracle_mod, name, make, tk, tname):
     assert relerr(eng.var(res), v, 1.0) < 1e-10
 
 
+@pytest.mark.parametrize("d", [16, 20, 33, 50])
+def test_results_moments_multi_tile(eng, oracle_mod, d):
+    # d + 1 > 16: the weighted moments span several 16x16 MFMA tiles
+    _say(f"case: results moments d={d}")
+    q = eng.DiagonalGaussian(np.linspace(-1.0, 1.0, d), np.linspace(0.8, 1.6, d))
+    L = np.tril(np.full((d, d), 0.05), -1) + 1.3 * np.eye(d)
+    tgt = eng.GaussianTarget(np.linspace(0.5, -0.5, d), L)
+    s = eng.prepare_sampler(3, tgt, eng.ImportanceSampling(q, 3000))
+    rnd = s.export_randomness(0)
+    res = eng.importance_sample_(s)
+    x, lw, fail = oracle_mod.plain_is(q, 1, tgt.params(d), rnd["normals"])
+    assert fail[0] == 0
+    assert relerr(res.logweights, lw, 1.0) < RTOL
+    m, v, c = oracle_mod.moments(x, lw)
+    assert relerr(eng.mean(res), m, 1.0) < 1e-10
+    assert relerr(eng.var(res), v, 1.0) < 1e-10
+    assert relerr(eng.cov(res), c, 1.0) < 1e-10
+
+
 def test_plain_is_student_t_known_answer(eng):
     # test/kernel_execution.jl:490-505
     _say("case: plain IS Student-t buffered known answer")
@@ -167,6 +186,7 @@ AMIS_CASES = [
     ("spherical2", 2, [17, 20, 23, 26, 29], "std", "spherical", None),
     ("scalar", 1, [64, 80, 96], "gauss", "scalar", None),
     ("d10-12rounds", 10, [2000] * 12, "gauss", "factor", None),
+    ("d20-factor", 20, [3000] * 4, "gauss", "factor", None),
     ("studentt5-d4", 4, [300, 400, 500], "gauss", "diagonal", 5.0),
 ]
 
@@ -433,3 +453,37 @@ def test_results_resample_matches_oracle(eng, oracle_mod):
     ref_s = oracle_mod.select_ancestors(oracle_mod.cdf_blocked(res.logweights), res.logweights,
                                         oracle_mod.systematic_points(0.37, 1000))
     assert np.array_equal(anc_s, ref_s)
+
+
+# ---------------------------------------------------------------- graph replay
+def _graph_cases(eng):
+    rng = np.random.default_rng(77)
+    p, tgt = _amis_problem(eng, 4, "gauss", "factor", None, rng)
+    yield "AMIS", tgt, eng.AMIS(p, rounds=4, round_size=[300, 200, 500, 100])
+    yield "NPMC", tgt, eng.NPMC(p, rounds=3, round_size=400)
+    bank = _dm_problem(eng, 3, 6, "factor", rng)
+    yield "DM-PMC", eng.GaussianTarget(np.zeros(3), 2.0 * np.eye(3)), \
+        eng.DeterministicMixturePMC(bank, rounds=3, round_size=[256, 300, 128],
+                                    method=eng.SystematicResampling())
+
+
+def test_graph_replays_match_eager_runs(eng):
+    """Each run's launch sequence is captured once and replayed; replays after the
+    first (with eager result kernels in between) must reproduce eager runs bit for bit."""
+    for name, tgt, alg in _graph_cases(eng):
+        _say(f"case: graph replay {name}")
+        g = eng.prepare_sampler(21, tgt, alg)
+        e = eng.prepare_sampler(21, tgt, alg)
+        assert e._lib.is_job_set_graph(e._job, 0) == 0
+        for rep in range(4):
+            rg = eng.importance_sample_(g)
+            lzg = eng.lognormalizer(rg)  # eager work between replays
+            re_ = eng.importance_sample_(e)
+            assert np.array_equal(rg.logweights, re_.logweights), (name, rep)
+            assert np.array_equal(rg.samples, re_.samples), (name, rep)
+            assert lzg == eng.lognormalizer(re_), (name, rep)
+            if name == "DM-PMC":
+                assert np.array_equal(g.ancestors(), e.ancestors()), (name, rep)
+            if rep == 1:
+                g.reset()
+                e.reset()
