@@ -282,6 +282,13 @@ struct is_job {
   std::vector<int64_t> plan_counts;      // K*T
   std::vector<double> plan_logcoef;      // K*T
   std::vector<double> tslot;             // Gaussian target slot
+  // batched regression target (logistic / linear): data on the device
+  int64_t tn = 0;                        // data rows
+  double t_inv_two_var = 0.0;            // linear regression 1 / (2 sigma^2)
+  double t_inv_two_prior = 0.0;          // 1 / (2 prior variance)
+  double t_const = 0.0;                  // log normalising constants
+  std::vector<double> tdata;             // X (n x p row-major) then y (n), FP64, until upload
+  DevBuf Xd, yd, tpre, tpart;
   // device buffers
   DevBuf X, logtargets, lognums, logweights, round_ids, proposal_ids;
   DevBuf slots64, slotsT, logmassT, zerosT, tslotT, assign, bank_ids_d;
@@ -342,7 +349,7 @@ constexpr int64_t kSweepPerBlock = 512;  // samples per sweep block (one partial
 
 constexpr size_t kNewSlotBudget = 48 * 1024;  // LDS bytes for staged slots per new-sample block
 
-template <class T, int D>
+template <class T, int D, int MODE>
 void launch_new_typed(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s) {
   const bool st = a.fam.student != 0;
   W = std::min(W, new_block_limit(D, (int)sizeof(T), st) / 64);
@@ -351,10 +358,12 @@ void launch_new_typed(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t 
   const size_t avail = kNewSlotBudget > tslot_bytes ? kNewSlotBudget - tslot_bytes : 0;
   a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / slot_bytes));
   const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
-  if (st)
-    hipLaunchKernelGGL((k_new<T, D, kDrawFused, true>), dim3(blocks), dim3(64 * W), lds, s, a);
+  if (MODE == kPreTarget)
+    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+  else if (st)
+    hipLaunchKernelGGL((k_new<T, D, MODE, true>), dim3(blocks), dim3(64 * W), lds, s, a);
   else
-    hipLaunchKernelGGL((k_new<T, D, kDrawFused, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
 }
 
 // Waves per new-sample block: enough that each wave evaluates a few mixture
@@ -368,16 +377,16 @@ int new_waves(int D, int d, int ncomp) {
   return W;
 }
 
-template <class T>
+template <class T, int MODE>
 void launch_mix(int D, int ncomp, const RoundArgs<T>& a, hipStream_t s) {
   const int W = new_waves(D, a.d, ncomp);
   const int blocks = (int)std::max<int64_t>(1, cdiv(a.n_new, 64));
   switch (D) {
-    case 4: launch_new_typed<T, 4>(a, ncomp, blocks, W, s); break;
-    case 8: launch_new_typed<T, 8>(a, ncomp, blocks, W, s); break;
-    case 16: launch_new_typed<T, 16>(a, ncomp, blocks, W, s); break;
-    case 32: launch_new_typed<T, 32>(a, ncomp, blocks, W, s); break;
-    default: launch_new_typed<T, 64>(a, ncomp, blocks, W, s); break;
+    case 4: launch_new_typed<T, 4, MODE>(a, ncomp, blocks, W, s); break;
+    case 8: launch_new_typed<T, 8, MODE>(a, ncomp, blocks, W, s); break;
+    case 16: launch_new_typed<T, 16, MODE>(a, ncomp, blocks, W, s); break;
+    case 32: launch_new_typed<T, 32, MODE>(a, ncomp, blocks, W, s); break;
+    default: launch_new_typed<T, 64, MODE>(a, ncomp, blocks, W, s); break;
   }
 }
 
@@ -424,6 +433,31 @@ int launch_sweep(int D, int nt, SweepArgs<T> a, hipStream_t s) {
   }
 #undef IS_SW
   return blocks;
+}
+
+// Row slices for the batched target: enough blocks to fill the chip when a round
+// has few sample tiles; each slice a whole number of 64-row chunks.
+int64_t target_splits(int64_t n_rows, int64_t n_new) {
+  const int64_t tiles = std::max<int64_t>(1, cdiv(n_new, 64));
+  const int64_t want = std::max<int64_t>(1, cdiv(1024, tiles));
+  return std::min(want, std::max<int64_t>(1, cdiv(n_rows, 64)));
+}
+
+template <class T, int KS>
+void launch_target_typed(const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
+  hipLaunchKernelGGL((k_target_gemm<T, KS>), dim3((unsigned)cdiv(t.n_new, 64), (unsigned)splits), dim3(256),
+                     target_lds_bytes(KS, (int)sizeof(T)), s, t);
+}
+
+template <class T>
+void launch_target(int D, const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
+  switch (D) {
+    case 4: launch_target_typed<T, 1>(t, splits, s); break;
+    case 8: launch_target_typed<T, 2>(t, splits, s); break;
+    case 16: launch_target_typed<T, 4>(t, splits, s); break;
+    case 32: launch_target_typed<T, 8>(t, splits, s); break;
+    default: launch_target_typed<T, 16>(t, splits, s); break;
+  }
 }
 
 // ---------------------------------------------------------------- job helpers
@@ -561,8 +595,34 @@ int run_rounds(is_job* j) {
   const int nv_mom = 2 + d + d * (d + 1) / 2;
   const int nt_mom = sweep_nt(d, true);
   unsigned int* counter = j->counter.as<unsigned int>();
+  const bool batch_target = j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG;
   auto mix = [&](RoundArgs<T>& a, int ncomp) {
-    launch_mix<T>(D, ncomp, a, s);
+    if (!batch_target) {
+      launch_mix<T, kDrawFused>(D, ncomp, a, s);
+      return;
+    }
+    // draw, batched target on MFMA, then the mixture with the precomputed target values
+    launch_mix<T, kDrawOnly>(D, ncomp, a, s);
+    const int64_t splits = target_splits(j->tn, a.n_new);
+    TargetArgs<T> ta{};
+    ta.kind = j->tkind;
+    ta.p = d;
+    ta.n = j->tn;
+    ta.Xd = j->Xd.as<T>();
+    ta.yd = j->yd.as<T>();
+    ta.inv_two_var = j->t_inv_two_var;
+    ta.beta = j->X.as<T>();
+    ta.cap = j->total;
+    ta.first = a.first;
+    ta.n_new = a.n_new;
+    ta.rows_per_split = cdiv(cdiv(j->tn, splits), 64) * 64;
+    ta.part = j->tpart.as<double>();
+    const int64_t used = cdiv(j->tn, ta.rows_per_split);
+    launch_target<T>(D, ta, used, s);
+    hipLaunchKernelGGL((k_target_combine<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, ta.part, (int)used,
+                       a.n_new, ta.beta, ta.cap, ta.first, d, j->t_inv_two_prior, j->t_const, j->tpre.as<T>());
+    a.pre_logtarget = j->tpre.as<T>();
+    launch_mix<T, kPreTarget>(D, ncomp, a, s);
   };
   auto reduce = [&](const DevBuf& partials, int nrec, int nv, DevBuf& reduced, const FinishArgs& f) {
     hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), 0, s, partials.as<double>(),
@@ -900,6 +960,12 @@ int alloc_job_buffers(is_job* j) {
   HIP_TRY(hipMemset(j->counter.p, 0, 64));
   std::vector<double> zeros(std::max(j->nslots, 1), 0.0);
   if ((rc = upload_T<T>(j->zerosT, zeros))) return rc;
+  if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
+    if ((rc = j->tpre.alloc((size_t)j->maxN * sizeof(T)))) return rc;
+    int64_t part = 1;
+    for (int64_t N : j->schedule) part = std::max(part, target_splits(j->tn, N) * N);
+    if ((rc = j->tpart.alloc((size_t)part * sizeof(double)))) return rc;
+  }
   if (j->method == IS_DMPMC) {
     if ((rc = j->cdf.alloc((size_t)j->maxN * sizeof(double)))) return rc;
     if ((rc = j->tile_total.alloc((size_t)(cdiv(j->maxN, kCdfThreads * kCdfPer) + 1) * sizeof(double)))) return rc;
@@ -947,8 +1013,8 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   if (desc->method < IS_PLAIN || desc->method > IS_DMPMC) return fail(IS_EINVAL, "unknown method");
   if (desc->family != IS_GAUSSIAN && desc->family != IS_STUDENT_T) return fail(IS_EINVAL, "unknown family");
   if (desc->precision != IS_FP64 && desc->precision != IS_FP32) return fail(IS_EINVAL, "unknown precision");
-  if (desc->target_kind != IS_TARGET_STANDARD && desc->target_kind != IS_TARGET_GAUSSIAN)
-    return fail(IS_EUNSUPPORTED, "target kind %d needs the batch-target path", desc->target_kind);
+  if (desc->target_kind < IS_TARGET_STANDARD || desc->target_kind > IS_TARGET_LINREG)
+    return fail(IS_EINVAL, "unknown target kind %d", desc->target_kind);
   const int T = desc->method == IS_PLAIN ? 1 : desc->rounds;
   if (T < 1 || !desc->round_size) return fail(IS_EINVAL, "rounds must be positive");
   auto j = new is_job();
@@ -1057,6 +1123,33 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
       if (!(tp.L[c + (size_t)c * d] != 0.0)) return fail(IS_EINVAL, "target factor diagonal must be nonzero");
     j->tslot.assign(j->stride, 0.0);
     pack_slot(tp, d, 0.0, j->tslot.data());
+  } else if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
+    // [n, p, prior_var, X, y] (logistic) or [n, p, sigma, tau, X, y] (linear regression)
+    const int head = j->tkind == IS_TARGET_LOGISTIC ? 3 : 4;
+    const double* tp = desc->target_params;
+    if (!tp || desc->target_params_len < head) return fail(IS_EINVAL, "regression target parameters are missing");
+    const double nd = tp[0], pd = tp[1];
+    if (!(nd >= 1.0) || nd != std::floor(nd) || pd != (double)d)
+      return fail(IS_EINVAL, "regression target needs n >= 1 rows and p == dim coefficients");
+    const int64_t n = (int64_t)nd;
+    if (desc->target_params_len < head + n * d + n) return fail(IS_EINVAL, "regression data is shorter than n x p + n");
+    j->tn = n;
+    const double pi2 = 2.0 * 3.141592653589793;
+    if (j->tkind == IS_TARGET_LOGISTIC) {
+      const double pv = tp[2];
+      if (!(pv > 0.0) || !std::isfinite(pv)) return fail(IS_EINVAL, "prior variance must be positive");
+      j->t_inv_two_prior = 0.5 / pv;
+      j->t_const = -0.5 * (double)d * std::log(pi2 * pv);
+    } else {
+      const double sigma = tp[2], tau = tp[3];
+      if (!(sigma > 0.0) || !(tau > 0.0) || !std::isfinite(sigma) || !std::isfinite(tau))
+        return fail(IS_EINVAL, "sigma and tau must be positive");
+      j->t_inv_two_var = 0.5 / (sigma * sigma);
+      j->t_inv_two_prior = 0.5 / (tau * tau);
+      j->t_const = -(double)n * std::log(sigma) - 0.5 * (double)n * std::log(pi2) - (double)d * std::log(tau) -
+                   0.5 * (double)d * std::log(pi2);
+    }
+    j->tdata.assign(tp + head, tp + head + n * d + n);
   }
   HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
   int rc = j->precision == IS_FP32 ? alloc_job_buffers<float>(j) : alloc_job_buffers<double>(j);
@@ -1076,6 +1169,15 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   if (!j->tslot.empty()) {
     rc = j->precision == IS_FP32 ? upload_T<float>(j->tslotT, j->tslot) : upload_T<double>(j->tslotT, j->tslot);
     if (rc) return rc;
+  }
+  if (!j->tdata.empty()) {
+    const size_t nx = (size_t)j->tn * d;
+    std::vector<double> xpart(j->tdata.begin(), j->tdata.begin() + nx), ypart(j->tdata.begin() + nx, j->tdata.end());
+    rc = j->precision == IS_FP32 ? upload_T<float>(j->Xd, xpart) : upload_T<double>(j->Xd, xpart);
+    if (!rc) rc = j->precision == IS_FP32 ? upload_T<float>(j->yd, ypart) : upload_T<double>(j->yd, ypart);
+    if (rc) return rc;
+    j->tdata.clear();
+    j->tdata.shrink_to_fit();
   }
   j->prefilled.assign(T, 0);
   j->prefilled_radial.assign(T, 0);

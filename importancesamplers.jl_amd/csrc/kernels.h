@@ -98,6 +98,7 @@ __device__ inline T inline_target(int tkind, int d, P tslot, const T (&x)[D], T 
 }
 
 #include "mix.h"
+#include "target.h"
 
 // ---------------------------------------------------------------- partial reduction + finish
 struct FinishArgs {

@@ -144,10 +144,16 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
   int gen = a.gen_slot;
   if (active && a.assign) gen = a.assign[j];
   if (MODE == kPreTarget) {
+    // the draw-only pass marks an invalid draw (already recorded) with a NaN numerator
     if (active) {
+      const T mark = a.lognums[gi];
+      if (mark != mark) {
+        ok = false;
+      } else {
 #pragma unroll
-      for (int c = 0; c < D; ++c)
-        if (c < d) x[c] = a.X[(int64_t)c * a.cap + gi];
+        for (int c = 0; c < D; ++c)
+          if (c < d) x[c] = a.X[(int64_t)c * a.cap + gi];
+      }
     }
   } else {
     int good = 0;
@@ -175,6 +181,7 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
         good = valid ? 1 : 0;
       }
       drawn_f[lane] = good;
+      if (MODE == kDrawOnly) a.lognums[gi] = good ? T(0) : T(__builtin_nan(""));
     }
     if (MODE == kDrawOnly) return;
     __syncthreads();

@@ -637,3 +637,53 @@ def test_dmpmc_scalar_oracle(oracle_mod):
     assert np.allclose(r["logweights"], lws, rtol=1e-12, atol=1e-13)
     assert (r["ancestors"] == ancs).all()
     assert np.allclose(r["final_locations"].ravel(), final)
+
+
+# ---------------------------------------------------------------- regression targets
+def test_linreg_target_bayes_identity(oracle_mod):
+    """log p(y | b) + log p(b) - log p(b | y) = log p(y) for every b: the conjugate
+    posterior and evidence come from numpy linear algebra, the densities from scipy."""
+    from scipy.stats import norm
+
+    rng = np.random.default_rng(7)
+    n, p, sigma, tau = 60, 4, 0.5, 2.0
+    X = rng.standard_normal((n, p))
+    y = X @ rng.normal(0, 1, p) + sigma * rng.standard_normal(n)
+    params = np.concatenate([[n, p, sigma, tau], X.ravel(), y])
+    prec = X.T @ X / sigma**2 + np.eye(p) / tau**2
+    S = np.linalg.inv(prec)
+    m = S @ X.T @ y / sigma**2
+    C = sigma**2 * np.eye(n) + tau**2 * X @ X.T
+    logev = -0.5 * (n * np.log(2 * np.pi) + np.linalg.slogdet(C)[1] + y @ np.linalg.solve(C, y))
+    L = np.linalg.cholesky(S)
+    for _ in range(5):
+        b = m + L @ rng.standard_normal(p) * 3.0
+        direct = norm.logpdf(y, X @ b, sigma).sum() + norm.logpdf(b, 0.0, tau).sum()
+        tv = oracle_mod.target(3, params, b)
+        assert abs(tv - direct) < 1e-10 * abs(direct)
+        r = np.linalg.solve(L, b - m)
+        logpost = -0.5 * r @ r - 0.5 * p * np.log(2 * np.pi) - np.log(np.diag(L)).sum()
+        assert abs(tv - logpost - logev) < 1e-9 * abs(logev)
+
+
+def test_logistic_target_against_log_expit(oracle_mod):
+    """Bernoulli log-likelihood through scipy's log_expit (a different formulation
+    from y*eta - log1pexp(eta)), the Gaussian prior through scipy's norm.logpdf,
+    and the b = 0 special case -n log 2."""
+    from scipy.special import log_expit
+    from scipy.stats import norm
+
+    rng = np.random.default_rng(8)
+    n, p, pv = 200, 5, 10.0
+    X = np.column_stack([rng.standard_normal((n, p - 1)), np.ones(n)])
+    y = (rng.uniform(size=n) < 0.4).astype(float)
+    params = np.concatenate([[n, p, pv], X.ravel(), y])
+    for scale in [0.0, 0.5, 3.0, 40.0]:
+        b = rng.normal(0, 1, p) * scale
+        eta = X @ b
+        ll = np.sum(y * log_expit(eta) + (1 - y) * log_expit(-eta))
+        direct = ll + norm.logpdf(b, 0.0, np.sqrt(pv)).sum()
+        tv = oracle_mod.target(2, params, b)
+        assert abs(tv - direct) < 1e-10 * max(1.0, abs(direct))
+        if scale == 0.0:
+            assert abs(ll + n * np.log(2.0)) < 1e-9
