@@ -1,37 +1,70 @@
-"""Compiles libissampler.so for gfx950 with hipcc (in-tree)."""
+"""Compiles libissampler.so for gfx950 with hipcc (in-tree).
+
+The host engine (csrc/engine.hip) and one object per dimension instantiation
+(csrc/dim.hip with -DIS_DIM=D, D in the IS_DIMS list of csrc/launch.h) are
+compiled in parallel, then linked into one shared library."""
 
 import os
+import re
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = [os.path.join(HERE, "csrc", "engine.hip")]
+CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libissampler.so")
+OBJDIR = os.path.join(HERE, "_build")
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # keep the compiler from emitting packed FP32 arithmetic
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",
 ]
+NOISE = "packed-fp32-ops"
 
 
-def build(verbose=False):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *FLAGS, "-o", OUT + ".tmp", *SOURCES]
+def dims():
+    with open(os.path.join(CSRC, "launch.h")) as f:
+        text = f.read()
+    body = text[text.index("#define IS_DIMS(X)"):]
+    body = body[: body.index("\n\n")]
+    return [int(v) for v in re.findall(r"X\((\d+)\)", body)]
+
+
+def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
-    noise = "packed-fp32-ops"
-    err = "\n".join(line for line in res.stderr.splitlines() if noise not in line)
-    if res.returncode != 0:
-        sys.stderr.write(err + "\n")
+    err = "\n".join(line for line in res.stderr.splitlines() if NOISE not in line)
+    return res.returncode, err
+
+
+def build(verbose=False, jobs=None):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJDIR, exist_ok=True)
+    units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), [])]
+    for d in dims():
+        units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}.o"), [f"-DIS_DIM={d}"]))
+    jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)))
+    cmds = [[hipcc, *FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(_run, cmds))
+    failed = [(cmd, err) for cmd, (rc, err) in zip(cmds, results) if rc != 0]
+    for cmd, err in failed:
+        sys.stderr.write(" ".join(cmd[-4:]) + "\n" + err + "\n")
+    if failed:
         raise RuntimeError("hipcc failed for libissampler.so")
-    if verbose and err.strip():
+    if verbose:
+        for (rc, err) in results:
+            if err.strip():
+                sys.stderr.write(err + "\n")
+    rc, err = _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *[o for _, o, _ in units]])
+    if rc != 0:
         sys.stderr.write(err + "\n")
+        raise RuntimeError("link failed for libissampler.so")
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

@@ -240,6 +240,15 @@ __device__ inline cptr<T> as_const(const T* p) {
   return (cptr<T>)p;
 }
 
+// Dimension instantiations: every d <= kExactDims has its own kernels (D == d,
+// loops fully unrolled with no guards); larger d run in a bucket D >= d with
+// the runtime d guarding each coordinate.
+constexpr int kExactDims = 16;
+template <int D>
+__host__ __device__ constexpr int exact_dim(int d) {
+  return D <= kExactDims ? D : d;
+}
+
 // ---------------------------------------------------------------- proposal slots
 // A slot packs one proposal for the density kernels:
 //   [ mu(d) | inv_diag(d) | L lower triangle incl. diagonal, row-major packed | lognorm | logmass ]

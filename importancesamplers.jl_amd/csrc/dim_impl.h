@@ -1,0 +1,82 @@
+// Definitions of DimKernels<D> (see launch.h); included only by dim.hip.
+#pragma once
+
+#include <algorithm>
+
+#include "launch.h"
+
+namespace issampler {
+
+template <class T, int D, int MODE>
+void launch_new_mode(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s) {
+  const bool st = a.fam.student != 0;
+  W = std::min(W, new_block_limit(D, (int)sizeof(T), st) / 64);
+  const size_t slot_bytes = (size_t)a.stride * sizeof(T);
+  const size_t tslot_bytes = a.tkind == kTGaussian ? slot_bytes : 0;
+  const size_t avail = kNewSlotBudget > tslot_bytes ? kNewSlotBudget - tslot_bytes : 0;
+  a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / slot_bytes));
+  const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
+  if (MODE == kPreTarget)
+    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+  else if (st)
+    hipLaunchKernelGGL((k_new<T, D, MODE, true>), dim3(blocks), dim3(64 * W), lds, s, a);
+  else
+    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+}
+
+template <class T, int D>
+void launch_new_any(int mode, const RoundArgs<T>& a, int ncomp, int blocks, int W, hipStream_t s) {
+  if (mode == kDrawFused)
+    launch_new_mode<T, D, kDrawFused>(a, ncomp, blocks, W, s);
+  else if (mode == kDrawOnly)
+    launch_new_mode<T, D, kDrawOnly>(a, ncomp, blocks, W, s);
+  else
+    launch_new_mode<T, D, kPreTarget>(a, ncomp, blocks, W, s);
+}
+
+// moment tile counts an instantiation serves: d in the bucket of D, d + 1 values
+constexpr int nt_lo(int D) { return D <= kExactDims ? (D + 16) / 16 : (D == 24 ? 2 : D == 32 ? 2 : D == 48 ? 3 : 4); }
+constexpr int nt_hi(int D) { return (D + 16) / 16; }
+
+template <class T, int D>
+void launch_sweep_any(int nt, const SweepArgs<T>& a, int blocks, int nw, size_t lds, hipStream_t s) {
+  if (nt == 0)
+    hipLaunchKernelGGL((k_sweep<T, D, 0>), dim3(blocks), dim3(64 * nw), lds, s, a);
+  else if (nt == nt_lo(D))
+    hipLaunchKernelGGL((k_sweep<T, D, nt_lo(D)>), dim3(blocks), dim3(64 * nw), lds, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep<T, D, nt_hi(D)>), dim3(blocks), dim3(64 * nw), lds, s, a);
+}
+
+template <class T, int D>
+void launch_target_any(const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
+  hipLaunchKernelGGL((k_target_gemm<T, D>), dim3((unsigned)((t.n_new + 63) / 64), (unsigned)splits), dim3(256),
+                     target_lds_bytes((D + 3) / 4, (int)sizeof(T)), s, t);
+}
+
+template <int D>
+void DimKernels<D>::new_samples(int mode, const RoundArgs<float>& a, int ncomp, int blocks, int W, hipStream_t s) {
+  launch_new_any<float, D>(mode, a, ncomp, blocks, W, s);
+}
+template <int D>
+void DimKernels<D>::new_samples(int mode, const RoundArgs<double>& a, int ncomp, int blocks, int W, hipStream_t s) {
+  launch_new_any<double, D>(mode, a, ncomp, blocks, W, s);
+}
+template <int D>
+void DimKernels<D>::sweep(int nt, const SweepArgs<float>& a, int blocks, int nw, size_t lds, hipStream_t s) {
+  launch_sweep_any<float, D>(nt, a, blocks, nw, lds, s);
+}
+template <int D>
+void DimKernels<D>::sweep(int nt, const SweepArgs<double>& a, int blocks, int nw, size_t lds, hipStream_t s) {
+  launch_sweep_any<double, D>(nt, a, blocks, nw, lds, s);
+}
+template <int D>
+void DimKernels<D>::target(const TargetArgs<float>& t, int64_t splits, hipStream_t s) {
+  launch_target_any<float, D>(t, splits, s);
+}
+template <int D>
+void DimKernels<D>::target(const TargetArgs<double>& t, int64_t splits, hipStream_t s) {
+  launch_target_any<double, D>(t, splits, s);
+}
+
+}  // namespace issampler

@@ -16,7 +16,9 @@
 #include <vector>
 
 #include "../../include/issampler.h"
-#include "kernels.h"
+#include "launch.h"
+
+#include <type_traits>
 
 using namespace issampler;
 
@@ -334,76 +336,41 @@ void tick(is_job* j, int t, int which, hipStream_t s) {
   if (j->timing && (size_t)(2 * t + which) < j->events.size()) (void)hipEventRecord(j->events[2 * t + which], s);
 }
 
-int dim_bucket(int d) {
-  if (d <= 4) return 4;
-  if (d <= 8) return 8;
-  if (d <= 16) return 16;
-  if (d <= 32) return 32;
-  return 64;
-}
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------- launch dispatch
 constexpr int64_t kSweepPerBlock = 512;  // samples per sweep block (one partial record)
 
-constexpr size_t kNewSlotBudget = 48 * 1024;  // LDS bytes for staged slots per new-sample block
-
-template <class T, int D, int MODE>
-void launch_new_typed(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s) {
-  const bool st = a.fam.student != 0;
-  W = std::min(W, new_block_limit(D, (int)sizeof(T), st) / 64);
-  const size_t slot_bytes = (size_t)a.stride * sizeof(T);
-  const size_t tslot_bytes = a.tkind == kTGaussian ? slot_bytes : 0;
-  const size_t avail = kNewSlotBudget > tslot_bytes ? kNewSlotBudget - tslot_bytes : 0;
-  a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / slot_bytes));
-  const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
-  if (MODE == kPreTarget)
-    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
-  else if (st)
-    hipLaunchKernelGGL((k_new<T, D, MODE, true>), dim3(blocks), dim3(64 * W), lds, s, a);
-  else
-    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+// Calls f(std::integral_constant<int, D>) for the job's dimension instantiation.
+template <class F>
+void with_dim(int D, F&& f) {
+  switch (D) {
+#define IS_DIM_CASE(v)                     \
+  case v:                                  \
+    f(std::integral_constant<int, v>{});   \
+    break;
+    IS_DIMS(IS_DIM_CASE)
+#undef IS_DIM_CASE
+    default:
+      break;
+  }
 }
 
 // Waves per new-sample block: enough that each wave evaluates a few mixture
 // components and the Box-Muller pairs of the block's 64 samples spread over
-// its threads (capped by the kernel's register-bound thread limit).
-int new_waves(int D, int d, int ncomp) {
+// its threads; at most 8, so that three blocks fit on a CU and a
+// 20,000-sample round (313 blocks) starts in one pass.
+int new_waves(int d, int ncomp) {
   const int np = (d + 1) / 2;
-  int W = 4;
-  if (ncomp >= 16 || np > 4) W = 8;
-  if (ncomp >= 40) W = 16;
-  return W;
+  return (ncomp >= 16 || np > 4) ? 8 : 4;
 }
 
 template <class T, int MODE>
 void launch_mix(int D, int ncomp, const RoundArgs<T>& a, hipStream_t s) {
-  const int W = new_waves(D, a.d, ncomp);
+  const int W = new_waves(a.d, ncomp);
   const int blocks = (int)std::max<int64_t>(1, cdiv(a.n_new, 64));
-  switch (D) {
-    case 4: launch_new_typed<T, 4, MODE>(a, ncomp, blocks, W, s); break;
-    case 8: launch_new_typed<T, 8, MODE>(a, ncomp, blocks, W, s); break;
-    case 16: launch_new_typed<T, 16, MODE>(a, ncomp, blocks, W, s); break;
-    case 32: launch_new_typed<T, 32, MODE>(a, ncomp, blocks, W, s); break;
-    default: launch_new_typed<T, 64, MODE>(a, ncomp, blocks, W, s); break;
-  }
-}
-
-// Moment tiles per side for the augmented vector (x - c, 1) of length d + 1.
-int sweep_nt(int d, bool moments) { return moments ? (d + 1 + 15) / 16 : 0; }
-int sweep_waves(int nt) { return nt <= 1 ? 4 : nt == 2 ? 3 : nt == 3 ? 2 : 1; }
-size_t sweep_lds(int nt, int nw) {
-  const size_t per_wave = nt > 0 ? (size_t)64 * sweep_row_stride(nt) * sizeof(double) : 0;
-  const size_t staging = nw * per_wave;
-  const size_t P = (size_t)nt * (nt + 1) / 2;
-  const size_t reduce = (16 + (size_t)nw * P * 256 + P * 256) * sizeof(double);
-  return std::max(staging, reduce);
-}
-
-template <class T, int D, int NT>
-void launch_sweep_typed(const SweepArgs<T>& a, int blocks, int nw, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_sweep<T, D, NT>), dim3(blocks), dim3(64 * nw), lds, s, a);
+  with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::new_samples(MODE, a, ncomp, blocks, W, s); });
 }
 
 // Launches one sweep over [a.lo, a.hi); returns the number of partial records.
@@ -413,26 +380,19 @@ int launch_sweep(int D, int nt, SweepArgs<T> a, hipStream_t s) {
   const int blocks = (int)std::max<int64_t>(1, cdiv(a.hi - a.lo, kSweepPerBlock));
   a.per_block = kSweepPerBlock;
   const size_t lds = sweep_lds(nt, nw);
-#define IS_SW(DD, NN) launch_sweep_typed<T, DD, NN>(a, blocks, nw, lds, s)
-  switch (D) {
-    case 4:
-      if (nt == 0) IS_SW(4, 0); else IS_SW(4, 1);
-      break;
-    case 8:
-      if (nt == 0) IS_SW(8, 0); else IS_SW(8, 1);
-      break;
-    case 16:
-      if (nt == 0) IS_SW(16, 0); else if (nt == 1) IS_SW(16, 1); else IS_SW(16, 2);
-      break;
-    case 32:
-      if (nt == 0) IS_SW(32, 0); else if (nt == 2) IS_SW(32, 2); else IS_SW(32, 3);
-      break;
-    default:
-      if (nt == 0) IS_SW(64, 0); else if (nt == 3) IS_SW(64, 3); else if (nt == 4) IS_SW(64, 4); else IS_SW(64, 5);
-      break;
-  }
-#undef IS_SW
+  with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::sweep(nt, a, blocks, nw, lds, s); });
   return blocks;
+}
+
+template <class T>
+void launch_target(int D, const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
+  with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::target(t, splits, s); });
+}
+
+// Summary / fit on a reduced record (one wavefront, any d <= 64).
+void launch_finish(int d, const double* reduced, const FinishArgs& f, hipStream_t s) {
+  (void)d;
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, reduced, f);
 }
 
 // Row slices for the batched target: enough blocks to fill the chip when a round
@@ -441,23 +401,6 @@ int64_t target_splits(int64_t n_rows, int64_t n_new) {
   const int64_t tiles = std::max<int64_t>(1, cdiv(n_new, 64));
   const int64_t want = std::max<int64_t>(1, cdiv(1024, tiles));
   return std::min(want, std::max<int64_t>(1, cdiv(n_rows, 64)));
-}
-
-template <class T, int KS>
-void launch_target_typed(const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
-  hipLaunchKernelGGL((k_target_gemm<T, KS>), dim3((unsigned)cdiv(t.n_new, 64), (unsigned)splits), dim3(256),
-                     target_lds_bytes(KS, (int)sizeof(T)), s, t);
-}
-
-template <class T>
-void launch_target(int D, const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
-  switch (D) {
-    case 4: launch_target_typed<T, 1>(t, splits, s); break;
-    case 8: launch_target_typed<T, 2>(t, splits, s); break;
-    case 16: launch_target_typed<T, 4>(t, splits, s); break;
-    case 32: launch_target_typed<T, 8>(t, splits, s); break;
-    default: launch_target_typed<T, 16>(t, splits, s); break;
-  }
 }
 
 // ---------------------------------------------------------------- job helpers
@@ -565,7 +508,6 @@ FinishArgs base_finish(is_job* j, int t, unsigned int* counter) {
   return f;
 }
 
-size_t finish_lds(int d) { return (size_t)finish_lds_doubles(d) * sizeof(double); }
 
 template <class T>
 SweepArgs<T> sweep_args(is_job* j, int t) {
@@ -627,7 +569,7 @@ int run_rounds(is_job* j) {
   auto reduce = [&](const DevBuf& partials, int nrec, int nv, DevBuf& reduced, const FinishArgs& f) {
     hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), 0, s, partials.as<double>(),
                        (int64_t)nrec, j->pcap, nv, reduced.as<double>(), f);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), finish_lds(d), s, reduced.as<double>(), f);
+    launch_finish(d, reduced.as<double>(), f, s);
   };
 
   if (j->method == IS_PLAIN) {
@@ -1567,9 +1509,9 @@ static int job_lognormalizer_typed(is_job* job, double* out3) {
   f.summary_count = n;
   f.summary_out = job->summary.as<double>();
   f.counter = job->counter.as<unsigned int>() + 1;
-  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), finish_lds(job->d), job->stream,
+  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), 0, job->stream,
                      job->partials2.as<double>(), (int64_t)blocks, job->pcap, 2, job->reduced2.as<double>(), f);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), finish_lds(job->d), job->stream, job->reduced2.as<double>(), f);
+  launch_finish(job->d, job->reduced2.as<double>(), f, job->stream);
   HIP_TRY(hipGetLastError());
   double s[5];
   HIP_TRY(hipMemcpyAsync(s, job->summary.p, sizeof(s), hipMemcpyDeviceToHost, job->stream));
@@ -1714,16 +1656,16 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   w.partials = partials.as<double>();
   w.pcap = pcap;
   w.nv = 2;
-  const int blocks = launch_sweep<double>(4, 0, w, s);
+  const int blocks = launch_sweep<double>(dim_bucket(1), 0, w, s);
   FinishArgs f{};
   f.d = 1;
   f.flags = kFinishSummary;
   f.summary_count = n;
   f.summary_out = summary.as<double>();
   f.counter = counter.as<unsigned int>();
-  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), finish_lds(1), s, partials.as<double>(),
+  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), 0, s, partials.as<double>(),
                      (int64_t)blocks, pcap, 2, reduced.as<double>(), f);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), finish_lds(1), s, reduced.as<double>(), f);
+  launch_finish(1, reduced.as<double>(), f, s);
   hipLaunchKernelGGL((k_cdf_tiles<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),
                      tiles.as<double>());
   hipLaunchKernelGGL((k_cdf_write<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),

@@ -126,7 +126,6 @@ constexpr int kFinishSummary = 1;
 constexpr int kFinishFit = 2;
 constexpr int kFinishAdaptEss = 4;
 
-__device__ void finish_block(const double* red, const FinishArgs& f, double* lds);
 
 // grid = nv blocks; block k reduces value k over all partial records.  Partial
 // records are stored column-major: column 0 holds the block maxima, column
@@ -136,7 +135,7 @@ __device__ void finish_block(const double* red, const FinishArgs& f, double* lds
 // to kBlock * kReduceUnroll records cost one memory round trip; the threads
 // are then merged in a fixed order (deterministic).
 constexpr int kReduceUnroll = 8;
-__global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
+static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
                                                           int64_t nrec, int64_t pcap, int nv,
                                                           double* __restrict__ reduced, FinishArgs f) {
   __shared__ double wm[kBlock / 64], ws[kBlock / 64];
@@ -195,179 +194,176 @@ __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restri
   (void)f;
 }
 
-// One workgroup: summary and fit on the reduced values (launched after
-// k_reduce_finish; the kernel boundary publishes the reduced column sums).
-__global__ __launch_bounds__(kBlock) void k_finish(const double* __restrict__ reduced, FinishArgs f) {
-  extern __shared__ double lds_raw[];
-  finish_block(reduced, f, lds_raw);
+// Summary (src/execution.jl:176-186): ESS = s0^2 / s2, log Z = M + log s0 - log n.
+__device__ inline void finish_summary(double M, double s0, double s2, const FinishArgs& f) {
+  double ess, logz;
+  if (s0 == 0.0 || M == -__builtin_huge_val()) {
+    ess = 0.0;
+    logz = -__builtin_huge_val();
+  } else {
+    ess = (s0 * s0) / s2;
+    logz = M + log(s0) - log((double)f.summary_count);
+  }
+  if (f.flags & kFinishSummary) {
+    if (f.round_ess) f.round_ess[f.round] = ess;
+    if (f.round_logz) f.round_logz[f.round] = logz;
+    if (f.summary_out) {
+      f.summary_out[0] = M;
+      f.summary_out[1] = s0;
+      f.summary_out[2] = s2;
+      f.summary_out[3] = ess;
+      f.summary_out[4] = logz;
+    }
+  }
+  if ((f.flags & kFinishAdaptEss) && f.adapt_ess) f.adapt_ess[f.round] = ess;
 }
 
-// Summary (src/execution.jl:176-186) and FP64 moment fit
+__device__ inline double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+constexpr int kFinishStride = kMaxDim + 1;  // LDS row stride of the covariance
+
+// One wavefront: summary (src/execution.jl:176-186) and FP64 moment fit
 // (src/adaptive_gaussian_execution.jl:303-425): mean = c + s1/s0,
-// cov = S/s0 - (s1/s0)(s1/s0)^T + ridge I, Cholesky, Student-t rescale, log-det.
-// Every global input (reduced sums, previous factor, centre) is fetched in one
-// batched round trip into LDS; the rest runs out of LDS with one barrier per
-// Cholesky column, so the kernel costs one memory latency plus O(d) barriers.
-__host__ __device__ constexpr int finish_lds_doubles(int d) {
-  return (3 + d + d * (d + 1) / 2) + d * (d + 1) / 2 + d + d * d + 3 * d + 8;
-}
-
-__device__ void finish_block(const double* __restrict__ red, const FinishArgs& f, double* lds) {
-  const int tid = threadIdx.x, nt = blockDim.x;
+// cov = S/s0 - (s1/s0)(s1/s0)^T + ridge I, Cholesky, Student-t rescale,
+// log-det.  Lane r owns row r (d <= 64).  All global inputs are fetched in one
+// batch into LDS; the Cholesky is the left-looking column order of
+// _gaussian_potrf! (src/adaptive_gaussian_execution.jl:427-434) with the rows
+// of each column computed in parallel, so a fit is d dependent steps of
+// (dot product, sqrt, divide) and no workgroup barrier.
+static __global__ __launch_bounds__(64) void k_finish(const double* __restrict__ red, FinishArgs f) {
+  __shared__ double A[kMaxDim * kFinishStride];
+  __shared__ double meanv[kMaxDim];
+  const int lane = threadIdx.x;
   const int d = f.d;
-  const bool fit = (f.flags & kFinishFit) != 0;
-  const int npk = d * (d + 1) / 2;
-  const int nv = fit ? 3 + d + npk : 3;
-  double* R = lds;                 // reduced values [M, s0, s2, s1(d), S pairs]
-  double* Lp = R + 3 + d + npk;    // previous factor, packed lower
-  double* C = Lp + npk;            // centre
-  double* A = C + d;               // d x d, row-major
-  double* mean = A + d * d;
-  double* dg = mean + d;           // Cholesky diagonal
-  double* lg = dg + d;             // log of the final diagonal
-  double* wred = lg + d;           // per-wave partials (4)
-  __shared__ int status_s;
-  const int nload = fit ? nv + npk + d : nv;
-  for (int q = tid; q < nload; q += nt) {
-    double v;
-    if (q < nv) v = red[q];
-    else if (q < nv + npk) v = f.prev_slot[2 * d + (q - nv)];
-    else v = f.center[q - nv - npk];
-    if (q < nv) R[q] = v;
-    else if (q < nv + npk) Lp[q - nv] = v;
-    else C[q - nv - npk] = v;
-  }
-  if (tid == 0) status_s = 0;
-  __syncthreads();
-  const double M = R[0], s0 = R[1], s2 = R[2];
-  if (tid == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) {
-    double ess, logz;
-    if (s0 == 0.0 || M == -__builtin_huge_val()) {
-      ess = 0.0;
-      logz = -__builtin_huge_val();
-    } else {
-      ess = (s0 * s0) / s2;
-      logz = M + log(s0) - log((double)f.summary_count);
-    }
-    if (f.flags & kFinishSummary) {
-      if (f.round_ess) f.round_ess[f.round] = ess;
-      if (f.round_logz) f.round_logz[f.round] = logz;
-      if (f.summary_out) {
-        f.summary_out[0] = M;
-        f.summary_out[1] = s0;
-        f.summary_out[2] = s2;
-        f.summary_out[3] = ess;
-        f.summary_out[4] = logz;
-      }
-    }
-    if ((f.flags & kFinishAdaptEss) && f.adapt_ess) f.adapt_ess[f.round] = ess;
-  }
-  if (!fit) return;
+  const double M = red[0], s0 = red[1], s2 = red[2];
+  if (lane == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) finish_summary(M, s0, s2, f);
+  if (!(f.flags & kFinishFit)) return;
   if (!(s0 > 0.0) || !isfinite(s0) || M == -__builtin_huge_val()) {
-    if (tid == 0) f.fit_status[f.round] = 1;
+    if (lane == 0) f.fit_status[f.round] = 1;
     return;
   }
-  const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
-  // scale-aware ridge from the previous proposal's factor
-  // (src/adaptive_gaussian_execution.jl:283-301): sqrt(eps) * tr(L L^T) * mult / d
+  const bool row = lane < d;
+  const int npk = d * (d + 1) / 2;
+  // ---- one batch of loads: S pairs and the previous factor (ridge), s1 and the centre
   double tr = 0.0;
-  for (int q = tid; q < npk; q += nt) tr += Lp[q] * Lp[q];
-  tr = wave_sum(tr);
-  if ((tid & 63) == 0) wred[tid >> 6] = tr;
-  for (int c = tid; c < d; c += nt) mean[c] = R[3 + c] / s0;
-  __syncthreads();
+  for (int p = lane; p < npk; p += 64) {
+    int r, c;
+    pair_index(p, r, c);
+    const double l = f.prev_slot[2 * d + p];
+    A[r * kFinishStride + c] = red[3 + d + p];
+    tr += l * l;
+  }
+  const double s1 = row ? red[3 + lane] : 0.0;
+  const double cr = row ? f.center[lane] : 0.0;
+  const double mean = s1 / s0;
+  meanv[lane] = mean;
+  const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
+  // scale-aware ridge (src/adaptive_gaussian_execution.jl:283-301): sqrt(eps) tr(L L^T) mult / d
   const double reg = 1.4901161193847656e-08;  // sqrt(eps(Float64))
-  double trace = 0.0;
-  for (int w = 0; w < (nt >> 6); ++w) trace += wred[w];
-  double ridge = reg * (trace * mult) / (double)d;
+  double ridge = reg * (wave_sum(tr) * mult) / (double)d;
   if (!isfinite(ridge)) {
-    // overflow-safe restatement: sum of (L_ij * sqrt(reg) * sqrt(mult / d))^2
     const double sc = sqrt(reg) * sqrt(mult / (double)d);
-    ridge = 0.0;
-    for (int q = 0; q < npk; ++q) {
-      const double v = Lp[q] * sc;
-      ridge += v * v;
+    double part = 0.0;
+    for (int p = lane; p < npk; p += 64) {
+      const double v = f.prev_slot[2 * d + p] * sc;
+      part += v * v;
+    }
+    ridge = wave_sum(part);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // ---- centred covariance, row `lane`
+  if (row) {
+    double* ar = A + lane * kFinishStride;
+    for (int c = 0; c <= lane; ++c) {
+      double v = ar[c] / s0 - mean * meanv[c];
+      if (c == lane) v += ridge;
+      ar[c] = v;
     }
   }
-  // centred covariance plus ridge
-  for (int p = tid; p < npk; p += nt) {
-    int r, c;
-    pair_index(p, r, c);
-    double v = R[3 + d + p] / s0 - mean[r] * mean[c];
-    if (r == c) v += ridge;
-    A[r * d + c] = v;
-    A[c * d + r] = v;
-  }
-  __syncthreads();
-  // left-looking Cholesky (LAPACK potrf order): each thread forms the column's
-  // diagonal itself from the finished columns, then the rows below it
+  // ---- Cholesky, column by column
+  int status = 0;
   for (int c = 0; c < d; ++c) {
-    double v = A[c * d + c];
-    for (int k = 0; k < c; ++k) v -= A[c * d + k] * A[c * d + k];
-    const double diag = sqrt(v);
-    if (tid == 0) {
-      if (!(v > 0.0) || !isfinite(v)) status_s = 2;
-      dg[c] = diag;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double v = 0.0;
+    if (lane >= c && row) {
+      const double* ar = A + lane * kFinishStride;
+      const double* ac = A + c * kFinishStride;
+      v = ar[c];
+      int k = 0;
+      for (; k + 4 <= c; k += 4) {
+        const double a0 = ar[k], a1 = ar[k + 1], a2 = ar[k + 2], a3 = ar[k + 3];
+        const double b0 = ac[k], b1 = ac[k + 1], b2 = ac[k + 2], b3 = ac[k + 3];
+        v -= a0 * b0;
+        v -= a1 * b1;
+        v -= a2 * b2;
+        v -= a3 * b3;
+      }
+      for (; k < c; ++k) v -= ar[k] * ac[k];
     }
-    for (int r = c + 1 + tid; r < d; r += nt) {
-      double w = A[r * d + c];
-      for (int k = 0; k < c; ++k) w -= A[r * d + k] * A[c * d + k];
-      A[r * d + c] = w / diag;
-    }
-    __syncthreads();
+    const double piv = readlane_f64(v, c);
+    if (!(piv > 0.0) || !isfinite(piv)) status = 2;
+    const double dg = sqrt(piv);
+    if (row && lane > c) A[lane * kFinishStride + c] = v / dg;
+    if (lane == c) A[c * kFinishStride + c] = dg;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // ---- outputs: lane r writes mean r, 1 / L_rr and row r of L (Student-t rescaled)
   const double scale = f.student ? sqrt(1.0 / mult) : 1.0;
-  double* out = f.next_slot64;
-  for (int r = tid; r < d; r += nt) {
-    const double dr = dg[r] * scale;
-    out[r] = C[r] + mean[r];
-    out[d + r] = 1.0 / dr;
-    lg[r] = log(fabs(dr));
-  }
   bool ok = true;
-  for (int p = tid; p < npk; p += nt) {
-    int r, c;
-    pair_index(p, r, c);
-    const double v = (r == c ? dg[r] : A[r * d + c]) * scale;
-    out[2 * d + p] = v;
-    ok &= isfinite(v);
-    if (r == c) ok &= v > 0.0;
+  double lg = 0.0, diag = 1.0;
+  double* out = f.next_slot64;
+  const bool tf = f.t_is_float != 0;
+  float* of = reinterpret_cast<float*>(f.next_slot_t);
+  double* od = reinterpret_cast<double*>(f.next_slot_t);
+  if (row) {
+    const double* ar = A + lane * kFinishStride;
+    const double mu = cr + mean;
+    diag = ar[lane] * scale;
+    ok = isfinite(mu) && diag > 0.0;
+    lg = log(fabs(diag));
+    out[lane] = mu;
+    out[d + lane] = 1.0 / diag;
+    if (tf) {
+      of[lane] = (float)mu;
+      of[d + lane] = (float)(1.0 / diag);
+    } else {
+      od[lane] = mu;
+      od[d + lane] = 1.0 / diag;
+    }
+    const int base = 2 * d + lane * (lane + 1) / 2;
+    for (int c = 0; c <= lane; ++c) {
+      const double v = ar[c] * scale;
+      ok &= isfinite(v);
+      out[base + c] = v;
+      if (tf)
+        of[base + c] = (float)v;
+      else
+        od[base + c] = v;
+    }
   }
-  for (int r = tid; r < d; r += nt) ok &= isfinite(C[r] + mean[r]);
-  if (!ok) status_s = status_s == 0 ? 3 : status_s;
-  __syncthreads();
-  if (tid == 0) {
-    int status = status_s;
-    double logabsdet = 0.0;
-    for (int r = 0; r < d; ++r) logabsdet += lg[r];
+  const double logabsdet = wave_sum(lg);
+  const bool all_ok = __all(ok);
+  if (lane == 0) {
     const double lognorm = f.student ? f.logconstant - logabsdet
                                      : -0.5 * (double)d * log(2.0 * 3.141592653589793) - logabsdet;
-    if (status == 0 && !isfinite(lognorm)) status = 3;
+    int st = status;
+    if (st == 0 && (!all_ok || !isfinite(lognorm))) st = 3;
     out[slot_lognorm_off(d)] = lognorm;
-    f.fit_status[f.round] = status;
-  }
-  // T copy of the new slot (logmass is owned by the host schedule); the values
-  // are recomputed rather than re-read so no barrier on the global writes is needed
-  const int nfill = slot_lognorm_off(d) + 1;
-  for (int q = tid; q < nfill; q += nt) {
-    double v;
-    if (q < d) {
-      v = C[q] + mean[q];
-    } else if (q < 2 * d) {
-      v = 1.0 / (dg[q - d] * scale);
-    } else if (q < 2 * d + npk) {
-      int r, c;
-      pair_index(q - 2 * d, r, c);
-      v = (r == c ? dg[r] : A[r * d + c]) * scale;
-    } else {
-      double la = 0.0;
-      for (int r = 0; r < d; ++r) la += lg[r];
-      v = f.student ? f.logconstant - la : -0.5 * (double)d * log(2.0 * 3.141592653589793) - la;
-    }
-    if (f.t_is_float)
-      reinterpret_cast<float*>(f.next_slot_t)[q] = (float)v;
+    if (tf)
+      of[slot_lognorm_off(d)] = (float)lognorm;
     else
-      reinterpret_cast<double*>(f.next_slot_t)[q] = v;
+      od[slot_lognorm_off(d)] = lognorm;
+    f.fit_status[f.round] = st;
   }
 }
 
@@ -540,19 +536,19 @@ __global__ void k_gather_locations(const T* __restrict__ X, int64_t cap, int64_t
 }
 
 // ---------------------------------------------------------------- RNG export (test seam)
-__global__ void k_export_normals(RngKey key, uint64_t round, int64_t n, int count, uint64_t purpose,
+static __global__ void k_export_normals(RngKey key, uint64_t round, int64_t n, int count, uint64_t purpose,
                                  double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int q = 0; q < count; ++q) out[i * count + q] = rng_normal(key, (uint64_t)i, round, purpose, q);
 }
-__global__ void k_export_uniforms(RngKey key, uint64_t round, int64_t n, int count, uint64_t purpose,
+static __global__ void k_export_uniforms(RngKey key, uint64_t round, int64_t n, int count, uint64_t purpose,
                                   double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int q = 0; q < count; ++q) out[i * count + q] = rng_uniform(key, (uint64_t)i, round, purpose, q);
 }
-__global__ void k_rng_raw(uint64_t k0, uint64_t k1, uint64_t c1, uint64_t c2, uint64_t c3, int64_t n,
+static __global__ void k_rng_raw(uint64_t k0, uint64_t k1, uint64_t c1, uint64_t c2, uint64_t c3, int64_t n,
                           unsigned long long* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -561,7 +557,7 @@ __global__ void k_rng_raw(uint64_t k0, uint64_t k1, uint64_t c1, uint64_t c2, ui
 }
 
 // Resampling uniforms of a round: counter (k, round, purpose 3, 0), word 0.
-__global__ void k_resampling_uniforms(RngKey key, const unsigned long long* run_hi, int round, int64_t count,
+static __global__ void k_resampling_uniforms(RngKey key, const unsigned long long* run_hi, int round, int64_t count,
                                       double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;

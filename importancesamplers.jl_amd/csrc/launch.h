@@ -1,0 +1,51 @@
+// Per-dimension kernel launchers.  Each dimension instantiation D (d = 1..16
+// exactly, then the buckets 24, 32, 48, 64) is compiled in its own
+// translation unit (dim.hip built with -DIS_DIM=D, in parallel) and linked
+// into the library; the engine dispatches on the job's D.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace issampler {
+
+#define IS_DIMS(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(24) X(32) X(48) X(64)
+
+inline int dim_bucket(int d) {
+  if (d <= kExactDims) return d;
+  if (d <= 24) return 24;
+  if (d <= 32) return 32;
+  if (d <= 48) return 48;
+  return 64;
+}
+
+// Moment tiles per side for the augmented vector (x - c, 1) of length d + 1.
+inline int sweep_nt(int d, bool moments) { return moments ? (d + 1 + 15) / 16 : 0; }
+inline int sweep_waves(int nt) { return nt <= 1 ? 4 : nt == 2 ? 3 : nt == 3 ? 2 : 1; }
+inline size_t sweep_lds(int nt, int nw) {
+  const size_t per_wave = nt > 0 ? (size_t)64 * sweep_row_stride(nt) * sizeof(double) : 0;
+  const size_t staging = nw * per_wave;
+  const size_t P = (size_t)nt * (nt + 1) / 2;
+  const size_t reduce = (16 + (size_t)nw * P * 256 + P * 256) * sizeof(double);
+  return staging > reduce ? staging : reduce;
+}
+
+constexpr size_t kNewSlotBudget = 48 * 1024;  // LDS bytes for staged slots per new-sample block
+
+template <int D>
+struct DimKernels {
+  // k_new in mode `mode` (kDrawFused / kDrawOnly / kPreTarget), 64 samples per block of W waves
+  static void new_samples(int mode, const RoundArgs<float>& a, int ncomp, int blocks, int W, hipStream_t s);
+  static void new_samples(int mode, const RoundArgs<double>& a, int ncomp, int blocks, int W, hipStream_t s);
+  // k_sweep with `nt` moment tiles per side (0: summary only)
+  static void sweep(int nt, const SweepArgs<float>& a, int blocks, int nw, size_t lds, hipStream_t s);
+  static void sweep(int nt, const SweepArgs<double>& a, int blocks, int nw, size_t lds, hipStream_t s);
+  // k_target_gemm over `splits` row slices
+  static void target(const TargetArgs<float>& t, int64_t splits, hipStream_t s);
+  static void target(const TargetArgs<double>& t, int64_t splits, hipStream_t s);
+};
+
+}  // namespace issampler

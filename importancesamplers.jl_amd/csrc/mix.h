@@ -53,7 +53,7 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
   constexpr int ZS = D + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int W = blockDim.x >> 6;
-  const int d = a.d;
+  const int d = exact_dim<D>(a.d);
   const int64_t j0 = (int64_t)blockIdx.x * 64;
   const int64_t j = j0 + lane;
   const bool active = j < a.n_new;
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
   T* tslot = reinterpret_cast<T*>(reinterpret_cast<char*>(lds) + new_lds_fixed(D, W, sizeof(T)));
   const bool staged_target = (MODE == kDrawFused && a.tkind == kTGaussian);
   T* comp = tslot + (staged_target ? a.stride : 0);
-  const int ncomp = (MODE == kDrawOnly) ? 0 : a.k1 - a.k0 + 1;
+  const int ncomp = a.k1 - a.k0 + 1;
   const int chunk = min(ncomp, a.slot_chunk);
 
   // ---- A: slot staging, normals and radial multipliers
@@ -168,9 +168,16 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
         T z[D];
 #pragma unroll
         for (int c = 0; c < D; ++c) z[c] = (c < d) ? T(zs[lane * ZS + c] * mult) : T(0);
+        // the generating slot is read from the staged chunk when it is there
+        const int gl = gen - a.k0;
         const int64_t goff = (int64_t)gen * a.stride;
-        const bool valid = a.assign ? slot_draw<T, D>(a.slots + goff, d, z, x)
-                                    : slot_draw<T, D>(as_const(a.slots + goff), d, z, x);
+        bool valid;
+        if (gl >= 0 && gl < chunk)
+          valid = slot_draw<T, D>(comp + gl * a.stride, d, z, x);
+        else if (a.assign)
+          valid = slot_draw<T, D>(a.slots + goff, d, z, x);
+        else
+          valid = slot_draw<T, D>(as_const(a.slots + goff), d, z, x);
 #pragma unroll
         for (int c = 0; c < D; ++c)
           if (c < d) {
@@ -323,7 +330,7 @@ struct SweepLoad {
 
 template <class T, int D, int NT>
 __device__ inline void sweep_load(const SweepArgs<T>& a, int64_t i, int64_t bend, SweepLoad<T, D>& L) {
-  const int d = a.d;
+  const int d = exact_dim<D>(a.d);
   const bool valid = i < bend;
   const bool app = a.append && i < a.append_hi;
 #pragma unroll
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
   const T ninf = -T(__builtin_huge_val());
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
-  const int d = a.d;
+  const int d = exact_dim<D>(a.d);
   double* rows = lds + wave * (64 * RS);
   if (NT > 0) {
     // columns above d stay zero for the whole kernel

@@ -53,17 +53,19 @@ struct TargetArgs {
 
 __host__ __device__ constexpr int target_row_stride(int ks) { return 4 * ks + 1; }
 
-// grid = (ceil(n_new / 64), splits), block = 256 (4 waves); KS = K steps (coefficients / 4, padded)
-template <class T, int KS>
+// grid = (ceil(n_new / 64), splits), block = 256 (4 waves); D = dimension
+// instantiation, KS = ceil(D / 4) K steps of the 16x16x4 MFMA
+template <class T, int D>
 __global__ __launch_bounds__(256) void k_target_gemm(TargetArgs<T> a) {
   typedef MfmaTile<T> M;
+  constexpr int KS = (D + 3) / 4;
   constexpr int PS = target_row_stride(KS);
   constexpr int PER = (64 * 4 * KS + 255) / 256;  // staged values per thread per chunk
   extern __shared__ double lds_raw[];
   T* xs = reinterpret_cast<T*>(lds_raw);  // 64 x PS
   T* ys = xs + 64 * PS;                   // 64
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int p = a.p;
+  const int p = exact_dim<D>(a.p);
   const int64_t s0 = (int64_t)blockIdx.x * 64;
   const int64_t r_lo = (int64_t)blockIdx.y * a.rows_per_split;
   const int64_t r_hi = min(a.n, r_lo + a.rows_per_split);
