@@ -44,4 +44,11 @@ from .api import (  # noqa: F401
     student_t_logconstant,
     var,
 )
+from .batch import (  # noqa: F401
+    BatchRoundErrors,
+    PreparedBatch,
+    autotune,
+    importance_sample_batch_,
+    prepare_batch,
+)
 from . import _lib  # noqa: F401

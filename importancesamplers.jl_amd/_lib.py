@@ -107,6 +107,28 @@ SIGNATURES = {
     "is_select": (ctypes.c_int, [VP, F64P, F64P, ctypes.c_int64, F64P, ctypes.c_int64, ctypes.c_int32, I64P]),
     "is_rng_raw": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_uint64, ctypes.c_int64, U64P]),
+    "is_batch_create": (ctypes.c_int, [VP, ctypes.POINTER(JobDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(VP)]),
+    "is_batch_destroy": (ctypes.c_int, [VP]),
+    "is_batch_run": (ctypes.c_int, [VP]),
+    "is_batch_set_tuning": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32]),
+    "is_batch_get_tuning": (ctypes.c_int, [VP, I32P, I32P]),
+    "is_batch_set_order": (ctypes.c_int, [VP, I32P]),
+    "is_batch_get_order": (ctypes.c_int, [VP, I32P]),
+    "is_batch_trial": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, F64P]),
+    "is_batch_set_timing": (ctypes.c_int, [VP, ctypes.c_int32]),
+    "is_batch_kernel_ms": (ctypes.c_int, [VP, F64P]),
+    "is_batch_status": (ctypes.c_int, [VP, I32P]),
+    "is_batch_job_total": (ctypes.c_int64, [VP, ctypes.c_int32]),
+    "is_batch_job_failure": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(Failure)]),
+    "is_batch_job_samples": (ctypes.c_int, [VP, ctypes.c_int32, F64P]),
+    "is_batch_job_logweights": (ctypes.c_int, [VP, ctypes.c_int32, F64P]),
+    "is_batch_job_round_summary": (ctypes.c_int, [VP, ctypes.c_int32, F64P, F64P]),
+    "is_batch_final_summary": (ctypes.c_int, [VP, F64P, F64P]),
+    "is_batch_job_history": (ctypes.c_int, [VP, ctypes.c_int32, F64P, F64P, F64P]),
+    "is_batch_job_proposal": (ctypes.c_int, [VP, ctypes.c_int32, F64P, F64P, F64P]),
+    "is_batch_export_normals": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, F64P, ctypes.c_int64]),
+    "is_batch_export_radial": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, F64P, ctypes.c_int64, F64P,
+                                              ctypes.c_int64]),
 }
 
 _lib = None

@@ -414,6 +414,55 @@ def context(device=0):
     return _ctx
 
 
+def _job_desc(rng, target, algorithm, precision):
+    """Builds the C descriptor of one prepared sampler; returns (desc, keep, proposal,
+    schedule) where `keep` holds the numpy buffers the descriptor points into."""
+    desc = JobDesc()
+    keep = []
+
+    def arr(a):
+        a = _f64(a)
+        keep.append(a)
+        return _p(a)
+
+    seed = rng.seed if isinstance(rng, PrefilledRNG) else int(rng)
+    desc.seed = seed & 0xFFFFFFFFFFFFFFFF
+    desc.stream = 0x6D693335357873  # fixed second key word
+    desc.precision = precision
+    if isinstance(algorithm, ImportanceSampling):
+        p, desc.method, sched = algorithm.proposal, PLAIN, [int(algorithm.nsamples)]
+    elif isinstance(algorithm, NPMC):
+        p, desc.method, sched = algorithm.proposal, NPMC_M, algorithm.schedule
+    elif isinstance(algorithm, AMIS):
+        p, desc.method, sched = algorithm.proposal, AMIS_M, algorithm.schedule
+    elif isinstance(algorithm, DeterministicMixturePMC):
+        p, desc.method, sched = algorithm.bank.proposals[0], DMPMC_M, algorithm.schedule
+    else:
+        raise TypeError("unsupported algorithm")
+    desc.family, desc.scale_kind, desc.scalar, desc.dim, desc.dof = p.family, p.kind, int(p.scalar), p.dim, p.dof
+    desc.rounds = len(sched)
+    rs = np.asarray(sched, dtype=np.int64)
+    keep.append(rs)
+    desc.round_size = rs.ctypes.data_as(I64P)
+    if desc.method == DMPMC_M:
+        bank = algorithm.bank
+        desc.bank_size = len(bank.proposals)
+        desc.bank_locations = arr(np.concatenate([q.location for q in bank.proposals]))
+        desc.bank_scales = arr(np.concatenate([q.scale if q.kind != FACTOR else q.factor().T.ravel()
+                                               for q in bank.proposals]))
+        desc.bank_masses = arr(bank.masses)
+        desc.resampling = algorithm.method.code
+    else:
+        desc.location = arr(p.location)
+        desc.scale = arr(p.scale if p.kind != FACTOR else p.factor().T.ravel())
+    desc.target_kind = target.kind
+    tp = _f64(target.params(p.dim))
+    keep.append(tp)
+    desc.target_params = _p(tp)
+    desc.target_params_len = tp.size
+    return desc, keep, p, sched
+
+
 # ---------------------------------------------------------------- prepared sampler
 class PreparedSampler:
     """`_PreparedImportanceSampler` on the MI355X device (one engine job)."""
@@ -424,53 +473,12 @@ class PreparedSampler:
         self.precision = precision
         lib = _lib.load()
         ctx = context()
-        desc = JobDesc()
-        keep = []  # keep numpy buffers alive across the call
-
-        def arr(a):
-            a = _f64(a)
-            keep.append(a)
-            return _p(a)
-
-        seed = rng.seed if isinstance(rng, PrefilledRNG) else int(rng)
-        desc.seed = seed & 0xFFFFFFFFFFFFFFFF
-        desc.stream = 0x6D693335357873  # fixed second key word
-        desc.precision = precision
-        if isinstance(algorithm, ImportanceSampling):
-            p, desc.method, sched = algorithm.proposal, PLAIN, [int(algorithm.nsamples)]
-        elif isinstance(algorithm, NPMC):
-            p, desc.method, sched = algorithm.proposal, NPMC_M, algorithm.schedule
-        elif isinstance(algorithm, AMIS):
-            p, desc.method, sched = algorithm.proposal, AMIS_M, algorithm.schedule
-        elif isinstance(algorithm, DeterministicMixturePMC):
-            p, desc.method, sched = algorithm.bank.proposals[0], DMPMC_M, algorithm.schedule
-        else:
-            raise TypeError("unsupported algorithm")
+        desc, keep, p, sched = _job_desc(rng, target, algorithm, precision)
+        self._keep = keep
         self.proposal0 = p
         self.dim = p.dim
         self.scalar = bool(p.scalar)
         self.schedule = list(sched)
-        desc.family, desc.scale_kind, desc.scalar, desc.dim, desc.dof = p.family, p.kind, int(p.scalar), p.dim, p.dof
-        desc.rounds = len(sched)
-        rs = np.asarray(sched, dtype=np.int64)
-        keep.append(rs)
-        desc.round_size = rs.ctypes.data_as(I64P)
-        if desc.method == DMPMC_M:
-            bank = algorithm.bank
-            desc.bank_size = len(bank.proposals)
-            desc.bank_locations = arr(np.concatenate([q.location for q in bank.proposals]))
-            desc.bank_scales = arr(np.concatenate([q.scale if q.kind != FACTOR else q.factor().T.ravel()
-                                                   for q in bank.proposals]))
-            desc.bank_masses = arr(bank.masses)
-            desc.resampling = algorithm.method.code
-        else:
-            desc.location = arr(p.location)
-            desc.scale = arr(p.scale if p.kind != FACTOR else p.factor().T.ravel())
-        desc.target_kind = target.kind
-        tp = _f64(target.params(p.dim))
-        keep.append(tp)
-        desc.target_params = _p(tp)
-        desc.target_params_len = tp.size
         h = VP()
         _check(lib.is_job_create(ctx, ctypes.byref(desc), ctypes.byref(h)))
         self._job = h
@@ -693,6 +701,8 @@ def lognormalizer(result: WeightedSamples):
         out = np.zeros(3)
         _check(s._lib.is_job_lognormalizer(s._job, _p(out)))
         return float(out[0])
+    if getattr(result, "_lognormalizer", None) is not None:
+        return result._lognormalizer  # batch jobs: the device's final-round reduction
     raise ValueError("result is detached from its device sampler")
 
 

@@ -1,8 +1,10 @@
 """Compiles libissampler.so for gfx950 with hipcc (in-tree).
 
-The host engine (csrc/engine.hip) and one object per dimension instantiation
-(csrc/dim.hip with -DIS_DIM=D, D in the IS_DIMS list of csrc/launch.h) are
-compiled in parallel, then linked into one shared library."""
+The host engine (csrc/engine.hip), the batched AMIS host side (csrc/batch.hip),
+one object per dimension instantiation (csrc/dim.hip with -DIS_DIM=D, D in the
+IS_DIMS list of csrc/launch.h) and one per batched-kernel shape (csrc/batch_k.hip
+with -DIS_BATCH_FP and -DIS_BATCH_W) are compiled in parallel, then linked into
+one shared library."""
 
 import os
 import re
@@ -26,6 +28,7 @@ FLAGS = [
     "-Wno-unused-variable",
 ]
 NOISE = "packed-fp32-ops"
+BATCH_WAVES = (2, 4, 8)  # waves per workgroup of the batched AMIS kernel (batch_k.hip)
 
 
 def dims():
@@ -45,7 +48,12 @@ def _run(cmd):
 def build(verbose=False, jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     os.makedirs(OBJDIR, exist_ok=True)
-    units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), [])]
+    units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), []),
+             (os.path.join(CSRC, "batch.hip"), os.path.join(OBJDIR, "batch.o"), [])]
+    for fp in (32, 64):
+        for w in BATCH_WAVES:
+            units.append((os.path.join(CSRC, "batch_k.hip"), os.path.join(OBJDIR, f"batch_k{fp}_w{w}.o"),
+                          [f"-DIS_BATCH_FP={fp}", f"-DIS_BATCH_W={w}"]))
     for d in dims():
         units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}.o"), [f"-DIS_DIM={d}"]))
     jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)))

@@ -1,0 +1,743 @@
+// Device code of the batched AMIS engine (see batch.h); included only by the
+// kernel translation units batch_k.hip.
+//
+// One workgroup of W waves runs one job for all of its rounds.  Per round t:
+//   pass A  each wave takes 64 of the round's N new samples at a time (one per
+//           lane): counter-RNG normals and the Student-t radial multiplier,
+//           the affine draw from slot t, the regression target on MFMA
+//           (X_data beta for the wave's 64 samples, streamed over the data rows
+//           in 16-row tiles), the t+1 term deterministic-mixture log-sum-exp
+//           over the slots staged in LDS, the weight, and the sample's
+//           contribution to the summary and the moments;
+//   pass B  the waves stream the t N retained samples: append of component t
+//           to the AMIS numerator (src/amis_execution.jl:37-65), weight
+//           re-formation (src/amis_execution.jl:107-132), summary and moments;
+//   finish  the waves' records are merged in wave order; wave 0 computes the
+//           round's ESS and log Z (src/execution.jl:176-186) and the FP64 fit
+//           (src/adaptive_gaussian_execution.jl:303-434): mean, covariance,
+//           scale-aware ridge, Cholesky (one row per lane, in registers),
+//           Student-t rescale and log-determinant, written as slot t + 1.
+//
+// Every loop over coordinates runs to the job's padded dimension P = d rounded
+// up to 4, a compile-time constant (one instantiation per P, chosen per job
+// inside the persistent kernel).  Padded coordinates are exact zeros: their
+// slot rows are mu = 0, L row = 0, 1/L_rr = 1, their data columns are zero,
+// so they add nothing to any sum, and the hot loops carry no per-coordinate
+// guards.  Moments accumulate about the centre mu_t on FP64 MFMA (16x16x4)
+// from the staged products sqrt(e)(x - mu_t).
+#pragma once
+
+#include "batch.h"
+#include "kernels.h"
+
+namespace issampler {
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 16-byte vectors: 4 floats or 2 doubles.
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float __attribute__((ext_vector_type(4))) type;
+  static constexpr int E = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double __attribute__((ext_vector_type(2))) type;
+  static constexpr int E = 2;
+};
+
+// Student-t radial randomness of one sample: the 8 buffered gamma normals and
+// uniforms of _buffered_unit_gamma (src/native_execution.jl:147-180), four per
+// Philox block, one block at a time (the scheduler barriers keep the blocks
+// from being interleaved, which would multiply the live registers).
+__device__ inline void batch_radial(const RngKey& key, uint64_t sample, uint64_t round, int rs, int us, double* rn,
+                                    double* ru) {
+#pragma unroll
+  for (int blk = 0; blk < kGammaAttempts / 4; ++blk) {
+    if (blk * 4 < rs) {
+      const U64x4 w = philox4x64(sample, round, kPurposeRadialNormals, (uint64_t)blk, key.k0, key.k1);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
+        double sn, cs;
+        sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
+        rn[4 * blk + 2 * p] = r * cs;
+        rn[4 * blk + 2 * p + 1] = r * sn;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int blk = 0; blk < (kGammaAttempts + 4) / 4; ++blk) {
+    if (blk * 4 < us) {
+      const U64x4 w = philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)blk, key.k0, key.k1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (blk * 4 + q <= kGammaAttempts) ru[blk * 4 + q] = u53(w.v[q]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Whitened density of a batch slot (layout in batch.h, padded dimension P)
+// at x; y receives L^{-1}(x - mu).  Same operation order as slot_logdensity.
+// The rows are software-pipelined: row r + 1 is loaded from LDS while row r
+// is reduced, and a scheduling barrier per row keeps the compiler from
+// hoisting every row's loads (which would hold the whole factor in registers).
+template <class T, int P>
+__device__ inline T bslot_logdensity(const T* s, const Family<T>& fam, const T (&x)[P], T (&y)[P]) {
+  typedef typename Vec16<T>::type V;
+  constexpr int E = Vec16<T>::E;
+  constexpr int NV = P / E;
+  const T* inv = s + P;
+  const T* L = s + 2 * P;
+  V buf[2][NV];
+  T mu_n = s[0], inv_n = inv[0];
+  T r2 = T(0);
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+    const T mu_r = mu_n, inv_r = inv_n;
+    if (r + 1 < P) {
+      const V* p = reinterpret_cast<const V*>(L + bslot_rowoff(r + 1));
+#pragma unroll
+      for (int q = 0; q < (r + E) / E; ++q) buf[(r + 1) & 1][q] = p[q];
+      mu_n = s[r + 1];
+      inv_n = inv[r + 1];
+    }
+    T v = x[r] - mu_r;
+#pragma unroll
+    for (int q = 0; q < (r + E - 1) / E; ++q)
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (q * E + e < r) v -= buf[r & 1][q][e] * y[q * E + e];
+    v *= inv_r;
+    y[r] = v;
+    r2 += v * v;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  return radial_logdensity<T, P>(fam, s[bslot_lognorm(P)], r2, y, P);
+}
+
+// acc[0..2]: moment tiles (0,0), (1,0), (1,1) of sum e (x - c)(x - c)^T;
+// acc[3..4]: first-order sums sum e (x - c) for coordinates 0..15 and 16..31
+// (every column of these tiles holds the same sums).
+constexpr int kBatchTiles = 5;
+struct BatchAcc {
+  double s0, s2;
+  double4_t acc[kBatchTiles];
+};
+
+// Adds one sample per lane (log weight lw, coordinates x) to the wave's
+// summary monoid (src/execution.jl:159-186) and moments about the centre.
+template <class T, int P>
+__device__ inline void batch_accumulate(T lw, const T (&x)[P], const T* centre, T* stage, int lane, T& mw,
+                                        BatchAcc& A) {
+  constexpr int NT = (P + 15) / 16;
+  const T ninf = -T(__builtin_huge_val());
+  if (__any(lw > mw)) {
+    const T wm = wave_max(lw);
+    const double f = (mw == ninf) ? 0.0 : exp((double)mw - (double)wm);
+    A.s0 *= f;
+    A.s2 *= f * f;
+#pragma unroll
+    for (int p = 0; p < kBatchTiles; ++p) A.acc[p] *= f;
+    mw = wm;
+  }
+  const T se = (lw == ninf) ? T(0) : exp(T(0.5) * (lw - mw));
+  const double sd = (double)se;
+  const double e = sd * sd;
+  A.s0 += e;
+  A.s2 += e * e;
+  T* row = stage + lane * kBatchStage;
+#pragma unroll
+  for (int c = 0; c < 16 * NT; ++c) {
+    double v = 0.0;
+    if (c < P) v = (sd != 0.0) ? sd * ((double)x[c] - (double)centre[c]) : 0.0;
+    row[c] = (T)v;
+  }
+  row[kBatchMaxDim] = se;
+  wave_sync();
+  // v_mfma_f64_16x16x4f64 over 4-sample steps: A = the staged column block of
+  // coordinates 16 ti + l % 16, B = the same block (second moments) or the
+  // samples' sqrt(e) (first moments)
+#pragma unroll
+  for (int st = 0; st < 16; ++st) {
+    const T* rs = stage + (4 * st + (lane >> 4)) * kBatchStage;
+    const double a0 = (double)rs[lane & 15];
+    const double b1 = (double)rs[kBatchMaxDim];
+    A.acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, A.acc[0], 0, 0, 0);
+    A.acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, A.acc[3], 0, 0, 0);
+    if (NT > 1) {
+      const double a1 = (double)rs[16 + (lane & 15)];
+      A.acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, A.acc[1], 0, 0, 0);
+      A.acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, A.acc[2], 0, 0, 0);
+      A.acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, A.acc[4], 0, 0, 0);
+    }
+  }
+  wave_sync();
+}
+
+// Regression log-likelihood of the wave's 64 staged samples (lane = sample):
+// eta = X_data beta on MFMA over 16-row tiles, epilogue per element, FP32
+// partial sums folded into FP64 every 8 tiles.
+template <class T, int P>
+__device__ inline double batch_target(const BatchJob<T>& J, const T* stage, int lane) {
+  typedef MfmaTile<T> M;
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  constexpr int KS = P / 4;
+  T breg[4][KS];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < KS; ++k) breg[q][k] = stage[(16 * q + (lane & 15)) * kBatchStage + 4 * k + (lane >> 4)];
+  const T* xp = J.xpack + lane;
+  const V4* yp = reinterpret_cast<const V4*>(J.ypack) + lane;
+  const bool logistic = J.tkind == kTLogistic;
+  double ll[4] = {0.0, 0.0, 0.0, 0.0};
+  T part[4] = {T(0), T(0), T(0), T(0)};
+  T an[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) an[k] = xp[k * 64];
+  V4 yn = yp[0];
+  for (int64_t rt = 0; rt < J.row_tiles; ++rt) {
+    T av[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) av[k] = an[k];
+    const V4 yv = yn;
+    if (rt + 1 < J.row_tiles) {
+      const T* xn = xp + (rt + 1) * (KS * 64);
+#pragma unroll
+      for (int k = 0; k < KS; ++k) an[k] = xn[k * 64];
+      yn = yp[(rt + 1) * 64];
+    }
+    typename M::acc_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = M::zero();
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = M::mma(av[k], breg[q][k], acc[q]);
+    if (!logistic) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const T res = yv[r] - acc[q][r];
+          part[q] += res * res;
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (16 * rt + M::row(lane, r) < J.n_rows) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const T eta = acc[q][r];
+            part[q] += yv[r] * eta - log1pexp(eta);
+          }
+        }
+      }
+    }
+    if ((rt & 7) == 7) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ll[q] += (double)part[q];
+        part[q] = T(0);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ll[q] += (double)part[q];
+    ll[q] += __shfl_xor(ll[q], 16, 64);
+    ll[q] += __shfl_xor(ll[q], 32, 64);
+  }
+  const int qq = lane >> 4;
+  const double mine = qq == 0 ? ll[0] : qq == 1 ? ll[1] : qq == 2 ? ll[2] : ll[3];
+  return logistic ? mine : -J.inv_two_var * mine;
+}
+
+// Wave 0: merge the W wave records, write the round summary and fit slot t + 1.
+// rec(w) = [mw, s0, s2, s1(32), tiles 3 x 256] as doubles in wave w's stage.
+template <class T, int W, int P>
+__device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, int rec_stride, T* slots_lds,
+                                    int lane) {
+  const int d = J.d;
+  const int SS = J.slot_elems;
+  const double ninf = -__builtin_huge_val();
+  double M = rec0[0];
+#pragma unroll
+  for (int w = 1; w < W; ++w) M = fmax(M, rec0[w * rec_stride]);
+  double f[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const double mw = rec0[w * rec_stride];
+    f[w] = (mw == ninf) ? 0.0 : exp(mw - M);
+  }
+  double s0 = 0.0, s2 = 0.0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    s0 += f[w] * rec0[w * rec_stride + 1];
+    s2 += f[w] * f[w] * rec0[w * rec_stride + 2];
+  }
+  // first-order sums (lane c) and the moment tiles (4 x 3 entries per lane), merged in place
+  double s1 = 0.0;
+  if (lane < kBatchMaxDim) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) s1 += f[w] * rec0[w * rec_stride + 3 + lane];
+  }
+  double tv[12];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) v += f[w] * rec0[w * rec_stride + 35 + p * 256 + lane * 4 + i];
+      tv[p * 4 + i] = v;
+    }
+  wave_sync();
+  if (lane < kBatchMaxDim) rec0[3 + lane] = s1;
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rec0[35 + p * 256 + lane * 4 + i] = tv[p * 4 + i];
+  wave_sync();
+  // summary (src/execution.jl:176-186)
+  const int64_t count = (int64_t)(t + 1) * J.N;
+  double ess, logz;
+  if (s0 == 0.0 || M == ninf) {
+    ess = 0.0;
+    logz = ninf;
+  } else {
+    ess = (s0 * s0) / s2;
+    logz = M + log(s0) - log((double)count);
+  }
+  if (lane == 0) {
+    J.round_ess[t] = ess;
+    J.round_logz[t] = logz;
+  }
+  if (!(s0 > 0.0) || !isfinite(s0) || M == ninf) {
+    if (lane == 0) J.fit_status[t] = 1;
+    return;
+  }
+  // ---- FP64 fit about the centre c = mu_t (the staged T values)
+  const T* slot_t = slots_lds + t * SS;
+  const bool row = lane < d;
+  const double* prev = J.slots64 + (int64_t)t * SS;
+  const double mean = row ? rec0[3 + lane] / s0 : 0.0;
+  const double cr = row ? (double)slot_t[lane] : 0.0;
+  const double mult = J.student ? J.dof / (J.dof - 2.0) : 1.0;
+  // scale-aware ridge (src/adaptive_gaussian_execution.jl:283-301): sqrt(eps) tr(L L^T) mult / d
+  double tr = 0.0;
+  if (row) {
+    const double* lr = prev + 2 * P + bslot_rowoff(lane);
+    for (int c = 0; c <= lane; ++c) tr += lr[c] * lr[c];
+  }
+  const double reg = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+  double ridge = reg * (wave_sum(tr) * mult) / (double)d;
+  if (!isfinite(ridge)) {
+    const double sc = sqrt(reg) * sqrt(mult / (double)d);
+    double part = 0.0;
+    if (row) {
+      const double* lr = prev + 2 * P + bslot_rowoff(lane);
+      for (int c = 0; c <= lane; ++c) {
+        const double v = lr[c] * sc;
+        part += v * v;
+      }
+    }
+    ridge = wave_sum(part);
+  }
+  // centred covariance, row `lane`, in registers
+  double a[P];
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    a[c] = 0.0;
+    if (c < d && row && c <= lane) {
+      const int r = lane;
+      const int ti = r >> 4, tj = c >> 4, ra = r & 15, cb = c & 15;
+      const int p = ti * (ti + 1) / 2 + tj;
+      const int ln = cb + 16 * (ra & 3), reg_i = ra >> 2;
+      const double S = rec0[35 + p * 256 + ln * 4 + reg_i];
+      double v = S / s0 - mean * (rec0[3 + c] / s0);
+      if (c == lane) v += ridge;
+      a[c] = v;
+    }
+  }
+  // Cholesky, left-looking by columns (the order of _gaussian_potrf!,
+  // src/adaptive_gaussian_execution.jl:427-434); row c of L sits in lane c
+  int status = 0;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    if (c < d) {
+      double v = a[c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) v -= a[k] * readlane_f64(a[k], c);
+      if (!(lane >= c && row)) v = 0.0;
+      const double piv = readlane_f64(v, c);
+      if (!(piv > 0.0) || !isfinite(piv)) status = 2;
+      const double dg = sqrt(piv);
+      if (row && lane > c) a[c] = v / dg;
+      if (lane == c) a[c] = dg;
+    }
+  }
+  // outputs: lane r < d writes mu_r, 1 / L_rr and row r of L (Student-t
+  // rescaled); lanes d <= r < P write the zero padding (mu 0, row 0, 1/L_rr 1)
+  const double scale = J.student ? sqrt(1.0 / mult) : 1.0;
+  double* o64 = J.slots64 + (int64_t)(t + 1) * SS;
+  T* oT = J.slotsT + (int64_t)(t + 1) * SS;
+  T* oL = (t + 1 < J.rounds) ? slots_lds + (t + 1) * SS : nullptr;
+  bool ok = true;
+  double lg = 0.0;
+  if (lane < P) {
+    double mu = 0.0, invd = 1.0;
+    if (row) {
+      double diag = 0.0;
+#pragma unroll
+      for (int c = 0; c < P; ++c)
+        if (c == lane) diag = a[c] * scale;
+      mu = cr + mean;
+      ok = isfinite(mu) && diag > 0.0;
+      lg = log(fabs(diag));
+      invd = 1.0 / diag;
+    }
+    o64[lane] = mu;
+    oT[lane] = (T)mu;
+    o64[P + lane] = invd;
+    oT[P + lane] = (T)invd;
+    if (oL) {
+      oL[lane] = (T)mu;
+      oL[P + lane] = (T)invd;
+    }
+    const int base = 2 * P + bslot_rowoff(lane);
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      if (c <= lane) {
+        const double v = row ? a[c] * scale : 0.0;
+        ok &= isfinite(v);
+        o64[base + c] = v;
+        oT[base + c] = (T)v;
+        if (oL) oL[base + c] = (T)v;
+      }
+    }
+  }
+  const double logabsdet = wave_sum(lg);
+  const bool all_ok = __all(ok);
+  if (lane == 0) {
+    const double lognorm =
+        J.student ? J.logconstant - logabsdet : -0.5 * (double)d * log(2.0 * 3.141592653589793) - logabsdet;
+    int st = status;
+    if (st == 0 && (!all_ok || !isfinite(lognorm))) st = 3;
+    const int off = bslot_lognorm(P);
+    o64[off] = lognorm;
+    oT[off] = (T)lognorm;
+    if (oL) oL[off] = (T)lognorm;
+    J.fit_status[t] = st;
+  }
+}
+
+// One job, all rounds, by the whole workgroup; P = the job's padded dimension.
+template <class T, int W, int P>
+__device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t, int slots_off) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const T ninf = -T(__builtin_huge_val());
+  const int SS = J.slot_elems;
+  T* stage = lds_t + wave * 64 * kBatchStage;
+  T* slots = lds_t + slots_off;
+  const RngKey key{J.key0, J.key1};
+  Family<T> fam;
+  fam.student = J.student;
+  fam.dof = J.fdof;
+  fam.half_dof_plus_d = J.fhalf;
+  // slot 0 (set from the committed proposal by k_batch_begin)
+  for (int q = tid; q < SS; q += 64 * W) slots[q] = J.slotsT[q];
+  __syncthreads();
+  for (int t = 0; t < J.rounds; ++t) {
+    const uint64_t rng_word = run_hi | (uint64_t)t;
+    const T* slot_t = slots + t * SS;
+    const int64_t first = (int64_t)t * J.N;
+    const T logtotal = (T)log((double)((int64_t)(t + 1) * J.N));
+    unsigned long long* fail_t = J.fail + 3 * t;
+    T mw = ninf;
+    BatchAcc A;
+    A.s0 = 0.0;
+    A.s2 = 0.0;
+#pragma unroll
+    for (int p = 0; p < kBatchTiles; ++p) A.acc[p] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+    // ---- pass A: the round's new samples, 64 per wave at a time
+    for (int64_t j0 = 64 * (int64_t)wave; j0 < J.N; j0 += 64 * W) {
+      const int64_t j = j0 + lane;
+      const bool active = j < J.N;
+      T x[P];
+      bool ok = active;
+      {
+        double mult = 1.0;
+        if (J.student) {
+          double rn[kGammaAttempts], ru[kGammaAttempts + 1];
+          batch_radial(key, (uint64_t)j, rng_word, J.rs, J.us, rn, ru);
+          if (!student_t_multiplier(J.dof, rn, ru, J.us > 0, &mult)) {
+            if (active) record_failure(fail_t, j + 1, 0, kDrawExhausted);
+            mult = __builtin_nan("");
+            ok = false;
+          }
+        }
+        // x = mu + L z, accumulated column by column as each Philox block
+        // yields four normals: row r still receives mu_r + L_r0 z_0 + L_r1 z_1
+        // + ... in that order (the order of slot_draw).  One block at a time.
+        const T* Ls = slot_t + 2 * P;
+#pragma unroll
+        for (int r = 0; r < P; ++r) x[r] = slot_t[r];
+#pragma unroll
+        for (int blk = 0; blk < P / 4; ++blk) {
+          const U64x4 w = philox4x64((uint64_t)j, rng_word, kPurposeNormals, (uint64_t)blk, key.k0, key.k1);
+          T zq[4];
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
+            double sn, cs;
+            sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
+            zq[2 * p] = T(r * cs * mult);
+            zq[2 * p + 1] = T(r * sn * mult);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int c = 4 * blk + e;
+#pragma unroll
+            for (int r = c; r < P; ++r) x[r] += Ls[bslot_rowoff(r) + c] * zq[e];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        bool valid = true;
+#pragma unroll
+        for (int r = 0; r < P; ++r) valid &= isfinite(x[r]);
+        if (!active) {
+#pragma unroll
+          for (int r = 0; r < P; ++r) x[r] = T(0);
+        }
+        T* rowp = stage + lane * kBatchStage;
+#pragma unroll
+        for (int c = 0; c < P; ++c) rowp[c] = x[c];
+        if (active) {
+          T* xb = J.X + bx_base(first + j, P);
+#pragma unroll
+          for (int c = 0; c < P; ++c) xb[c * 64] = x[c];
+          if (!valid) record_failure(fail_t, j + 1, 0, kGeneratedNonfinite);
+        }
+        ok = ok && valid;
+      }
+      // target on MFMA for the wave's 64 samples (coordinates staged in LDS)
+      wave_sync();
+      const double ll = batch_target<T, P>(J, stage, lane);
+      wave_sync();
+      double b2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < P; ++c) b2 += (double)x[c] * (double)x[c];
+      const T tv = (T)(ll - b2 * J.inv_two_prior + J.tconst);
+      if (ok) {
+        const uint16_t tr = target_reason(tv);
+        if (tr) {
+          record_failure(fail_t, j + 1, 0, tr);
+          ok = false;
+        }
+      }
+      // deterministic mixture over slots 0..t (src/mis_execution.jl:30-95)
+      T m = ninf, sum = T(0);
+      bool genbad = false;
+      if (ok) {
+        T y[P];
+        for (int k = 0; k <= t; ++k) {
+          const T q = bslot_logdensity<T, P>(slots + k * SS, fam, x, y);
+          if (k == t && (q != q || q == ninf)) genbad = true;
+          const T v = J.logN + q;
+          if (v > m) {
+            sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
+            m = v;
+          } else {
+            sum += LseAcc<T>::fast_exp(v - m);
+          }
+        }
+      }
+      T ltv = ninf, lnv = T(0), lwv = ninf;
+      if (ok) {
+        const T value = m + log(sum);
+        if (genbad || value != value || value == ninf) {
+          record_failure(fail_t, j + 1, 0, kProposalInvalid);
+        } else {
+          const T w = tv - (value - logtotal);
+          const uint16_t wr = logweight_reason(w);
+          if (wr) {
+            record_failure(fail_t, j + 1, 0, wr);
+          } else {
+            ltv = tv;
+            lnv = value;
+            lwv = w;
+          }
+        }
+      }
+      if (active) {
+        J.lt[first + j] = ltv;
+        J.ln[first + j] = lnv;
+        J.lw[first + j] = lwv;
+      }
+      batch_accumulate<T, P>(active ? lwv : ninf, x, slot_t, stage, lane, mw, A);
+    }
+
+    // ---- pass B: append component t to the t N retained samples
+    for (int64_t base = 64 * (int64_t)wave; base < first; base += 64 * W) {
+      const int64_t i = base + lane;
+      const bool v = i < first;
+      T cx[P];
+      {
+        const T* xb = J.X + bx_base(v ? i : 0, P);
+#pragma unroll
+        for (int c = 0; c < P; ++c) cx[c] = v ? xb[c * 64] : T(0);
+      }
+      const T cln = v ? J.ln[i] : T(0);
+      const T clt = v ? J.lt[i] : ninf;
+      T lwv = ninf;
+      if (v) {
+        T y[P];
+        const T q = bslot_logdensity<T, P>(slot_t, fam, cx, y);
+        uint16_t reason = (q != q) ? kProposalInvalid : 0;
+        T num = cln;
+        if (!reason) {
+          const T value = logaddexp<T>(num, J.logN + q);
+          reason = proposal_reason(value);
+          if (!reason) {
+            num = value;
+            J.ln[i] = value;
+          }
+        }
+        if (reason) record_failure(fail_t, i + 1, 0, reason);
+        const T w = clt - (num - logtotal);
+        const uint16_t wr = logweight_reason(w);
+        if (wr) {
+          record_failure(fail_t, i + 1, 0, wr);
+        } else {
+          J.lw[i] = w;
+          lwv = w;
+        }
+      }
+      batch_accumulate<T, P>(lwv, cx, slot_t, stage, lane, mw, A);
+    }
+
+    // ---- round end: wave records, merge and fit on wave 0
+    const double s0 = wave_sum(A.s0);
+    const double s2 = wave_sum(A.s2);
+    __syncthreads();  // every wave is done with its stage and the slots of this round
+    const int rec_stride = 64 * kBatchStage * (int)sizeof(T) / 8;
+    double* rec = reinterpret_cast<double*>(lds_t) + wave * rec_stride;
+    if (lane == 0) {
+      rec[0] = (double)mw;
+      rec[1] = s0;
+      rec[2] = s2;
+    }
+    // first-order sums: tile entry (row i, column 0) sits in lane 16 (i & 3), register i >> 2
+    if ((lane & 15) == 0) {
+      const int g = lane >> 4;
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rec[3 + 16 * ti + 4 * q + g] = A.acc[3 + ti][q];
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) rec[35 + p * 256 + lane * 4 + i] = A.acc[p][i];
+    __syncthreads();
+    if (wave == 0) batch_finish<T, W, P>(J, t, reinterpret_cast<double*>(lds_t), rec_stride, slots, lane);
+    __syncthreads();
+  }
+}
+
+// Padded dimensions with their own job runner (every multiple of 4 up to 32).
+#define IS_BATCH_DIMS(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32)
+
+// Persistent kernel: every workgroup pulls jobs from the queue until it is empty.
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void k_batch_amis(BatchArgs<T> a) {
+  extern __shared__ double lds_raw[];
+  __shared__ int item_s;
+  T* lds_t = reinterpret_cast<T*>(lds_raw);
+  const BatchJob<T>* __restrict__ jobs = a.jobs;
+  const uint64_t run_hi = *a.run_hi;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int item = item_s;
+    __syncthreads();
+    if (item >= a.njobs) break;
+    const int jid = __builtin_amdgcn_readfirstlane(a.order[item]);
+    // the descriptor is uniform: copy it word by word through the constant
+    // address space so it lands in scalar registers
+    BatchJob<T> J;
+    {
+      const cptr<uint32_t> src = as_const(reinterpret_cast<const uint32_t*>(jobs + jid));
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&J);
+#pragma unroll
+      for (int q = 0; q < (int)(sizeof(BatchJob<T>) / 4); ++q) dst[q] = src[q];
+    }
+    switch (J.dp) {
+#define IS_BATCH_CASE(v)                               \
+  case v:                                              \
+    batch_job<T, W, v>(J, run_hi, lds_t, a.slots_off); \
+    break;
+      IS_BATCH_DIMS(IS_BATCH_CASE)
+#undef IS_BATCH_CASE
+      default:
+        break;
+    }
+  }
+}
+
+// Before the persistent launch: clear each job's failure words and fit
+// status, put the committed proposal into slot 0, reset the queue head.
+template <class T>
+__global__ void k_batch_begin(const BatchJob<T>* jobs, int njobs, unsigned int* queue) {
+  const int j = blockIdx.x;
+  if (j == 0 && threadIdx.x == 0) *queue = 0u;
+  if (j >= njobs) return;
+  const BatchJob<T>& J = jobs[j];
+  for (int q = threadIdx.x; q < 3 * J.rounds; q += blockDim.x) J.fail[q] = 0ull;
+  for (int q = threadIdx.x; q < J.rounds; q += blockDim.x) J.fit_status[q] = 0;
+  for (int q = threadIdx.x; q < J.slot_elems; q += blockDim.x) {
+    J.slots64[q] = J.committed64[q];
+    J.slotsT[q] = J.committedT[q];
+  }
+}
+
+// After the launch: per-job status (0 ok, else the first failing round) and,
+// when `commit`, the transactional commit of the last fit
+// (src/adaptive_gaussian_execution.jl:1101-1111); advances the run word.
+template <class T>
+__global__ void k_batch_end(const BatchJob<T>* jobs, int njobs, int commit, int* status,
+                            unsigned long long* run_word) {
+  __shared__ int st_s;
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const BatchJob<T>& J = jobs[j];
+  if (threadIdx.x == 0) {
+    int st = 0;
+    for (int t = 0; t < J.rounds && st == 0; ++t)
+      if (J.fail[3 * t] != 0ull || J.fit_status[t] != 0) st = t + 1;
+    st_s = st;
+    status[j] = st;
+  }
+  __syncthreads();
+  if (commit && st_s == 0) {
+    const int64_t off = (int64_t)J.rounds * J.slot_elems;
+    for (int q = threadIdx.x; q < J.slot_elems; q += blockDim.x) {
+      J.committed64[q] = J.slots64[off + q];
+      J.committedT[q] = J.slotsT[off + q];
+    }
+  }
+  if (commit && j == 0 && threadIdx.x == 0) *run_word += (1ull << 32);
+}
+
+}  // namespace issampler

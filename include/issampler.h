@@ -205,6 +205,47 @@ int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t 
 int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3,
                int64_t n, uint64_t* out);
 
+/* ---- batched AMIS: many independent posteriors in one persistent launch ----------
+ * A batch is a vector of prepared AMIS samplers run together; job j means
+ * importance_sample!(prepare_sampler(rng_j, target_j, AMIS(q_j; rounds, round_size)))
+ * (src/methods/amis.jl:42-47, src/adaptive_gaussian_execution.jl:897-1112) and its
+ * results are those of the job run alone on the same counter-RNG key.  Jobs must be
+ * AMIS with a built-in regression target (linear or logistic), d <= 32, equal round
+ * sizes; precision is per batch (IS_FP32 or IS_FP64).  One workgroup runs one job for
+ * all of its rounds; workgroups pull jobs from a device queue in `order`.
+ * No reference counterpart exists for the batch itself: the reference runs such jobs
+ * one prepared sampler at a time. */
+typedef struct is_batch is_batch;
+int is_batch_create(is_ctx* ctx, const is_job_desc* descs, int32_t njobs, int32_t precision, is_batch** out);
+int is_batch_destroy(is_batch* batch);
+/* importance_sample! over every job; IS_EFAILED when any job failed (per-job status
+ * via is_batch_status / is_batch_job_failure; failed jobs keep their committed proposal). */
+int is_batch_run(is_batch* batch);
+/* Launch shape: waves per workgroup (2, 4, 8) and workgroups per CU; job order (a
+ * permutation of 0..njobs-1, NULL = longest estimated job first). */
+int is_batch_set_tuning(is_batch* batch, int32_t waves, int32_t blocks_per_cu);
+int is_batch_get_tuning(const is_batch* batch, int32_t* waves, int32_t* blocks_per_cu);
+int is_batch_set_order(is_batch* batch, const int32_t* order);
+int is_batch_get_order(const is_batch* batch, int32_t* order);
+/* Autotuning trial: one run with the given launch shape, event-timed, without commit
+ * (the next run draws the same counter-RNG values); kernel_ms = persistent-kernel time. */
+int is_batch_trial(is_batch* batch, int32_t waves, int32_t blocks_per_cu, double* kernel_ms);
+int is_batch_set_timing(is_batch* batch, int32_t enable);
+int is_batch_kernel_ms(const is_batch* batch, double* ms);
+/* per-job results of the last run */
+int is_batch_status(const is_batch* batch, int32_t* status);  /* njobs: 0 ok, else failing round */
+int64_t is_batch_job_total(const is_batch* batch, int32_t job);
+int is_batch_job_failure(is_batch* batch, int32_t job, is_failure* out);
+int is_batch_job_samples(is_batch* batch, int32_t job, double* out);     /* d x total */
+int is_batch_job_logweights(is_batch* batch, int32_t job, double* out);  /* total */
+int is_batch_job_round_summary(is_batch* batch, int32_t job, double* ess, double* lognormalizers);
+int is_batch_final_summary(is_batch* batch, double* ess, double* lognormalizers);  /* njobs each */
+int is_batch_job_history(is_batch* batch, int32_t job, double* means, double* factors, double* lognormalizers);
+int is_batch_job_proposal(is_batch* batch, int32_t job, double* location, double* factor, double* lognormalizer);
+int is_batch_export_normals(is_batch* batch, int32_t job, int32_t round, double* out, int64_t count);
+int is_batch_export_radial(is_batch* batch, int32_t job, int32_t round, double* normals, int64_t ncount,
+                           double* uniforms, int64_t ucount);
+
 #ifdef __cplusplus
 }
 #endif
