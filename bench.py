@@ -1,4 +1,6 @@
-"""Benchmark: AMIS on a 10-D correlated Gaussian, 50 rounds x 20,000 samples.
+"""Benchmark: AMIS on a 10-D correlated Gaussian, 50 rounds x 20,000 samples
+(default workload); `--workload batch` runs 4,096 linear-regression posteriors
+of uneven size in one persistent launch instead.
 
 One step = one complete AMIS solve (draw, target, the N x T deterministic-mixture
 log-sum-exp, weights, FP64 fit per round) from the initial proposal
@@ -72,6 +74,137 @@ def cpu_baseline(pkg, target, proposal, threads):
                       f"{threads} threads, {dt:.2f} s, ESS {ess:.0f}"}
 
 
+# ---------------------------------------------------------------- batched posteriors
+BATCH_JOBS, BATCH_T, BATCH_N, BATCH_DOF = 4096, 8, 4096, 5.0
+FP32_MFMA_PEAK = 157.3  # TFLOP/s, dense
+
+
+def batch_problem(pkg, njobs=BATCH_JOBS, seed=4096):
+    """4,096 Bayesian linear regressions: n ~ U{20..2000}, d ~ U{2..32}, X ~ N(0,1),
+    sigma = 0.5, prior N(0, 4 I).  Each runs AMIS with Student-t (nu = 5) proposals
+    from a Laplace initialisation (the posterior mode and inverse Hessian, here the
+    conjugate posterior, with the scale inflated 1.5x)."""
+    rng = np.random.default_rng(seed)
+    rngs, targets, algs, flops = [], [], [], 0.0
+    for j in range(njobs):
+        n = int(rng.integers(20, 2001))
+        d = int(rng.integers(2, 33))
+        X = rng.standard_normal((n, d))
+        y = X @ rng.normal(0.0, 2.0, d) + 0.5 * rng.standard_normal(n)
+        prec = X.T @ X / 0.25 + np.eye(d) / 4.0
+        S = np.linalg.inv(prec)
+        m = S @ (X.T @ y) / 0.25
+        L = np.linalg.cholesky(0.5 * (S + S.T))
+        rngs.append(1 + j)
+        targets.append(pkg.LinearRegressionTarget(X, y, 0.5, 2.0))
+        algs.append(pkg.AMIS(pkg.FactorStudentT(BATCH_DOF, m, 1.5 * L), rounds=BATCH_T, round_size=BATCH_N))
+        flops += 2.0 * n * d * BATCH_T * BATCH_N  # target X beta per sample (algorithmic)
+    return rngs, targets, algs, flops
+
+
+def batch_cpu_baseline(pkg, batch, targets, algs, threads, jobs=8):
+    import oracle
+
+    t_cpu, ess = 0.0, 0.0
+    for j in range(jobs):
+        sched = algs[j].schedule
+        d, N = batch.dims[j], sched[0]
+        normals = np.zeros((len(sched), d * N))
+        rn = np.zeros((len(sched), 8 * N))
+        ru = np.zeros((len(sched), 8 * N))
+        for t in range(len(sched)):
+            r = batch.export_randomness(j, t)
+            normals[t], rn[t], ru[t] = r["normals"], r["radial_normals"], r["radial_uniforms"]
+        t0 = time.perf_counter()
+        o = oracle.adaptive(0, algs[j].proposal, sched, normals, targets[j].kind, targets[j].params(d), rn, ru,
+                            nthreads=threads)
+        t_cpu += time.perf_counter() - t0
+        ess += float(o["round_ess"][-1])
+    return {"value": ess / t_cpu, "unit": "effective samples/s", "cores": threads, "kind": "port",
+            "sample": f"the first {jobs} of the {len(algs)} posteriors (each the full {BATCH_T} x {BATCH_N} AMIS), "
+                      f"C++ FP64 oracle, {threads} threads, {t_cpu:.2f} s"}
+
+
+def main_batch(args):
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    t0 = time.perf_counter()
+    rngs, targets, algs, flops = batch_problem(pkg)
+    b64 = pkg.prepare_batch(rngs, targets, algs, precision="fp64")
+    b = pkg.prepare_batch(rngs, targets, algs, precision="fp32")
+    t_prep = time.perf_counter() - t0
+    print(f"batch prepared in {t_prep:.1f} s", file=sys.stderr, flush=True)
+    tune = pkg.autotune(b, candidates=[(w, c, "lpt") for w in (2, 4, 8) for c in (1, 2)], repeats=1)
+    # FP64 reference solve and the FP32 solve on the same counter-RNG streams (no commit)
+    b64.trial(*b64.tuning)
+    b.trial(*b.tuning)
+    lz64, lz32 = b64.lognormalizers(), b.lognormalizers()
+    print(f"tuned {tune}; fp64 and fp32 solves done", file=sys.stderr, flush=True)
+    dlz = float(np.max(np.abs(lz32 - lz64)))
+    del b64
+    for _ in range(args.warmup):
+        b.run()
+    lat, ess_total = [], 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        st = b.run()
+        lat.append(time.perf_counter() - t1)
+        ess_total += float(np.sum(b.final_summary()[0]))
+    elapsed = time.perf_counter() - t_start
+    failed = int(np.count_nonzero(st))
+    b.set_timing(True)
+    kms = []
+    for _ in range(3):
+        b.run()
+        kms.append(b.kernel_ms)
+    b.set_timing(False)
+    kernel_ms = float(np.mean(kms))
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": ess_total / elapsed,
+        "unit": "effective samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched AMIS: {BATCH_JOBS} Bayesian linear-regression posteriors, n ~ U[20, 2000], "
+                        f"d ~ U[2, 32], X ~ N(0,1), sigma 0.5, prior N(0, 4I); {BATCH_T} rounds x {BATCH_N} "
+                        f"Student-t (nu = {BATCH_DOF:g}) samples each, Laplace-initialised; one persistent launch",
+            "precision": "fp32", "fit_and_logz": "fp64",
+            "tuning": {"waves": tune["waves"], "blocks_per_cu": tune["blocks_per_cu"], "order": tune["order"]},
+        },
+        "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+        "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+        "dlogz_vs_fp64": dlz,
+        "failed_jobs": failed,
+        "weighted_samples_per_s": BATCH_JOBS * BATCH_T * BATCH_N * args.steps / elapsed,
+        "prepare_s": t_prep,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP32_MFMA_PEAK,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_MFMA_PEAK,
+            "traffic": None,
+            "kernel": "k_batch_amis (persistent; algorithmic flops = target X beta, 2 n d per sample)",
+            "mean_launch_ms": kernel_ms,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(pkg, b, targets, algs, min(16, os.cpu_count() or 1))
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,11 +212,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="amis", choices=["amis", "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return main_batch(args)
 
     import __graft_entry__
 
