@@ -51,4 +51,5 @@ from .batch import (  # noqa: F401
     importance_sample_batch_,
     prepare_batch,
 )
+from .serve import Server  # noqa: F401
 from . import _lib  # noqa: F401

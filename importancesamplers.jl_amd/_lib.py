@@ -53,6 +53,20 @@ class JobDesc(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_int64), ("max_active", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
+class ServerTiming(ctypes.Structure):
+    _fields_ = [("submit_ns", ctypes.c_int64), ("admit_ns", ctypes.c_int64), ("done_ns", ctypes.c_int64),
+                ("device_bytes", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "failed", "active", "queued", "peak_active",
+                                              "bytes_in_use", "peak_bytes")]
+
+
 class Failure(ctypes.Structure):
     _fields_ = [
         ("round", ctypes.c_int32),
@@ -74,6 +88,7 @@ SIGNATURES = {
     "is_ctx_destroy": (ctypes.c_int, [VP]),
     "is_job_create": (ctypes.c_int, [VP, ctypes.POINTER(JobDesc), ctypes.POINTER(VP)]),
     "is_job_destroy": (ctypes.c_int, [VP]),
+    "is_job_device_bytes": (ctypes.c_int64, [ctypes.POINTER(JobDesc)]),
     "is_job_run": (ctypes.c_int, [VP]),
     "is_job_set_normals": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64]),
     "is_job_set_radial": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64, F64P, ctypes.c_int64]),
@@ -107,6 +122,13 @@ SIGNATURES = {
     "is_select": (ctypes.c_int, [VP, F64P, F64P, ctypes.c_int64, F64P, ctypes.c_int64, ctypes.c_int32, I64P]),
     "is_rng_raw": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_uint64, ctypes.c_int64, U64P]),
+    "is_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
+    "is_server_submit": (ctypes.c_int, [VP, ctypes.POINTER(JobDesc), U64P]),
+    "is_server_poll": (ctypes.c_int, [VP, ctypes.c_uint64, I32P]),
+    "is_server_wait": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_int32, I32P]),
+    "is_server_take": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(VP), ctypes.POINTER(ServerTiming)]),
+    "is_server_get_stats": (ctypes.c_int, [VP, ctypes.POINTER(ServerStats)]),
+    "is_server_destroy": (ctypes.c_int, [VP]),
     "is_batch_create": (ctypes.c_int, [VP, ctypes.POINTER(JobDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(VP)]),
     "is_batch_destroy": (ctypes.c_int, [VP]),
     "is_batch_run": (ctypes.c_int, [VP]),

@@ -474,22 +474,34 @@ class PreparedSampler:
         lib = _lib.load()
         ctx = context()
         desc, keep, p, sched = _job_desc(rng, target, algorithm, precision)
+        h = VP()
+        _check(lib.is_job_create(ctx, ctypes.byref(desc), ctypes.byref(h)))
+        self._adopt(h, desc, keep, p, sched)
+        if isinstance(rng, PrefilledRNG):
+            self.set_prefilled(rng)
+
+    def _adopt(self, handle, desc, keep, p, sched):
         self._keep = keep
         self.proposal0 = p
         self.dim = p.dim
         self.scalar = bool(p.scalar)
         self.schedule = list(sched)
-        h = VP()
-        _check(lib.is_job_create(ctx, ctypes.byref(desc), ctypes.byref(h)))
-        self._job = h
-        self._lib = lib
-        self.total = int(lib.is_job_total(h))
+        self._job = handle
+        self._lib = _lib.load()
+        self.total = int(self._lib.is_job_total(handle))
         if desc.method == DMPMC_M:
-            order = sorted([i for i, m in enumerate(algorithm.bank.masses) if m != 0],
-                           key=lambda i: algorithm.bank.masses[i])
+            order = sorted([i for i, m in enumerate(self.algorithm.bank.masses) if m != 0],
+                           key=lambda i: self.algorithm.bank.masses[i])
             self.active_ids = [i + 1 for i in order]
-        if isinstance(rng, PrefilledRNG):
-            self.set_prefilled(rng)
+
+    @classmethod
+    def from_handle(cls, handle, rng, target, algorithm, precision):
+        """Wraps a job created elsewhere (the serving loop) with the same descriptor."""
+        self = cls.__new__(cls)
+        self.algorithm, self.target, self.precision = algorithm, target, precision
+        desc, keep, p, sched = _job_desc(rng, target, algorithm, precision)
+        self._adopt(handle, desc, keep, p, sched)
+        return self
 
     def __del__(self):
         try:

@@ -1,6 +1,7 @@
 """Compiles libissampler.so for gfx950 with hipcc (in-tree).
 
 The host engine (csrc/engine.hip), the batched AMIS host side (csrc/batch.hip),
+the serving loop (csrc/serve.hip),
 one object per dimension instantiation (csrc/dim.hip with -DIS_DIM=D, D in the
 IS_DIMS list of csrc/launch.h) and one per batched-kernel shape (csrc/batch_k.hip
 with -DIS_BATCH_FP and -DIS_BATCH_W) are compiled in parallel, then linked into
@@ -49,7 +50,8 @@ def build(verbose=False, jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     os.makedirs(OBJDIR, exist_ok=True)
     units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), []),
-             (os.path.join(CSRC, "batch.hip"), os.path.join(OBJDIR, "batch.o"), [])]
+             (os.path.join(CSRC, "batch.hip"), os.path.join(OBJDIR, "batch.o"), []),
+             (os.path.join(CSRC, "serve.hip"), os.path.join(OBJDIR, "serve.o"), [])]
     for fp in (32, 64):
         for w in BATCH_WAVES:
             units.append((os.path.join(CSRC, "batch_k.hip"), os.path.join(OBJDIR, f"batch_k{fp}_w{w}.o"),

@@ -955,6 +955,41 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   return IS_OK;
 }
 
+// Device bytes a job of this descriptor allocates (alloc_job_buffers plus the
+// target data and the DM-PMC plan); the serving loop admits jobs against it.
+int64_t is_job_device_bytes(const is_job_desc* desc) {
+  if (!desc || !desc->round_size || desc->dim < 1 || desc->dim > kMaxDim) return -1;
+  const int d = desc->dim;
+  const int T = desc->method == IS_PLAIN ? 1 : desc->rounds;
+  if (T < 1) return -1;
+  int64_t total = 0, maxN = 0;
+  for (int t = 0; t < T; ++t) {
+    if (desc->round_size[t] <= 0) return -1;
+    total += desc->round_size[t];
+    maxN = std::max<int64_t>(maxN, desc->round_size[t]);
+  }
+  const int64_t ts = desc->precision == IS_FP32 ? 4 : 8;
+  const int stride = slot_stride(d);
+  const int K = desc->method == IS_DMPMC ? std::max(1, desc->bank_size) : 1;
+  const int nslots = desc->method == IS_DMPMC ? K : desc->method == IS_PLAIN ? 1 : T + 1;
+  const int nv_mom = 2 + d + d * (d + 1) / 2;
+  const int64_t pcap = cdiv(total, kSweepPerBlock) + 1;
+  int64_t bytes = total * (d * ts + 3 * ts + 2 * 4);         // samples, log arrays, ids
+  bytes += (int64_t)nslots * stride * (8 + ts);               // slot tables
+  bytes += 2 * pcap * (1 + nv_mom) * 8 + 2 * (1 + nv_mom) * 8; // partial records, reduced records
+  bytes += (int64_t)T * (5 * 8 + 3 * 8 + 2 * 4 + 3 * 8) + 4 * 64;
+  if (desc->target_kind == IS_TARGET_LOGISTIC || desc->target_kind == IS_TARGET_LINREG) {
+    const int64_t n = desc->target_params && desc->target_params_len > 0 ? (int64_t)desc->target_params[0] : 0;
+    bytes += n * (d + 1) * ts + maxN * ts;
+    bytes += std::max<int64_t>(1, target_splits(n, maxN) * maxN) * 8;
+  }
+  if (desc->method == IS_DMPMC) {
+    bytes += (int64_t)T * maxN * 4 + maxN * 8 + (int64_t)K * T * 12;
+    bytes += (cdiv(maxN, kCdfThreads * kCdfPer) + 1) * 8;
+  }
+  return bytes;
+}
+
 int is_job_destroy(is_job* job) {
   if (!job) return IS_OK;
   if (job->stream) (void)hipStreamSynchronize(job->stream);

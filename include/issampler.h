@@ -126,6 +126,9 @@ const char* is_last_error(void);
  * (src/methods/importance_sampling.jl:357-404, 599-719). */
 int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out);
 int is_job_destroy(is_job* job);
+/* Device memory (bytes) a job of this descriptor allocates, or -1 for an invalid
+ * descriptor; no reference counterpart (the serving loop's admission estimate). */
+int64_t is_job_device_bytes(const is_job_desc* desc);
 
 /* importance_sample! -> _execute_prepared_sampler! (src/methods/importance_sampling.jl:912-953);
  * adaptive loops src/adaptive_gaussian_execution.jl:897-1112 and src/population_execution.jl:407-567.
@@ -245,6 +248,41 @@ int is_batch_job_proposal(is_batch* batch, int32_t job, double* location, double
 int is_batch_export_normals(is_batch* batch, int32_t job, int32_t round, double* out, int64_t count);
 int is_batch_export_radial(is_batch* batch, int32_t job, int32_t round, double* normals, int64_t ncount,
                            double* uniforms, int64_t ucount);
+
+/* ---- serving: a request loop with continuous batching -----------------------------
+ * Requests are job descriptors (any method); each is copied at submission, admitted
+ * in arrival order when one of `max_active` execution slots is free and its device
+ * memory (is_job_device_bytes) fits the budget beside every admitted and untaken job,
+ * created (its proposal slots, mixture sums and samples stay resident on the device)
+ * and run on its own stream, so jobs in flight overlap on the GPU.  Results are those
+ * of the same descriptor run alone.  No reference counterpart: the reference runs
+ * importance_sample! (src/methods/importance_sampling.jl:912-930) one call at a time. */
+typedef struct is_server is_server;
+enum is_request_state { IS_REQ_QUEUED = 0, IS_REQ_RUNNING = 1, IS_REQ_DONE = 2, IS_REQ_FAILED = 3 };
+typedef struct is_server_config {
+  int64_t memory_budget;   /* bytes of device memory admitted jobs may hold at once */
+  int32_t max_active;      /* jobs running at once (1..64) */
+  int32_t use_graph;       /* replay each job's launches from a captured graph (0: eager launches) */
+} is_server_config;
+typedef struct is_server_timing {
+  int64_t submit_ns, admit_ns, done_ns;  /* steady-clock times of the request */
+  int64_t device_bytes;                  /* admission estimate */
+} is_server_timing;
+typedef struct is_server_stats {
+  int64_t submitted, completed, failed, active, queued, peak_active, bytes_in_use, peak_bytes;
+} is_server_stats;
+int is_server_create(is_ctx* ctx, const is_server_config* cfg, is_server** out);
+/* IS_ENOMEM when the job alone exceeds the budget */
+int is_server_submit(is_server* server, const is_job_desc* desc, uint64_t* ticket);
+int is_server_poll(is_server* server, uint64_t ticket, int32_t* state);
+/* waits until the request finished (timeout_ms < 0: no limit); *state as of the return */
+int is_server_wait(is_server* server, uint64_t ticket, int32_t timeout_ms, int32_t* state);
+/* hands the finished job (results, failure record) to the caller, who destroys it with
+ * is_job_destroy; returns the job's run status (IS_EFAILED: see is_job_failure) */
+int is_server_take(is_server* server, uint64_t ticket, is_job** job, is_server_timing* timing);
+int is_server_get_stats(is_server* server, is_server_stats* out);
+/* finishes the admitted requests, drops the queued ones, frees untaken jobs */
+int is_server_destroy(is_server* server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,131 @@
+"""GPU tests of the serving loop (continuous batching): a concurrent mix of
+plain IS, AMIS, NPMC and DM-PMC requests gives, for every request, the same
+results as the request run alone (bit-identical log-weights and log Z);
+admission keeps the admitted jobs within the memory budget; oversized
+requests are refused; a failing request raises its own error while the rest of
+the mix completes."""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng(pkg):
+    pkg.context(0)
+    return pkg
+
+
+def _gauss_target(eng, rng, d):
+    A = rng.standard_normal((d, d))
+    S = A @ A.T / d + np.eye(d)
+    return eng.GaussianTarget(rng.normal(0, 1, d), np.linalg.cholesky(S))
+
+
+def _logistic(eng, rng, d, n):
+    X = rng.standard_normal((n, d))
+    beta = rng.normal(0, 1 / np.sqrt(d), d)
+    y = (rng.random(n) < 1 / (1 + np.exp(-X @ beta))).astype(float)
+    return eng.LogisticRegressionTarget(X, y, 10.0)
+
+
+def request_mix(eng, seed=3):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    reqs.append((11, eng.StandardGaussianTarget(), eng.ImportanceSampling(eng.SphericalGaussian(np.zeros(3), 1.5), 20000),
+                 "fp64"))
+    reqs.append((12, _gauss_target(eng, rng, 10), eng.AMIS(eng.FactorGaussian(np.zeros(10), 4 * np.eye(10)), 8, 5000),
+                 "fp32"))
+    reqs.append((13, _gauss_target(eng, rng, 5), eng.NPMC(eng.DiagonalStudentT(5.0, np.zeros(5), np.full(5, 2.0)), 6, 4000),
+                 "fp64"))
+    reqs.append((14, _logistic(eng, rng, 6, 500), eng.AMIS(eng.SphericalGaussian(np.zeros(6), 1.0), 5, 3000), "fp32"))
+    reqs.append((15, _logistic(eng, rng, 12, 1000),
+                 eng.ImportanceSampling(eng.SphericalStudentT(5.0, np.zeros(12), 0.5), 8192), "fp64"))
+    bank = eng.ProposalBank([eng.SphericalGaussian(rng.normal(0, 2, 4), 1.0) for _ in range(8)])
+    reqs.append((16, _gauss_target(eng, rng, 4), eng.DeterministicMixturePMC(bank, 4, 2048), "fp64"))
+    reqs.append((17, _gauss_target(eng, rng, 24), eng.AMIS(eng.SphericalStudentT(6.0, np.zeros(24), 3.0), 6, 4096),
+                 "fp32"))
+    reqs.append((18, eng.StandardGaussianTarget(), eng.ImportanceSampling(eng.SphericalGaussian(np.zeros(40), 1.2), 50000),
+                 "fp32"))
+    return reqs
+
+
+def test_serve_results_same_alone_and_in_mix(eng):
+    _say("case: serving mix of 8 requests (plain IS, AMIS, NPMC, DM-PMC; FP32 and FP64), 4 at once")
+    reqs = request_mix(eng)
+    srv = eng.Server(memory_budget=64 << 30, max_active=4)
+    tickets = [srv.submit(r, t, a, p) for r, t, a, p in reqs]
+    mixed = [srv.result(tk, timeout=120) for tk in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert 1 <= st["peak_active"] <= 4
+    for (r, t, a, p), (res, timing) in zip(reqs, mixed):
+        alone = eng.importance_sample(r, t, a, precision=p)
+        assert np.array_equal(res.logweights, alone.logweights), type(a).__name__
+        assert np.array_equal(res.samples, alone.samples)
+        assert eng.lognormalizer(res) == eng.lognormalizer(alone)
+        assert timing["latency_s"] >= timing["queue_s"] >= 0.0
+
+
+def test_serve_admission_respects_memory_budget(eng):
+    _say("case: serving admission against a small memory budget")
+    reqs = request_mix(eng, seed=5)
+    lib = eng._lib.load()
+    from importancesamplers_jl_amd.api import _job_desc, FP32, FP64
+
+    sizes = []
+    for r, t, a, p in reqs:
+        desc, keep, _, _ = _job_desc(r, t, a, FP32 if p == "fp32" else FP64)
+        sizes.append(int(lib.is_job_device_bytes(ctypes.byref(desc))))
+    assert min(sizes) > 0
+    budget = int(1.2 * max(sizes))
+    srv = eng.Server(memory_budget=budget, max_active=8)
+    tickets = [srv.submit(r, t, a, p) for r, t, a, p in reqs]
+    for tk in tickets:  # taking a result releases its memory for the queued requests
+        srv.result(tk, timeout=120)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs)
+    assert st["peak_bytes"] <= budget
+    assert st["bytes_in_use"] == 0
+
+
+def test_serve_refuses_request_over_budget(eng):
+    _say("case: serving refuses a request larger than the budget")
+    srv = eng.Server(memory_budget=1 << 20, max_active=2)
+    with pytest.raises(MemoryError):
+        srv.submit(1, eng.StandardGaussianTarget(), eng.ImportanceSampling(eng.SphericalGaussian(np.zeros(8), 1.0),
+                                                                          1_000_000))
+    srv.close()
+
+
+def test_serve_failing_request_in_mix(eng):
+    _say("case: serving mix with one failing AMIS request")
+    reqs = request_mix(eng, seed=7)[:4]
+    bad = (99, eng.GaussianTarget(np.zeros(3), np.eye(3)),
+           eng.AMIS(eng.SphericalGaussian(np.zeros(3), 1e200), 3, 256), "fp64")
+    reqs.insert(2, bad)
+    srv = eng.Server(memory_budget=64 << 30, max_active=3)
+    tickets = [srv.submit(r, t, a, p) for r, t, a, p in reqs]
+    for i, tk in enumerate(tickets):
+        if i == 2:
+            with pytest.raises(eng.AMISRoundError) as ei:
+                srv.result(tk, timeout=120)
+            assert ei.value.round == 1
+        else:
+            res, _ = srv.result(tk, timeout=120)
+            r, t, a, p = reqs[i]
+            assert np.array_equal(res.logweights, eng.importance_sample(r, t, a, precision=p).logweights)
+    st = srv.stats()
+    srv.close()
+    assert st["failed"] == 1 and st["completed"] == len(reqs) - 1
