@@ -212,6 +212,123 @@ def main_batch(args):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------- serving
+SERVE_REQUESTS, SERVE_RATE, SERVE_BUDGET, SERVE_SLOTS = 32, 2.0, 200 << 30, 8
+
+
+def serve_stream(pkg, nreq=SERVE_REQUESTS, rate=SERVE_RATE, seed=555):
+    """Poisson arrivals of mixed requests: plain IS, AMIS and NPMC on Gaussian
+    (correlated, as the AMIS workload) or logistic (n = 2,000 rows, prior N(0, 10 I))
+    targets; d ~ U{2..64}, N log-uniform on [1e3, 1e6] samples (per round for the
+    adaptive methods), T ~ U{5..50} rounds."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs = []
+    for k in range(nreq):
+        d = int(rng.integers(2, 65))
+        n_samp = int(round(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))))
+        T = int(rng.integers(5, 51))
+        method = ("plain", "amis", "npmc")[int(rng.integers(0, 3))]
+        if rng.random() < 0.5:
+            G = rng.standard_normal((d, d))
+            Q, R = np.linalg.qr(G)
+            Q = Q * np.sign(np.diag(R))
+            lam = np.exp(rng.uniform(math.log(0.1), math.log(10.0), d))
+            target = pkg.GaussianTarget(rng.normal(0.0, 2.0, d), np.linalg.cholesky(Q @ np.diag(lam) @ Q.T))
+            q0 = pkg.FactorGaussian(np.zeros(d), 5.0 * np.eye(d))
+        else:
+            X = rng.standard_normal((2000, d))
+            beta = rng.normal(0.0, 1.0 / math.sqrt(d), d)
+            y = (rng.random(2000) < 1.0 / (1.0 + np.exp(-X @ beta))).astype(float)
+            target = pkg.LogisticRegressionTarget(X, y, 10.0)
+            q0 = pkg.FactorGaussian(np.zeros(d), 0.5 * np.eye(d))
+        if method == "plain":
+            alg = pkg.ImportanceSampling(q0, n_samp)
+        else:
+            alg = (pkg.AMIS if method == "amis" else pkg.NPMC)(q0, T, n_samp)
+        reqs.append((1000 + k, target, alg))
+    return arrivals, reqs
+
+
+def serve_run(pkg, arrivals, reqs, precision, slots):
+    """Submits each request at its arrival time; returns per-request latency, samples and ESS."""
+    import threading
+
+    srv = pkg.Server(memory_budget=SERVE_BUDGET, max_active=slots)
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for k, (r, t, a) in enumerate(reqs):
+            delay = arrivals[k] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets[k] = srv.submit(r, t, a, precision)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, ess, samples, ok = [], 0.0, 0, True
+    last_done = 0
+    first_submit = None
+    for k in range(len(reqs)):
+        while tickets[k] is None:
+            time.sleep(0.0005)
+        e, z, timing, good = srv.summary(tickets[k])
+        ok &= good
+        lat.append(timing["latency_s"])
+        ess += float(e[-1]) if good else 0.0
+        samples += int(sum(getattr(reqs[k][2], "schedule", [getattr(reqs[k][2], "nsamples", 0)])))
+        first_submit = timing["submit_ns"] if first_submit is None else min(first_submit, timing["submit_ns"])
+        last_done = max(last_done, timing["done_ns"])
+        print(f"request {k} done: latency {timing['latency_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    th.join()
+    stats = srv.stats()
+    srv.close()
+    span = (last_done - first_submit) * 1e-9
+    return {"p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "requests_per_s": len(reqs) / span, "weighted_samples_per_s": samples / span,
+            "effective_samples_per_s": ess / span, "span_s": span, "all_ok": bool(ok),
+            "peak_active": stats["peak_active"], "peak_bytes": stats["peak_bytes"]}
+
+
+def main_serve(args):
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    arrivals, reqs = serve_stream(pkg)
+    served = serve_run(pkg, arrivals, reqs, "fp32", SERVE_SLOTS)
+    plain = serve_run(pkg, arrivals, reqs, "fp64", 1)
+    out = {
+        "metric": METRIC,
+        "value": served["effective_samples_per_s"],
+        "unit": "effective samples/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": 1e3 * served["span_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"serving: {SERVE_REQUESTS} Poisson arrivals at {SERVE_RATE:g}/s of plain IS, AMIS and NPMC "
+                        "requests on Gaussian or logistic (n = 2000) targets, d ~ U[2, 64], N log-uniform "
+                        f"[1e3, 1e6] per round, T ~ U[5, 50]; {SERVE_SLOTS} execution slots, 200 GiB admission "
+                        "budget; baseline: the same arrivals run one at a time in FP64",
+            "precision": "fp32", "fit_and_logz": "fp64",
+        },
+        "p50_ms": served["p50_ms"],
+        "p99_ms": served["p99_ms"],
+        "served": served,
+        "unbatched_fp64_one_at_a_time": plain,
+        "p99_speedup_vs_unbatched": plain["p99_ms"] / served["p99_ms"],
+        "throughput_ratio_vs_unbatched": served["requests_per_s"] / plain["requests_per_s"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,7 +336,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="amis", choices=["amis", "batch"])
+    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve"])
     ap.add_argument("--tuning", default=None, help="batch workload: 'waves,blocks_per_cu' instead of autotuning")
     ap.add_argument("--skip-fp64-check", action="store_true", help="batch workload: skip the FP64 reference solve")
     args = ap.parse_args()
@@ -229,6 +346,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return main_batch(args)
+    if args.workload == "serve":
+        return main_serve(args)
 
     import __graft_entry__
 

@@ -75,6 +75,23 @@ class Server:
             sampler._raise_failure()
         return sampler.collect(), timing
 
+    def summary(self, ticket, timeout=None):
+        """Waits for the request, returns (round ESS, round log Z, timing, ok) and frees
+        the job without copying its samples to the host."""
+        import numpy as np
+
+        from ._lib import F64P
+
+        self.wait(ticket, timeout)
+        sampler, timing, ok = self.take(ticket)
+        T = len(sampler.schedule)
+        ess, lz, aess = np.zeros(T), np.zeros(T), np.zeros(T)
+        if ok:
+            _check(sampler._lib.is_job_get_round_summary(sampler._job, ess.ctypes.data_as(F64P),
+                                                         lz.ctypes.data_as(F64P), aess.ctypes.data_as(F64P)))
+        del sampler  # destroys the job and its device buffers
+        return ess, lz, timing, ok
+
     def stats(self):
         s = ServerStats()
         _check(self._lib.is_server_get_stats(self._h, ctypes.byref(s)))
