@@ -20,6 +20,9 @@ import sys
 import time
 
 import numpy as np
+import ctypes
+
+F64P_ = ctypes.POINTER(ctypes.c_double)
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -329,6 +332,138 @@ def main_serve(args):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------- plain IS (config 1) and DM-PMC (config 3)
+def _solve_stats(pkg, sampler, steps, warmup, ess_fn):
+    for _ in range(warmup):
+        sampler.run()
+    lat, ess_total = [], 0.0
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        sampler.run()
+        lat.append(time.perf_counter() - t0)
+        ess_total += ess_fn(sampler)
+    elapsed = time.perf_counter() - t_start
+    return ess_total / elapsed, elapsed, lat
+
+
+def _round_ess(sampler):
+    T_ = len(sampler.schedule)
+    ess, lz = np.zeros(T_), np.zeros(T_)
+    sampler._lib.is_job_get_round_summary(sampler._job, ess.ctypes.data_as(F64P_), lz.ctypes.data_as(F64P_), None)
+    return ess, lz
+
+
+def main_plain(args):
+    """Config 1: plain IS, d = 3, -|x|^2/2 under SphericalGaussian(0, 1.5), N = 10,000;
+    true log Z = 1.5 log(2 pi).  One step = one importance_sample! call."""
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    q = pkg.SphericalGaussian(np.zeros(3), 1.5)
+    alg = pkg.ImportanceSampling(q, 10_000)
+    tgt = pkg.StandardGaussianTarget()
+    res = {}
+    for prec in ("fp64", "fp32"):
+        s = pkg.prepare_sampler(42, tgt, alg, precision=prec)
+        ess_s, elapsed, lat = _solve_stats(pkg, s, args.steps * 10, args.warmup, lambda x: float(_round_ess(x)[0][0]))
+        res[prec] = {"ess_per_s": ess_s, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                     "p99_ms": 1e3 * float(np.percentile(lat, 99)), "logz": float(_round_ess(s)[1][0]),
+                     "ms_per_step": 1e3 * elapsed / (args.steps * 10)}
+    true_lz = 1.5 * math.log(2 * math.pi)
+    out = _line("plain importance sampling, d=3, -|x|^2/2 under SphericalGaussian(0, 1.5), N=10,000 "
+                "(latency-bound: one fused launch plus a summary per step)", "f32", res["fp32"], args.steps * 10,
+                args.warmup)
+    out.update({"logz_fp32": res["fp32"]["logz"], "logz_fp64": res["fp64"]["logz"], "logz_true": true_lz,
+                "dlogz_vs_fp64": abs(res["fp32"]["logz"] - res["fp64"]["logz"]),
+                "fp64_plain_form": res["fp64"],
+                "speedup_vs_fp64_plain_form": res["fp32"]["ess_per_s"] / res["fp64"]["ess_per_s"]})
+    print(json.dumps(out))
+
+
+def _line(workload, dtype, r, steps, warmup):
+    return {"metric": METRIC, "value": r["ess_per_s"], "unit": "effective samples/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": {"workload": workload},
+            "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"]}
+
+
+def dmpmc_problem(pkg, seed=51):
+    """Config 3: X 100,000 x 50 N(0,1) plus an intercept column (p = 51); beta ~ N(0, I/50);
+    y ~ Bernoulli(sigmoid(X beta)); prior N(0, 10 I).  Bank: K = 64 Gaussian proposals from the
+    Laplace approximation (MAP by Newton, L the factor of the inverse Hessian): factors 0.5 L
+    and centres MAP + 0.87 L z, so the mixture's spread matches the posterior's."""
+    rng = np.random.default_rng(seed)
+    n, p = 100_000, 51
+    X = np.hstack([rng.standard_normal((n, p - 1)), np.ones((n, 1))])
+    beta = rng.normal(0.0, math.sqrt(1.0 / 50.0), p)
+    y = (rng.random(n) < 1.0 / (1.0 + np.exp(-X @ beta))).astype(float)
+    b = np.zeros(p)
+    for _ in range(25):
+        mu = 1.0 / (1.0 + np.exp(-X @ b))
+        g = X.T @ (y - mu) - b / 10.0
+        H = (X * (mu * (1 - mu))[:, None]).T @ X + np.eye(p) / 10.0
+        step = np.linalg.solve(H, g)
+        b = b + step
+        if np.max(np.abs(step)) < 1e-10:
+            break
+    L = np.linalg.cholesky(np.linalg.inv(H))
+    props = [pkg.FactorGaussian(b + 0.87 * (L @ rng.standard_normal(p)), 0.5 * L) for _ in range(64)]
+    bank = pkg.ProposalBank(props, np.ones(64))
+    alg = pkg.DeterministicMixturePMC(bank, rounds=30, round_size=64 * 512, method=pkg.SystematicResampling())
+    return pkg.LogisticRegressionTarget(X, y, 10.0), alg, n, p
+
+
+def main_dmpmc(args):
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    tgt, alg, n, p = dmpmc_problem(pkg)
+    S, T_ = 64 * 512, 30
+    flops = 2.0 * n * p * S * T_  # target X beta per solve (algorithmic)
+
+    def full_ess(sampler):
+        lw = np.zeros(sampler.total)
+        sampler._lib.is_job_get_logweights(sampler._job, lw.ctypes.data_as(F64P_))
+        w = np.exp(lw - lw.max())
+        return float(w.sum() ** 2 / (w * w).sum())
+
+    res, lz = {}, {}
+    for prec in ("fp64", "fp32"):
+        s = pkg.prepare_sampler(3, tgt, alg, precision=prec)
+        ess_s, elapsed, lat = _solve_stats(pkg, s, args.steps, args.warmup, full_ess)
+        out3 = np.zeros(3)
+        s._lib.is_job_lognormalizer(s._job, out3.ctypes.data_as(F64P_))
+        lz[prec] = float(out3[0])
+        s._lib.is_job_set_timing(s._job, 1)
+        s.reset()
+        s.run()
+        ms = np.zeros(T_)
+        s._lib.is_job_round_kernel_ms(s._job, ms.ctypes.data_as(F64P_))
+        s._lib.is_job_set_timing(s._job, 0)
+        res[prec] = {"ess_per_s": ess_s, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                     "p99_ms": 1e3 * float(np.percentile(lat, 99)), "ms_per_step": 1e3 * elapsed / args.steps,
+                     "round_phase_ms": float(np.sum(ms))}
+        del s
+    peak = 157.3
+    achieved = flops / (res["fp32"]["round_phase_ms"] * 1e-3) / 1e12
+    out = _line("DM-PMC on a Bayesian logistic regression: X 100,000 x 51 (50 N(0,1) columns + intercept), "
+                "beta ~ N(0, I/50), prior N(0, 10I); K = 64 Gaussian proposals x 512 samples, 30 rounds, "
+                "systematic resampling; bank of 0.5 L factors around Laplace draws", "f32", res["fp32"], args.steps,
+                args.warmup)
+    out.update({"logz_fp32": lz["fp32"], "logz_fp64": lz["fp64"], "dlogz_vs_fp64": abs(lz["fp32"] - lz["fp64"]),
+                "fp64_plain_form": res["fp64"],
+                "speedup_vs_fp64_plain_form": res["fp32"]["ess_per_s"] / res["fp64"]["ess_per_s"],
+                "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                             "frac": achieved / peak, "traffic": None,
+                             "kernel": "per-round draw + target GEMM (X beta, 2 n p per sample) + mixture phase; "
+                                       "achieved = GEMM flops / that phase's event time (a lower bound for the GEMM)",
+                             "mean_launch_ms": res["fp32"]["round_phase_ms"] / T_}})
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,7 +471,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve"])
+    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc"])
     ap.add_argument("--tuning", default=None, help="batch workload: 'waves,blocks_per_cu' instead of autotuning")
     ap.add_argument("--skip-fp64-check", action="store_true", help="batch workload: skip the FP64 reference solve")
     args = ap.parse_args()
@@ -348,6 +483,10 @@ def main():
         return main_batch(args)
     if args.workload == "serve":
         return main_serve(args)
+    if args.workload == "plain":
+        return main_plain(args)
+    if args.workload == "dmpmc":
+        return main_dmpmc(args)
 
     import __graft_entry__
 
