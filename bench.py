@@ -391,9 +391,11 @@ def _line(workload, dtype, r, steps, warmup):
 
 def dmpmc_problem(pkg, seed=51):
     """Config 3: X 100,000 x 50 N(0,1) plus an intercept column (p = 51); beta ~ N(0, I/50);
-    y ~ Bernoulli(sigmoid(X beta)); prior N(0, 10 I).  Bank: K = 64 Gaussian proposals from the
-    Laplace approximation (MAP by Newton, L the factor of the inverse Hessian): factors 0.5 L
-    and centres MAP + 0.87 L z, so the mixture's spread matches the posterior's."""
+    y ~ Bernoulli(sigmoid(X beta)); prior N(0, 10 I).  Bank: K = 64 copies of the Laplace
+    approximation (MAP by Newton, L the factor of the inverse Hessian), so round 0 is near-exact
+    importance sampling; systematic resampling then moves the 64 centres over the posterior.
+    With fixed scales the deterministic mixture degenerates after the first resampling in 51
+    dimensions (round ESS falls from ~32,600 to a few), for any bank scale tried (0.5-1.5 L)."""
     rng = np.random.default_rng(seed)
     n, p = 100_000, 51
     X = np.hstack([rng.standard_normal((n, p - 1)), np.ones((n, 1))])
@@ -409,7 +411,7 @@ def dmpmc_problem(pkg, seed=51):
         if np.max(np.abs(step)) < 1e-10:
             break
     L = np.linalg.cholesky(np.linalg.inv(H))
-    props = [pkg.FactorGaussian(b + 0.87 * (L @ rng.standard_normal(p)), 0.5 * L) for _ in range(64)]
+    props = [pkg.FactorGaussian(b, L) for _ in range(64)]
     bank = pkg.ProposalBank(props, np.ones(64))
     alg = pkg.DeterministicMixturePMC(bank, rounds=30, round_size=64 * 512, method=pkg.SystematicResampling())
     return pkg.LogisticRegressionTarget(X, y, 10.0), alg, n, p
@@ -430,7 +432,7 @@ def main_dmpmc(args):
         w = np.exp(lw - lw.max())
         return float(w.sum() ** 2 / (w * w).sum())
 
-    res, lz = {}, {}
+    res, lz, rounds = {}, {}, {}
     for prec in ("fp64", "fp32"):
         s = pkg.prepare_sampler(3, tgt, alg, precision=prec)
         ess_s, elapsed, lat = _solve_stats(pkg, s, args.steps, args.warmup, full_ess)
@@ -440,6 +442,10 @@ def main_dmpmc(args):
         s._lib.is_job_set_timing(s._job, 1)
         s.reset()
         s.run()
+        # a run from the initial bank: the precisions draw the same randomness
+        rlz = np.zeros(T_)
+        s._lib.is_job_get_round_summary(s._job, None, rlz.ctypes.data_as(F64P_), None)
+        rounds[prec] = (rlz, s.ancestors())
         ms = np.zeros(T_)
         s._lib.is_job_round_kernel_ms(s._job, ms.ctypes.data_as(F64P_))
         s._lib.is_job_set_timing(s._job, 0)
@@ -451,11 +457,24 @@ def main_dmpmc(args):
     achieved = flops / (res["fp32"]["round_phase_ms"] * 1e-3) / 1e12
     out = _line("DM-PMC on a Bayesian logistic regression: X 100,000 x 51 (50 N(0,1) columns + intercept), "
                 "beta ~ N(0, I/50), prior N(0, 10I); K = 64 Gaussian proposals x 512 samples, 30 rounds, "
-                "systematic resampling; bank of 0.5 L factors around Laplace draws", "f32", res["fp32"], args.steps,
+                "systematic resampling; bank of 64 copies of the Laplace approximation", "f32", res["fp32"], args.steps,
                 args.warmup)
-    out.update({"logz_fp32": lz["fp32"], "logz_fp64": lz["fp64"], "dlogz_vs_fp64": abs(lz["fp32"] - lz["fp64"]),
+    # rounds 0..t* draw from identical proposal banks, t* the first round whose
+    # resampled ancestors differ between the precisions (after it the runs
+    # follow different adaptive trajectories and differ by Monte Carlo error)
+    (rlz64, anc64), (rlz32, anc32) = rounds["fp64"], rounds["fp32"]
+    diff = [t for t in range(T_) if not np.array_equal(anc64[t], anc32[t])]
+    shared = (diff[0] + 1) if diff else T_
+    dlz_rounds = np.abs(rlz32 - rlz64)
+    out.update({"dlogz_vs_fp64": float(dlz_rounds[:shared].max()),
+                "dlogz_vs_fp64_definition": "max over rounds with identical proposal banks of |round log Z FP32 - "
+                                            "FP64|",
+                "rounds_with_identical_banks": shared,
+                "logz_fp32": lz["fp32"], "logz_fp64": lz["fp64"], "dlogz_final": abs(lz["fp32"] - lz["fp64"]),
                 "fp64_plain_form": res["fp64"],
                 "speedup_vs_fp64_plain_form": res["fp32"]["ess_per_s"] / res["fp64"]["ess_per_s"],
+                "solve_time_speedup_vs_fp64_plain_form": res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"],
+                "weighted_samples_per_s": S * T_ / (res["fp32"]["ms_per_step"] * 1e-3),
                 "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                              "frac": achieved / peak, "traffic": None,
                              "kernel": "per-round draw + target GEMM (X beta, 2 n p per sample) + mixture phase; "

@@ -112,6 +112,7 @@ SIGNATURES = {
     "is_job_get_locations": (ctypes.c_int, [VP, F64P]),
     "is_job_get_ancestors": (ctypes.c_int, [VP, I64P]),
     "is_job_device_results": (ctypes.c_int, [VP, ctypes.POINTER(VP), I64P, ctypes.POINTER(VP), I32P]),
+    "is_job_logweight_offset": (ctypes.c_int, [VP, F64P]),
     "is_job_reset": (ctypes.c_int, [VP]),
     "is_job_lognormalizer": (ctypes.c_int, [VP, F64P]),
     "is_job_moments": (ctypes.c_int, [VP, F64P, F64P, F64P]),

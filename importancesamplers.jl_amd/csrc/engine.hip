@@ -117,6 +117,7 @@ struct is_job {
   double t_inv_two_var = 0.0;            // linear regression 1 / (2 sigma^2)
   double t_inv_two_prior = 0.0;          // 1 / (2 prior variance)
   double t_const = 0.0;                  // log normalising constants
+  double t_shift = 0.0;                  // log-target offset kept off the device values (FP32 regression)
   std::vector<double> tdata;             // X (n x p row-major) then y (n), FP64, until upload
   DevBuf Xd, yd, tpre, tpart;
   // device buffers
@@ -231,6 +232,31 @@ int64_t target_splits(int64_t n_rows, int64_t n_new) {
 }
 
 // ---------------------------------------------------------------- job helpers
+// log pi(beta) of the job's regression target at one point, FP64 on the host:
+// logistic sum_i y_i eta_i - log(1 + e^eta_i), or linear -(y_i - eta_i)^2 / (2
+// sigma^2), plus the Gaussian log prior and the normalising constants.
+double regression_logtarget(const is_job* j, const double* beta) {
+  const int64_t n = j->tn;
+  const int p = j->d;
+  const double* X = j->tdata.data();
+  const double* y = X + n * p;
+  double ll = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double eta = 0.0;
+    for (int c = 0; c < p; ++c) eta += X[i * p + c] * beta[c];
+    if (j->tkind == IS_TARGET_LOGISTIC) {
+      const double sp = eta > 0.0 ? eta + std::log1p(std::exp(-eta)) : std::log1p(std::exp(eta));
+      ll += y[i] * eta - sp;
+    } else {
+      const double r = y[i] - eta;
+      ll -= r * r * j->t_inv_two_var;
+    }
+  }
+  double b2 = 0.0;
+  for (int c = 0; c < p; ++c) b2 += beta[c] * beta[c];
+  return ll - b2 * j->t_inv_two_prior + j->t_const;
+}
+
 int64_t isqrt64(int64_t n) {
   int64_t r = (int64_t)std::sqrt((double)n);
   while (r * r > n) --r;
@@ -389,7 +415,8 @@ int run_rounds(is_job* j) {
     const int64_t used = cdiv(j->tn, ta.rows_per_split);
     launch_target<T>(D, ta, used, s);
     hipLaunchKernelGGL((k_target_combine<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, ta.part, (int)used,
-                       a.n_new, ta.beta, ta.cap, ta.first, d, j->t_inv_two_prior, j->t_const, j->tpre.as<T>());
+                       a.n_new, ta.beta, ta.cap, ta.first, d, j->t_inv_two_prior, j->t_const - j->t_shift,
+                       j->tpre.as<T>());
     a.pre_logtarget = j->tpre.as<T>();
     launch_mix<T, kPreTarget>(D, ncomp, a, s);
   };
@@ -919,6 +946,21 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
                    0.5 * (double)d * std::log(pi2);
     }
     j->tdata.assign(tp + head, tp + head + n * d + n);
+    if (j->precision == IS_FP32) {
+      // FP32 log-targets of a large data set sit near log pi(x0) ~ -n; storing them
+      // minus that FP64 constant keeps their FP32 rounding at the scale of the
+      // differences between samples, which is all the weights depend on
+      std::vector<double> x0(d, 0.0);
+      if (j->method == IS_DMPMC) {
+        const int K = (int)(j->committed_locs.size() / d);
+        for (int k = 0; k < K; ++k)
+          for (int c = 0; c < d; ++c) x0[c] += j->committed_locs[(size_t)k * d + c] / K;
+      } else {
+        x0 = j->initial.mu;
+      }
+      const double c0 = regression_logtarget(j, x0.data());
+      j->t_shift = std::isfinite(c0) ? c0 : 0.0;
+    }
   }
   HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
   int rc = j->precision == IS_FP32 ? alloc_job_buffers<float>(j) : alloc_job_buffers<double>(j);
@@ -1266,7 +1308,16 @@ static int copy_as_double(is_job* job, const DevBuf& src, int64_t n, double* out
 
 int is_job_get_logweights(is_job* job, double* out) {
   REQUIRE_RESULTS(job);
-  return copy_as_double(job, job->logweights, job->total, out);
+  const int rc = copy_as_double(job, job->logweights, job->total, out);
+  if (rc == IS_OK && job->t_shift != 0.0)
+    for (int64_t i = 0; i < job->total; ++i) out[i] += job->t_shift;
+  return rc;
+}
+
+int is_job_logweight_offset(const is_job* job, double* out) {
+  if (!job || !out) return fail(IS_EINVAL, "null argument");
+  *out = job->t_shift;
+  return IS_OK;
 }
 
 static int copy_ids(is_job* job, const DevBuf& src, int64_t* out) {
@@ -1290,8 +1341,10 @@ int is_job_get_proposal_ids(is_job* job, int64_t* out) {
 int is_job_get_round_summary(is_job* job, double* ess, double* lognormalizers, double* adaptation_ess) {
   REQUIRE_RESULTS(job);
   if (ess) HIP_TRY(hipMemcpy(ess, job->round_ess.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
-  if (lognormalizers)
+  if (lognormalizers) {
     HIP_TRY(hipMemcpy(lognormalizers, job->round_logz.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
+    for (int t = 0; t < job->T; ++t) lognormalizers[t] += job->t_shift;
+  }
   if (adaptation_ess)
     HIP_TRY(hipMemcpy(adaptation_ess, job->adapt_ess.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
   return IS_OK;
@@ -1378,8 +1431,8 @@ static int job_lognormalizer_typed(is_job* job, double* out3) {
   double s[5];
   HIP_TRY(hipMemcpyAsync(s, job->summary.p, sizeof(s), hipMemcpyDeviceToHost, job->stream));
   HIP_TRY(hipStreamSynchronize(job->stream));
-  out3[0] = s[4];
-  out3[1] = s[0];
+  out3[0] = s[4] + job->t_shift;
+  out3[1] = s[0] + job->t_shift;
   out3[2] = s[1];
   return IS_OK;
 }

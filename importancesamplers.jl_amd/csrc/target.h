@@ -5,11 +5,11 @@
 // eta = X beta is a dense contraction and runs on MFMA (16x16x4, FP32 in FP32
 // mode, FP64 in FP64 mode).  A block owns 64 samples (four 16-column tiles)
 // and a slice of the data rows.  The samples' coefficients are loaded once
-// into registers as the B operands.  The data rows stream through LDS in
-// 64-row chunks (the next chunk is prefetched into registers while the
-// current one is multiplied); wave w takes the chunk's 16-row tile w.  The
-// epilogue maps each eta to its log-likelihood term and accumulates it per
-// sample in FP64; partial sums per row slice are combined in a fixed order.
+// into registers as the B operands.  The data rows stream through two LDS
+// buffers in 64-row chunks; wave w takes a chunk's 16-row tile w.  The
+// epilogue of one chunk (each eta to its log-likelihood term, accumulated per
+// sample in FP64) runs beside the next chunk's MFMAs; partial sums per row
+// slice are combined in a fixed order.
 #pragma once
 
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -53,22 +53,125 @@ struct TargetArgs {
 
 __host__ __device__ constexpr int target_row_stride(int ks) { return 4 * ks + 1; }
 
-// grid = (ceil(n_new / 64), splits), block = 256 (4 waves); D = dimension
-// instantiation, KS = ceil(D / 4) K steps of the 16x16x4 MFMA
-template <class T, int D>
-__global__ __launch_bounds__(256) void k_target_gemm(TargetArgs<T> a) {
+// log1p(t) for t in [0, 1] as 2 atanh(s), s = t / (2 + t) <= 1/3:
+// 2 s (1 + s^2/3 + s^4/5 + ...), NC terms after the leading one.  The first
+// omitted term is below 1.3e-10 of the sum with NC = 8 and at FP64 rounding
+// with NC = 16.  Branch-free: s from one reciprocal and Newton / exact-residual
+// corrections, so that its rounding is unbiased.
+template <int NC>
+__device__ inline double log1p_unit(double t) {
+  const double w = 2.0 + t;
+  double r = __builtin_amdgcn_rcp(w);
+  r = fma(r, fma(-w, r, 1.0), r);
+  r = fma(r, fma(-w, r, 1.0), r);
+  double sv = t * r;
+  sv = fma(fma(-sv, w, t), r, sv);
+  const double s2 = sv * sv;
+  double q = 1.0 / (2 * NC + 1);
+#pragma unroll
+  for (int k = NC - 1; k >= 1; --k) q = fma(q, s2, 1.0 / (2 * k + 1));
+  const double s2v = 2.0 * sv;
+  return fma(s2v * s2, q, s2v);
+}
+__device__ inline float log1p_unit_f32(float t) {
+  const float w = 2.0f + t;
+  const float w_lo = t - (w - 2.0f);  // 2 + t = w + w_lo exactly
+  const float r = __builtin_amdgcn_rcpf(w);
+  const float s0 = t * r;
+  const float sv = fmaf(fmaf(-s0, w, t) - s0 * w_lo, r, s0);
+  const float s2 = sv * sv;
+  float q = 1.0f / 17.0f;
+#pragma unroll
+  for (int k = 7; k >= 1; --k) q = fmaf(q, s2, 1.0f / (2 * k + 1));
+  const float s2v = 2.0f * sv;
+  return fmaf(s2v * s2, q, s2v);
+}
+
+// One row's log-likelihood term for linear predictor eta and response y,
+// branch-free.  Logistic: y eta - log(1 + e^eta) = y eta - max(eta, 0) -
+// log1p(e^-|eta|); linear regression: -(y - eta)^2 / (2 sigma^2), the
+// constants added once per sample.
+//   FP64: e^-|eta| from exp and the 16-term series.
+//   FP32: e^-|eta| = 2^(-|eta| log2 e) from the hardware base-2 exponential,
+//   log2 e split in two floats (the product's rounding carried as a
+//   first-order correction), and the 8-term series.  Every step rounds without
+//   bias: summed over 1e5 rows, a per-row bias of a tenth of an FP32 ulp would
+//   move log pi by 5e-4, and the hardware logarithm and a float log1p carry
+//   biases of that size; this form's is about 4e-11 per row.
+template <class T, int KIND>
+__device__ inline T target_term(T y, T eta, double inv_two_var);
+template <>
+__device__ inline double target_term<double, kTLogistic>(double y, double eta, double) {
+  return fma(y, eta, -fmax(eta, 0.0)) - log1p_unit<16>(exp(-fabs(eta)));
+}
+template <>
+__device__ inline float target_term<float, kTLogistic>(float y, float eta, double) {
+  const float a = fabsf(eta);
+  const float L2E_HI = 1.44269502162933349609375f, L2E_LO = 1.925962989e-08f;
+  const float e_hi = -a * L2E_HI;
+  const float e_lo = fmaf(-a, L2E_HI, -e_hi) - a * L2E_LO;
+  const float t0 = __builtin_amdgcn_exp2f(e_hi);
+  const float t = fmaf(t0 * 0.6931471805599453f, e_lo, t0);
+  // y eta - max(eta, 0) is exact for y in {0, 1}
+  return fmaf(y, eta, -fmaxf(eta, 0.0f)) - log1p_unit_f32(t);
+}
+template <>
+__device__ inline double target_term<double, kTLinreg>(double y, double eta, double inv_two_var) {
+  const double res = y - eta;
+  return -(res * res) * inv_two_var;
+}
+template <>
+__device__ inline float target_term<float, kTLinreg>(float y, float eta, double inv_two_var) {
+  const float res = y - eta;
+  return -(res * res) * (float)inv_two_var;
+}
+
+// LDS of the target kernel: two chunk buffers, each 65 rows of PS values (row
+// 64 takes the staging writes that fall outside the chunk) and 64 responses.
+__host__ __device__ constexpr int target_buf_values(int ks) { return 65 * target_row_stride(ks) + 64; }
+__host__ __device__ constexpr size_t target_lds_bytes(int ks, int tsize) {
+  return (size_t)2 * target_buf_values(ks) * tsize > 256 * 8 ? (size_t)2 * target_buf_values(ks) * tsize
+                                                                : (size_t)256 * 8;
+}
+
+// One block's work with KS = ceil(p / 4) K steps of the 16x16x4 MFMA.
+// A chunk of 64 rows is the contiguous span X[r0 p, (r0 + 64) p): thread tid
+// stages span values tid + 256 u (u < KS), value e going to LDS row e / p (for
+// a bucket, a multiply-high by ceil(2^32 / p), exact for e p < 2^32; an exact
+// instantiation divides by its constant p), column e mod p; values past the
+// span go to the spare row 64.  The padding columns [p, 4 KS) stay zero.
+// Software pipeline, one barrier per chunk: iteration c reads chunk c's A
+// operands (buffer c mod 2), stages chunk c + 1 (loaded from global memory
+// during iteration c - 1) into the other buffer, issues the loads of chunk
+// c + 2, runs chunk c's MFMAs and the epilogue of chunk c - 1.  A wave does
+// not issue vector work while its own MFMAs run; the matrix core and the
+// vector ALU of a SIMD overlap across the waves of different blocks.  Only the
+// last chunk can be short; its rows past the end are masked.
+template <class T, int KS, bool EXACT, int KIND>
+__device__ inline void target_tile(const TargetArgs<T>& a, int p) {
   typedef MfmaTile<T> M;
-  constexpr int KS = (D + 3) / 4;
+  typedef typename M::acc_t acc_t;
   constexpr int PS = target_row_stride(KS);
-  constexpr int PER = (64 * 4 * KS + 255) / 256;  // staged values per thread per chunk
+  constexpr int BV = target_buf_values(KS);
   extern __shared__ double lds_raw[];
-  T* xs = reinterpret_cast<T*>(lds_raw);  // 64 x PS
-  T* ys = xs + 64 * PS;                   // 64
+  T* const lds = reinterpret_cast<T*>(lds_raw);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int p = exact_dim<D>(a.p);
+  const double inv_two_var = a.inv_two_var;
   const int64_t s0 = (int64_t)blockIdx.x * 64;
   const int64_t r_lo = (int64_t)blockIdx.y * a.rows_per_split;
   const int64_t r_hi = min(a.n, r_lo + a.rows_per_split);
+  const int64_t nchunks = r_hi > r_lo ? (r_hi - r_lo + 63) >> 6 : 0;
+  const int span = 64 * p;
+  const uint32_t pmagic = EXACT ? 0u : 0xffffffffu / (uint32_t)p + 1u;
+  for (int q = tid; q < 2 * BV; q += 256) lds[q] = T(0);
+  // staging destinations, the same for every chunk
+  int soff[KS];
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    const int e = tid + 256 * u;
+    const int row = EXACT ? e / p : (int)__umulhi((uint32_t)e, pmagic);
+    soff[u] = e < span ? row * PS + (e - row * p) : 64 * PS;
+  }
 
   // B operands: lane l holds beta[s0 + 16 q + l % 16][4 k + l / 16]
   T breg[4][KS];
@@ -83,61 +186,86 @@ __global__ __launch_bounds__(256) void k_target_gemm(TargetArgs<T> a) {
   }
   double ll[4] = {0.0, 0.0, 0.0, 0.0};
 
-  // chunk staging: value v = 64-row chunk element (row v / (4 KS), coefficient v % (4 KS))
-  T pre[PER], prey = T(0);
-  auto fetch = [&](int64_t r0) {
+  T pre[KS], prey;
+  // loads of chunk c (clamped into the slice, so a fetch past the end is harmless)
+  auto fetch = [&](int64_t c) {
+    const int64_t r0 = r_lo + 64 * min(c, nchunks - 1);
+    const int rows = (int)min((int64_t)64, r_hi - r0);
+    const int last = rows * p - 1;
+    const T* src = a.Xd + r0 * p;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int v = tid + 256 * u;
-      const int row = v / (4 * KS), c = v - row * (4 * KS);
-      const int64_t r = r0 + row;
-      pre[u] = (v < 64 * 4 * KS && r < r_hi && c < p) ? a.Xd[r * p + c] : T(0);
-    }
-    prey = (tid < 64 && r0 + tid < r_hi) ? a.yd[r0 + tid] : T(0);
+    for (int u = 0; u < KS; ++u) pre[u] = src[min(tid + 256 * u, last)];
+    prey = a.yd[r0 + min(lane, rows - 1)];
   };
-  if (r_lo < r_hi) fetch(r_lo);
-  for (int64_t r0 = r_lo; r0 < r_hi; r0 += 64) {
-    __syncthreads();  // the previous chunk's LDS reads are done
+  auto stage = [&](T* buf) {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int v = tid + 256 * u;
-      if (v < 64 * 4 * KS) {
-        const int row = v / (4 * KS), c = v - row * (4 * KS);
-        xs[row * PS + c] = pre[u];
-      }
+    for (int u = 0; u < KS; ++u) buf[soff[u]] = pre[u];
+    buf[65 * PS + lane] = prey;
+  };
+  const int arow = (16 * wave + (lane & 15)) * PS + (lane >> 4);
+  // log-likelihood terms of a finished chunk, accumulated per sample in FP64
+  // (a lane's four rows of a column summed in the working precision)
+  auto epilogue = [&](const acc_t (&acc)[4], const T (&yv)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T part = target_term<T, KIND>(yv[0], acc[q][0], inv_two_var);
+#pragma unroll
+      for (int r = 1; r < 4; ++r) part += target_term<T, KIND>(yv[r], acc[q][r], inv_two_var);
+      ll[q] += (double)part;
     }
-    if (tid < 64) ys[tid] = prey;
-    __syncthreads();
-    if (r0 + 64 < r_hi) fetch(r0 + 64);
-    typename M::acc_t acc[4];
+  };
+  auto epilogue_masked = [&](const acc_t (&acc)[4], const T (&yv)[4], int valid_rows) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T part = T(0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool valid = 16 * wave + M::row(lane, r) < valid_rows;
+        const T term = target_term<T, KIND>(yv[r], acc[q][r], inv_two_var);
+        part += valid ? term : T(0);
+      }
+      ll[q] += (double)part;
+    }
+  };
+  // chunk c: its A operands and responses from LDS, the staging of chunk c + 1,
+  // the loads of chunk c + 2 and its MFMAs
+  auto chunk = [&](int64_t c, acc_t (&acc)[4], T (&yv)[4]) {
+    const T* cur = lds + (c & 1) * BV;
+    T av[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) av[k] = cur[arow + 4 * k];  // A lane l: X[16 w + l % 16][4 k + l / 16]
+#pragma unroll
+    for (int r = 0; r < 4; ++r) yv[r] = cur[65 * PS + 16 * wave + M::row(lane, r)];
+    stage(lds + ((c + 1) & 1) * BV);
+    fetch(c + 2);
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = M::zero();
-    const T* arow = xs + (16 * wave + (lane & 15)) * PS + (lane >> 4);
 #pragma unroll
-    for (int k = 0; k < KS; ++k) {
-      const T av = arow[4 * k];  // A lane l: X[row 16 w + l % 16][coefficient 4 k + l / 16]
+    for (int k = 0; k < KS; ++k)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = M::mma(av, breg[q][k], acc[q]);
+      for (int q = 0; q < 4; ++q) acc[q] = M::mma(av[k], breg[q][k], acc[q]);
+  };
+
+  if (nchunks > 0) {
+    acc_t accP[4], accC[4];
+    T yP[4], yC[4];
+    fetch(0);
+    __syncthreads();  // zero fill done
+    stage(lds);
+    fetch(1);
+    __syncthreads();
+    chunk(0, accP, yP);
+    __syncthreads();
+    for (int64_t c = 1; c < nchunks; ++c) {
+      chunk(c, accC, yC);
+      epilogue(accP, yP);  // chunk c - 1, full
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) accP[q] = accC[q];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yP[r] = yC[r];
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = 16 * wave + M::row(lane, r);
-      if (r0 + i < r_hi) {
-        const T y = ys[i];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const T eta = acc[q][r];
-          double term;
-          if (a.kind == kTLogistic) {
-            term = (double)(y * eta) - (double)log1pexp(eta);
-          } else {
-            const double res = (double)y - (double)eta;
-            term = -(res * res) * a.inv_two_var;
-          }
-          ll[q] += term;
-        }
-      }
-    }
+    epilogue_masked(accP, yP, (int)(r_hi - (r_lo + 64 * (nchunks - 1))));
   }
   // sum the four row groups of each column (lanes l, l^16, l^32, l^48), then the waves
 #pragma unroll
@@ -158,9 +286,34 @@ __global__ __launch_bounds__(256) void k_target_gemm(TargetArgs<T> a) {
   }
 }
 
-__host__ __device__ constexpr size_t target_lds_bytes(int ks, int tsize) {
-  return (size_t)(64 * target_row_stride(ks) + 64) * tsize > 256 * 8 ? (size_t)(64 * target_row_stride(ks) + 64) * tsize
-                                                                     : (size_t)256 * 8;
+template <class T, int KS, bool EXACT>
+__device__ inline void target_kind(const TargetArgs<T>& a, int p) {
+  if (a.kind == kTLogistic)
+    target_tile<T, KS, EXACT, kTLogistic>(a, p);
+  else
+    target_tile<T, KS, EXACT, kTLinreg>(a, p);
+}
+
+// grid = (ceil(n_new / 64), splits), block = 256 (4 waves).  D is the dimension
+// instantiation; a bucket (D > 16) serves the p above the next smaller
+// instantiation, at most four K-step counts, each with its own unrolled body.
+// FP32 asks for 3 waves per SIMD (a few registers spill outside the chunk
+// loop); FP64, whose B operands take twice the registers, runs 1.
+template <class T, int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_target_gemm(
+    TargetArgs<T> a) {
+  constexpr int KS = (D + 3) / 4;
+  const int p = exact_dim<D>(a.p);
+  if constexpr (D <= kExactDims) {
+    target_kind<T, KS, true>(a, p);
+  } else {
+    switch ((p + 3) >> 2) {
+      case KS - 3: target_kind<T, KS - 3, false>(a, p); break;
+      case KS - 2: target_kind<T, KS - 2, false>(a, p); break;
+      case KS - 1: target_kind<T, KS - 1, false>(a, p); break;
+      default: target_kind<T, KS, false>(a, p); break;
+    }
+  }
 }
 
 // Per new sample: the row-slice partials in slice order, plus the log prior

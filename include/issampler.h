@@ -176,9 +176,17 @@ int is_job_get_history(is_job* job, double* means, double* factors, double* logn
 /* DM-PMC: current_proposal locations (d x K, active order) and per-round ancestors (K x T, 1-based). */
 int is_job_get_locations(is_job* job, double* out);
 int is_job_get_ancestors(is_job* job, int64_t* out);
-/* Device-resident logweights / samples (SoA, coordinate stride = capacity) of the last run. */
+/* Device-resident logweights / samples (SoA, coordinate stride = capacity) of the last run.
+ * The device log-weights are the log-weights minus is_job_logweight_offset. */
 int is_job_device_results(is_job* job, const void** samples, int64_t* coord_stride,
                           const void** logweights, int32_t* is_float);
+/* Offset c kept off the device-resident log-targets and log-weights: in FP32 mode
+ * with a regression target (n data rows) they are stored minus c = log pi(x0), the
+ * FP64 log target at the initial location (mean of the bank locations for DM-PMC),
+ * so their FP32 rounding stays at the scale of the differences between samples.
+ * Every result getter adds c back; 0 in every other case.  No reference
+ * counterpart (mixed-precision storage). */
+int is_job_logweight_offset(const is_job* job, double* out);
 /* Reset the committed proposal (AMIS/NPMC) or locations (DM-PMC) to the descriptor's. */
 int is_job_reset(is_job* job);
 
