@@ -195,3 +195,26 @@ def test_logistic_config3_shape_sampled_parity(eng, oracle_mod):
         dens = [oracle_mod.proposal_logdensity(q, xs[i]) for q in props]
         mix = np.logaddexp.reduce(dens) - math.log(K)
         assert abs(lw[i] - (tv - mix)) < 1e-9 * max(1.0, abs(tv))
+
+
+def test_logistic_fp32_log_targets_unbiased_at_config3_shape(eng):
+    """FP32 log-weights of 100,000-row logistic targets against FP64 at the same draws.
+    Each log target sums 1e5 FP32 terms: a per-row bias of 1e-9 (a tenth of an FP32 ulp,
+    the size of the bias of a float log1p or the hardware logarithm) would shift every
+    log-weight by 1e-4, so the mean shift is held to 3e-5 (the per-row rounding noise of
+    a sum of 1e5 terms is ~1e-5 per sample and averages out over the samples)."""
+    _say("case: logistic FP32 vs FP64 log-weights, 100,000 x 51, mean shift")
+    rng = np.random.default_rng(8)
+    n, p = 100_000, 51
+    X, y, beta = _logistic_data(rng, n, p)
+    tgt = eng.LogisticRegressionTarget(X, y, prior_var=10.0)
+    alg = eng.ImportanceSampling(eng.SphericalGaussian(beta, 0.003), 4096)
+    r64 = eng.importance_sample(5, tgt, alg, precision="fp64")
+    r32 = eng.importance_sample(5, tgt, alg, precision="fp32")
+    x64 = r64.samples.reshape(len(r64), -1)
+    x32 = r32.samples.reshape(len(r32), -1)
+    assert np.max(np.abs(x32 - x64)) < 1e-6  # the same counter-RNG draws
+    d = r32.logweights - r64.logweights
+    assert abs(float(np.mean(d))) < 3e-5, float(np.mean(d))
+    assert float(np.max(np.abs(d))) < 2e-3, float(np.max(np.abs(d)))
+    assert abs(eng.lognormalizer(r32) - eng.lognormalizer(r64)) < 1e-4
