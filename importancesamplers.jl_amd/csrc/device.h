@@ -269,7 +269,7 @@ struct Family {
 
 // _radial_logdensity (src/proposals.jl:24-55) for the finite-range paths.
 template <class T, int D>
-__device__ inline T radial_logdensity(const Family<T>& fam, T lognorm, T r2, const T (&y)[D], int d) {
+__device__ __forceinline__ T radial_logdensity(const Family<T>& fam, T lognorm, T r2, const T (&y)[D], int d) {
   if (!fam.student) {
     T half = T(0.5) * r2;
     if (!isfinite(r2)) {
@@ -321,7 +321,7 @@ __device__ inline T radial_logdensity(const Family<T>& fam, T lognorm, T r2, con
 // (cptr<T>) for a slot shared by the whole wave, whose values then arrive by
 // scalar loads and feed the FMAs as SGPR operands.
 template <class T, int D, class P>
-__device__ inline T slot_logdensity(P s, int d, const Family<T>& fam, const T (&x)[D], T (&y)[D]) {
+__device__ __forceinline__ T slot_logdensity(P s, int d, const Family<T>& fam, const T (&x)[D], T (&y)[D]) {
   T r2 = T(0);
 #pragma unroll
   for (int r = 0; r < D; ++r) {
@@ -340,7 +340,7 @@ __device__ inline T slot_logdensity(P s, int d, const Family<T>& fam, const T (&
 
 // Packed affine draw x = mu + L z (src/adaptive_gaussian_execution.jl:49-62).
 template <class T, int D, class P>
-__device__ inline bool slot_draw(P s, int d, const T (&z)[D], T (&x)[D]) {
+__device__ __forceinline__ bool slot_draw(P s, int d, const T (&z)[D], T (&x)[D]) {
   bool valid = true;
 #pragma unroll
   for (int r = 0; r < D; ++r) {

@@ -81,7 +81,7 @@ __device__ inline void pair_index(int p, int& r, int& c) {
 }
 
 template <class T, int D, class P>
-__device__ inline T inline_target(int tkind, int d, P tslot, const T (&x)[D], T (&y)[D]) {
+__device__ __forceinline__ T inline_target(int tkind, int d, P tslot, const T (&x)[D], T (&y)[D]) {
   if (tkind == kTStandard) {
     T s = T(0);
 #pragma unroll

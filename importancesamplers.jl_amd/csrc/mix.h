@@ -329,7 +329,7 @@ struct SweepLoad {
 };
 
 template <class T, int D, int NT>
-__device__ inline void sweep_load(const SweepArgs<T>& a, int64_t i, int64_t bend, SweepLoad<T, D>& L) {
+__device__ __forceinline__ void sweep_load(const SweepArgs<T>& a, int64_t i, int64_t bend, SweepLoad<T, D>& L) {
   const int d = exact_dim<D>(a.d);
   const bool valid = i < bend;
   const bool app = a.append && i < a.append_hi;
@@ -369,9 +369,8 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
     // columns above d stay zero for the whole kernel
     for (int q = lane; q < 64 * RS; q += 64) rows[q] = 0.0;
   }
-  double cen[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) cen[c] = (NT > 0 && c < d) ? as_const(a.center)[c] : 0.0;
+  // the centre is uniform: scalar loads, read where it is used
+  const cptr<double> cen = as_const(a.center);
   const cptr<T> slot = as_const(a.slot);
   const double clip = a.clip ? *a.clip : 0.0;
   T mw = ninf;

@@ -76,8 +76,9 @@ class Server:
         return sampler.collect(), timing
 
     def summary(self, ticket, timeout=None):
-        """Waits for the request, returns (round ESS, round log Z, timing, ok) and frees
-        the job without copying its samples to the host."""
+        """Waits for the request, returns (round ESS, round log Z, timing, error) and frees
+        the job without copying its samples to the host; error is None, or the exception
+        the request's own failure would raise."""
         import numpy as np
 
         from ._lib import F64P
@@ -86,11 +87,17 @@ class Server:
         sampler, timing, ok = self.take(ticket)
         T = len(sampler.schedule)
         ess, lz, aess = np.zeros(T), np.zeros(T), np.zeros(T)
+        error = None
         if ok:
             _check(sampler._lib.is_job_get_round_summary(sampler._job, ess.ctypes.data_as(F64P),
                                                          lz.ctypes.data_as(F64P), aess.ctypes.data_as(F64P)))
+        else:
+            try:
+                sampler._raise_failure()
+            except Exception as exc:  # the request's failure, reported to the caller
+                error = exc
         del sampler  # destroys the job and its device buffers
-        return ess, lz, timing, ok
+        return ess, lz, timing, error
 
     def stats(self):
         s = ServerStats()
