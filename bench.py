@@ -575,6 +575,23 @@ def main():
         ess_total += ess
     elapsed = time.perf_counter() - t_start
 
+    # the plain FP64 form: the same solves in FP64 storage and arithmetic
+    fp64_form = None
+    if chosen != "fp64":
+        for _ in range(args.warmup):
+            solve(pkg, ref)
+        lat64, ess64 = [], 0.0
+        t64 = time.perf_counter()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            solve(pkg, ref)
+            lat64.append(time.perf_counter() - t0)
+            ess64 += final_summary(pkg, ref)[0]
+        el64 = time.perf_counter() - t64
+        fp64_form = {"ess_per_s": ess64 / el64, "ms_per_step": 1e3 * el64 / args.steps,
+                     "p50_ms": 1e3 * float(np.percentile(lat64, 50)), "p99_ms": 1e3 * float(np.percentile(lat64, 99))}
+    del ref
+
     # kernel durations: separate eager solves with HIP events around the sweep
     # launch of every round, recorded on the job's stream
     lib.is_job_set_timing(sampler._job, 1)
@@ -623,6 +640,8 @@ def main():
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "dlogz_vs_fp64": dlz,
         "moment_err_vs_fp64": dmom,
+        "fp64_plain_form": fp64_form,
+        "speedup_vs_fp64_plain_form": (ess_total / elapsed) / fp64_form["ess_per_s"] if fp64_form else None,
         "logz_fp64": lz64,
         "weighted_samples_per_s": T * N * args.steps / elapsed,
         "roofline": {
