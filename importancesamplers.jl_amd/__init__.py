@@ -23,13 +23,17 @@ from .api import (  # noqa: F401
     LogisticRegressionTarget,
     MultinomialResampling,
     NPMCRoundError,
+    PartialDeterministicMixture,
     PrefilledRNG,
+    RandomMixture,
     ProposalBank,
     SamplerBusyError,
     SamplerExecutionError,
     SphericalGaussian,
     SphericalStudentT,
     StandardGaussianTarget,
+    StandardMIS,
+    StratifiedMixture,
     SystematicResampling,
     WeightedSamples,
     context,

@@ -50,6 +50,10 @@ class JobDesc(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("stream", ctypes.c_uint64),
+        ("mis_scheme", ctypes.c_int32),
+        ("mis_group_count", ctypes.c_int32),
+        ("mis_group_offsets", I32P),
+        ("mis_group_members", I32P),
     ]
 
 

@@ -17,11 +17,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import F64P, I64P, VP, JobDesc
+from ._lib import F64P, I32P, I64P, VP, JobDesc
 
 GAUSSIAN, STUDENT_T = 0, 1
 SPHERICAL, DIAGONAL, FACTOR = 0, 1, 2
-PLAIN, AMIS_M, NPMC_M, DMPMC_M = 0, 1, 2, 3
+PLAIN, AMIS_M, NPMC_M, DMPMC_M, MIS_M = 0, 1, 2, 3, 4
 FP64, FP32 = 0, 1
 
 PHASES = {0: None, 1: "sampling", 2: "target", 3: "denominator", 4: "weight", 5: "moment",
@@ -327,12 +327,74 @@ def _schedule(rounds, round_size):
 
 @dataclass
 class ImportanceSampling:
-    proposal: NativeProposal
+    """`ImportanceSampling(proposal; nsamples)`, or static MIS with a `ProposalBank`
+    and a `mis_scheme` (src/methods/static_mis.jl; default StratifiedMixture)."""
+
+    proposal: object
     nsamples: int
+    mis_scheme: object = None
 
     def __post_init__(self):
         if int(self.nsamples) <= 0:
             raise ValueError("nsamples must be positive")
+        if isinstance(self.proposal, ProposalBank):
+            if self.mis_scheme is None:
+                self.mis_scheme = StratifiedMixture()
+            if not isinstance(self.mis_scheme, (StratifiedMixture, RandomMixture, StandardMIS,
+                                                PartialDeterministicMixture)):
+                raise ValueError(f"{type(self.mis_scheme).__name__} static-MIS execution is not implemented")
+            if isinstance(self.mis_scheme, PartialDeterministicMixture):
+                self.mis_scheme.validate(len(self.proposal.proposals))
+        elif self.mis_scheme is not None:
+            raise ValueError("an MIS scheme needs a ProposalBank")
+
+
+# ---------------------------------------------------------------- static MIS schemes
+class StratifiedMixture:
+    """Stratified slot assignment, full deterministic-mixture denominator."""
+
+    code, name = 0, "stratified_mixture"
+
+
+class RandomMixture:
+    """Independent slot assignment, full deterministic-mixture denominator."""
+
+    code, name = 1, "random_mixture"
+
+
+class StandardMIS:
+    """Stratified slot assignment, the generating proposal alone in the denominator."""
+
+    code, name = 2, "standard_mis"
+
+
+class PartialDeterministicMixture:
+    """Stratified slot assignment; the denominator is the mixture of the generating
+    proposal's group (`groups`: tuples of 1-based proposal ids, each id exactly once;
+    src/methods/static_mis.jl:245-310)."""
+
+    code, name = 3, "partial_deterministic_mixture"
+
+    def __init__(self, groups):
+        self.groups = tuple(tuple(g) for g in groups)
+
+    def validate(self, nproposals):
+        if not self.groups:
+            raise ValueError("partial-MIS groups cannot be empty")
+        seen = set()
+        for g in self.groups:
+            if not g:
+                raise ValueError("partial-MIS groups cannot be empty")
+            for pid in g:
+                if isinstance(pid, bool) or not isinstance(pid, (int, np.integer)):
+                    raise ValueError("partial-MIS proposal IDs must be integers")
+                if not 1 <= pid <= nproposals:
+                    raise ValueError(f"partial-MIS proposal ID {pid} is out of range")
+                if pid in seen:
+                    raise ValueError(f"partial-MIS proposal ID {pid} appears more than once")
+                seen.add(int(pid))
+        if len(seen) != nproposals:
+            raise ValueError("partial-MIS groups must include every proposal ID exactly once")
 
 
 @dataclass
@@ -429,7 +491,9 @@ def _job_desc(rng, target, algorithm, precision):
     desc.seed = seed & 0xFFFFFFFFFFFFFFFF
     desc.stream = 0x6D693335357873  # fixed second key word
     desc.precision = precision
-    if isinstance(algorithm, ImportanceSampling):
+    if isinstance(algorithm, ImportanceSampling) and isinstance(algorithm.proposal, ProposalBank):
+        p, desc.method, sched = algorithm.proposal.proposals[0], MIS_M, [int(algorithm.nsamples)]
+    elif isinstance(algorithm, ImportanceSampling):
         p, desc.method, sched = algorithm.proposal, PLAIN, [int(algorithm.nsamples)]
     elif isinstance(algorithm, NPMC):
         p, desc.method, sched = algorithm.proposal, NPMC_M, algorithm.schedule
@@ -444,14 +508,25 @@ def _job_desc(rng, target, algorithm, precision):
     rs = np.asarray(sched, dtype=np.int64)
     keep.append(rs)
     desc.round_size = rs.ctypes.data_as(I64P)
-    if desc.method == DMPMC_M:
-        bank = algorithm.bank
+    if desc.method in (DMPMC_M, MIS_M):
+        bank = algorithm.bank if desc.method == DMPMC_M else algorithm.proposal
         desc.bank_size = len(bank.proposals)
         desc.bank_locations = arr(np.concatenate([q.location for q in bank.proposals]))
         desc.bank_scales = arr(np.concatenate([q.scale if q.kind != FACTOR else q.factor().T.ravel()
                                                for q in bank.proposals]))
         desc.bank_masses = arr(bank.masses)
-        desc.resampling = algorithm.method.code
+        if desc.method == DMPMC_M:
+            desc.resampling = algorithm.method.code
+        else:
+            scheme = algorithm.mis_scheme
+            desc.mis_scheme = scheme.code
+            if isinstance(scheme, PartialDeterministicMixture):
+                offs = np.cumsum([0] + [len(g) for g in scheme.groups]).astype(np.int32)
+                mem = np.asarray([int(pid) for g in scheme.groups for pid in g], dtype=np.int32)
+                keep.extend([offs, mem])
+                desc.mis_group_count = len(scheme.groups)
+                desc.mis_group_offsets = offs.ctypes.data_as(I32P)
+                desc.mis_group_members = mem.ctypes.data_as(I32P)
     else:
         desc.location = arr(p.location)
         desc.scale = arr(p.scale if p.kind != FACTOR else p.factor().T.ravel())
@@ -489,9 +564,9 @@ class PreparedSampler:
         self._job = handle
         self._lib = _lib.load()
         self.total = int(self._lib.is_job_total(handle))
-        if desc.method == DMPMC_M:
-            order = sorted([i for i, m in enumerate(self.algorithm.bank.masses) if m != 0],
-                           key=lambda i: self.algorithm.bank.masses[i])
+        if desc.method in (DMPMC_M, MIS_M):
+            bank = self.algorithm.bank if desc.method == DMPMC_M else self.algorithm.proposal
+            order = sorted([i for i, m in enumerate(bank.masses) if m != 0], key=lambda i: bank.masses[i])
             self.active_ids = [i + 1 for i in order]
 
     @classmethod
@@ -515,7 +590,7 @@ class PreparedSampler:
     def method(self):
         a = self.algorithm
         if isinstance(a, ImportanceSampling):
-            return PLAIN
+            return MIS_M if isinstance(a.proposal, ProposalBank) else PLAIN
         if isinstance(a, NPMC):
             return NPMC_M
         if isinstance(a, AMIS):
@@ -570,6 +645,10 @@ class PreparedSampler:
             u = np.zeros(K)
             _check(lib.is_job_export_uniforms(self._job, round, _p(u), K))
             out["uniforms"] = u
+        if self.method == MIS_M:
+            u = np.zeros(n)  # one assignment uniform per sample
+            _check(lib.is_job_export_uniforms(self._job, round, _p(u), n))
+            out["uniforms"] = u
         return out
 
     def clear_prefilled(self):
@@ -622,8 +701,8 @@ class PreparedSampler:
         ess, lz, aess = np.zeros(T), np.zeros(T), np.zeros(T)
         _check(self._lib.is_job_get_round_summary(self._job, _p(ess), _p(lz), _p(aess)))
         method = self.method
-        diag = {"method": {PLAIN: "importance_sampling", AMIS_M: "amis", NPMC_M: "npmc",
-                           DMPMC_M: "deterministic_mixture_pmc"}[method],
+        diag = {"method": {PLAIN: "importance_sampling", MIS_M: "importance_sampling", AMIS_M: "amis",
+                           NPMC_M: "npmc", DMPMC_M: "deterministic_mixture_pmc"}[method],
                 "execution": "mi355x", "rounds": T, "round_sizes": list(self.schedule),
                 "round_ess": ess, "round_lognormalizers": lz,
                 "target_evaluations": n,
@@ -635,6 +714,11 @@ class PreparedSampler:
         if method == DMPMC_M:
             prov["proposal_id"] = self._get("is_job_get_proposal_ids", n, np.int64)
             diag["resampling"] = "global"
+        if method == MIS_M:
+            prov = {"proposal_id": self._get("is_job_get_proposal_ids", n, np.int64)}
+            diag["mis_scheme"] = self.algorithm.mis_scheme.name
+            if isinstance(self.algorithm.mis_scheme, (StratifiedMixture, RandomMixture)):
+                diag["proposal_evaluations"] = n * len(self.active_ids)
         if self.scalar:
             samples = samples[:, 0].copy()
         return WeightedSamples(samples, logw, prov, diag, _sampler=self)

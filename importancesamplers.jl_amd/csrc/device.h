@@ -26,6 +26,7 @@ constexpr uint64_t kPurposeNormals = 0;
 constexpr uint64_t kPurposeRadialNormals = 1;
 constexpr uint64_t kPurposeRadialUniforms = 2;
 constexpr uint64_t kPurposeResampling = 3;
+constexpr uint64_t kPurposeAssignment = 4;  // static MIS slot assignment, one uniform per sample
 
 // ---------------------------------------------------------------- Philox4x64-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11).

@@ -111,6 +111,12 @@ struct is_job {
   std::vector<int32_t> bank_ids;         // 1-based configured ids of active slots
   std::vector<int64_t> plan_counts;      // K*T
   std::vector<double> plan_logcoef;      // K*T
+  // static MIS: scheme, active-slot cumulative masses and log masses, partial groups
+  int mis_scheme = 0;
+  std::vector<double> mis_cdf, mis_logmass;
+  std::vector<int32_t> mis_slot_group, mis_goff, mis_gmem;
+  std::vector<double> mis_gcoef;
+  DevBuf mis_cdf_d, mis_ints_d, mis_gcoefT;
   std::vector<double> tslot;             // Gaussian target slot
   // batched regression target (logistic / linear): data on the device
   int64_t tn = 0;                        // data rows
@@ -257,6 +263,96 @@ double regression_logtarget(const is_job* j, const double* beta) {
   return ll - b2 * j->t_inv_two_prior + j->t_const;
 }
 
+double host_logaddexp(double a, double b) {
+  if (a == -HUGE_VAL) return b;
+  if (b == -HUGE_VAL) return a;
+  const double m = std::max(a, b);
+  return m + std::log1p(std::exp(-std::fabs(a - b)));
+}
+
+// Static MIS design over the active slots (configured order of the nonzero
+// masses, stably sorted by mass): cumulative masses with the last exactly 1
+// and the log interval masses (src/proposal_banks.jl:301-326); for the partial
+// mixture, the active members of each group in the listed order, empty groups
+// dropped, with log coefficients log m - log sum_group m
+// (src/methods/static_mis.jl:245-310).  Uploads the cumulative masses, the
+// group tables and allocates the assignment buffer.
+int setup_mis(is_job* j, const is_job_desc* desc, const std::vector<double>& masses) {
+  if (desc->mis_scheme < IS_MIS_STRATIFIED || desc->mis_scheme > IS_MIS_PARTIAL)
+    return fail(IS_EINVAL, "unknown MIS scheme");
+  j->mis_scheme = desc->mis_scheme;
+  const int K = j->K;
+  double tot = 0.0;
+  for (double m : masses) tot += m;
+  j->mis_cdf.assign(K, 0.0);
+  double acc = 0.0;
+  for (int s = 0; s < K; ++s) {
+    acc += masses[s] / tot;
+    j->mis_cdf[s] = acc;
+  }
+  j->mis_cdf[K - 1] = 1.0;
+  j->mis_logmass.assign(K, 0.0);
+  double prev = 0.0, mx = -HUGE_VAL;
+  for (int s = 0; s < K; ++s) {
+    if (!(j->mis_cdf[s] > prev))
+      return fail(IS_EINVAL, "every positive proposal mass must retain a positive assignment interval");
+    j->mis_logmass[s] = std::log(j->mis_cdf[s] - prev);
+    prev = j->mis_cdf[s];
+    mx = std::max(mx, j->mis_logmass[s]);
+  }
+  double se = 0.0;
+  for (int s = 0; s < K; ++s) se += std::exp(j->mis_logmass[s] - mx);
+  const double lse = mx + std::log(se);
+  for (int s = 0; s < K; ++s) j->mis_logmass[s] -= lse;
+  if (j->mis_scheme == IS_MIS_PARTIAL) {
+    const int G = desc->mis_group_count, Kc = desc->bank_size;
+    if (G < 1 || !desc->mis_group_offsets || !desc->mis_group_members)
+      return fail(IS_EINVAL, "partial-MIS groups cannot be empty");
+    if (desc->mis_group_offsets[0] != 0) return fail(IS_EINVAL, "partial-MIS group offsets must start at 0");
+    std::vector<int> slot_of_id(Kc + 1, -1);
+    for (int s = 0; s < K; ++s) slot_of_id[j->bank_ids[s]] = s;
+    std::vector<char> seen(Kc + 1, 0);
+    j->mis_slot_group.assign(K, 0);
+    j->mis_goff.assign(1, 0);
+    j->mis_gmem.clear();
+    j->mis_gcoef.clear();
+    for (int g = 0; g < G; ++g) {
+      const int e0 = desc->mis_group_offsets[g], e1 = desc->mis_group_offsets[g + 1];
+      if (e1 <= e0) return fail(IS_EINVAL, "partial-MIS groups cannot be empty");
+      const size_t first = j->mis_gmem.size();
+      for (int e = e0; e < e1; ++e) {
+        const int id = desc->mis_group_members[e];
+        if (id < 1 || id > Kc) return fail(IS_EINVAL, "partial-MIS proposal ID %d is out of range", id);
+        if (seen[id]) return fail(IS_EINVAL, "partial-MIS proposal ID %d appears more than once", id);
+        seen[id] = 1;
+        if (slot_of_id[id] >= 0) j->mis_gmem.push_back(slot_of_id[id]);
+      }
+      if (j->mis_gmem.size() == first) continue;
+      const int ag = (int)j->mis_goff.size() - 1;
+      double glm = -HUGE_VAL;
+      for (size_t m = first; m < j->mis_gmem.size(); ++m) glm = host_logaddexp(glm, j->mis_logmass[j->mis_gmem[m]]);
+      for (size_t m = first; m < j->mis_gmem.size(); ++m) {
+        j->mis_slot_group[j->mis_gmem[m]] = ag;
+        j->mis_gcoef.push_back(j->mis_logmass[j->mis_gmem[m]] - glm);
+      }
+      j->mis_goff.push_back((int32_t)j->mis_gmem.size());
+    }
+    for (int id = 1; id <= Kc; ++id)
+      if (!seen[id]) return fail(IS_EINVAL, "partial-MIS groups must include every proposal ID exactly once");
+    std::vector<int32_t> ints(j->mis_slot_group);
+    ints.insert(ints.end(), j->mis_goff.begin(), j->mis_goff.end());
+    ints.insert(ints.end(), j->mis_gmem.begin(), j->mis_gmem.end());
+    int rc;
+    if ((rc = j->mis_ints_d.alloc(ints.size() * sizeof(int32_t)))) return rc;
+    HIP_TRY(hipMemcpy(j->mis_ints_d.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  int rc;
+  if ((rc = j->mis_cdf_d.alloc((size_t)K * sizeof(double)))) return rc;
+  HIP_TRY(hipMemcpy(j->mis_cdf_d.p, j->mis_cdf.data(), (size_t)K * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = j->assign.alloc((size_t)j->schedule[0] * sizeof(int32_t)))) return rc;
+  return IS_OK;
+}
+
 int64_t isqrt64(int64_t n) {
   int64_t r = (int64_t)std::sqrt((double)n);
   while (r * r > n) --r;
@@ -307,7 +403,7 @@ RoundArgs<T> base_args(is_job* j, int t) {
   a.lognums = j->lognums.as<T>();
   a.logweights = j->logweights.as<T>();
   a.round_ids = j->round_ids.as<int32_t>();
-  a.proposal_ids = j->method == IS_DMPMC ? j->proposal_ids.as<int32_t>() : nullptr;
+  a.proposal_ids = (j->method == IS_DMPMC || j->method == IS_MIS) ? j->proposal_ids.as<int32_t>() : nullptr;
   a.fail = j->fail_rec.as<unsigned long long>() + 3 * t;
   a.partials = j->partials.as<double>();
   return a;
@@ -426,6 +522,55 @@ int run_rounds(is_job* j) {
     launch_finish(d, reduced.as<double>(), f, s);
   };
 
+  if (j->method == IS_MIS) {
+    // static MIS (src/methods/static_mis.jl:310-340,553-592): slot assignment,
+    // draws from the assigned slots, the design's denominator, one summary
+    const int64_t N = j->schedule[0];
+    hipLaunchKernelGGL(k_mis_assign, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, s, j->key,
+                       j->run_dev.as<unsigned long long>(), N, j->mis_scheme == IS_MIS_RANDOM ? 0 : 1,
+                       j->mis_cdf_d.as<double>(), j->K,
+                       j->prefilled_uniforms[0] ? j->pre_uniforms.as<double>() : nullptr, j->assign.as<int32_t>());
+    RoundArgs<T> a = base_args<T>(j, 0);
+    set_prefilled(j, 0, a);
+    a.first = 0;
+    a.n_new = N;
+    a.assign = j->assign.as<int32_t>();
+    a.bank_ids = j->bank_ids_d.as<int32_t>();
+    a.gen_slot = 0;
+    a.logmass = j->logmassT.as<T>();
+    a.logtotal = T(0);
+    a.record_nonfinite_after_exhaust = 0;
+    int ncomp = j->K;
+    if (j->mis_scheme == IS_MIS_STANDARD) {
+      a.mis_terms = 1;
+      ncomp = 1;
+    } else if (j->mis_scheme == IS_MIS_PARTIAL) {
+      const int32_t* ints = j->mis_ints_d.as<int32_t>();
+      a.mis_terms = 2;
+      a.mis_slot_group = ints;
+      a.mis_goff = ints + j->K;
+      a.mis_gmem = a.mis_goff + j->mis_goff.size();
+      a.mis_gcoef = j->mis_gcoefT.as<T>();
+      ncomp = 1;
+    }
+    a.k0 = 0;
+    a.k1 = ncomp - 1;
+    tick(j, 0, 0, s);
+    mix(a, ncomp);
+    tick(j, 0, 1, s);
+    SweepArgs<T> w = sweep_args<T>(j, 0);
+    w.lo = 0;
+    w.hi = N;
+    w.nv = 2;
+    const int nrec = launch_sweep<T>(D, 0, w, s);
+    FinishArgs f = base_finish(j, 0, counter);
+    f.flags = kFinishSummary;
+    f.summary_count = N;
+    f.summary_out = j->summary.as<double>();
+    reduce(j->partials, nrec, 2, j->reduced, f);
+    HIP_TRY(hipGetLastError());
+    return IS_OK;
+  }
   if (j->method == IS_PLAIN) {
     const int64_t N = j->schedule[0];
     RoundArgs<T> a = base_args<T>(j, 0);
@@ -604,9 +749,9 @@ int job_run_typed(is_job* j) {
   HIP_TRY(hipMemsetAsync(j->fit_status.p, 0, j->fit_status.bytes, s));
   HIP_TRY(hipMemsetAsync(j->cdf_status.p, 0, j->cdf_status.bytes, s));
   HIP_TRY(hipMemsetAsync(j->counter.p, 0, j->counter.bytes, s));
-  // committed state into slot 0 (AMIS / NPMC) or the bank slots (DM-PMC)
+  // committed state into slot 0 (AMIS / NPMC) or the bank slots (DM-PMC, static MIS)
   const int d = j->d;
-  if (j->method == IS_DMPMC) {
+  if (j->method == IS_DMPMC || j->method == IS_MIS) {
     std::vector<double> slots(j->bank_slots_init);
     for (int k = 0; k < j->K; ++k)
       for (int c = 0; c < d; ++c) slots[(size_t)k * j->stride + c] = j->committed_locs[(size_t)k * d + c];
@@ -806,12 +951,12 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   const int d = desc->dim;
   if (d < 1 || d > kMaxDim) return fail(IS_EUNSUPPORTED, "dimension %d outside [1, %d]", d, kMaxDim);
   if (desc->scalar && d != 1) return fail(IS_EINVAL, "scalar proposals have dimension 1");
-  if (desc->method < IS_PLAIN || desc->method > IS_DMPMC) return fail(IS_EINVAL, "unknown method");
+  if (desc->method < IS_PLAIN || desc->method > IS_MIS) return fail(IS_EINVAL, "unknown method");
   if (desc->family != IS_GAUSSIAN && desc->family != IS_STUDENT_T) return fail(IS_EINVAL, "unknown family");
   if (desc->precision != IS_FP64 && desc->precision != IS_FP32) return fail(IS_EINVAL, "unknown precision");
   if (desc->target_kind < IS_TARGET_STANDARD || desc->target_kind > IS_TARGET_LINREG)
     return fail(IS_EINVAL, "unknown target kind %d", desc->target_kind);
-  const int T = desc->method == IS_PLAIN ? 1 : desc->rounds;
+  const int T = (desc->method == IS_PLAIN || desc->method == IS_MIS) ? 1 : desc->rounds;
   if (T < 1 || !desc->round_size) return fail(IS_EINVAL, "rounds must be positive");
   auto j = new is_job();
   std::unique_ptr<is_job> guard(j);
@@ -845,10 +990,10 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   j->total = j->offsets[T];
   if (j->total >= ((int64_t)1 << 31)) return fail(IS_EUNSUPPORTED, "sample budget exceeds 2^31 - 1");
   // proposals
-  if (j->method == IS_DMPMC) {
+  if (j->method == IS_DMPMC || j->method == IS_MIS) {
     const int Kc = desc->bank_size;
     if (Kc < 1 || !desc->bank_locations || !desc->bank_scales || !desc->bank_masses)
-      return fail(IS_EINVAL, "DM-PMC needs a proposal bank");
+      return fail(IS_EINVAL, "%s needs a proposal bank", j->method == IS_MIS ? "static MIS" : "DM-PMC");
     std::vector<int> active;
     for (int k = 0; k < Kc; ++k) {
       const double m = desc->bank_masses[k];
@@ -876,28 +1021,34 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
       j->bank_ids.push_back(k + 1);
       masses[s] = desc->bank_masses[k];
     }
-    std::vector<u128> I;
-    if (!exact_integer_masses(masses, I))
-      return fail(IS_EUNSUPPORTED, "mass ratios too wide for the exact allocation plan");
-    j->plan_counts.assign((size_t)K * T, 0);
-    j->plan_logcoef.assign((size_t)K * T, 0.0);
-    std::vector<int32_t> assign((size_t)T * j->maxN, 0);
-    for (int t = 0; t < T; ++t) {
-      std::vector<int64_t> cnt;
-      if (!round_counts(I, j->schedule[t], t + 1, cnt)) return fail(IS_EINVAL, "invalid DM-PMC allocation remainder");
-      int64_t pos = 0;
-      for (int s = 0; s < K; ++s) {
-        if (cnt[s] <= 0)
-          return fail(IS_EINVAL, "round %d of size %lld cannot allocate at least one sample to every positive-mass proposal",
-                      t + 1, (long long)j->schedule[t]);
-        j->plan_counts[(size_t)t * K + s] = cnt[s];
-        j->plan_logcoef[(size_t)t * K + s] = std::log((double)cnt[s] / (double)j->schedule[t]);
-        for (int64_t q = 0; q < cnt[s]; ++q) assign[(size_t)t * j->maxN + pos++] = s;
+    if (j->method == IS_MIS) {
+      int rc = setup_mis(j, desc, masses);
+      if (rc) return rc;
+    } else {
+      std::vector<u128> I;
+      if (!exact_integer_masses(masses, I))
+        return fail(IS_EUNSUPPORTED, "mass ratios too wide for the exact allocation plan");
+      j->plan_counts.assign((size_t)K * T, 0);
+      j->plan_logcoef.assign((size_t)K * T, 0.0);
+      std::vector<int32_t> assign((size_t)T * j->maxN, 0);
+      for (int t = 0; t < T; ++t) {
+        std::vector<int64_t> cnt;
+        if (!round_counts(I, j->schedule[t], t + 1, cnt)) return fail(IS_EINVAL, "invalid DM-PMC allocation remainder");
+        int64_t pos = 0;
+        for (int s = 0; s < K; ++s) {
+          if (cnt[s] <= 0)
+            return fail(IS_EINVAL, "round %d of size %lld cannot allocate at least one sample to every positive-mass proposal",
+                        t + 1, (long long)j->schedule[t]);
+          j->plan_counts[(size_t)t * K + s] = cnt[s];
+          j->plan_logcoef[(size_t)t * K + s] = std::log((double)cnt[s] / (double)j->schedule[t]);
+          for (int64_t q = 0; q < cnt[s]; ++q) assign[(size_t)t * j->maxN + pos++] = s;
+        }
       }
+      int rc;
+      if ((rc = j->assign.alloc(assign.size() * sizeof(int32_t)))) return rc;
+      HIP_TRY(hipMemcpy(j->assign.p, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     int rc;
-    if ((rc = j->assign.alloc(assign.size() * sizeof(int32_t)))) return rc;
-    HIP_TRY(hipMemcpy(j->assign.p, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     if ((rc = j->bank_ids_d.alloc(j->bank_ids.size() * sizeof(int32_t)))) return rc;
     HIP_TRY(hipMemcpy(j->bank_ids_d.p, j->bank_ids.data(), j->bank_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   } else {
@@ -972,6 +1123,13 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
     for (int t = 0; t < T; ++t) logmass[t] = std::log((double)j->schedule[t]);
   } else if (j->method == IS_DMPMC) {
     logmass = j->plan_logcoef;
+  } else if (j->method == IS_MIS) {
+    logmass = j->mis_logmass;
+    if (!j->mis_gcoef.empty()) {
+      rc = j->precision == IS_FP32 ? upload_T<float>(j->mis_gcoefT, j->mis_gcoef)
+                                   : upload_T<double>(j->mis_gcoefT, j->mis_gcoef);
+      if (rc) return rc;
+    }
   } else {
     logmass.assign(std::max(j->nslots, 1), 0.0);
   }
@@ -1002,7 +1160,7 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
 int64_t is_job_device_bytes(const is_job_desc* desc) {
   if (!desc || !desc->round_size || desc->dim < 1 || desc->dim > kMaxDim) return -1;
   const int d = desc->dim;
-  const int T = desc->method == IS_PLAIN ? 1 : desc->rounds;
+  const int T = (desc->method == IS_PLAIN || desc->method == IS_MIS) ? 1 : desc->rounds;
   if (T < 1) return -1;
   int64_t total = 0, maxN = 0;
   for (int t = 0; t < T; ++t) {
@@ -1012,8 +1170,9 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
   }
   const int64_t ts = desc->precision == IS_FP32 ? 4 : 8;
   const int stride = slot_stride(d);
-  const int K = desc->method == IS_DMPMC ? std::max(1, desc->bank_size) : 1;
-  const int nslots = desc->method == IS_DMPMC ? K : desc->method == IS_PLAIN ? 1 : T + 1;
+  const bool banked = desc->method == IS_DMPMC || desc->method == IS_MIS;
+  const int K = banked ? std::max(1, desc->bank_size) : 1;
+  const int nslots = banked ? K : desc->method == IS_PLAIN ? 1 : T + 1;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
   const int64_t pcap = cdiv(total, kSweepPerBlock) + 1;
   int64_t bytes = total * (d * ts + 3 * ts + 2 * 4);         // samples, log arrays, ids
@@ -1029,6 +1188,7 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
     bytes += (int64_t)T * maxN * 4 + maxN * 8 + (int64_t)K * T * 12;
     bytes += (cdiv(maxN, kCdfThreads * kCdfPer) + 1) * 8;
   }
+  if (desc->method == IS_MIS) bytes += maxN * 4 + (int64_t)K * (8 + 4 * 4 + ts) + 64;  // assignment, design tables
   return bytes;
 }
 
@@ -1133,10 +1293,20 @@ int is_job_set_radial(is_job* job, int32_t round, const double* normals, int64_t
 }
 
 int is_job_set_uniforms(is_job* job, int32_t round, const double* uniforms, int64_t count) {
-  if (!job || !uniforms || round < 0 || round >= job->T || job->method != IS_DMPMC)
+  if (!job || !uniforms || round < 0 || round >= job->T || (job->method != IS_DMPMC && job->method != IS_MIS))
     return fail(IS_EINVAL, "bad arguments");
   BusyGuard g(job);
   if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  if (job->method == IS_MIS) {
+    // static MIS: one assignment uniform per sample
+    const int64_t N = job->schedule[0];
+    if (count != N) return fail(IS_EINVAL, "static MIS assignment uniforms: one per sample (%lld)", (long long)N);
+    int rc = upload_prefill(job->pre_uniforms, (size_t)N, 0, uniforms, (size_t)N);
+    if (rc) return rc;
+    job->prefilled_uniforms[0] = 1;
+    job->graph_dirty = true;
+    return IS_OK;
+  }
   if (count < 1 || count > job->K) return fail(IS_EINVAL, "uniform count must be in [1, K]");
   int rc = upload_prefill(job->pre_uniforms, (size_t)job->K * job->T, (size_t)job->K * round, uniforms, count);
   if (rc) return rc;
@@ -1219,7 +1389,17 @@ int is_job_export_radial(is_job* job, int32_t round, double* normals, int64_t nc
 }
 
 int is_job_export_uniforms(is_job* job, int32_t round, double* out, int64_t count) {
-  if (!job || !out || round < 0 || round >= job->T || job->method != IS_DMPMC) return fail(IS_EINVAL, "bad arguments");
+  if (!job || !out || round < 0 || round >= job->T || (job->method != IS_DMPMC && job->method != IS_MIS))
+    return fail(IS_EINVAL, "bad arguments");
+  if (job->method == IS_MIS) {
+    const int64_t N = job->schedule[0];
+    if (count != N) return fail(IS_EINVAL, "static MIS assignment uniforms: one per sample");
+    std::vector<double> u;
+    int rc = export_purpose(job, 0, N, 1, kPurposeAssignment, true, u);
+    if (rc) return rc;
+    std::memcpy(out, u.data(), u.size() * sizeof(double));
+    return IS_OK;
+  }
   if (count != job->K) return fail(IS_EINVAL, "resampling uniforms: K values");
   std::vector<double> u;
   int rc = export_purpose(job, round, job->K, 1, kPurposeResampling, true, u);
@@ -1334,7 +1514,8 @@ int is_job_get_round_ids(is_job* job, int64_t* out) {
 
 int is_job_get_proposal_ids(is_job* job, int64_t* out) {
   REQUIRE_RESULTS(job);
-  if (job->method != IS_DMPMC) return fail(IS_ESTATE, "proposal ids exist for DM-PMC only");
+  if (job->method != IS_DMPMC && job->method != IS_MIS)
+    return fail(IS_ESTATE, "proposal ids exist for DM-PMC and static MIS only");
   return copy_ids(job, job->proposal_ids, out);
 }
 
@@ -1352,7 +1533,8 @@ int is_job_get_round_summary(is_job* job, double* ess, double* lognormalizers, d
 
 int is_job_get_proposal(is_job* job, double* location, double* factor, double* lognormalizer) {
   if (!job) return fail(IS_EINVAL, "null job");
-  if (job->method == IS_PLAIN || job->method == IS_DMPMC) return fail(IS_ESTATE, "no adaptive proposal");
+  if (job->method == IS_PLAIN || job->method == IS_DMPMC || job->method == IS_MIS)
+    return fail(IS_ESTATE, "no adaptive proposal");
   const int d = job->d;
   if (location) std::memcpy(location, job->committed.mu.data(), d * sizeof(double));
   if (factor) {

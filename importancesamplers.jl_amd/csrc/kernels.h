@@ -37,8 +37,17 @@ struct RoundArgs {
   int append_slot;    // component appended to retained samples
   const T* logmass;   // per-slot log mixture mass (indexed by slot)
   int gen_slot;       // generating slot when assign == nullptr
-  const int32_t* assign;    // per-new-sample generating slot (DM-PMC)
-  const int32_t* bank_ids;  // per-slot proposal id (DM-PMC)
+  const int32_t* assign;    // per-new-sample generating slot (DM-PMC, static MIS)
+  const int32_t* bank_ids;  // per-slot proposal id (DM-PMC, static MIS)
+  // per-sample mixture terms (static MIS): 0 the slots [k0, k1] for every sample;
+  // 1 the generating slot alone (log coefficient 0); 2 the members of the
+  // generating slot's group, mis_gmem[mis_goff[g] .. mis_goff[g + 1]) with log
+  // coefficients mis_gcoef, g = mis_slot_group[generating slot]
+  int mis_terms;
+  const int32_t* mis_slot_group;
+  const int32_t* mis_goff;
+  const int32_t* mis_gmem;
+  const T* mis_gcoef;
   T logtotal;
   Family<T> fam;
   double dof;
@@ -365,6 +374,32 @@ static __global__ __launch_bounds__(64) void k_finish(const double* __restrict__
       od[slot_lognorm_off(d)] = lognorm;
     f.fit_status[f.round] = st;
   }
+}
+
+// ---------------------------------------------------------------- static MIS assignment
+// Generating slot of every sample (src/methods/static_mis.jl:172-218): the
+// sample's uniform u (counter RNG, or the prefilled buffer), mapped to
+// (i + u) / n under stratified assignment or kept as is, then the first slot
+// whose cumulative mass is >= the mapped value (binary search over the K
+// active slots; the last cumulative mass is exactly 1).
+static __global__ __launch_bounds__(256) void k_mis_assign(RngKey key, const unsigned long long* __restrict__ run_hi,
+                                                          int64_t n, int stratified,
+                                                          const double* __restrict__ cdf, int K,
+                                                          const double* __restrict__ pre_uniforms,
+                                                          int32_t* __restrict__ assign) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double u = pre_uniforms ? pre_uniforms[i] : rng_uniform(key, (uint64_t)i, *run_hi, kPurposeAssignment, 0);
+  const double v = stratified ? ((double)i + u) / (double)n : u;
+  int lo = 0, hi = K - 1;
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if (v <= cdf[mid])
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  assign[i] = lo;
 }
 
 // ---------------------------------------------------------------- order statistic (NPMC clip)

@@ -218,27 +218,52 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
   }
   T m = -T(__builtin_huge_val()), sum = T(0);
   int genbad = 0;
-  for (int base = 0; base < ncomp; base += chunk) {
-    const int cnt = min(chunk, ncomp - base);
-    if (base > 0) {
-      __syncthreads();
-      const T* src = a.slots + (int64_t)(a.k0 + base) * a.stride;
-      for (int q = tid; q < cnt * a.stride; q += blockDim.x) comp[q] = src[q];
-      __syncthreads();
-    }
-    if (ok) {
-      // first kk >= base with (kk + 1) mod W == wave
-      int kk = base + ((wave + W - 1 - base % W) % W + W) % W;
-      for (; kk < base + cnt; kk += W) {
-        const int k = a.k0 + kk;
-        const T q = slot_logdensity<T, D>(comp + (kk - base) * a.stride, d, a.fam, x, y);
+  if (a.mis_terms != 0) {
+    // static MIS: each sample's own terms (the generating slot, or its group;
+    // src/methods/static_mis.jl:383-468), evaluated by wave 0 from the slot table
+    if (wave == 0 && ok) {
+      int mb = gen, me = gen + 1;
+      if (a.mis_terms == 2) {
+        const int g = a.mis_slot_group[gen];
+        mb = a.mis_goff[g];
+        me = a.mis_goff[g + 1];
+      }
+      for (int mi = mb; mi < me; ++mi) {
+        const int k = a.mis_terms == 2 ? a.mis_gmem[mi] : gen;
+        const T q = slot_logdensity<T, D>(a.slots + (int64_t)k * a.stride, d, a.fam, x, y);
         if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
-        const T v = a.logmass[k] + q;
+        const T v = (a.mis_terms == 2 ? a.mis_gcoef[mi] : T(0)) + q;
         if (v > m) {
           sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
           m = v;
         } else {
           sum += LseAcc<T>::fast_exp(v - m);
+        }
+      }
+    }
+  } else {
+    for (int base = 0; base < ncomp; base += chunk) {
+      const int cnt = min(chunk, ncomp - base);
+      if (base > 0) {
+        __syncthreads();
+        const T* src = a.slots + (int64_t)(a.k0 + base) * a.stride;
+        for (int q = tid; q < cnt * a.stride; q += blockDim.x) comp[q] = src[q];
+        __syncthreads();
+      }
+      if (ok) {
+        // first kk >= base with (kk + 1) mod W == wave
+        int kk = base + ((wave + W - 1 - base % W) % W + W) % W;
+        for (; kk < base + cnt; kk += W) {
+          const int k = a.k0 + kk;
+          const T q = slot_logdensity<T, D>(comp + (kk - base) * a.stride, d, a.fam, x, y);
+          if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
+          const T v = a.logmass[k] + q;
+          if (v > m) {
+            sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
+            m = v;
+          } else {
+            sum += LseAcc<T>::fast_exp(v - m);
+          }
         }
       }
     }

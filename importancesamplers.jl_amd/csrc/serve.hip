@@ -37,21 +37,30 @@ struct OwnedDesc {
   is_job_desc d{};
   std::vector<int64_t> round_size;
   std::vector<double> location, scale, bank_locations, bank_scales, bank_masses, target_params;
+  std::vector<int32_t> group_offsets, group_members;
 
   int copy_from(const is_job_desc& s) {
     d = s;
-    const int T = s.method == IS_PLAIN ? 1 : s.rounds;
+    const int T = (s.method == IS_PLAIN || s.method == IS_MIS) ? 1 : s.rounds;
     const int dim = s.dim;
     if (T < 1 || !s.round_size || dim < 1) return fail(IS_EINVAL, "descriptor needs rounds, round sizes and dim");
     round_size.assign(s.round_size, s.round_size + T);
     const int scale_len = s.scale_kind == IS_SPHERICAL ? 1 : s.scale_kind == IS_DIAGONAL ? dim : dim * dim;
-    if (s.method == IS_DMPMC) {
+    if (s.method == IS_DMPMC || s.method == IS_MIS) {
       const int K = s.bank_size;
       if (K < 1 || !s.bank_locations || !s.bank_scales || !s.bank_masses)
-        return fail(IS_EINVAL, "DM-PMC needs a proposal bank");
+        return fail(IS_EINVAL, "a proposal bank is required");
       bank_locations.assign(s.bank_locations, s.bank_locations + (size_t)K * dim);
       bank_scales.assign(s.bank_scales, s.bank_scales + (size_t)K * scale_len);
       bank_masses.assign(s.bank_masses, s.bank_masses + K);
+      if (s.method == IS_MIS && s.mis_scheme == IS_MIS_PARTIAL) {
+        const int G = s.mis_group_count;
+        if (G < 1 || !s.mis_group_offsets || !s.mis_group_members) return fail(IS_EINVAL, "partial-MIS groups cannot be empty");
+        group_offsets.assign(s.mis_group_offsets, s.mis_group_offsets + G + 1);
+        const int nm = group_offsets[G];
+        if (nm < 0) return fail(IS_EINVAL, "invalid partial-MIS group offsets");
+        group_members.assign(s.mis_group_members, s.mis_group_members + nm);
+      }
     } else {
       if (!s.location || !s.scale) return fail(IS_EINVAL, "proposal location and scale are required");
       location.assign(s.location, s.location + dim);
@@ -66,6 +75,8 @@ struct OwnedDesc {
     d.bank_scales = bank_scales.empty() ? nullptr : bank_scales.data();
     d.bank_masses = bank_masses.empty() ? nullptr : bank_masses.data();
     d.target_params = target_params.empty() ? nullptr : target_params.data();
+    d.mis_group_offsets = group_offsets.empty() ? nullptr : group_offsets.data();
+    d.mis_group_members = group_members.empty() ? nullptr : group_members.data();
     return IS_OK;
   }
 };

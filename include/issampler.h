@@ -43,12 +43,19 @@ enum is_method {
   IS_PLAIN = 0,   /* ImportanceSampling(proposal; nsamples), src/methods/importance_sampling.jl */
   IS_AMIS = 1,    /* AMIS(proposal; rounds, round_size), src/methods/amis.jl:42-47 */
   IS_NPMC = 2,    /* NPMC(proposal; rounds, round_size), src/methods/npmc.jl:21-32 */
-  IS_DMPMC = 3    /* DeterministicMixturePMC(bank; rounds, round_size), src/methods/dm_pmc.jl:72-120 */
+  IS_DMPMC = 3,   /* DeterministicMixturePMC(bank; rounds, round_size), src/methods/dm_pmc.jl:72-120 */
+  IS_MIS = 4      /* ImportanceSampling(bank::ProposalBank; nsamples, mis_scheme), src/methods/static_mis.jl */
 };
 enum is_family { IS_GAUSSIAN = 0, IS_STUDENT_T = 1 };             /* GaussianFamily / StudentTFamily */
 enum is_scale_kind { IS_SPHERICAL = 0, IS_DIAGONAL = 1, IS_FACTOR = 2 }; /* src/proposals.jl:158-169 */
 enum is_precision { IS_FP64 = 0, IS_FP32 = 1 };
 enum is_resampling { IS_MULTINOMIAL = 0, IS_SYSTEMATIC = 1 };     /* MultinomialResampling; systematic is new */
+enum is_mis_scheme {       /* static MIS designs (src/methods/static_mis.jl:100-147)             */
+  IS_MIS_STRATIFIED = 0,   /* StratifiedMixture: stratified assignment, full mixture             */
+  IS_MIS_RANDOM = 1,       /* RandomMixture: independent assignment, full mixture                */
+  IS_MIS_STANDARD = 2,     /* StandardMIS: stratified assignment, generating proposal only       */
+  IS_MIS_PARTIAL = 3       /* PartialDeterministicMixture(groups): stratified, the slot's group  */
+};
 enum is_target_kind {
   IS_TARGET_STANDARD = 0,  /* -sum(abs2, x)/2                                              */
   IS_TARGET_GAUSSIAN = 1,  /* params: mean[d], L[d*d] column-major lower, lognormalizer     */
@@ -84,7 +91,7 @@ typedef struct is_job_desc {
   /* single proposal (plain, AMIS, NPMC) */
   const double* location;  /* d */
   const double* scale;     /* spherical: 1, diagonal: d, factor: d*d column-major lower */
-  /* proposal bank (DM-PMC): configured order, zero masses are inert */
+  /* proposal bank (DM-PMC, static MIS): configured order, zero masses are inert */
   int32_t bank_size;       /* K */
   const double* bank_locations; /* d*K column-major */
   const double* bank_scales;    /* spherical: K, diagonal: d*K, factor: d*d*K */
@@ -98,6 +105,11 @@ typedef struct is_job_desc {
   int32_t precision;       /* enum is_precision */
   uint64_t seed;           /* counter-RNG key word 0 */
   uint64_t stream;         /* counter-RNG key word 1 */
+  /* static MIS (method IS_MIS, the bank fields above hold the ProposalBank) */
+  int32_t mis_scheme;      /* enum is_mis_scheme */
+  int32_t mis_group_count; /* partial mixture: number of groups G */
+  const int32_t* mis_group_offsets; /* G + 1 offsets into the members, starting at 0 */
+  const int32_t* mis_group_members; /* configured 1-based proposal ids, in group order, each id once */
 } is_job_desc;
 
 /* Failure snapshot: _device_failure_snapshot (src/execution.jl:719-734) plus

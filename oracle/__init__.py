@@ -522,3 +522,121 @@ def ridge_factor(factor, family=0, dof=0.0):
     d = f.shape[0]
     a = _f64(f.T.ravel())
     return L.or_ridge_factor(_p(a), d, int(family), float(dof))
+
+
+# ---------------------------------------------------------------- static MIS (numpy restatement)
+# src/proposal_banks.jl:301-326 (active bank), src/methods/static_mis.jl:172-218
+# (assignment), :245-310 (partial groups), :383-468 (denominators), :526-592
+# (log-weights).  Pinned in tests/test_oracle_static_mis.py by the reference's
+# assignment truth table and partial-group compilation (test/static_mis.jl:31-67,
+# 380-432), the zero-mass inertness and mixture identities (:185-247, :345-378)
+# and a brute-force partial denominator (:434-477).
+def mis_active_bank(masses):
+    """Active slots (configured ids with nonzero mass, stably sorted by mass), cumulative
+    masses with the last one exactly 1, and log interval masses less their logsumexp."""
+    masses = _f64(masses)
+    ids = [i + 1 for i in sorted((i for i in range(masses.size) if masses[i] != 0), key=lambda i: masses[i])]
+    m = np.array([masses[i - 1] for i in ids])
+    m = m / m.sum()
+    cdf = np.cumsum(m)
+    cdf[-1] = 1.0
+    prev = np.concatenate([[0.0], cdf[:-1]])
+    if np.any(cdf <= prev):
+        raise ValueError("every positive proposal mass must retain a positive assignment interval")
+    lm = np.log(cdf - prev)
+    mx = lm.max()
+    lm = lm - (mx + math.log(np.sum(np.exp(lm - mx))))
+    return ids, cdf, lm
+
+
+def mis_assignment(cdf, u, i, n, stratified):
+    """0-based slot of sample i (0-based): the first slot whose cumulative mass is >= the
+    mapped uniform, (i + u) / n when stratified, u otherwise."""
+    v = ((i + u) / n) if stratified else u
+    lo, hi = 0, len(cdf) - 1
+    while lo < hi:
+        mid = lo + (hi - lo) // 2
+        if v <= cdf[mid]:
+            hi = mid
+        else:
+            lo = mid + 1
+    return lo
+
+
+def mis_partial_groups(ids, logmass, groups, nproposals):
+    """Per active slot its group, the group offsets, the members (0-based slots, in the
+    listed order, inactive ids skipped, empty groups dropped) and log coefficients."""
+    slot_of = {pid: s for s, pid in enumerate(ids)}
+    seen = set()
+    slot_group = [0] * len(ids)
+    offsets, members, coef = [0], [], []
+    for g in groups:
+        if not g:
+            raise ValueError("partial-MIS groups cannot be empty")
+        first = len(members)
+        for pid in g:
+            if not 1 <= pid <= nproposals or pid in seen:
+                raise ValueError("invalid partial-MIS proposal ID")
+            seen.add(pid)
+            if pid in slot_of:
+                members.append(slot_of[pid])
+        if len(members) == first:
+            continue
+        glm = -math.inf
+        for s in members[first:]:
+            glm = logaddexp(glm, logmass[s])
+        for s in members[first:]:
+            slot_group[s] = len(offsets) - 1
+            coef.append(logmass[s] - glm)
+        offsets.append(len(members))
+    if len(seen) != nproposals:
+        raise ValueError("partial-MIS groups must include every proposal ID exactly once")
+    return slot_group, offsets, members, coef
+
+
+def _draw_factor(p):
+    s = as_slot(p)
+    if s.kind == 0:
+        return np.eye(s.d) * s.scale[0]
+    if s.kind == 1:
+        return np.diag(s.scale)
+    return s.scale.reshape(s.d, s.d).T  # lower factor (stored column-major)
+
+
+def static_mis(props, masses, scheme, groups, normals, uniforms, tkind, tparams,
+               radial_normals=None, radial_uniforms=None):
+    """scheme 0 stratified mixture, 1 random mixture, 2 standard MIS, 3 partial mixture.
+    normals (n, d), uniforms (n,): the sample's standard normals and assignment uniform;
+    Student-t banks take rs radial normals (and us uniforms) per sample.  Returns the
+    samples, log-weights, log targets and 1-based proposal ids."""
+    ids, cdf, lm = mis_active_bank(masses)
+    n = len(uniforms)
+    normals = _f64(normals).reshape(n, -1)
+    slots = [props[pid - 1] for pid in ids]
+    if scheme == 3:
+        slot_group, offsets, members, coef = mis_partial_groups(ids, lm, groups, len(props))
+    d = as_slot(slots[0]).d
+    xs, lw, lt, pid = np.zeros((n, d)), np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        g = mis_assignment(cdf, uniforms[i], i, n, scheme != 1)
+        sl = as_slot(slots[g])
+        mult = 1.0
+        if sl.family == 1:
+            mult = student_t_multiplier(sl.dof, radial_normals[i],
+                                        None if radial_uniforms is None else radial_uniforms[i])
+        x = sl.location + _draw_factor(sl) @ (normals[i] * mult)
+        if scheme in (0, 1):
+            terms = [lm[k] + proposal_logdensity(slots[k], x) for k in range(len(slots))]
+        elif scheme == 2:
+            terms = [proposal_logdensity(slots[g], x)]
+        else:
+            grp = slot_group[g]
+            terms = [coef[m] + proposal_logdensity(slots[members[m]], x)
+                     for m in range(offsets[grp], offsets[grp + 1])]
+        den = -math.inf
+        for t_ in terms:
+            den = logaddexp(den, t_)
+        tv = target(tkind, tparams, x)
+        xs[i], lt[i], lw[i], pid[i] = x, tv, tv - den, ids[g]
+    return {"samples": xs, "logweights": lw, "logtargets": lt, "proposal_ids": pid, "ids": ids, "cdf": cdf,
+            "logmass": lm}
