@@ -40,6 +40,13 @@ def dims():
     return [int(v) for v in re.findall(r"X\((\d+)\)", body)]
 
 
+def _unit_cost(extra):
+    for flag in extra:
+        if flag.startswith("-DIS_DIM="):
+            return int(flag.split("=")[1])
+    return 20 if extra else 10  # batch kernels, then the host-side units
+
+
 def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(line for line in res.stderr.splitlines() if NOISE not in line)
@@ -59,7 +66,10 @@ def build(verbose=False, jobs=None):
     for d in dims():
         units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}.o"), [f"-DIS_DIM={d}"]))
     jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)))
-    cmds = [[hipcc, *FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
+    # the wide dimension units take the longest (their density loops unroll over up to
+    # 64 coordinates): start them first so the pool stays full to the end
+    order = sorted(range(len(units)), key=lambda i: -_unit_cost(units[i][2]))
+    cmds = [[hipcc, *FLAGS, *units[i][2], "-c", units[i][0], "-o", units[i][1]] for i in order]
     with ThreadPoolExecutor(max_workers=jobs) as pool:
         results = list(pool.map(_run, cmds))
     failed = [(cmd, err) for cmd, (rc, err) in zip(cmds, results) if rc != 0]
