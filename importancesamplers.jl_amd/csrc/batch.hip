@@ -155,9 +155,9 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
   if ((rc = b->status_dev.alloc((size_t)b->njobs * sizeof(int)))) return rc;
   if ((rc = b->queue.alloc(64))) return rc;
   if ((rc = b->run_dev.alloc(64))) return rc;
-  HIP_TRY(hipMemset(b->run_dev.p, 0, 64));
-  HIP_TRY(hipMemset(b->slots64.p, 0, b->slots64.bytes));
-  HIP_TRY(hipMemset(b->slotsT.p, 0, b->slotsT.bytes));
+  HIP_TRY(locked_memset(b->run_dev.p, 0, 64));
+  HIP_TRY(locked_memset(b->slots64.p, 0, b->slots64.bytes));
+  HIP_TRY(locked_memset(b->slotsT.p, 0, b->slotsT.bytes));
   // data in MFMA fragment order, committed slots, job table
   std::vector<T> xp(nxp), yp(nyp);
   std::vector<double> c64(ncom);
@@ -206,12 +206,12 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
     J.fail = b->fail.as<unsigned long long>() + 3 * H.round_off;
   }
   std::vector<T> cT(c64.begin(), c64.end());
-  HIP_TRY(hipMemcpy(b->xpack.p, xp.data(), nxp * sizeof(T), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->ypack.p, yp.data(), nyp * sizeof(T), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->com64.p, c64.data(), ncom * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->comT.p, cT.data(), ncom * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(b->xpack.p, xp.data(), nxp * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(b->ypack.p, yp.data(), nyp * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(b->com64.p, c64.data(), ncom * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(b->comT.p, cT.data(), ncom * sizeof(T), hipMemcpyHostToDevice));
   if ((rc = b->jobs_dev.alloc(table.size() * sizeof(BatchJob<T>)))) return rc;
-  HIP_TRY(hipMemcpy(b->jobs_dev.p, table.data(), table.size() * sizeof(BatchJob<T>), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(b->jobs_dev.p, table.data(), table.size() * sizeof(BatchJob<T>), hipMemcpyHostToDevice));
   return IS_OK;
 }
 
@@ -232,7 +232,7 @@ void lpt_order(is_batch* b) {
 int upload_order(is_batch* b) {
   int rc = b->order_dev.alloc((size_t)b->njobs * sizeof(int32_t));
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(b->order_dev.p, b->order.data(), (size_t)b->njobs * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(b->order_dev.p, b->order.data(), (size_t)b->njobs * sizeof(int32_t), hipMemcpyHostToDevice));
   b->graph_dirty = true;
   return IS_OK;
 }
@@ -275,6 +275,7 @@ int batch_run_typed(is_batch* b, int W, int bpc, int commit, bool eager) {
   const bool timed = b->timing || eager;
   if (!timed) {
     if (b->graph_dirty || !b->graph_exec) {
+      CaptureGuard cg(capture_lock());
       if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->graph_exec = nullptr;
@@ -322,7 +323,7 @@ bool tuning_fits(const is_batch* b, int W, int bpc) {
 template <class T>
 int copy_job_T(is_batch* b, const DevBuf& pool, size_t off, int64_t n, double* out) {
   std::vector<T> h((size_t)n);
-  HIP_TRY(hipMemcpy(h.data(), pool.as<T>() + off, (size_t)n * sizeof(T), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(h.data(), pool.as<T>() + off, (size_t)n * sizeof(T), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < n; ++i) out[i] = (double)h[(size_t)i];
   return IS_OK;
 }
@@ -418,7 +419,10 @@ int is_batch_create(is_ctx* ctx, const is_job_desc* descs, int32_t njobs, int32_
   if (batch_lds(b, 2) > (size_t)160 * 1024) return fail(IS_EUNSUPPORTED, "the largest job's slots do not fit in LDS");
   b->waves = batch_lds(b, 4) <= (size_t)160 * 1024 ? 4 : 2;
   b->blocks_per_cu = 1;
-  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  {
+    CaptureGuard cg(capture_lock());
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  }
   HIP_TRY(hipEventCreate(&b->ev0));
   HIP_TRY(hipEventCreate(&b->ev1));
   int rc = precision == IS_FP32 ? batch_upload<float>(b, Xs, ys) : batch_upload<double>(b, Xs, ys);
@@ -432,6 +436,7 @@ int is_batch_create(is_ctx* ctx, const is_job_desc* descs, int32_t njobs, int32_
 int is_batch_destroy(is_batch* b) {
   if (!b) return IS_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  CaptureGuard cg(capture_lock());
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -528,8 +533,8 @@ int is_batch_job_failure(is_batch* b, int32_t job, is_failure* out) {
   const BatchJobHost& H = b->jobs[job];
   std::vector<uint64_t> rec((size_t)3 * H.rounds);
   std::vector<int> fit(H.rounds);
-  HIP_TRY(hipMemcpy(rec.data(), b->fail.as<uint64_t>() + 3 * H.round_off, rec.size() * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(fit.data(), b->fit.as<int>() + H.round_off, fit.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(rec.data(), b->fail.as<uint64_t>() + 3 * H.round_off, rec.size() * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(fit.data(), b->fit.as<int>() + H.round_off, fit.size() * sizeof(int), hipMemcpyDeviceToHost));
   is_failure f{};
   for (int t = 0; t < H.rounds; ++t) {
     const uint64_t* w = rec.data() + 3 * t;
@@ -595,9 +600,9 @@ int is_batch_job_round_summary(is_batch* b, int32_t job, double* ess, double* lo
   if (!b->have_results) return fail(IS_ESTATE, "the batch has not run");
   const BatchJobHost& H = b->jobs[job];
   if (ess)
-    HIP_TRY(hipMemcpy(ess, b->ress.as<double>() + H.round_off, H.rounds * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(locked_memcpy(ess, b->ress.as<double>() + H.round_off, H.rounds * sizeof(double), hipMemcpyDeviceToHost));
   if (lognormalizers)
-    HIP_TRY(hipMemcpy(lognormalizers, b->rlogz.as<double>() + H.round_off, H.rounds * sizeof(double),
+    HIP_TRY(locked_memcpy(lognormalizers, b->rlogz.as<double>() + H.round_off, H.rounds * sizeof(double),
                       hipMemcpyDeviceToHost));
   return IS_OK;
 }
@@ -607,8 +612,8 @@ int is_batch_final_summary(is_batch* b, double* ess, double* lognormalizers) {
   if (!b->have_results) return fail(IS_ESTATE, "the batch has not run");
   const size_t nround = b->ress.bytes / sizeof(double);
   std::vector<double> e(nround), z(nround);
-  HIP_TRY(hipMemcpy(e.data(), b->ress.p, nround * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(z.data(), b->rlogz.p, nround * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(e.data(), b->ress.p, nround * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(z.data(), b->rlogz.p, nround * sizeof(double), hipMemcpyDeviceToHost));
   for (int j = 0; j < b->njobs; ++j) {
     const BatchJobHost& H = b->jobs[j];
     if (ess) ess[j] = e[H.round_off + H.rounds - 1];
@@ -622,7 +627,7 @@ int is_batch_job_history(is_batch* b, int32_t job, double* means, double* factor
   if (!b->have_results) return fail(IS_ESTATE, "the batch has not run");
   const BatchJobHost& H = b->jobs[job];
   std::vector<double> slots((size_t)(H.rounds + 1) * H.SS);
-  HIP_TRY(hipMemcpy(slots.data(), b->slots64.as<double>() + H.slot_off, slots.size() * sizeof(double),
+  HIP_TRY(locked_memcpy(slots.data(), b->slots64.as<double>() + H.slot_off, slots.size() * sizeof(double),
                     hipMemcpyDeviceToHost));
   const int d = H.d;
   for (int s = 0; s <= H.rounds; ++s)
@@ -635,7 +640,7 @@ int is_batch_job_proposal(is_batch* b, int32_t job, double* location, double* fa
   BATCH_JOB_CHECK(b, job);
   const BatchJobHost& H = b->jobs[job];
   std::vector<double> slot(H.SS);
-  HIP_TRY(hipMemcpy(slot.data(), b->com64.as<double>() + H.com_off, slot.size() * sizeof(double),
+  HIP_TRY(locked_memcpy(slot.data(), b->com64.as<double>() + H.com_off, slot.size() * sizeof(double),
                     hipMemcpyDeviceToHost));
   unpack_bslot(slot.data(), H.d, H.P, location, factor, lognormalizer);
   return IS_OK;

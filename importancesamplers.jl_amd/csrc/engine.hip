@@ -129,8 +129,18 @@ struct is_job {
   // device buffers
   DevBuf X, logtargets, lognums, logweights, round_ids, proposal_ids;
   DevBuf slots64, slotsT, logmassT, zerosT, tslotT, assign, bank_ids_d;
-  DevBuf partials, partials2, reduced, reduced2, summary, counter, fail_rec, fit_status, cdf_status;
-  DevBuf round_ess, round_logz, adapt_ess, clip, cdf, tile_total, uniforms, ancestors, scratch;
+  DevBuf partials, partials2, reduced, reduced2, summary, clip, cdf, tile_total, uniforms, ancestors, scratch;
+  // per-run control block: one device allocation holding the run word, the
+  // finish counters, the failure records and the round summaries.  Each run
+  // uploads [0, ctl_up_bytes) (run word + zeroed status) from a pinned image and
+  // reads [ctl_down_off, ctl.bytes) (status + round summaries) back in one copy.
+  DevBuf ctl;
+  DevView run_dev, counter, fail_rec, fit_status, cdf_status, round_ess, round_logz, adapt_ess;
+  size_t ctl_up_bytes = 0, ctl_down_off = 0;
+  HostPinned ctl_up, ctl_down;
+  // pinned staging for the committed slot upload and the commit read-back
+  HostPinned slot_up, slot_down;
+  bool slots_resident = false;  // plain IS / static MIS: slot table already holds the committed state
   DevBuf pre_normals, pre_rn, pre_ru, pre_uniforms;
   std::vector<int> prefilled, prefilled_radial, prefilled_uniforms;
   // last run
@@ -138,10 +148,8 @@ struct is_job {
   int last_status = 0;
   is_failure failure{};
   std::vector<int64_t> ancestors_host;
-  // run counter word on the device (run index << 32): RNG counters read it, so the
-  // captured launch sequence stays valid across runs
-  DevBuf run_dev;
-  uint64_t run_word_host = 0;
+  // the run word (run index << 32) lives in the control block: RNG counters read
+  // it from the device, so the captured launch sequence stays valid across runs
   // the run's launch sequence captured as a HIP graph (replayed while valid)
   int use_graph = 1;
   bool graph_dirty = true;
@@ -344,11 +352,11 @@ int setup_mis(is_job* j, const is_job_desc* desc, const std::vector<double>& mas
     ints.insert(ints.end(), j->mis_gmem.begin(), j->mis_gmem.end());
     int rc;
     if ((rc = j->mis_ints_d.alloc(ints.size() * sizeof(int32_t)))) return rc;
-    HIP_TRY(hipMemcpy(j->mis_ints_d.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(locked_memcpy(j->mis_ints_d.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   }
   int rc;
   if ((rc = j->mis_cdf_d.alloc((size_t)K * sizeof(double)))) return rc;
-  HIP_TRY(hipMemcpy(j->mis_cdf_d.p, j->mis_cdf.data(), (size_t)K * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(j->mis_cdf_d.p, j->mis_cdf.data(), (size_t)K * sizeof(double), hipMemcpyHostToDevice));
   if ((rc = j->assign.alloc((size_t)j->schedule[0] * sizeof(int32_t)))) return rc;
   return IS_OK;
 }
@@ -363,20 +371,10 @@ int64_t isqrt64(int64_t n) {
 uint64_t round_word(const is_job* j, int t) { return ((uint64_t)j->run_index << 32) | (uint64_t)t; }
 
 template <class T>
-int upload_slots(is_job* j) {
-  // FP64 master -> T copy
-  std::vector<double> h((size_t)j->nslots * j->stride);
-  HIP_TRY(hipMemcpy(h.data(), j->slots64.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-  std::vector<T> t(h.begin(), h.end());
-  HIP_TRY(hipMemcpy(j->slotsT.p, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
-  return IS_OK;
-}
-
-template <class T>
 int upload_T(DevBuf& buf, const std::vector<double>& v) {
   std::vector<T> t(v.begin(), v.end());
   if (buf.alloc(std::max<size_t>(1, t.size()) * sizeof(T))) return IS_ENOMEM;
-  if (!t.empty()) HIP_TRY(hipMemcpy(buf.p, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!t.empty()) HIP_TRY(locked_memcpy(buf.p, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
   return IS_OK;
 }
 
@@ -745,35 +743,36 @@ template <class T>
 int job_run_typed(is_job* j) {
   hipStream_t s = j->stream;
   const int T_ = j->T;
-  HIP_TRY(hipMemsetAsync(j->fail_rec.p, 0, j->fail_rec.bytes, s));
-  HIP_TRY(hipMemsetAsync(j->fit_status.p, 0, j->fit_status.bytes, s));
-  HIP_TRY(hipMemsetAsync(j->cdf_status.p, 0, j->cdf_status.bytes, s));
-  HIP_TRY(hipMemsetAsync(j->counter.p, 0, j->counter.bytes, s));
-  // committed state into slot 0 (AMIS / NPMC) or the bank slots (DM-PMC, static MIS)
+  // run word and zeroed status from the pinned image (one copy, no host wait)
+  j->ctl_up.as<uint64_t>()[0] = (uint64_t)j->run_index << 32;
+  HIP_TRY(hipMemcpyAsync(j->ctl.p, j->ctl_up.p, j->ctl_up_bytes, hipMemcpyHostToDevice, s));
+  // committed state into slot 0 (AMIS / NPMC) or the bank slots (DM-PMC, static MIS);
+  // plain IS and static MIS never change it, so it is uploaded once
   const int d = j->d;
-  if (j->method == IS_DMPMC || j->method == IS_MIS) {
-    std::vector<double> slots(j->bank_slots_init);
-    for (int k = 0; k < j->K; ++k)
-      for (int c = 0; c < d; ++c) slots[(size_t)k * j->stride + c] = j->committed_locs[(size_t)k * d + c];
-    HIP_TRY(hipMemcpyAsync(j->slots64.p, slots.data(), slots.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    std::vector<T> st(slots.begin(), slots.end());
-    HIP_TRY(hipMemcpyAsync(j->slotsT.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-  } else {
-    std::vector<double> slot(j->stride);
-    pack_slot(j->committed, d, 0.0, slot.data());
-    std::vector<T> st(slot.begin(), slot.end());
-    HIP_TRY(hipMemcpyAsync(j->slots64.p, slot.data(), slot.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(j->slotsT.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+  const bool fixed_state = j->method == IS_PLAIN || j->method == IS_MIS;
+  if (!(fixed_state && j->slots_resident)) {
+    const bool bank = j->method == IS_DMPMC || j->method == IS_MIS;
+    const size_t n = (size_t)(bank ? j->K : 1) * j->stride;
+    double* h64 = j->slot_up.as<double>();
+    T* hT = reinterpret_cast<T*>(h64 + n);
+    if (bank) {
+      std::copy(j->bank_slots_init.begin(), j->bank_slots_init.begin() + n, h64);
+      for (int k = 0; k < j->K; ++k)
+        for (int c = 0; c < d; ++c) h64[(size_t)k * j->stride + c] = j->committed_locs[(size_t)k * d + c];
+    } else {
+      pack_slot(j->committed, d, 0.0, h64);
+    }
+    for (size_t i = 0; i < n; ++i) hT[i] = (T)h64[i];
+    HIP_TRY(hipMemcpyAsync(j->slots64.p, h64, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(j->slotsT.p, hT, n * sizeof(T), hipMemcpyHostToDevice, s));
+    j->slots_resident = fixed_state;
   }
-  j->run_word_host = (uint64_t)j->run_index << 32;
-  HIP_TRY(hipMemcpyAsync(j->run_dev.p, &j->run_word_host, sizeof(uint64_t), hipMemcpyHostToDevice, s));
   int rc = IS_OK;
   // per-round event timing records events between launches, so a timed run is issued
   // eagerly; otherwise the whole launch sequence is replayed from one captured graph
   if (j->use_graph && !j->timing) {
     if (j->graph_dirty || !j->graph_exec) {
+      CaptureGuard cg(capture_lock());
       if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
       if (j->graph) (void)hipGraphDestroy(j->graph);
       j->graph_exec = nullptr;
@@ -796,13 +795,20 @@ int job_run_typed(is_job* j) {
     rc = run_rounds<T>(j);
     if (rc) return rc;
   }
-  // failure snapshot (one transfer per run)
-  std::vector<uint64_t> rec((size_t)3 * T_);
-  std::vector<int> fit(T_), cdfs(T_);
-  HIP_TRY(hipMemcpyAsync(rec.data(), j->fail_rec.p, rec.size() * 8, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(fit.data(), j->fit_status.p, fit.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(cdfs.data(), j->cdf_status.p, cdfs.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  // status and round summaries (one transfer), plus the state a successful run commits
+  HIP_TRY(hipMemcpyAsync(j->ctl_down.as<char>() + j->ctl_down_off, j->ctl.as<char>() + j->ctl_down_off,
+                         j->ctl.bytes - j->ctl_down_off, hipMemcpyDeviceToHost, s));
+  if (j->method == IS_AMIS || j->method == IS_NPMC)
+    HIP_TRY(hipMemcpyAsync(j->slot_down.p, j->slots64.as<double>() + (size_t)T_ * j->stride,
+                           (size_t)j->stride * sizeof(double), hipMemcpyDeviceToHost, s));
+  else if (j->method == IS_DMPMC)
+    HIP_TRY(hipMemcpyAsync(j->slot_down.p, j->slots64.p, (size_t)j->K * j->stride * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  const char* down = j->ctl_down.as<const char>();
+  const uint64_t* rec = reinterpret_cast<const uint64_t*>(down + ((const char*)j->fail_rec.p - j->ctl.as<char>()));
+  const int* fit = reinterpret_cast<const int*>(down + ((const char*)j->fit_status.p - j->ctl.as<char>()));
+  const int* cdfs = reinterpret_cast<const int*>(down + ((const char*)j->cdf_status.p - j->ctl.as<char>()));
   j->run_index += 1;
   if (j->timing) {
     j->round_ms.assign(T_, 0.0);
@@ -813,7 +819,7 @@ int job_run_typed(is_job* j) {
   }
   is_failure f{};
   for (int t = 0; t < T_; ++t) {
-    const uint64_t* w = rec.data() + 3 * t;
+    const uint64_t* w = rec + 3 * t;
     if (w[0] != 0) {
       f.round = t + 1;
       f.count = w[0];
@@ -849,13 +855,9 @@ int job_run_typed(is_job* j) {
   }
   // transactional commit (src/adaptive_gaussian_execution.jl:1101-1111)
   if (j->method == IS_AMIS || j->method == IS_NPMC) {
-    std::vector<double> slot(j->stride);
-    HIP_TRY(hipMemcpy(slot.data(), j->slots64.as<double>() + (size_t)T_ * j->stride,
-                      slot.size() * sizeof(double), hipMemcpyDeviceToHost));
-    unpack_slot(slot.data(), d, j->committed);
+    unpack_slot(j->slot_down.as<const double>(), d, j->committed);
   } else if (j->method == IS_DMPMC) {
-    std::vector<double> slots((size_t)j->K * j->stride);
-    HIP_TRY(hipMemcpy(slots.data(), j->slots64.p, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double* slots = j->slot_down.as<const double>();
     for (int k = 0; k < j->K; ++k)
       for (int c = 0; c < d; ++c) j->committed_locs[(size_t)k * d + c] = slots[(size_t)k * j->stride + c];
   }
@@ -875,11 +877,11 @@ int alloc_job_buffers(is_job* j) {
   if ((rc = j->logweights.alloc(cap * sizeof(T)))) return rc;
   if ((rc = j->round_ids.alloc(cap * sizeof(int32_t)))) return rc;
   if ((rc = j->proposal_ids.alloc(cap * sizeof(int32_t)))) return rc;
-  HIP_TRY(hipMemset(j->round_ids.p, 0, cap * sizeof(int32_t)));
+  HIP_TRY(locked_memset(j->round_ids.p, 0, cap * sizeof(int32_t)));
   if ((rc = j->slots64.alloc((size_t)j->nslots * j->stride * sizeof(double)))) return rc;
   if ((rc = j->slotsT.alloc((size_t)j->nslots * j->stride * sizeof(T)))) return rc;
-  HIP_TRY(hipMemset(j->slots64.p, 0, j->slots64.bytes));
-  HIP_TRY(hipMemset(j->slotsT.p, 0, j->slotsT.bytes));
+  HIP_TRY(locked_memset(j->slots64.p, 0, j->slots64.bytes));
+  HIP_TRY(locked_memset(j->slotsT.p, 0, j->slotsT.bytes));
   // partial records (column-major): one per sweep block over at most `cap` samples
   j->pcap = cdiv((int64_t)cap, kSweepPerBlock) + 1;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
@@ -888,17 +890,39 @@ int alloc_job_buffers(is_job* j) {
   if ((rc = j->reduced.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->reduced2.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->summary.alloc((size_t)5 * std::max(1, j->T) * sizeof(double)))) return rc;
-  if ((rc = j->counter.alloc(64))) return rc;
-  if ((rc = j->fail_rec.alloc((size_t)3 * j->T * sizeof(uint64_t)))) return rc;
-  if ((rc = j->fit_status.alloc((size_t)j->T * sizeof(int)))) return rc;
-  if ((rc = j->cdf_status.alloc((size_t)j->T * sizeof(int)))) return rc;
-  if ((rc = j->round_ess.alloc((size_t)j->T * sizeof(double)))) return rc;
-  if ((rc = j->round_logz.alloc((size_t)j->T * sizeof(double)))) return rc;
-  if ((rc = j->adapt_ess.alloc((size_t)j->T * sizeof(double)))) return rc;
   if ((rc = j->clip.alloc(64))) return rc;
-  if ((rc = j->run_dev.alloc(64))) return rc;
-  HIP_TRY(hipMemset(j->run_dev.p, 0, 64));
-  HIP_TRY(hipMemset(j->counter.p, 0, 64));
+  {
+    // control block layout (bytes): run word 64 | counters 64 | failure records 24 T
+    // | fit status 4 T | resampling status 4 T (padded to 8) | ESS 8 T | log Z 8 T | adaptation ESS 8 T
+    const size_t Tn = (size_t)j->T;
+    const size_t o_fail = 128, o_fit = o_fail + 24 * Tn, o_cdf = o_fit + 4 * Tn;
+    const size_t o_ess = (o_cdf + 4 * Tn + 7) & ~(size_t)7, o_logz = o_ess + 8 * Tn, o_aess = o_logz + 8 * Tn;
+    const size_t total = o_aess + 8 * Tn;
+    if ((rc = j->ctl.alloc(total))) return rc;
+    HIP_TRY(locked_memset(j->ctl.p, 0, total));
+    char* base = j->ctl.as<char>();
+    auto view = [&](DevView& v, size_t off, size_t n) {
+      v.p = base + off;
+      v.bytes = n;
+    };
+    view(j->run_dev, 0, 64);
+    view(j->counter, 64, 64);
+    view(j->fail_rec, o_fail, 24 * Tn);
+    view(j->fit_status, o_fit, 4 * Tn);
+    view(j->cdf_status, o_cdf, 4 * Tn);
+    view(j->round_ess, o_ess, 8 * Tn);
+    view(j->round_logz, o_logz, 8 * Tn);
+    view(j->adapt_ess, o_aess, 8 * Tn);
+    j->ctl_up_bytes = o_ess;
+    j->ctl_down_off = o_fail;
+    if ((rc = j->ctl_up.alloc(total)) || (rc = j->ctl_down.alloc(total))) return rc;
+    const bool bank = j->method == IS_DMPMC || j->method == IS_MIS;
+    const size_t nup = (size_t)(bank ? j->K : 1) * j->stride;
+    if ((rc = j->slot_up.alloc(nup * (sizeof(double) + sizeof(T))))) return rc;
+    const size_t ndown = j->method == IS_DMPMC ? (size_t)j->K * j->stride : (size_t)j->stride;
+    if ((rc = j->slot_down.alloc(ndown * sizeof(double)))) return rc;
+    j->slots_resident = false;
+  }
   std::vector<double> zeros(std::max(j->nslots, 1), 0.0);
   if ((rc = upload_T<T>(j->zerosT, zeros))) return rc;
   if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
@@ -1046,11 +1070,11 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
       }
       int rc;
       if ((rc = j->assign.alloc(assign.size() * sizeof(int32_t)))) return rc;
-      HIP_TRY(hipMemcpy(j->assign.p, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIP_TRY(locked_memcpy(j->assign.p, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     int rc;
     if ((rc = j->bank_ids_d.alloc(j->bank_ids.size() * sizeof(int32_t)))) return rc;
-    HIP_TRY(hipMemcpy(j->bank_ids_d.p, j->bank_ids.data(), j->bank_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(locked_memcpy(j->bank_ids_d.p, j->bank_ids.data(), j->bank_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   } else {
     if (!desc->location || !desc->scale) return fail(IS_EINVAL, "proposal location and scale are required");
     std::string err = make_prop(j->family, j->dof, j->kind, d, desc->location, desc->scale, j->initial);
@@ -1113,7 +1137,10 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
       j->t_shift = std::isfinite(c0) ? c0 : 0.0;
     }
   }
-  HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+  {
+    CaptureGuard cg(capture_lock());
+    HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+  }
   int rc = j->precision == IS_FP32 ? alloc_job_buffers<float>(j) : alloc_job_buffers<double>(j);
   if (rc) return rc;
   // logmass tables: AMIS log N_t per slot; DM-PMC per-round log coefficients
@@ -1195,6 +1222,7 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
 int is_job_destroy(is_job* job) {
   if (!job) return IS_OK;
   if (job->stream) (void)hipStreamSynchronize(job->stream);
+  CaptureGuard cg(capture_lock());
   if (job->graph_exec) (void)hipGraphExecDestroy(job->graph_exec);
   if (job->graph) (void)hipGraphDestroy(job->graph);
   for (auto& e : job->events) (void)hipEventDestroy(e);
@@ -1227,10 +1255,10 @@ static int upload_prefill(DevBuf& buf, size_t total_doubles, size_t offset, cons
   if (!buf.p) {
     int rc = buf.alloc(total_doubles * sizeof(double));
     if (rc) return rc;
-    HIP_TRY(hipMemset(buf.p, 0, buf.bytes));
+    HIP_TRY(locked_memset(buf.p, 0, buf.bytes));
   }
   if (offset + n > total_doubles) return fail(IS_EINVAL, "prefilled buffer too large");
-  HIP_TRY(hipMemcpy(buf.as<double>() + offset, src, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(buf.as<double>() + offset, src, n * sizeof(double), hipMemcpyHostToDevice));
   return IS_OK;
 }
 
@@ -1502,7 +1530,7 @@ int is_job_logweight_offset(const is_job* job, double* out) {
 
 static int copy_ids(is_job* job, const DevBuf& src, int64_t* out) {
   std::vector<int32_t> h((size_t)job->total);
-  HIP_TRY(hipMemcpy(h.data(), src.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(h.data(), src.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
   return IS_OK;
 }
@@ -1521,13 +1549,18 @@ int is_job_get_proposal_ids(is_job* job, int64_t* out) {
 
 int is_job_get_round_summary(is_job* job, double* ess, double* lognormalizers, double* adaptation_ess) {
   REQUIRE_RESULTS(job);
-  if (ess) HIP_TRY(hipMemcpy(ess, job->round_ess.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
+  // the run's own read-back holds them (the round kernels are their only writers)
+  const char* down = job->ctl_down.as<const char>();
+  auto host_copy = [&](const DevView& v) {
+    return reinterpret_cast<const double*>(down + ((const char*)v.p - job->ctl.as<char>()));
+  };
+  const size_t n = (size_t)job->T;
+  if (ess) std::memcpy(ess, host_copy(job->round_ess), n * sizeof(double));
   if (lognormalizers) {
-    HIP_TRY(hipMemcpy(lognormalizers, job->round_logz.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
-    for (int t = 0; t < job->T; ++t) lognormalizers[t] += job->t_shift;
+    std::memcpy(lognormalizers, host_copy(job->round_logz), n * sizeof(double));
+    for (size_t t = 0; t < n; ++t) lognormalizers[t] += job->t_shift;
   }
-  if (adaptation_ess)
-    HIP_TRY(hipMemcpy(adaptation_ess, job->adapt_ess.p, (size_t)job->T * sizeof(double), hipMemcpyDeviceToHost));
+  if (adaptation_ess) std::memcpy(adaptation_ess, host_copy(job->adapt_ess), n * sizeof(double));
   return IS_OK;
 }
 
@@ -1552,7 +1585,7 @@ int is_job_get_history(is_job* job, double* means, double* factors, double* logn
   if (job->method != IS_AMIS && job->method != IS_NPMC) return fail(IS_ESTATE, "no proposal history");
   const int d = job->d;
   std::vector<double> slots((size_t)job->nslots * job->stride);
-  HIP_TRY(hipMemcpy(slots.data(), job->slots64.p, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(slots.data(), job->slots64.p, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int s = 0; s < job->nslots; ++s) {
     HostProp p;
     unpack_slot(slots.data() + (size_t)s * job->stride, d, p);
@@ -1574,7 +1607,7 @@ int is_job_get_ancestors(is_job* job, int64_t* out) {
   REQUIRE_RESULTS(job);
   if (job->method != IS_DMPMC) return fail(IS_ESTATE, "ancestors exist for DM-PMC only");
   std::vector<int32_t> h((size_t)job->K * job->T);
-  HIP_TRY(hipMemcpy(h.data(), job->ancestors.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(h.data(), job->ancestors.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
   return IS_OK;
 }
@@ -1643,7 +1676,7 @@ static int job_moments_typed(is_job* job, double* mean, double* var, double* cov
     c0 = job->committed.mu;
   else
     for (int c = 0; c < d; ++c) c0[c] = job->committed_locs[c];
-  HIP_TRY(hipMemcpy(center.p, c0.data(), d * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(center.p, c0.data(), d * sizeof(double), hipMemcpyHostToDevice));
   SweepArgs<T> w = sweep_args<T>(job, 0);
   w.lo = 0;
   w.hi = n;
@@ -1705,7 +1738,7 @@ static int job_resample_typed(is_job* job, const double* uniforms, int64_t count
       (rc = u.alloc((size_t)ucount * sizeof(double))) || (rc = anc.alloc((size_t)count * sizeof(int32_t))) ||
       (rc = st.alloc(sizeof(int))))
     return rc;
-  HIP_TRY(hipMemcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
   hipLaunchKernelGGL((k_cdf_tiles<T>), dim3(ntiles), dim3(kCdfThreads), 0, job->stream, job->logweights.as<T>(), n,
                      job->summary.as<double>(), tiles.as<double>());
   hipLaunchKernelGGL((k_cdf_write<T>), dim3(ntiles), dim3(kCdfThreads), 0, job->stream, job->logweights.as<T>(), n,
@@ -1793,10 +1826,10 @@ int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf) {
   DevBuf lw, out;
   int rc;
   if ((rc = lw.alloc((size_t)n * sizeof(double))) || (rc = out.alloc((size_t)n * sizeof(double)))) return rc;
-  HIP_TRY(hipMemcpy(lw.p, logweights, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(lw.p, logweights, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
   rc = is_cdf_device(ctx, lw.as<double>(), n, out.as<double>(), nullptr);
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(cdf, out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(cdf, out.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   return IS_OK;
 }
 
@@ -1811,14 +1844,14 @@ int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t 
   if ((rc = c.alloc((size_t)n * sizeof(double))) || (rc = lw.alloc((size_t)n * sizeof(double))) ||
       (rc = u.alloc((size_t)ucount * sizeof(double))) || (rc = a.alloc((size_t)count * sizeof(int32_t))))
     return rc;
-  HIP_TRY(hipMemcpy(c.p, cdf, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(lw.p, logweights, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(c.p, cdf, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(lw.p, logweights, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
   rc = is_select_device(ctx, c.as<double>(), lw.as<double>(), n, u.as<double>(), count, method, a.as<int32_t>(),
                         nullptr);
   if (rc) return rc;
   std::vector<int32_t> h((size_t)count);
-  HIP_TRY(hipMemcpy(h.data(), a.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(h.data(), a.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < count; ++i) ancestors[i] = h[i];
   return IS_OK;
 }
@@ -1833,7 +1866,7 @@ int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t 
   hipLaunchKernelGGL(k_rng_raw, dim3(cdiv(n, 256)), dim3(256), 0, 0, key0, key1, c1, c2, c3, n,
                      tmp.as<unsigned long long>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpy(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost));
+  HIP_TRY(locked_memcpy(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost));
   return IS_OK;
 }
 

@@ -172,14 +172,39 @@ inline std::string make_prop(int family, double dof, int kind, int d, const doub
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ---------------------------------------------------------------- capture lock
+// A stream capture on one thread must not overlap a device-wide runtime call on
+// another: hipMalloc, hipFree and the blocking copies and fills may synchronise
+// every stream of the device, which invalidates a capture in progress.  Every
+// capture section and every such call takes this one process-wide lock (it is
+// recursive, so a capture section may reach an allocation on its own thread).
+inline std::recursive_mutex& capture_lock() {
+  static std::recursive_mutex m;
+  return m;
+}
+using CaptureGuard = std::lock_guard<std::recursive_mutex>;
+
+inline hipError_t locked_memcpy(void* dst, const void* src, size_t n, hipMemcpyKind kind) {
+  CaptureGuard g(capture_lock());
+  return hipMemcpy(dst, src, n, kind);
+}
+inline hipError_t locked_memset(void* dst, int v, size_t n) {
+  CaptureGuard g(capture_lock());
+  return hipMemset(dst, v, n);
+}
+
 // ---------------------------------------------------------------- device buffer
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      CaptureGuard g(capture_lock());
+      (void)hipFree(p);
+    }
   }
   int alloc(size_t n) {
+    CaptureGuard g(capture_lock());
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = n;
@@ -188,6 +213,49 @@ struct DevBuf {
       p = nullptr;
       return fail(IS_ENOMEM, "hipMalloc(%zu) failed", n);
     }
+    return IS_OK;
+  }
+  template <class U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+// A part of a DevBuf (no ownership; same accessors).
+struct DevView {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+// Page-locked host memory from hipHostMalloc: copies to and from it are
+// asynchronous on the issuing stream, with no staging through pageable memory.
+struct HostPinned {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostPinned() = default;
+  HostPinned(const HostPinned&) = delete;
+  HostPinned& operator=(const HostPinned&) = delete;
+  ~HostPinned() {
+    if (p) {
+      CaptureGuard g(capture_lock());
+      (void)hipHostFree(p);
+    }
+  }
+  int alloc(size_t n) {
+    CaptureGuard g(capture_lock());
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = n;
+    if (n == 0) return IS_OK;
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return fail(IS_ENOMEM, "hipHostMalloc(%zu) failed", n);
+    }
+    std::memset(p, 0, n);
     return IS_OK;
   }
   template <class U>

@@ -1,5 +1,5 @@
 """GPU tests of the serving loop (continuous batching): a concurrent mix of
-plain IS, AMIS, NPMC and DM-PMC requests gives, for every request, the same
+plain IS, AMIS, NPMC, DM-PMC and static-MIS requests gives, for every request, the same
 results as the request run alone (bit-identical log-weights and log Z);
 admission keeps the admitted jobs within the memory budget; oversized
 requests are refused; a failing request raises its own error while the rest of
@@ -56,11 +56,14 @@ def request_mix(eng, seed=3):
                  "fp32"))
     reqs.append((18, eng.StandardGaussianTarget(), eng.ImportanceSampling(eng.SphericalGaussian(np.zeros(40), 1.2), 50000),
                  "fp32"))
+    mis_bank = eng.ProposalBank([eng.SphericalGaussian(rng.normal(0, 1, 6), 1.3) for _ in range(4)], [1, 2, 0, 3])
+    reqs.append((19, _gauss_target(eng, rng, 6),
+                 eng.ImportanceSampling(mis_bank, 6000, eng.PartialDeterministicMixture(((1, 3), (2, 4)))), "fp64"))
     return reqs
 
 
 def test_serve_results_same_alone_and_in_mix(eng):
-    _say("case: serving mix of 8 requests (plain IS, AMIS, NPMC, DM-PMC; FP32 and FP64), 4 at once")
+    _say("case: serving mix of 9 requests (plain IS, AMIS, NPMC, DM-PMC, static MIS; FP32 and FP64), 4 at once")
     reqs = request_mix(eng)
     srv = eng.Server(memory_budget=64 << 30, max_active=4)
     tickets = [srv.submit(r, t, a, p) for r, t, a, p in reqs]
