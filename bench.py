@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "effective samples/sec at |ΔlogZ|<1e-3 vs FP64; p50/p99 solve latency"
 D, T, N = 10, 50, 20000
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30  # counter-calibration copy size
 
 
 def problem(pkg, seed=20260):
@@ -410,6 +411,40 @@ def main_plain(args):
     print(json.dumps(out))
 
 
+def _traffic(name):
+    """HBM bytes per launch of a kernel from its committed FETCH_SIZE / WRITE_SIZE passes
+    (profiles/<name>_traffic.json, counters scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main_copy(args):
+    """Counter calibration: a plain device-to-device copy of COPY_BYTES, `steps` times.
+    Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE the copy kernel's per-dispatch
+    reading against its known bytes gives the scale that turns the same counters
+    read for a sampler kernel into HBM bytes."""
+    import torch
+
+    n = COPY_BYTES // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert torch.equal(dst[-4:], src[-4:])
+    print(json.dumps({"workload": "device copy (counter calibration)", "bytes_read_per_copy": COPY_BYTES,
+                      "bytes_written_per_copy": COPY_BYTES, "copies": args.steps,
+                      "GB_per_s": 2 * COPY_BYTES * args.steps / el / 1e9}))
+
+
 def _line(workload, dtype, r, steps, warmup):
     return {"metric": METRIC, "value": r["ess_per_s"], "unit": "effective samples/s", "n_gpus": 1, "steps": steps,
             "warmup": warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -556,7 +591,7 @@ def main_mis(args):
                 "fp64_plain_form": res["fp64"],
                 "speedup_vs_fp64_plain_form": res["fp32"]["ess_per_s"] / res["fp64"]["ess_per_s"],
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                             "frac": achieved / 8000.0, "traffic": None,
+                             "frac": achieved / 8000.0, "traffic": _traffic(f"k_new_static_mis_d{D}_K{K}_N{n}_fp32"),
                              "kernel": "k_new (assignment, draw, target, 16-term mixture log-sum-exp); "
                                        "compute-bound: 16 whitened quadratic forms (~ 130 flop each) per sample",
                              "mean_launch_ms": res["fp32"]["draw_mixture_ms"]}})
@@ -584,7 +619,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc", "mis"])
+    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc", "mis", "copy"])
     ap.add_argument("--tuning", default=None, help="batch workload: 'waves,blocks_per_cu' instead of autotuning")
     ap.add_argument("--skip-fp64-check", action="store_true", help="batch workload: skip the FP64 reference solve")
     args = ap.parse_args()
@@ -600,6 +635,8 @@ def main():
         return main_plain(args)
     if args.workload == "dmpmc":
         return main_dmpmc(args)
+    if args.workload == "copy":
+        return main_copy(args)
     if args.workload == "mis":
         return main_mis(args)
 
@@ -678,11 +715,7 @@ def main():
     bytes_per_launch = per_sample * retained  # one launch per round
     kms = np.mean(np.array(kernel_ms), axis=0)
     achieved = float(np.sum(bytes_per_launch) / (np.sum(kms) * 1e-3) / 1e9)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"amis_round_kernel_{chosen}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    traffic = _traffic(f"k_sweep_amis_d{D}_T{T}_N{N}_{chosen}")
 
     out = {
         "metric": METRIC,
