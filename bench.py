@@ -708,14 +708,18 @@ def main():
         kernel_ms.append(ms)
     lib.is_job_set_timing(sampler._job, 0)
 
-    # roofline of the round kernel: algorithmic bytes = each retained sample once per
-    # round, (8d + 24) B in FP64 storage, (4d + 12) B in FP32
-    per_sample = (8 * D + 24) if chosen == "fp64" else (4 * D + 12)
-    retained = np.cumsum([N] * T)
-    bytes_per_launch = per_sample * retained  # one launch per round
+    # roofline of the round kernel (one launch per round): algorithmic bytes = each
+    # retained sample once (read x, log-numerator, log-target, round id; write
+    # log-numerator, log-weight) plus each new sample once (write x, log-target,
+    # log-numerator, log-weight, round id); ts = storage bytes per value
+    ts = 8 if chosen == "fp64" else 4
+    old_bytes = D * ts + 4 * ts + 4
+    new_bytes = D * ts + 3 * ts + 4
+    retained_before = N * np.arange(T)
+    bytes_per_launch = old_bytes * retained_before + new_bytes * N
     kms = np.mean(np.array(kernel_ms), axis=0)
     achieved = float(np.sum(bytes_per_launch) / (np.sum(kms) * 1e-3) / 1e9)
-    traffic = _traffic(f"k_sweep_amis_d{D}_T{T}_N{N}_{chosen}")
+    traffic = _traffic(f"k_round_amis_d{D}_T{T}_N{N}_{chosen}")
 
     out = {
         "metric": METRIC,
@@ -752,9 +756,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_sweep (AMIS append of the newest component to every retained sample, "
-                      "weight re-formation, summary and FP64-MFMA moment partials), mean of "
-                      f"{T} launches per solve",
+            "kernel": "k_round (one AMIS round per launch: new-sample blocks draw, evaluate the target "
+                      "and the (t+1)-term mixture and write their moment records, beside the sweep of the "
+                      "retained samples: append of the newest component, weight re-formation, summary and "
+                      f"FP64-MFMA moment partials), mean of {T} launches per solve; "
+                      f"{old_bytes} B per retained and {new_bytes} B per new sample",
             "mean_launch_ms": float(np.mean(kms)),
         },
     }

@@ -226,6 +226,16 @@ int launch_sweep(int D, int nt, SweepArgs<T> a, hipStream_t s) {
   return blocks;
 }
 
+// Launches one fused AMIS round (k_round); false when the job's dimension
+// instantiation has none, and nothing was launched.
+template <class T>
+bool launch_round(int D, int ncomp, const RoundArgs<T>& a, const SweepArgs<T>& w, int nb_new, int nrec_old,
+                  hipStream_t s) {
+  bool done = false;
+  with_dim(D, [&](auto k) { done = DimKernels<decltype(k)::value>::round(a, w, ncomp, nb_new, nrec_old, s); });
+  return done;
+}
+
 template <class T>
 void launch_target(int D, const TargetArgs<T>& t, int64_t splits, hipStream_t s) {
   with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::target(t, splits, s); });
@@ -233,8 +243,27 @@ void launch_target(int D, const TargetArgs<T>& t, int64_t splits, hipStream_t s)
 
 // Summary / fit on a reduced record (one wavefront, any d <= 64).
 void launch_finish(int d, const double* reduced, const FinishArgs& f, hipStream_t s) {
-  (void)d;
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, reduced, f);
+  if (d <= 16)
+    hipLaunchKernelGGL(k_finish_small<16>, dim3(1), dim3(64), 0, s, reduced, f);
+  else if (d <= 32)
+    hipLaunchKernelGGL(k_finish_small<32>, dim3(1), dim3(64), 0, s, reduced, f);
+  else
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, reduced, f);
+}
+
+// Reduction of `nrec` partial records into `reduced`, then summary / fit: one
+// launch for d <= 32 (the last reduction block fits; f.counter is its ticket
+// word), a reduction and a finish launch above.
+void launch_reduce_finish(int d, const double* partials, int64_t nrec, int64_t pcap, int nv, double* reduced,
+                          const FinishArgs& f, hipStream_t s) {
+  if (d <= 16) {
+    hipLaunchKernelGGL(k_reduce_finish<16>, dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, reduced, f);
+  } else if (d <= 32) {
+    hipLaunchKernelGGL(k_reduce_finish<32>, dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, reduced, f);
+  } else {
+    hipLaunchKernelGGL(k_reduce_finish<0>, dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, reduced, f);
+    launch_finish(d, reduced, f, s);
+  }
 }
 
 // Row slices for the batched target: enough blocks to fill the chip when a round
@@ -515,9 +544,7 @@ int run_rounds(is_job* j) {
     launch_mix<T, kPreTarget>(D, ncomp, a, s);
   };
   auto reduce = [&](const DevBuf& partials, int nrec, int nv, DevBuf& reduced, const FinishArgs& f) {
-    hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), 0, s, partials.as<double>(),
-                       (int64_t)nrec, j->pcap, nv, reduced.as<double>(), f);
-    launch_finish(d, reduced.as<double>(), f, s);
+    launch_reduce_finish(d, partials.as<double>(), (int64_t)nrec, j->pcap, nv, reduced.as<double>(), f, s);
   };
 
   if (j->method == IS_MIS) {
@@ -615,7 +642,6 @@ int run_rounds(is_job* j) {
       a.gen_slot = t;
       a.logmass = j->logmassT.as<T>();
       a.logtotal = (T)std::log((double)last);
-      mix(a, t + 1);
       SweepArgs<T> w = sweep_args<T>(j, t);
       w.lo = 0;
       w.hi = last;
@@ -626,9 +652,27 @@ int run_rounds(is_job* j) {
       w.logtotal = a.logtotal;
       w.center = slot64_t;
       w.nv = nv_mom;
-      tick(j, t, 0, s);
-      const int nrec = launch_sweep<T>(D, nt_mom, w, s);
-      tick(j, t, 1, s);
+      int nrec = 0;
+      bool fused = false;
+      if (!batch_target && !w.clip) {
+        // one launch: the new samples' blocks (with their own moment records) beside
+        // the sweep of the retained samples [0, first)
+        SweepArgs<T> wo = w;
+        wo.hi = first;
+        wo.per_block = kSweepPerBlock;
+        const int nrec_old = first > 0 ? (int)cdiv(first, kSweepPerBlock) : 0;
+        const int nb_new = (int)cdiv(N, 64);
+        tick(j, t, 0, s);
+        fused = launch_round<T>(D, t + 1, a, wo, nb_new, nrec_old, s);
+        tick(j, t, 1, s);
+        nrec = nrec_old + nb_new;
+      }
+      if (!fused) {
+        mix(a, t + 1);
+        tick(j, t, 0, s);
+        nrec = launch_sweep<T>(D, nt_mom, w, s);
+        tick(j, t, 1, s);
+      }
       FinishArgs f = base_finish(j, t, counter);
       f.flags = kFinishSummary | kFinishFit;
       f.summary_count = last;
@@ -883,7 +927,8 @@ int alloc_job_buffers(is_job* j) {
   HIP_TRY(locked_memset(j->slots64.p, 0, j->slots64.bytes));
   HIP_TRY(locked_memset(j->slotsT.p, 0, j->slotsT.bytes));
   // partial records (column-major): one per sweep block over at most `cap` samples
-  j->pcap = cdiv((int64_t)cap, kSweepPerBlock) + 1;
+  // (a fused AMIS round adds one record per 64 new samples)
+  j->pcap = cdiv((int64_t)cap, kSweepPerBlock) + cdiv(std::max<int64_t>(1, j->maxN), 64) + 1;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
   if ((rc = j->partials.alloc((size_t)j->pcap * (1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->partials2.alloc((size_t)j->pcap * (1 + nv_mom) * sizeof(double)))) return rc;
@@ -1201,7 +1246,7 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
   const int K = banked ? std::max(1, desc->bank_size) : 1;
   const int nslots = banked ? K : desc->method == IS_PLAIN ? 1 : T + 1;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
-  const int64_t pcap = cdiv(total, kSweepPerBlock) + 1;
+  const int64_t pcap = cdiv(total, kSweepPerBlock) + cdiv(maxN, 64) + 1;
   int64_t bytes = total * (d * ts + 3 * ts + 2 * 4);         // samples, log arrays, ids
   bytes += (int64_t)nslots * stride * (8 + ts);               // slot tables
   bytes += 2 * pcap * (1 + nv_mom) * 8 + 2 * (1 + nv_mom) * 8; // partial records, reduced records
@@ -1639,7 +1684,7 @@ static int job_lognormalizer_typed(is_job* job, double* out3) {
   f.summary_count = n;
   f.summary_out = job->summary.as<double>();
   f.counter = job->counter.as<unsigned int>() + 1;
-  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), 0, job->stream,
+  hipLaunchKernelGGL(k_reduce_finish<0>, dim3(2), dim3(kBlock), 0, job->stream,
                      job->partials2.as<double>(), (int64_t)blocks, job->pcap, 2, job->reduced2.as<double>(), f);
   launch_finish(job->d, job->reduced2.as<double>(), f, job->stream);
   HIP_TRY(hipGetLastError());
@@ -1689,7 +1734,7 @@ static int job_moments_typed(is_job* job, double* mean, double* var, double* cov
   f.d = d;
   f.flags = 0;
   f.counter = job->counter.as<unsigned int>() + 2;
-  hipLaunchKernelGGL(k_reduce_finish, dim3(nv), dim3(kBlock), 0, job->stream,
+  hipLaunchKernelGGL(k_reduce_finish<0>, dim3(nv), dim3(kBlock), 0, job->stream,
                      job->partials2.as<double>(), (int64_t)blocks, job->pcap, nv, job->reduced2.as<double>(), f);
   HIP_TRY(hipGetLastError());
   std::vector<double> red(1 + nv);
@@ -1793,7 +1838,7 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   f.summary_count = n;
   f.summary_out = summary.as<double>();
   f.counter = counter.as<unsigned int>();
-  hipLaunchKernelGGL(k_reduce_finish, dim3(2), dim3(kBlock), 0, s, partials.as<double>(),
+  hipLaunchKernelGGL(k_reduce_finish<0>, dim3(2), dim3(kBlock), 0, s, partials.as<double>(),
                      (int64_t)blocks, pcap, 2, reduced.as<double>(), f);
   launch_finish(1, reduced.as<double>(), f, s);
   hipLaunchKernelGGL((k_cdf_tiles<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),

@@ -143,7 +143,14 @@ constexpr int kFinishAdaptEss = 4;
 // kReduceUnroll records whose loads are all issued before any is used, so up
 // to kBlock * kReduceUnroll records cost one memory round trip; the threads
 // are then merged in a fixed order (deterministic).
+// With DR > 0 the block that finishes last (a ticket on f.counter, which it
+// returns to zero) also runs the summary and fit of k_finish_small<DR> in its
+// first wave, so a round needs no separate finish launch.
 constexpr int kReduceUnroll = 8;
+template <int DR>
+__device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f);
+
+template <int DR>
 static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
                                                           int64_t nrec, int64_t pcap, int nv,
                                                           double* __restrict__ reduced, FinishArgs f) {
@@ -200,7 +207,22 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
     reduced[1 + k] = tot;
     if (k == 0) reduced[0] = M;
   }
-  (void)f;
+  if constexpr (DR > 0) {
+    __shared__ int last;
+    if (tid == 0) {
+      // release: this block's values before the ticket; acquire: every earlier block's
+      const unsigned int t = __hip_atomic_fetch_add(f.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      last = t == gridDim.x - 1;
+      if (last) __hip_atomic_store(f.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (last && tid < 64) {
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);
+      fit_small<DR>(reduced, f);
+    }
+  } else {
+    (void)f;
+  }
 }
 
 // Summary (src/execution.jl:176-186): ESS = s0^2 / s2, log Z = M + log s0 - log n.
@@ -374,6 +396,140 @@ static __global__ __launch_bounds__(64) void k_finish(const double* __restrict__
       od[slot_lognorm_off(d)] = lognorm;
     f.fit_status[f.round] = st;
   }
+}
+
+// The same fit for d <= DR with the covariance held in registers: lane r keeps
+// row r, every global input is fetched in one batch, and the Cholesky takes
+// row c of the factor from lane c by readlane, so no LDS round trip or wave
+// barrier sits between the d column steps.  Operation order and rounding are
+// those of k_finish (left-looking column order, the same expressions).
+template <int DR>
+__device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f) {
+  const int lane = threadIdx.x & 63;
+  const int d = f.d;
+  const bool row = lane < d;
+  const bool fit = (f.flags & kFinishFit) != 0;
+  // ---- one batch of loads: summary, row `lane` of S and of the previous factor, s1, centre
+  const double M = red[0], s0 = red[1], s2 = red[2];
+  double a[DR], l[DR];
+  const int rs = 3 + d + lane * (lane + 1) / 2;
+  const int rl = 2 * d + lane * (lane + 1) / 2;
+#pragma unroll
+  for (int c = 0; c < DR; ++c) {
+    const bool in = fit && row && c <= lane;
+    a[c] = in ? red[rs + c] : 0.0;
+    l[c] = in ? f.prev_slot[rl + c] : 0.0;
+  }
+  const double s1 = (fit && row) ? red[3 + lane] : 0.0;
+  const double cr = (fit && row) ? f.center[lane] : 0.0;
+  if (lane == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) finish_summary(M, s0, s2, f);
+  if (!fit) return;
+  if (!(s0 > 0.0) || !isfinite(s0) || M == -__builtin_huge_val()) {
+    if (lane == 0) f.fit_status[f.round] = 1;
+    return;
+  }
+  const double mean = s1 / s0;
+  const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
+  // scale-aware ridge (src/adaptive_gaussian_execution.jl:283-301): sqrt(eps) tr(L L^T) mult / d
+  const double reg = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+  double tr = 0.0;
+#pragma unroll
+  for (int c = 0; c < DR; ++c) tr += l[c] * l[c];
+  double ridge = reg * (wave_sum(tr) * mult) / (double)d;
+  if (!isfinite(ridge)) {
+    const double sc = sqrt(reg) * sqrt(mult / (double)d);
+    double part = 0.0;
+#pragma unroll
+    for (int c = 0; c < DR; ++c) {
+      const double v = l[c] * sc;
+      part += v * v;
+    }
+    ridge = wave_sum(part);
+  }
+  // ---- centred covariance, row `lane`
+#pragma unroll
+  for (int c = 0; c < DR; ++c) {
+    if (c < d) {
+      const double mc = readlane_f64(mean, c);
+      double v = a[c] / s0 - mean * mc;
+      if (c == lane) v += ridge;
+      if (row && c <= lane) a[c] = v;
+    }
+  }
+  // ---- Cholesky, column by column; lane c's a[k] (k < c) already holds L[c][k]
+  int status = 0;
+#pragma unroll
+  for (int c = 0; c < DR; ++c) {
+    if (c < d) {
+      double v = a[c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) v -= a[k] * readlane_f64(a[k], c);
+      const double piv = readlane_f64(v, c);
+      if (!(piv > 0.0) || !isfinite(piv)) status = 2;
+      const double dg = sqrt(piv);
+      if (row && lane > c) a[c] = v / dg;
+      if (lane == c) a[c] = dg;
+    }
+  }
+  // ---- outputs: lane r writes mean r, 1 / L_rr and row r of L (Student-t rescaled)
+  const double scale = f.student ? sqrt(1.0 / mult) : 1.0;
+  bool ok = true;
+  double lg = 0.0;
+  double* out = f.next_slot64;
+  const bool tf = f.t_is_float != 0;
+  float* of = reinterpret_cast<float*>(f.next_slot_t);
+  double* od = reinterpret_cast<double*>(f.next_slot_t);
+  if (row) {
+    double dl = 0.0;
+#pragma unroll
+    for (int c = 0; c < DR; ++c)
+      if (c == lane) dl = a[c];
+    const double mu = cr + mean;
+    const double diag = dl * scale;
+    ok = isfinite(mu) && diag > 0.0;
+    lg = log(fabs(diag));
+    out[lane] = mu;
+    out[d + lane] = 1.0 / diag;
+    if (tf) {
+      of[lane] = (float)mu;
+      of[d + lane] = (float)(1.0 / diag);
+    } else {
+      od[lane] = mu;
+      od[d + lane] = 1.0 / diag;
+    }
+    const int base = 2 * d + lane * (lane + 1) / 2;
+#pragma unroll
+    for (int c = 0; c < DR; ++c) {
+      if (c <= lane) {
+        const double v = a[c] * scale;
+        ok &= isfinite(v);
+        out[base + c] = v;
+        if (tf)
+          of[base + c] = (float)v;
+        else
+          od[base + c] = v;
+      }
+    }
+  }
+  const double logabsdet = wave_sum(lg);
+  const bool all_ok = __all(ok);
+  if (lane == 0) {
+    const double lognorm = f.student ? f.logconstant - logabsdet
+                                     : -0.5 * (double)d * log(2.0 * 3.141592653589793) - logabsdet;
+    int st = status;
+    if (st == 0 && (!all_ok || !isfinite(lognorm))) st = 3;
+    out[slot_lognorm_off(d)] = lognorm;
+    if (tf)
+      of[slot_lognorm_off(d)] = (float)lognorm;
+    else
+      od[slot_lognorm_off(d)] = lognorm;
+    f.fit_status[f.round] = st;
+  }
+}
+
+template <int DR>
+static __global__ __launch_bounds__(64) void k_finish_small(const double* __restrict__ red, FinishArgs f) {
+  fit_small<DR>(red, f);
 }
 
 // ---------------------------------------------------------------- static MIS assignment

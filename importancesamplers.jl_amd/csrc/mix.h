@@ -47,14 +47,17 @@ __host__ __device__ constexpr size_t new_lds_fixed(int D, int W, int tsize) {
            64 * 4) + 15) & ~(size_t)15;
 }
 
+// The body of one new-sample block `blk` (samples 64 blk .. 64 blk + 63).  On
+// return, wave 0's active lanes hold their sample in out_x and its log-weight
+// (-inf when it failed) in out_lw; other lanes leave both untouched.
 template <class T, int D, int MODE, bool ST>
-__global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(RoundArgs<T> a) {
+__device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T& out_lw, T (&out_x)[D]) {
   extern __shared__ double lds[];
   constexpr int ZS = D + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int W = blockDim.x >> 6;
   const int d = exact_dim<D>(a.d);
-  const int64_t j0 = (int64_t)blockIdx.x * 64;
+  const int64_t j0 = blk * 64;
   const int64_t j = j0 + lane;
   const bool active = j < a.n_new;
   const int64_t gi = a.first + j;
@@ -306,6 +309,16 @@ __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(Round
   a.logweights[gi] = lwv;
   a.round_ids[gi] = rid;
   if (a.proposal_ids) a.proposal_ids[gi] = pid;
+  out_lw = lwv;
+#pragma unroll
+  for (int c = 0; c < D; ++c) out_x[c] = x[c];
+}
+
+template <class T, int D, int MODE, bool ST>
+__global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(RoundArgs<T> a) {
+  T lw;
+  T x[D];
+  new_block<T, D, MODE, ST>(a, blockIdx.x, lw, x);
 }
 
 // ---------------------------------------------------------------- streaming sweep
@@ -380,11 +393,155 @@ __device__ __forceinline__ void sweep_load(const SweepArgs<T>& a, int64_t i, int
   }
 }
 
+// Per-wave running state of the summary monoid and the moment tiles.
+template <class T, int NT>
+struct SweepAcc {
+  static constexpr int P = NT * (NT + 1) / 2;
+  T mw;
+  double s0, s2;
+  double4_t acc[P > 0 ? P : 1];
+};
+
+template <class T, int NT>
+__device__ __forceinline__ void sweep_acc_init(SweepAcc<T, NT>& st) {
+  st.mw = -T(__builtin_huge_val());
+  st.s0 = 0.0;
+  st.s2 = 0.0;
+#pragma unroll
+  for (int p = 0; p < (SweepAcc<T, NT>::P > 0 ? SweepAcc<T, NT>::P : 1); ++p)
+    st.acc[p] = (double4_t){0.0, 0.0, 0.0, 0.0};
+}
+
+// One 64-sample step of a wave: lane `lane` contributes log-weight lw (-inf: no
+// contribution) of sample x.  `rows` is the wave's 64 x (16 NT + 1) LDS staging
+// area, whose columns above d are zero.
+template <class T, int D, int NT, class CP>
+__device__ __forceinline__ void sweep_accumulate(SweepAcc<T, NT>& st, T lw, const T (&x)[D], int d, CP cen,
+                                                 double* rows, int lane) {
+  constexpr int RS = sweep_row_stride(NT);
+  constexpr int P = SweepAcc<T, NT>::P;
+  const T ninf = -T(__builtin_huge_val());
+  // running wave maximum, recomputed only when some lane exceeds it
+  if (__any(lw > st.mw)) {
+    const T wm = wave_max(lw);
+    const double f = (st.mw == ninf) ? 0.0 : exp((double)st.mw - (double)wm);
+    st.s0 *= f;
+    st.s2 *= f * f;
+#pragma unroll
+    for (int p = 0; p < (P > 0 ? P : 1); ++p) st.acc[p] *= f;
+    st.mw = wm;
+  }
+  const T se = (lw == ninf) ? T(0) : exp(T(0.5) * (lw - st.mw));
+  const double sd = (double)se;
+  const double e = sd * sd;
+  st.s0 += e;
+  st.s2 += e * e;
+  if (NT > 0) {
+    double* row = rows + lane * RS;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      if (c < d) row[c] = sd * ((double)x[c] - cen[c]);
+    row[d] = sd;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int stp = 0; stp < 16; ++stp) {
+      const double* r = rows + (4 * stp + (lane >> 4)) * RS + (lane & 15);
+      double av[NT > 0 ? NT : 1];
+#pragma unroll
+      for (int ti = 0; ti < NT; ++ti) av[ti] = r[ti * 16];
+      int p = 0;
+#pragma unroll
+      for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj <= ti; ++tj) {
+          st.acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], av[tj], st.acc[p], 0, 0, 0);
+          ++p;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Block reduction of the waves' states into one column-major partial record at
+// `out` (column stride pcap): rescale the waves to the block maximum, sum in
+// wave order.  Every thread of the block calls it (it holds barriers).
+template <class T, int NT>
+__device__ __forceinline__ void sweep_epilogue(SweepAcc<T, NT>& st, int d, int nv, double* out, int64_t pcap,
+                                               double* lds) {
+  constexpr int P = SweepAcc<T, NT>::P;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  double s0 = wave_sum(st.s0);
+  double s2 = wave_sum(st.s2);
+  __syncthreads();
+  double* wmax = lds;                    // nw
+  double* ws0 = lds + 4;                 // nw
+  double* ws2 = lds + 8;                 // nw
+  double* tiles = lds + 16;              // nw * P * 256
+  if (lane == 0) {
+    wmax[wave] = (double)st.mw;
+    ws0[wave] = s0;
+    ws2[wave] = s2;
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tiles[((size_t)wave * P + p) * 256 + lane * 4 + r] = st.acc[p][r];
+  __syncthreads();
+  double M = wmax[0];
+  for (int w = 1; w < nw; ++w) M = fmax(M, wmax[w]);
+  double fw[4];
+  for (int w = 0; w < 4; ++w)
+    fw[w] = (w < nw && wmax[w] != -__builtin_huge_val()) ? exp(wmax[w] - M) : 0.0;
+  // summed tiles go after the per-wave tiles
+  double* sumt = tiles + (size_t)nw * P * 256;
+  for (int q = tid; q < P * 256; q += blockDim.x) {
+    double v = 0.0;
+    for (int w = 0; w < nw; ++w) v += fw[w] * tiles[((size_t)w * P) * 256 + q];
+    sumt[q] = v;
+  }
+  __syncthreads();
+  if (tid == 0) out[0] = M;
+  for (int v = tid; v < nv; v += blockDim.x) {
+    double val;
+    if (v == 1) {
+      val = 0.0;
+      for (int w = 0; w < nw; ++w) val += fw[w] * fw[w] * ws2[w];
+    } else if (NT == 0) {
+      val = 0.0;
+      for (int w = 0; w < nw; ++w) val += fw[w] * ws0[w];
+    } else {
+      // moment entry C[r][c], r >= c, of the augmented matrix: s0 = C[d][d],
+      // s1[r] = C[d][r], S pairs C[r][c]
+      int r, c;
+      if (v == 0) {
+        r = c = d;
+      } else if (v < 2 + d) {
+        r = d;
+        c = v - 2;
+      } else {
+        pair_index(v - 2 - d, r, c);
+      }
+      // v_mfma_f64_16x16x4f64 result layout: lane l, register i holds tile
+      // entry (row 4 i + l / 16, column l % 16)
+      const int ti = r >> 4, tj = c >> 4, ra = r & 15, cb = c & 15;
+      const int p = ti * (ti + 1) / 2 + tj;
+      const int ln = cb + 16 * (ra & 3), reg = ra >> 2;
+      val = sumt[(size_t)p * 256 + ln * 4 + reg];
+    }
+    out[(int64_t)(1 + v) * pcap] = val;
+  }
+}
+
+// The body of sweep block `blk` over [a.lo + blk * per_block, ...).
 template <class T, int D, int NT>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
+__device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) {
   extern __shared__ double lds[];
   constexpr int RS = sweep_row_stride(NT);
-  constexpr int P = NT * (NT + 1) / 2;
   const T ninf = -T(__builtin_huge_val());
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
@@ -398,12 +555,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
   const cptr<double> cen = as_const(a.center);
   const cptr<T> slot = as_const(a.slot);
   const double clip = a.clip ? *a.clip : 0.0;
-  T mw = ninf;
-  double s0 = 0.0, s2 = 0.0;
-  double4_t acc[P > 0 ? P : 1];
-#pragma unroll
-  for (int p = 0; p < (P > 0 ? P : 1); ++p) acc[p] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  const int64_t bstart = a.lo + (int64_t)blockIdx.x * a.per_block;
+  SweepAcc<T, NT> st;
+  sweep_acc_init(st);
+  const int64_t bstart = a.lo + blk * a.per_block;
   const int64_t bend = min(a.hi, bstart + a.per_block);
   const int64_t step = (int64_t)nw * 64;
   SweepLoad<T, D> cur, nxt;
@@ -443,111 +597,48 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
       }
       if (a.clip) lw = fmin(lw, (T)clip);
     }
-    // running wave maximum, recomputed only when some lane exceeds it
-    if (__any(lw > mw)) {
-      const T wm = wave_max(lw);
-      const double f = (mw == ninf) ? 0.0 : exp((double)mw - (double)wm);
-      s0 *= f;
-      s2 *= f * f;
-#pragma unroll
-      for (int p = 0; p < (P > 0 ? P : 1); ++p) acc[p] *= f;
-      mw = wm;
-    }
-    const T se = (lw == ninf) ? T(0) : exp(T(0.5) * (lw - mw));
-    const double sd = (double)se;
-    const double e = sd * sd;
-    s0 += e;
-    s2 += e * e;
-    if (NT > 0) {
-      double* row = rows + lane * RS;
-#pragma unroll
-      for (int c = 0; c < D; ++c)
-        if (c < d) row[c] = sd * ((double)cur.x[c] - cen[c]);
-      row[d] = sd;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-      for (int st = 0; st < 16; ++st) {
-        const double* r = rows + (4 * st + (lane >> 4)) * RS + (lane & 15);
-        double av[NT > 0 ? NT : 1];
-#pragma unroll
-        for (int ti = 0; ti < NT; ++ti) av[ti] = r[ti * 16];
-        int p = 0;
-#pragma unroll
-        for (int ti = 0; ti < NT; ++ti)
-#pragma unroll
-          for (int tj = 0; tj <= ti; ++tj) {
-            acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], av[tj], acc[p], 0, 0, 0);
-            ++p;
-          }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    sweep_accumulate<T, D, NT>(st, lw, cur.x, d, cen, rows, lane);
     cur = nxt;
   }
-  // ---- block reduction: rescale the waves to the block maximum, sum in wave order
-  s0 = wave_sum(s0);
-  s2 = wave_sum(s2);
-  __syncthreads();
-  double* wmax = lds;                    // nw
-  double* ws0 = lds + 4;                 // nw
-  double* ws2 = lds + 8;                 // nw
-  double* tiles = lds + 16;              // nw * P * 256
-  if (lane == 0) {
-    wmax[wave] = (double)mw;
-    ws0[wave] = s0;
-    ws2[wave] = s2;
-  }
+  sweep_epilogue<T, NT>(st, d, a.nv, a.partials + blk, a.pcap, lds);
+}
+
+template <class T, int D, int NT>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
+  sweep_block<T, D, NT>(a, blockIdx.x);
+}
+
+// One AMIS round in one launch (samples are disjoint between the two roles):
+//   blocks [0, nb_new)   draw the round's new samples (new_block, 64 per block,
+//                        4 waves) and write the moment partial record of their
+//                        64 samples at nrec_old + block;
+//   blocks [nb_new, ...) sweep the retained samples [w.lo, w.hi) (append of the
+//                        newest component, weights, summary, moments), records
+//                        [0, nrec_old).
+// The new-sample blocks (few, and bound by their mixture latency) then run
+// beside the sweep instead of before it.
+template <class T, int D, int NT, bool ST>
+__global__ __launch_bounds__(256) void k_round(RoundArgs<T> a, SweepArgs<T> w, int nb_new, int nrec_old) {
+  extern __shared__ double lds[];
+  if ((int)blockIdx.x < nb_new) {
+    constexpr int RS = sweep_row_stride(NT);
+    T lw = -T(__builtin_huge_val());
+    T x[D];
 #pragma unroll
-  for (int p = 0; p < P; ++p)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) tiles[((size_t)wave * P + p) * 256 + lane * 4 + r] = acc[p][r];
-  __syncthreads();
-  double M = wmax[0];
-  for (int w = 1; w < nw; ++w) M = fmax(M, wmax[w]);
-  double fw[4];
-  for (int w = 0; w < 4; ++w)
-    fw[w] = (w < nw && wmax[w] != -__builtin_huge_val()) ? exp(wmax[w] - M) : 0.0;
-  // summed tiles go after the per-wave tiles
-  double* sumt = tiles + (size_t)nw * P * 256;
-  for (int q = tid; q < P * 256; q += blockDim.x) {
-    double v = 0.0;
-    for (int w = 0; w < nw; ++w) v += fw[w] * tiles[((size_t)w * P) * 256 + q];
-    sumt[q] = v;
-  }
-  __syncthreads();
-  double* out = a.partials + blockIdx.x;
-  if (tid == 0) out[0] = M;
-  for (int v = tid; v < a.nv; v += blockDim.x) {
-    double val;
-    if (v == 1) {
-      val = 0.0;
-      for (int w = 0; w < nw; ++w) val += fw[w] * fw[w] * ws2[w];
-    } else if (NT == 0) {
-      val = 0.0;
-      for (int w = 0; w < nw; ++w) val += fw[w] * ws0[w];
-    } else {
-      // moment entry C[r][c], r >= c, of the augmented matrix: s0 = C[d][d],
-      // s1[r] = C[d][r], S pairs C[r][c]
-      int r, c;
-      if (v == 0) {
-        r = c = d;
-      } else if (v < 2 + d) {
-        r = d;
-        c = v - 2;
-      } else {
-        pair_index(v - 2 - d, r, c);
-      }
-      // v_mfma_f64_16x16x4f64 result layout: lane l, register i holds tile
-      // entry (row 4 i + l / 16, column l % 16)
-      const int ti = r >> 4, tj = c >> 4, ra = r & 15, cb = c & 15;
-      const int p = ti * (ti + 1) / 2 + tj;
-      const int ln = cb + 16 * (ra & 3), reg = ra >> 2;
-      val = sumt[(size_t)p * 256 + ln * 4 + reg];
+    for (int c = 0; c < D; ++c) x[c] = T(0);
+    new_block<T, D, kDrawFused, ST>(a, blockIdx.x, lw, x);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int d = exact_dim<D>(a.d);
+    SweepAcc<T, NT> st;
+    sweep_acc_init(st);
+    if (wave == 0) {
+      if (NT > 0)
+        for (int q = lane; q < 64 * RS; q += 64) lds[q] = 0.0;
+      sweep_accumulate<T, D, NT>(st, lw, x, d, as_const(w.center), lds, lane);
     }
-    out[(int64_t)(1 + v) * a.pcap] = val;
+    sweep_epilogue<T, NT>(st, d, w.nv, w.partials + nrec_old + blockIdx.x, w.pcap, lds);
+  } else {
+    sweep_block<T, D, NT>(w, (int64_t)blockIdx.x - nb_new);
   }
 }
