@@ -148,7 +148,12 @@ constexpr int kFinishAdaptEss = 4;
 // first wave, so a round needs no separate finish launch.
 constexpr int kReduceUnroll = 8;
 template <int DR>
-__device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f);
+struct FitPrefetch;
+template <int DR>
+__device__ __forceinline__ void fit_prefetch(const FinishArgs& f, FitPrefetch<DR>& pf);
+template <int DR>
+__device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f,
+                                          const FitPrefetch<DR>& pf);
 
 template <int DR>
 static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
@@ -161,6 +166,11 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
   const double ninf = -__builtin_huge_val();
   const double* maxima = partials;
   const double* column = partials + (int64_t)(1 + k) * pcap;
+  // the fit's inputs from earlier launches, fetched now in case this block is last
+  FitPrefetch<(DR > 0 ? DR : 1)> pf;
+  if constexpr (DR > 0) {
+    if (tid < 64) fit_prefetch<DR>(f, pf);
+  }
   double m = ninf, s = 0.0;
   for (int64_t base = 0; base < nrec; base += (int64_t)kBlock * kReduceUnroll) {
     double mv[kReduceUnroll], cv[kReduceUnroll];
@@ -218,7 +228,7 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
     __syncthreads();
     if (last && tid < 64) {
       __atomic_thread_fence(__ATOMIC_ACQUIRE);
-      fit_small<DR>(reduced, f);
+      fit_small<DR>(reduced, f, pf);
     }
   } else {
     (void)f;
@@ -399,49 +409,73 @@ static __global__ __launch_bounds__(64) void k_finish(const double* __restrict__
 }
 
 // The same fit for d <= DR with the covariance held in registers: lane r keeps
-// row r, every global input is fetched in one batch, and the Cholesky takes
-// row c of the factor from lane c by readlane, so no LDS round trip or wave
-// barrier sits between the d column steps.  Operation order and rounding are
-// those of k_finish (left-looking column order, the same expressions).
+// row r and the Cholesky takes row c of the factor from lane c by readlane, so
+// no LDS round trip or wave barrier sits between the d column steps.  Each
+// column step scales by 1 / sqrt(pivot) from the hardware reciprocal square
+// root and two Newton steps (relative error a few ulp) instead of a square root
+// and a division; the covariance scales by 1 / s0 once.  The previous factor's
+// row and the centre (written by earlier launches) are fetched by fit_prefetch
+// before the reduced record is available.
 template <int DR>
-__device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f) {
+struct FitPrefetch {
+  double l[DR];
+  double cr;
+};
+
+template <int DR>
+__device__ __forceinline__ void fit_prefetch(const FinishArgs& f, FitPrefetch<DR>& pf) {
+  const int lane = threadIdx.x & 63;
+  const int d = f.d;
+  const bool row = lane < d && (f.flags & kFinishFit) != 0;
+  const int rl = 2 * d + lane * (lane + 1) / 2;
+#pragma unroll
+  for (int c = 0; c < DR; ++c) pf.l[c] = (row && c <= lane) ? f.prev_slot[rl + c] : 0.0;
+  pf.cr = row ? f.center[lane] : 0.0;
+}
+
+__device__ __forceinline__ double rsqrt_nr(double v) {
+  double r = __builtin_amdgcn_rsq(v);
+  const double h = 0.5 * v;
+  r = r * fma(-h * r, r, 1.5);
+  r = r * fma(-h * r, r, 1.5);
+  return r;
+}
+
+template <int DR>
+__device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f,
+                                          const FitPrefetch<DR>& pf) {
   const int lane = threadIdx.x & 63;
   const int d = f.d;
   const bool row = lane < d;
   const bool fit = (f.flags & kFinishFit) != 0;
-  // ---- one batch of loads: summary, row `lane` of S and of the previous factor, s1, centre
+  // ---- one batch of loads: summary, row `lane` of S, s1
   const double M = red[0], s0 = red[1], s2 = red[2];
-  double a[DR], l[DR];
+  double a[DR];
   const int rs = 3 + d + lane * (lane + 1) / 2;
-  const int rl = 2 * d + lane * (lane + 1) / 2;
 #pragma unroll
-  for (int c = 0; c < DR; ++c) {
-    const bool in = fit && row && c <= lane;
-    a[c] = in ? red[rs + c] : 0.0;
-    l[c] = in ? f.prev_slot[rl + c] : 0.0;
-  }
+  for (int c = 0; c < DR; ++c) a[c] = (fit && row && c <= lane) ? red[rs + c] : 0.0;
   const double s1 = (fit && row) ? red[3 + lane] : 0.0;
-  const double cr = (fit && row) ? f.center[lane] : 0.0;
   if (lane == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) finish_summary(M, s0, s2, f);
   if (!fit) return;
   if (!(s0 > 0.0) || !isfinite(s0) || M == -__builtin_huge_val()) {
     if (lane == 0) f.fit_status[f.round] = 1;
     return;
   }
-  const double mean = s1 / s0;
+  const double is0 = 1.0 / s0;
+  const double mean = s1 * is0;
   const double mult = f.student ? f.dof / (f.dof - 2.0) : 1.0;
   // scale-aware ridge (src/adaptive_gaussian_execution.jl:283-301): sqrt(eps) tr(L L^T) mult / d
   const double reg = 1.4901161193847656e-08;  // sqrt(eps(Float64))
   double tr = 0.0;
 #pragma unroll
-  for (int c = 0; c < DR; ++c) tr += l[c] * l[c];
+  for (int c = 0; c < DR; ++c) tr += pf.l[c] * pf.l[c];
   double ridge = reg * (wave_sum(tr) * mult) / (double)d;
   if (!isfinite(ridge)) {
     const double sc = sqrt(reg) * sqrt(mult / (double)d);
     double part = 0.0;
 #pragma unroll
     for (int c = 0; c < DR; ++c) {
-      const double v = l[c] * sc;
+      const double v = pf.l[c] * sc;
       part += v * v;
     }
     ridge = wave_sum(part);
@@ -451,13 +485,16 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
   for (int c = 0; c < DR; ++c) {
     if (c < d) {
       const double mc = readlane_f64(mean, c);
-      double v = a[c] / s0 - mean * mc;
+      double v = a[c] * is0 - mean * mc;
       if (c == lane) v += ridge;
       if (row && c <= lane) a[c] = v;
     }
   }
-  // ---- Cholesky, column by column; lane c's a[k] (k < c) already holds L[c][k]
+  // ---- Cholesky, column by column (left-looking order of _gaussian_potrf!,
+  // src/adaptive_gaussian_execution.jl:427-434); lane c's a[k] (k < c) already
+  // holds L[c][k]
   int status = 0;
+  double rdiag = 0.0;  // 1 / L[lane][lane]
 #pragma unroll
   for (int c = 0; c < DR; ++c) {
     if (c < d) {
@@ -466,9 +503,12 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
       for (int k = 0; k < c; ++k) v -= a[k] * readlane_f64(a[k], c);
       const double piv = readlane_f64(v, c);
       if (!(piv > 0.0) || !isfinite(piv)) status = 2;
-      const double dg = sqrt(piv);
-      if (row && lane > c) a[c] = v / dg;
-      if (lane == c) a[c] = dg;
+      const double r = rsqrt_nr(piv);
+      if (row && lane > c) a[c] = v * r;
+      if (lane == c) {
+        a[c] = piv * r;
+        rdiag = r;
+      }
     }
   }
   // ---- outputs: lane r writes mean r, 1 / L_rr and row r of L (Student-t rescaled)
@@ -484,18 +524,19 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
 #pragma unroll
     for (int c = 0; c < DR; ++c)
       if (c == lane) dl = a[c];
-    const double mu = cr + mean;
+    const double mu = pf.cr + mean;
     const double diag = dl * scale;
+    const double idiag = f.student ? rdiag / scale : rdiag;
     ok = isfinite(mu) && diag > 0.0;
     lg = log(fabs(diag));
     out[lane] = mu;
-    out[d + lane] = 1.0 / diag;
+    out[d + lane] = idiag;
     if (tf) {
       of[lane] = (float)mu;
-      of[d + lane] = (float)(1.0 / diag);
+      of[d + lane] = (float)idiag;
     } else {
       od[lane] = mu;
-      od[d + lane] = 1.0 / diag;
+      od[d + lane] = idiag;
     }
     const int base = 2 * d + lane * (lane + 1) / 2;
 #pragma unroll
@@ -529,7 +570,9 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
 
 template <int DR>
 static __global__ __launch_bounds__(64) void k_finish_small(const double* __restrict__ red, FinishArgs f) {
-  fit_small<DR>(red, f);
+  FitPrefetch<DR> pf;
+  fit_prefetch<DR>(f, pf);
+  fit_small<DR>(red, f, pf);
 }
 
 // ---------------------------------------------------------------- static MIS assignment
