@@ -121,6 +121,40 @@ __device__ inline double rng_uniform(const RngKey& key, uint64_t sample, uint64_
 }
 
 // ---------------------------------------------------------------- log-sum-exp helpers
+// log1p(t) for t in [0, 1] as 2 atanh(s), s = t / (2 + t) <= 1/3:
+// 2 s (1 + s^2/3 + s^4/5 + ...), NC terms after the leading one.  The first
+// omitted term is below 1.3e-10 of the sum with NC = 8 and at FP64 rounding
+// with NC = 16.  Branch-free: s from one reciprocal and Newton / exact-residual
+// corrections, so that its rounding is unbiased.
+template <int NC>
+__device__ inline double log1p_unit(double t) {
+  const double w = 2.0 + t;
+  double r = __builtin_amdgcn_rcp(w);
+  r = fma(r, fma(-w, r, 1.0), r);
+  r = fma(r, fma(-w, r, 1.0), r);
+  double sv = t * r;
+  sv = fma(fma(-sv, w, t), r, sv);
+  const double s2 = sv * sv;
+  double q = 1.0 / (2 * NC + 1);
+#pragma unroll
+  for (int k = NC - 1; k >= 1; --k) q = fma(q, s2, 1.0 / (2 * k + 1));
+  const double s2v = 2.0 * sv;
+  return fma(s2v * s2, q, s2v);
+}
+__device__ inline float log1p_unit_f32(float t) {
+  const float w = 2.0f + t;
+  const float w_lo = t - (w - 2.0f);  // 2 + t = w + w_lo exactly
+  const float r = __builtin_amdgcn_rcpf(w);
+  const float s0 = t * r;
+  const float sv = fmaf(fmaf(-s0, w, t) - s0 * w_lo, r, s0);
+  const float s2 = sv * sv;
+  float q = 1.0f / 17.0f;
+#pragma unroll
+  for (int k = 7; k >= 1; --k) q = fmaf(q, s2, 1.0f / (2 * k + 1));
+  const float s2v = 2.0f * sv;
+  return fmaf(s2v * s2, q, s2v);
+}
+
 // LogExpFunctions 1.0.1 log1pexp with the Maechler thresholds for the
 // working precision (53 or 24 mantissa bits).
 __device__ inline double log1pexp(double x) {
@@ -142,6 +176,12 @@ __device__ inline float log1pexp(float x) {
   return x;
 }
 
+// log(1 + e^x) for x <= 0 (or NaN), as logaddexp needs it.  FP32: branch-free
+// through the unbiased 8-term series (for x below -24 log 2 it returns e^x to
+// within e^2x, as the threshold branch would).
+__device__ inline double log1pexp_nonpos(double x) { return log1pexp(x); }
+__device__ inline float log1pexp_nonpos(float x) { return log1p_unit_f32(__expf(x)); }
+
 template <class T>
 __device__ inline T logaddexp(T a, T b) {
   T diff, mx;
@@ -152,7 +192,7 @@ __device__ inline T logaddexp(T a, T b) {
     diff = (a == b) ? T(0) : b - a;
     mx = (b == b) ? a : b;
   }
-  return mx + log1pexp(diff);
+  return mx + log1pexp_nonpos(diff);
 }
 
 // Online (running max, scaled sum) log-sum-exp accumulator.

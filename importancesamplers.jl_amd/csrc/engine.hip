@@ -183,6 +183,9 @@ void tick(is_job* j, int t, int which, hipStream_t s) {
 
 // ---------------------------------------------------------------- launch dispatch
 constexpr int64_t kSweepPerBlock = 512;  // samples per sweep block (one partial record)
+// samples per sweep block in the fused AMIS round: more 64-sample steps per wave
+// amortise the block's start and its reduction
+constexpr int64_t kRoundPerBlock = 1024;
 
 // Calls f(std::integral_constant<int, D>) for the job's dimension instantiation.
 template <class F>
@@ -659,8 +662,8 @@ int run_rounds(is_job* j) {
         // the sweep of the retained samples [0, first)
         SweepArgs<T> wo = w;
         wo.hi = first;
-        wo.per_block = kSweepPerBlock;
-        const int nrec_old = first > 0 ? (int)cdiv(first, kSweepPerBlock) : 0;
+        wo.per_block = kRoundPerBlock;
+        const int nrec_old = first > 0 ? (int)cdiv(first, kRoundPerBlock) : 0;
         const int nb_new = (int)cdiv(N, 64);
         tick(j, t, 0, s);
         fused = launch_round<T>(D, t + 1, a, wo, nb_new, nrec_old, s);

@@ -53,40 +53,6 @@ struct TargetArgs {
 
 __host__ __device__ constexpr int target_row_stride(int ks) { return 4 * ks + 1; }
 
-// log1p(t) for t in [0, 1] as 2 atanh(s), s = t / (2 + t) <= 1/3:
-// 2 s (1 + s^2/3 + s^4/5 + ...), NC terms after the leading one.  The first
-// omitted term is below 1.3e-10 of the sum with NC = 8 and at FP64 rounding
-// with NC = 16.  Branch-free: s from one reciprocal and Newton / exact-residual
-// corrections, so that its rounding is unbiased.
-template <int NC>
-__device__ inline double log1p_unit(double t) {
-  const double w = 2.0 + t;
-  double r = __builtin_amdgcn_rcp(w);
-  r = fma(r, fma(-w, r, 1.0), r);
-  r = fma(r, fma(-w, r, 1.0), r);
-  double sv = t * r;
-  sv = fma(fma(-sv, w, t), r, sv);
-  const double s2 = sv * sv;
-  double q = 1.0 / (2 * NC + 1);
-#pragma unroll
-  for (int k = NC - 1; k >= 1; --k) q = fma(q, s2, 1.0 / (2 * k + 1));
-  const double s2v = 2.0 * sv;
-  return fma(s2v * s2, q, s2v);
-}
-__device__ inline float log1p_unit_f32(float t) {
-  const float w = 2.0f + t;
-  const float w_lo = t - (w - 2.0f);  // 2 + t = w + w_lo exactly
-  const float r = __builtin_amdgcn_rcpf(w);
-  const float s0 = t * r;
-  const float sv = fmaf(fmaf(-s0, w, t) - s0 * w_lo, r, s0);
-  const float s2 = sv * sv;
-  float q = 1.0f / 17.0f;
-#pragma unroll
-  for (int k = 7; k >= 1; --k) q = fmaf(q, s2, 1.0f / (2 * k + 1));
-  const float s2v = 2.0f * sv;
-  return fmaf(s2v * s2, q, s2v);
-}
-
 // One row's log-likelihood term for linear predictor eta and response y,
 // branch-free.  Logistic: y eta - log(1 + e^eta) = y eta - max(eta, 0) -
 // log1p(e^-|eta|); linear regression: -(y - eta)^2 / (2 sigma^2), the
