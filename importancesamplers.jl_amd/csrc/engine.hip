@@ -245,26 +245,48 @@ void launch_target(int D, const TargetArgs<T>& t, int64_t splits, hipStream_t s)
 }
 
 // Summary / fit on a reduced record (one wavefront, any d <= 64).
+#define IS_EXACT_DIMS(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
 void launch_finish(int d, const double* reduced, const FinishArgs& f, hipStream_t s) {
-  if (d <= 16)
-    hipLaunchKernelGGL(k_finish_small<16>, dim3(1), dim3(64), 0, s, reduced, f);
-  else if (d <= 32)
-    hipLaunchKernelGGL(k_finish_small<32>, dim3(1), dim3(64), 0, s, reduced, f);
+  switch (d) {
+#define IS_FINISH_CASE(v)                                                                     \
+  case v:                                                                                     \
+    hipLaunchKernelGGL((k_finish_small<v, true>), dim3(1), dim3(64), 0, s, reduced, f); \
+    return;
+    IS_EXACT_DIMS(IS_FINISH_CASE)
+#undef IS_FINISH_CASE
+    default:
+      break;
+  }
+  if (d <= 32)
+    hipLaunchKernelGGL((k_finish_small<32, false>), dim3(1), dim3(64), 0, s, reduced, f);
   else
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, reduced, f);
 }
 
 // Reduction of `nrec` partial records into `reduced`, then summary / fit: one
 // launch for d <= 32 (the last reduction block fits; f.counter is its ticket
-// word), a reduction and a finish launch above.
+// word; d <= 16 compiled for the exact d), a reduction and a finish launch above.
 void launch_reduce_finish(int d, const double* partials, int64_t nrec, int64_t pcap, int nv, double* reduced,
                           const FinishArgs& f, hipStream_t s) {
-  if (d <= 16) {
-    hipLaunchKernelGGL(k_reduce_finish<16>, dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, reduced, f);
-  } else if (d <= 32) {
-    hipLaunchKernelGGL(k_reduce_finish<32>, dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, reduced, f);
+  switch (d) {
+#define IS_REDUCE_CASE(v)                                                                          \
+  case v:                                                                                          \
+    hipLaunchKernelGGL((k_reduce_finish<v, true>), dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, \
+                       reduced, f);                                                                \
+    return;
+    IS_EXACT_DIMS(IS_REDUCE_CASE)
+#undef IS_REDUCE_CASE
+    default:
+      break;
+  }
+  if (d <= 32) {
+    hipLaunchKernelGGL((k_reduce_finish<32, false>), dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv,
+                       reduced, f);
   } else {
-    hipLaunchKernelGGL(k_reduce_finish<0>, dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv, reduced, f);
+    hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(nv), dim3(kBlock), 0, s, partials, nrec, pcap, nv,
+                       reduced, f);
     launch_finish(d, reduced, f, s);
   }
 }
@@ -1687,7 +1709,7 @@ static int job_lognormalizer_typed(is_job* job, double* out3) {
   f.summary_count = n;
   f.summary_out = job->summary.as<double>();
   f.counter = job->counter.as<unsigned int>() + 1;
-  hipLaunchKernelGGL(k_reduce_finish<0>, dim3(2), dim3(kBlock), 0, job->stream,
+  hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(2), dim3(kBlock), 0, job->stream,
                      job->partials2.as<double>(), (int64_t)blocks, job->pcap, 2, job->reduced2.as<double>(), f);
   launch_finish(job->d, job->reduced2.as<double>(), f, job->stream);
   HIP_TRY(hipGetLastError());
@@ -1737,7 +1759,7 @@ static int job_moments_typed(is_job* job, double* mean, double* var, double* cov
   f.d = d;
   f.flags = 0;
   f.counter = job->counter.as<unsigned int>() + 2;
-  hipLaunchKernelGGL(k_reduce_finish<0>, dim3(nv), dim3(kBlock), 0, job->stream,
+  hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(nv), dim3(kBlock), 0, job->stream,
                      job->partials2.as<double>(), (int64_t)blocks, job->pcap, nv, job->reduced2.as<double>(), f);
   HIP_TRY(hipGetLastError());
   std::vector<double> red(1 + nv);
@@ -1841,7 +1863,7 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   f.summary_count = n;
   f.summary_out = summary.as<double>();
   f.counter = counter.as<unsigned int>();
-  hipLaunchKernelGGL(k_reduce_finish<0>, dim3(2), dim3(kBlock), 0, s, partials.as<double>(),
+  hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(2), dim3(kBlock), 0, s, partials.as<double>(),
                      (int64_t)blocks, pcap, 2, reduced.as<double>(), f);
   launch_finish(1, reduced.as<double>(), f, s);
   hipLaunchKernelGGL((k_cdf_tiles<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),

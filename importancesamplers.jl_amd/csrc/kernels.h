@@ -147,15 +147,16 @@ constexpr int kFinishAdaptEss = 4;
 // returns to zero) also runs the summary and fit of k_finish_small<DR> in its
 // first wave, so a round needs no separate finish launch.
 constexpr int kReduceUnroll = 8;
+__device__ inline void finish_summary(double M, double s0, double s2, const FinishArgs& f);
 template <int DR>
 struct FitPrefetch;
-template <int DR>
+template <int DR, bool EXACT>
 __device__ __forceinline__ void fit_prefetch(const FinishArgs& f, FitPrefetch<DR>& pf);
-template <int DR>
+template <int DR, bool EXACT>
 __device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f,
-                                          const FitPrefetch<DR>& pf);
+                                          const FitPrefetch<DR>& pf, bool summary);
 
-template <int DR>
+template <int DR, bool EXACT>
 static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
                                                           int64_t nrec, int64_t pcap, int nv,
                                                           double* __restrict__ reduced, FinishArgs f) {
@@ -169,7 +170,7 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
   // the fit's inputs from earlier launches, fetched now in case this block is last
   FitPrefetch<(DR > 0 ? DR : 1)> pf;
   if constexpr (DR > 0) {
-    if (tid < 64) fit_prefetch<DR>(f, pf);
+    if (tid < 64) fit_prefetch<DR, EXACT>(f, pf);
   }
   double m = ninf, s = 0.0;
   for (int64_t base = 0; base < nrec; base += (int64_t)kBlock * kReduceUnroll) {
@@ -226,9 +227,13 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
       if (last) __hip_atomic_store(f.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    if (last && tid < 64) {
+    if (last) {
       __atomic_thread_fence(__ATOMIC_ACQUIRE);
-      fit_small<DR>(reduced, f, pf);
+      // wave 0 fits; the summary runs beside it on wave 1
+      if (tid < 64)
+        fit_small<DR, EXACT>(reduced, f, pf, false);
+      else if (tid == 64 && (f.flags & (kFinishSummary | kFinishAdaptEss)))
+        finish_summary(reduced[0], reduced[1], reduced[2], f);
     }
   } else {
     (void)f;
@@ -422,10 +427,10 @@ struct FitPrefetch {
   double cr;
 };
 
-template <int DR>
+template <int DR, bool EXACT>
 __device__ __forceinline__ void fit_prefetch(const FinishArgs& f, FitPrefetch<DR>& pf) {
   const int lane = threadIdx.x & 63;
-  const int d = f.d;
+  const int d = EXACT ? DR : f.d;
   const bool row = lane < d && (f.flags & kFinishFit) != 0;
   const int rl = 2 * d + lane * (lane + 1) / 2;
 #pragma unroll
@@ -441,11 +446,11 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
   return r;
 }
 
-template <int DR>
+template <int DR, bool EXACT>
 __device__ __forceinline__ void fit_small(const double* __restrict__ red, const FinishArgs& f,
-                                          const FitPrefetch<DR>& pf) {
+                                          const FitPrefetch<DR>& pf, bool summary) {
   const int lane = threadIdx.x & 63;
-  const int d = f.d;
+  const int d = EXACT ? DR : f.d;
   const bool row = lane < d;
   const bool fit = (f.flags & kFinishFit) != 0;
   // ---- one batch of loads: summary, row `lane` of S, s1
@@ -455,7 +460,7 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
 #pragma unroll
   for (int c = 0; c < DR; ++c) a[c] = (fit && row && c <= lane) ? red[rs + c] : 0.0;
   const double s1 = (fit && row) ? red[3 + lane] : 0.0;
-  if (lane == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) finish_summary(M, s0, s2, f);
+  if (summary && lane == 0 && (f.flags & (kFinishSummary | kFinishAdaptEss))) finish_summary(M, s0, s2, f);
   if (!fit) return;
   if (!(s0 > 0.0) || !isfinite(s0) || M == -__builtin_huge_val()) {
     if (lane == 0) f.fit_status[f.round] = 1;
@@ -568,11 +573,12 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
   }
 }
 
-template <int DR>
+// EXACT: d == DR, compiled without per-column guards (d <= 16)
+template <int DR, bool EXACT>
 static __global__ __launch_bounds__(64) void k_finish_small(const double* __restrict__ red, FinishArgs f) {
   FitPrefetch<DR> pf;
-  fit_prefetch<DR>(f, pf);
-  fit_small<DR>(red, f, pf);
+  fit_prefetch<DR, EXACT>(f, pf);
+  fit_small<DR, EXACT>(red, f, pf, true);
 }
 
 // ---------------------------------------------------------------- static MIS assignment
