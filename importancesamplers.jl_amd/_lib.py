@@ -120,6 +120,9 @@ SIGNATURES = {
     "is_job_reset": (ctypes.c_int, [VP]),
     "is_job_lognormalizer": (ctypes.c_int, [VP, F64P]),
     "is_job_moments": (ctypes.c_int, [VP, F64P, F64P, F64P]),
+    "is_job_functional_moments": (ctypes.c_int, [VP, VP, ctypes.c_int32, ctypes.c_int32, F64P, F64P]),
+    "is_functional_moments_device": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32, ctypes.c_int64,
+                                                    F64P, F64P, VP]),
     "is_job_resample": (ctypes.c_int, [VP, F64P, ctypes.c_int64, ctypes.c_int32, I64P]),
     "is_cdf_device": (ctypes.c_int, [VP, VP, ctypes.c_int64, VP, VP]),
     "is_select_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int32, VP, VP]),

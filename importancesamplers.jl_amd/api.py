@@ -813,16 +813,59 @@ def _device_moments(result):
     return mean, var, cov.reshape(d, d).T.copy()
 
 
-def mean(result: WeightedSamples):
-    """Self-normalized weighted mean (src/statistics.jl:11-15)."""
+@dataclass
+class DeviceValues:
+    """Per-sample values f(x_i) already in device memory (a pointer on the engine's HIP
+    runtime, e.g. written by the caller's own kernel over is_job_device_results'
+    samples), FP64 when fp64 else FP32, in sample order."""
+    ptr: int
+    fp64: bool = True
+
+
+def _functional_moments(f, result, want_var):
+    """mean(f, result) / var(f, result) (src/statistics.jl:17-31,50-64,144-199) against the
+    device-resident log-weights, reduced in FP64 on the device.  f is a DeviceValues, or
+    a callable mapping the samples (n,) or (n, d) to one number per sample, evaluated on
+    the result's host samples and uploaded."""
+    s = result._sampler
+    if s is None or s._job is None:
+        raise ValueError("result is detached from its device sampler")
+    m, v = np.zeros(1), np.zeros(1)
+    if isinstance(f, DeviceValues):
+        rc = s._lib.is_job_functional_moments(s._job, VP(int(f.ptr)), int(bool(f.fp64)), 1, _p(m),
+                                              _p(v) if want_var else None)
+    else:
+        vals = np.asarray(f(result.samples), dtype=np.float64)
+        if vals.size != len(result) or (vals.ndim > 1 and vals.shape[0] != len(result)):
+            raise ValueError("functionals must return one number per sample")
+        vals = np.ascontiguousarray(vals.reshape(-1))
+        rc = s._lib.is_job_functional_moments(s._job, vals.ctypes.data_as(VP), 1, 0, _p(m),
+                                              _p(v) if want_var else None)
+    if rc == _lib.IS_EFAILED:
+        raise AllZeroWeightsError()
+    _check(rc)
+    return float(m[0]), float(v[0])
+
+
+def mean(f_or_result, result: WeightedSamples = None):
+    """mean(result): self-normalized weighted mean (src/statistics.jl:11-15);
+    mean(f, result): weighted expectation of the scalar functional f, evaluated on the
+    device samples (src/statistics.jl:17-31)."""
+    if result is not None:
+        return _functional_moments(f_or_result, result, False)[0]
+    result = f_or_result
     m, _, _ = _device_moments(result)
     return float(m[0]) if result.samples.ndim == 1 else m
 
 
-def var(result: WeightedSamples, corrected=False):
-    """Uncorrected weighted variance (src/statistics.jl:39-46)."""
+def var(f_or_result, result: WeightedSamples = None, corrected=False):
+    """var(result): uncorrected weighted variance (src/statistics.jl:39-46); var(f, result):
+    that of the scalar functional f (src/statistics.jl:50-64)."""
     if corrected:
         raise ValueError("corrected=true is unsupported")
+    if result is not None:
+        return _functional_moments(f_or_result, result, True)[1]
+    result = f_or_result
     _, v, _ = _device_moments(result)
     return float(v[0]) if result.samples.ndim == 1 else v
 

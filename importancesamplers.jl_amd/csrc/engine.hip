@@ -1890,6 +1890,81 @@ int is_select_device(is_ctx* ctx, const double* cdf, const double* lw, int64_t n
   return IS_OK;
 }
 
+// Statistics.mean(f, r) / var(f, r) on device arrays: pass 0 records and their
+// merge give the mean; pass 1 (centre read from the pass-0 merge on the device)
+// gives the variance.  One host wait, for both results.
+extern "C++" {
+template <class T, class V>
+static int functional_moments(const void* lw, const void* val, int64_t n, double* mean, double* var,
+                              hipStream_t s) {
+  const int64_t blocks = cdiv(n, kFuncPerBlock);
+  const int64_t pcap = blocks;
+  DevBuf partials, red;
+  int rc;
+  if ((rc = partials.alloc((size_t)pcap * 4 * sizeof(double))) || (rc = red.alloc(8 * sizeof(double)))) return rc;
+  double* r0 = red.as<double>();
+  double* r1 = r0 + 4;
+  FinishArgs f{};
+  hipLaunchKernelGGL((k_functional<T, V>), dim3((unsigned)blocks), dim3(kBlock), 0, s, (const T*)lw, (const V*)val, n,
+                     (const double*)nullptr, partials.as<double>(), pcap);
+  hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(3), dim3(kBlock), 0, s, partials.as<double>(), blocks, pcap, 3,
+                     r0, f);
+  if (var) {
+    hipLaunchKernelGGL((k_functional<T, V>), dim3((unsigned)blocks), dim3(kBlock), 0, s, (const T*)lw, (const V*)val,
+                       n, (const double*)r0, partials.as<double>(), pcap);
+    hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(3), dim3(kBlock), 0, s, partials.as<double>(), blocks, pcap,
+                       3, r1, f);
+  }
+  HIP_TRY(hipGetLastError());
+  double h[8];
+  HIP_TRY(hipMemcpyAsync(h, r0, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (!(h[1] > 0.0)) return fail(IS_EFAILED, "all weights are zero");
+  if (mean) *mean = h[3] / h[1];
+  if (var) *var = h[7] / h[5];
+  return IS_OK;
+}
+
+static int functional_dispatch(const void* lw, int lw_fp64, const void* val, int val_fp64, int64_t n, double* mean,
+                               double* var, hipStream_t s) {
+  if (lw_fp64)
+    return val_fp64 ? functional_moments<double, double>(lw, val, n, mean, var, s)
+                    : functional_moments<double, float>(lw, val, n, mean, var, s);
+  return val_fp64 ? functional_moments<float, double>(lw, val, n, mean, var, s)
+                  : functional_moments<float, float>(lw, val, n, mean, var, s);
+}
+}  // extern "C++"
+
+int is_functional_moments_device(is_ctx* ctx, const void* logweights_device, int32_t logweights_fp64,
+                                 const void* values_device, int32_t values_fp64, int64_t n, double* mean, double* var,
+                                 void* stream) {
+  if (!ctx || !logweights_device || !values_device || n < 1) return fail(IS_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  return functional_dispatch(logweights_device, logweights_fp64, values_device, values_fp64, n, mean, var,
+                             (hipStream_t)stream);
+}
+
+int is_job_functional_moments(is_job* job, const void* values, int32_t values_fp64, int32_t values_on_device,
+                              double* mean, double* var) {
+  REQUIRE_RESULTS(job);
+  if (!values) return fail(IS_EINVAL, "null values");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  HIP_TRY(hipSetDevice(job->ctx->device));
+  const void* dv = values;
+  DevBuf up;
+  if (!values_on_device) {
+    const size_t bytes = (size_t)job->total * (values_fp64 ? sizeof(double) : sizeof(float));
+    int rc = up.alloc(bytes);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(up.p, values, bytes, hipMemcpyHostToDevice, job->stream));
+    dv = up.p;
+  }
+  // the device log-weights carry the offset t_shift, which normalised weights cancel
+  return functional_dispatch(job->logweights.p, job->precision == IS_FP64, dv, values_fp64, job->total, mean, var,
+                             job->stream);
+}
+
 int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf) {
   if (!ctx || !logweights || !cdf || n < 1) return fail(IS_EINVAL, "bad arguments");
   HIP_TRY(hipSetDevice(ctx->device));

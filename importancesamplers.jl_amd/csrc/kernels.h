@@ -240,6 +240,92 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
   }
 }
 
+// Weighted functional partial records for Statistics.mean(f, r) and var(f, r)
+// (src/statistics.jl:17-31,50-64,170-199) over device arrays: block b covers
+// samples [b kFuncPerBlock, ...) and writes the column-major record
+// [M, sum e, sum e^2, sum e g], e = exp(lw - M), with g = f (pass 0: zero-weight
+// samples enter as 0 f, so a non-finite f there gives NaN, as dot(weights,
+// values) does) or, when `center_rec` is given (pass 1), the squared deviation
+// about the centre center_rec[3] / center_rec[1], zero-weight samples skipped and
+// a non-finite deviation recomputed as (sqrt(e) f - sqrt(e) c)^2.  A block whose
+// weights are all zero stores the finite maximum -DBL_MAX so the merge keeps its
+// sums (and any NaN).  k_reduce_finish<0> merges the records (nv = 3).
+constexpr int kFuncPerThread = 8;
+constexpr int64_t kFuncPerBlock = (int64_t)kBlock * kFuncPerThread;
+template <class T, class V>
+static __global__ __launch_bounds__(kBlock) void k_functional(const T* __restrict__ lw, const V* __restrict__ val,
+                                                              int64_t n, const double* __restrict__ center_rec,
+                                                              double* __restrict__ partials, int64_t pcap) {
+  __shared__ double wm[kBlock / 64], w0[kBlock / 64], w2[kBlock / 64], w1[kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double ninf = -__builtin_huge_val();
+  const bool dev_pass = center_rec != nullptr;
+  const double c = dev_pass ? center_rec[3] / center_rec[1] : 0.0;
+  const int64_t base = (int64_t)blockIdx.x * kFuncPerBlock;
+  double l[kFuncPerThread], v[kFuncPerThread];
+#pragma unroll
+  for (int u = 0; u < kFuncPerThread; ++u) {
+    const int64_t i = base + (int64_t)u * kBlock + tid;
+    l[u] = i < n ? (double)lw[i] : ninf;
+    v[u] = i < n ? (double)val[i] : 0.0;
+  }
+  double m = ninf;
+#pragma unroll
+  for (int u = 0; u < kFuncPerThread; ++u) m = fmax(m, l[u]);
+  double s0 = 0.0, s2 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int u = 0; u < kFuncPerThread; ++u) {
+    const double e = (l[u] == ninf) ? 0.0 : exp(l[u] - m);
+    s0 += e;
+    s2 += e * e;
+    if (!dev_pass) {
+      s1 += e * v[u];
+    } else if (e != 0.0) {
+      const double dv = v[u] - c;
+      double t;
+      if (isfinite(dv)) {
+        t = dv * (e * dv);
+      } else {
+        const double r = sqrt(e);
+        const double q = r * v[u] - r * c;
+        t = q * q;
+      }
+      s1 += t;
+    }
+  }
+  // block merge: the block maximum, each thread rescaled to it, sums in thread order
+  double M = wave_max(m);
+  if (lane == 0) wm[wave] = M;
+  __syncthreads();
+  M = wm[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) M = fmax(M, wm[w]);
+  const double sc = (m == ninf) ? 0.0 : exp(m - M);
+  s0 = wave_sum(s0 * sc);
+  s2 = wave_sum(s2 * sc * sc);
+  s1 = wave_sum(s1 * sc);
+  if (lane == 0) {
+    w0[wave] = s0;
+    w2[wave] = s2;
+    w1[wave] = s1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t0 = 0.0, t2 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t0 += w0[w];
+      t2 += w2[w];
+      t1 += w1[w];
+    }
+    double* out = partials + blockIdx.x;
+    out[0] = (M == ninf) ? -1.7976931348623157e308 : M;
+    out[1 * pcap] = t0;
+    out[2 * pcap] = t2;
+    out[3 * pcap] = t1;
+  }
+}
+
 // Summary (src/execution.jl:176-186): ESS = s0^2 / s2, log Z = M + log s0 - log n.
 __device__ inline void finish_summary(double M, double s0, double s2, const FinishArgs& f) {
   double ess, logz;

@@ -207,6 +207,15 @@ int is_job_reset(is_job* job);
  * mean / var / cov (src/statistics.jl:11-103): FP64 device reductions. */
 int is_job_lognormalizer(is_job* job, double* out3);
 int is_job_moments(is_job* job, double* mean, double* var, double* cov);
+/* mean(f, result) / var(f, result) (src/statistics.jl:17-31,50-64,144-199) for a scalar
+ * functional: values holds f(x_i) for the last run's samples in sample order (total
+ * samples; FP64 when values_fp64, else FP32), in device memory when values_on_device
+ * (the device functional path: the caller evaluates f on is_job_device_results'
+ * samples), else in host memory (uploaded first).  mean = sum_i w_i f_i and
+ * var = sum_i w_i (f_i - mean)^2 with w the normalised weights of the device-resident
+ * log-weights, reduced in FP64; var may be NULL.  IS_EFAILED when every weight is zero. */
+int is_job_functional_moments(is_job* job, const void* values, int32_t values_fp64, int32_t values_on_device,
+                              double* mean, double* var);
 /* resample (src/resampling.jl:245-283): multinomial or systematic ancestors of the
  * last run's logweights for host uniforms (count, or 1 for systematic). */
 int is_job_resample(is_job* job, const double* uniforms, int64_t count, int32_t method,
@@ -220,6 +229,15 @@ int is_cdf_device(is_ctx* ctx, const double* logweights_device, int64_t n, doubl
 int is_select_device(is_ctx* ctx, const double* cdf_device, const double* logweights_device, int64_t n,
                      const double* uniforms_device, int64_t count, int32_t method,
                      int32_t* ancestors_device, void* stream);
+/* mean(f, result) / var(f, result) over device arrays (the device functional path of
+ * src/statistics.jl:17-31,50-64,144-163 for a result adopted from device arrays,
+ * src/results.jl:485-546): log-weights and per-sample values, each FP64 or FP32.
+ * Zero-weight samples enter the mean as 0 * f_i (a non-finite f_i there gives NaN, as
+ * dot(weights, values) does) and are skipped by the variance
+ * (src/statistics.jl:188-197). */
+int is_functional_moments_device(is_ctx* ctx, const void* logweights_device, int32_t logweights_fp64,
+                                 const void* values_device, int32_t values_fp64, int64_t n, double* mean,
+                                 double* var, void* stream);
 /* Host-buffer forms of the two primitives above (copy in, run, copy out); ancestors 1-based. */
 int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf);
 int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t n, const double* uniforms,

@@ -40,6 +40,10 @@ Parity pins (what checks each oracle function; tests/test_oracle_golden.py):
 * ``logweight_summary`` / ``lognormalizer`` / ``moments`` --
   ``test/dm_pmc_kernel.jl:274-296`` and ``test/results.jl:9-75,357-388``.
 * ``cholesky`` -- numpy.linalg.cholesky on random SPD matrices.
+* ``functional_moments`` -- the known answers of ``test/results.jl:9-38``
+  (mean/var of abs2 and of x1 + 2 x2) and the variance range case of
+  ``test/cuda_statistics.jl:27-38`` (exact rationals), plus the zero-weight
+  rules of ``src/statistics.jl:188-197``.
 """
 
 from __future__ import annotations
@@ -276,6 +280,26 @@ def moments(samples, lw):
     if lib().or_moments(_p(x), _p(lw), n, d, _p(mean), _p(var), _p(cov)):
         raise ZeroDivisionError("all weights are zero")
     return mean, var, cov.reshape(d, d).T.copy()
+
+
+def functional_moments(lw, values):
+    """mean(f, result) and var(f, result) of src/statistics.jl:17-31,50-64,170-199 for
+    per-sample values f_i: w = normalized_weights (src/results.jl:589-598),
+    mean = dot(w, f) (a zero weight times a non-finite value gives NaN), and
+    var = sum of _weighted_squared_deviation(f_i, mean, w_i): (f - c) (w (f - c)),
+    recomputed as (sqrt(w) f - sqrt(w) c)^2 when f - c is not finite, 0 when w = 0."""
+    w = normalized_weights(lw)
+    f = _f64(values)
+    with np.errstate(invalid="ignore", over="ignore"):
+        mean = float(np.dot(w, f))
+        dev = f - mean
+        contrib = dev * (w * dev)
+        bad = ~np.isfinite(dev)
+        r = np.sqrt(w)
+        contrib[bad] = (r[bad] * f[bad] - r[bad] * mean) ** 2
+        contrib[w == 0.0] = 0.0
+        var = float(np.sum(contrib))
+    return mean, var
 
 
 def cholesky(A):
