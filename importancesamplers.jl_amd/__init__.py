@@ -38,6 +38,7 @@ from .api import (  # noqa: F401
     WeightedSamples,
     context,
     cov,
+    BatchTarget,
     DeviceValues,
     current_proposal,
     importance_sample,

@@ -27,6 +27,11 @@ I32P = ctypes.POINTER(ctypes.c_int32)
 VP = ctypes.c_void_p
 
 
+# is_batch_target_fn: (user, samples, coord_stride, dim, count, values, is_float, stream) -> int
+BATCH_TARGET_FN = ctypes.CFUNCTYPE(ctypes.c_int, VP, VP, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, VP,
+                                   ctypes.c_int32, VP)
+
+
 class JobDesc(ctypes.Structure):
     _fields_ = [
         ("method", ctypes.c_int32),
@@ -54,6 +59,8 @@ class JobDesc(ctypes.Structure):
         ("mis_group_count", ctypes.c_int32),
         ("mis_group_offsets", I32P),
         ("mis_group_members", I32P),
+        ("target_batch", BATCH_TARGET_FN),
+        ("target_user", VP),
     ]
 
 

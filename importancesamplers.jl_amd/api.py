@@ -281,6 +281,33 @@ class GaussianTarget:
         return np.concatenate([self.mean, self.factor.T.ravel(), [self.lognormalizer]])
 
 
+class BatchTarget:
+    """A caller's batch target (the `batch=batch!` contract, src/targets.jl:17-43,
+    src/batched_targets.jl:92-114): batch(values, samples, coord_stride, dim, count,
+    is_float, stream) sets values[i], i < count, to the log density of sample i, whose
+    coordinate c is element c * coord_stride + i of `samples`.  values and samples are
+    device pointers (FP32 when is_float, else FP64) ordered on the job's HIP stream
+    `stream`; values start as NaN, -Inf is valid, NaN and +Inf fail the round.  Return 0,
+    or raise / return nonzero to abort the run."""
+
+    kind: int = 4
+
+    def __init__(self, batch):
+        self.batch = batch
+
+        def _cb(user, samples, stride, dim, count, values, is_float, stream):
+            try:
+                rc = self.batch(values, samples, int(stride), int(dim), int(count), bool(is_float), stream)
+                return int(rc or 0)
+            except Exception:  # the callback's own failure aborts the run
+                return -1
+
+        self._fn = _lib.BATCH_TARGET_FN(_cb)
+
+    def params(self, d):
+        return np.zeros(1)
+
+
 @dataclass
 class LogisticRegressionTarget:
     """Bayesian logistic regression with prior N(0, prior_var I)."""
@@ -535,6 +562,9 @@ def _job_desc(rng, target, algorithm, precision):
     keep.append(tp)
     desc.target_params = _p(tp)
     desc.target_params_len = tp.size
+    if isinstance(target, BatchTarget):
+        desc.target_batch = target._fn
+        keep.append(target)
     return desc, keep, p, sched
 
 

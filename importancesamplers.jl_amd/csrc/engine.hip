@@ -152,6 +152,10 @@ struct is_job {
   // it from the device, so the captured launch sequence stays valid across runs
   // the run's launch sequence captured as a HIP graph (replayed while valid)
   int use_graph = 1;
+  // caller's batch target (IS_TARGET_CALLBACK): called between the draw and the
+  // mixture pass of every round, on the job's stream, outside any graph capture
+  is_batch_target_fn tbatch = nullptr;
+  void* tuser = nullptr;
   bool graph_dirty = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -538,11 +542,36 @@ int run_rounds(is_job* j) {
   const int nv_mom = 2 + d + d * (d + 1) / 2;
   const int nt_mom = sweep_nt(d, true);
   unsigned int* counter = j->counter.as<unsigned int>();
-  const bool batch_target = j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG;
-  auto mix = [&](RoundArgs<T>& a, int ncomp) {
+  const bool batch_target =
+      j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG || j->tkind == IS_TARGET_CALLBACK;
+  auto mix = [&](RoundArgs<T>& a, int ncomp) -> int {
     if (!batch_target) {
       launch_mix<T, kDrawFused>(D, ncomp, a, s);
-      return;
+      return IS_OK;
+    }
+    if (j->tkind == IS_TARGET_CALLBACK) {
+      // draw, the caller's batch target over the round's samples (values start as
+      // NaN, src/batched_targets.jl:96-98), then the mixture with those values; the
+      // mixture pass checks them (NaN / +Inf fail in the target phase)
+      launch_mix<T, kDrawOnly>(D, ncomp, a, s);
+      HIP_TRY(hipMemsetAsync(j->tpre.p, 0xff, (size_t)a.n_new * sizeof(T), s));
+      HIP_TRY(hipGetLastError());
+      const int cb = j->tbatch(j->tuser, j->X.as<T>() + a.first, j->total, d, a.n_new, j->tpre.p,
+                               sizeof(T) == 4 ? 1 : 0, (void*)s);
+      if (cb != 0) {
+        // the run fails in the target phase of this round and commits nothing
+        (void)hipStreamSynchronize(s);
+        is_failure fr{};
+        fr.round = a.round + 1;
+        fr.phase = IS_PHASE_TARGET;
+        j->failure = fr;
+        j->have_results = false;
+        j->last_status = IS_EFAILED;
+        return fail(IS_EFAILED, "batch target callback returned %d in round %d", cb, a.round + 1);
+      }
+      a.pre_logtarget = j->tpre.as<T>();
+      launch_mix<T, kPreTarget>(D, ncomp, a, s);
+      return IS_OK;
     }
     // draw, batched target on MFMA, then the mixture with the precomputed target values
     launch_mix<T, kDrawOnly>(D, ncomp, a, s);
@@ -567,6 +596,7 @@ int run_rounds(is_job* j) {
                        j->tpre.as<T>());
     a.pre_logtarget = j->tpre.as<T>();
     launch_mix<T, kPreTarget>(D, ncomp, a, s);
+    return IS_OK;
   };
   auto reduce = [&](const DevBuf& partials, int nrec, int nv, DevBuf& reduced, const FinishArgs& f) {
     launch_reduce_finish(d, partials.as<double>(), (int64_t)nrec, j->pcap, nv, reduced.as<double>(), f, s);
@@ -606,7 +636,7 @@ int run_rounds(is_job* j) {
     a.k0 = 0;
     a.k1 = ncomp - 1;
     tick(j, 0, 0, s);
-    mix(a, ncomp);
+    if (const int mrc = mix(a, ncomp)) return mrc;
     tick(j, 0, 1, s);
     SweepArgs<T> w = sweep_args<T>(j, 0);
     w.lo = 0;
@@ -633,7 +663,7 @@ int run_rounds(is_job* j) {
     a.logtotal = T(0);
     a.record_nonfinite_after_exhaust = 0;
     tick(j, 0, 0, s);
-    mix(a, 1);
+    if (const int mrc = mix(a, 1)) return mrc;
     tick(j, 0, 1, s);
     SweepArgs<T> w = sweep_args<T>(j, 0);
     w.lo = 0;
@@ -693,7 +723,7 @@ int run_rounds(is_job* j) {
         nrec = nrec_old + nb_new;
       }
       if (!fused) {
-        mix(a, t + 1);
+        if (const int mrc = mix(a, t + 1)) return mrc;
         tick(j, t, 0, s);
         nrec = launch_sweep<T>(D, nt_mom, w, s);
         tick(j, t, 1, s);
@@ -712,7 +742,7 @@ int run_rounds(is_job* j) {
       a.logmass = j->zerosT.as<T>();
       a.logtotal = T(0);
       tick(j, t, 0, s);
-      mix(a, 1);
+      if (const int mrc = mix(a, 1)) return mrc;
       tick(j, t, 1, s);
       SweepArgs<T> w = sweep_args<T>(j, t);
       w.lo = first;
@@ -752,7 +782,7 @@ int run_rounds(is_job* j) {
       a.logmass = j->logmassT.as<T>() + (size_t)t * j->K;
       a.logtotal = T(0);
       tick(j, t, 0, s);
-      mix(a, j->K);
+      if (const int mrc = mix(a, j->K)) return mrc;
       tick(j, t, 1, s);
       SweepArgs<T> w = sweep_args<T>(j, t);
       w.lo = first;
@@ -839,7 +869,7 @@ int job_run_typed(is_job* j) {
   int rc = IS_OK;
   // per-round event timing records events between launches, so a timed run is issued
   // eagerly; otherwise the whole launch sequence is replayed from one captured graph
-  if (j->use_graph && !j->timing) {
+  if (j->use_graph && !j->timing && j->tkind != IS_TARGET_CALLBACK) {
     if (j->graph_dirty || !j->graph_exec) {
       CaptureGuard cg(capture_lock());
       if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
@@ -995,6 +1025,7 @@ int alloc_job_buffers(is_job* j) {
   }
   std::vector<double> zeros(std::max(j->nslots, 1), 0.0);
   if ((rc = upload_T<T>(j->zerosT, zeros))) return rc;
+  if (j->tkind == IS_TARGET_CALLBACK && (rc = j->tpre.alloc((size_t)j->maxN * sizeof(T)))) return rc;
   if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
     if ((rc = j->tpre.alloc((size_t)j->maxN * sizeof(T)))) return rc;
     int64_t part = 1;
@@ -1048,7 +1079,9 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   if (desc->method < IS_PLAIN || desc->method > IS_MIS) return fail(IS_EINVAL, "unknown method");
   if (desc->family != IS_GAUSSIAN && desc->family != IS_STUDENT_T) return fail(IS_EINVAL, "unknown family");
   if (desc->precision != IS_FP64 && desc->precision != IS_FP32) return fail(IS_EINVAL, "unknown precision");
-  if (desc->target_kind < IS_TARGET_STANDARD || desc->target_kind > IS_TARGET_LINREG)
+  if (desc->target_kind == IS_TARGET_CALLBACK && !desc->target_batch)
+    return fail(IS_EINVAL, "a callback target needs target_batch");
+  if (desc->target_kind < IS_TARGET_STANDARD || desc->target_kind > IS_TARGET_CALLBACK)
     return fail(IS_EINVAL, "unknown target kind %d", desc->target_kind);
   const int T = (desc->method == IS_PLAIN || desc->method == IS_MIS) ? 1 : desc->rounds;
   if (T < 1 || !desc->round_size) return fail(IS_EINVAL, "rounds must be positive");
@@ -1065,6 +1098,8 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   j->dof = desc->dof;
   j->resampling = desc->resampling;
   j->tkind = desc->target_kind;
+  j->tbatch = desc->target_batch;
+  j->tuser = desc->target_user;
   j->precision = desc->precision;
   j->key = RngKey{desc->seed, desc->stream};
   j->stride = slot_stride(d);
@@ -1276,6 +1311,7 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
   bytes += (int64_t)nslots * stride * (8 + ts);               // slot tables
   bytes += 2 * pcap * (1 + nv_mom) * 8 + 2 * (1 + nv_mom) * 8; // partial records, reduced records
   bytes += (int64_t)T * (5 * 8 + 3 * 8 + 2 * 4 + 3 * 8) + 4 * 64;
+  if (desc->target_kind == IS_TARGET_CALLBACK) bytes += maxN * ts;
   if (desc->target_kind == IS_TARGET_LOGISTIC || desc->target_kind == IS_TARGET_LINREG) {
     const int64_t n = desc->target_params && desc->target_params_len > 0 ? (int64_t)desc->target_params[0] : 0;
     bytes += n * (d + 1) * ts + maxN * ts;

@@ -60,8 +60,20 @@ enum is_target_kind {
   IS_TARGET_STANDARD = 0,  /* -sum(abs2, x)/2                                              */
   IS_TARGET_GAUSSIAN = 1,  /* params: mean[d], L[d*d] column-major lower, lognormalizer     */
   IS_TARGET_LOGISTIC = 2,  /* params: n, p, prior_var, X[n*p] row-major, y[n]               */
-  IS_TARGET_LINREG = 3     /* params: n, p, sigma, tau, X[n*p] row-major, y[n]              */
+  IS_TARGET_LINREG = 3,    /* params: n, p, sigma, tau, X[n*p] row-major, y[n]              */
+  IS_TARGET_CALLBACK = 4   /* a caller's batch target: target_batch, target_user             */
 };
+/* Batch target callback (batch!(values, samples), src/targets.jl:17-43,
+ * src/batched_targets.jl:92-114): set values[i], i < count, to the log density of
+ * sample i, whose coordinate c is element c * coord_stride + i of `samples` (FP32
+ * when is_float, else FP64, as `values`).  Both are device memory of the job and
+ * stream-ordered on `stream` (a hipStream_t): enqueue the work there, or synchronize
+ * it before reading and finish private work before returning.  values start as NaN;
+ * -Inf is a valid value, NaN and +Inf fail the round in the target phase.  Inputs are
+ * borrowed; a nonzero return aborts the run.  Jobs with a callback run eagerly (no
+ * graph replay). */
+typedef int (*is_batch_target_fn)(void* user, const void* samples, int64_t coord_stride, int32_t dim,
+                                  int64_t count, void* values, int32_t is_float, void* stream);
 enum is_phase {            /* AMISRoundError / NPMCRoundError / DMPMCRoundError phases       */
   IS_PHASE_NONE = 0,
   IS_PHASE_SAMPLING = 1,   /* :sampling / :proposal_draw                                     */
@@ -110,6 +122,9 @@ typedef struct is_job_desc {
   int32_t mis_group_count; /* partial mixture: number of groups G */
   const int32_t* mis_group_offsets; /* G + 1 offsets into the members, starting at 0 */
   const int32_t* mis_group_members; /* configured 1-based proposal ids, in group order, each id once */
+  /* caller's batch target (target_kind IS_TARGET_CALLBACK) */
+  is_batch_target_fn target_batch;
+  void* target_user;       /* passed back as the callback's first argument */
 } is_job_desc;
 
 /* Failure snapshot: _device_failure_snapshot (src/execution.jl:719-734) plus
