@@ -188,6 +188,12 @@ AMIS_CASES = [
     ("d10-12rounds", 10, [2000] * 12, "gauss", "factor", None),
     ("d20-factor", 20, [3000] * 4, "gauss", "factor", None),
     ("studentt5-d4", 4, [300, 400, 500], "gauss", "diagonal", 5.0),
+    # the fused round kernel's edges: d = 15 (largest fused d) with ragged round sizes,
+    # d = 16 (first unfused exact d) and Student-t factor draws at d = 15
+    # (rounds of several thousand samples keep the 15-d fits well conditioned)
+    ("d15-ragged", 15, [4001, 5537, 4777, 6048, 4640, 5999], "std", "factor", None),
+    ("d16-factor", 16, [1500] * 6, "gauss", "factor", None),
+    ("studentt3-d15", 15, [4700, 4900, 4650, 4800, 4700, 4900], "gauss", "factor", 3.0),
 ]
 
 
