@@ -53,25 +53,33 @@ void launch_sweep_any(int nt, const SweepArgs<T>& a, int blocks, int nw, size_t 
 constexpr size_t kRoundLds = 40 * 1024;
 
 template <class T, int D>
-bool launch_round_any(const RoundArgs<T>& a0, const SweepArgs<T>& w, int ncomp, int nb_new, int nrec_old,
+bool launch_round_any(int nt, const RoundArgs<T>& a0, const SweepArgs<T>& w, int ncomp, int nb_new, int nrec_old,
                       hipStream_t s) {
   if constexpr (D > kExactDims || nt_lo(D) != 1) {
     return false;
   } else {
+    if (nt != 0 && nt != 1) return false;
     RoundArgs<T> a = a0;
     const bool st = a.fam.student != 0;
     const size_t slot_bytes = (size_t)a.stride * sizeof(T);
     const size_t tslot_bytes = a.tkind == kTGaussian ? slot_bytes : 0;
     const size_t fixed = new_lds_fixed(D, 4, (int)sizeof(T)) + tslot_bytes;
-    const size_t cap = std::max(sweep_lds(1, 4), kRoundLds);
+    const size_t cap = std::max(sweep_lds(nt, 4), kRoundLds);
     const size_t room = cap > fixed ? cap - fixed : 0;
     a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), room / slot_bytes));
-    const size_t lds = std::max(sweep_lds(1, 4), fixed + (size_t)a.slot_chunk * slot_bytes);
+    const size_t lds = std::max(sweep_lds(nt, 4), fixed + (size_t)a.slot_chunk * slot_bytes);
     const int blocks = nb_new + nrec_old;
-    if (st)
-      hipLaunchKernelGGL((k_round<T, D, 1, true>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
-    else
-      hipLaunchKernelGGL((k_round<T, D, 1, false>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
+    if (nt == 1) {
+      if (st)
+        hipLaunchKernelGGL((k_round<T, D, 1, true>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
+      else
+        hipLaunchKernelGGL((k_round<T, D, 1, false>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
+    } else {
+      if (st)
+        hipLaunchKernelGGL((k_round<T, D, 0, true>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
+      else
+        hipLaunchKernelGGL((k_round<T, D, 0, false>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
+    }
     return true;
   }
 }
@@ -99,14 +107,14 @@ void DimKernels<D>::sweep(int nt, const SweepArgs<double>& a, int blocks, int nw
   launch_sweep_any<double, D>(nt, a, blocks, nw, lds, s);
 }
 template <int D>
-bool DimKernels<D>::round(const RoundArgs<float>& a, const SweepArgs<float>& w, int ncomp, int nb_new, int nrec_old,
-                          hipStream_t s) {
-  return launch_round_any<float, D>(a, w, ncomp, nb_new, nrec_old, s);
+bool DimKernels<D>::round(int nt, const RoundArgs<float>& a, const SweepArgs<float>& w, int ncomp, int nb_new,
+                          int nrec_old, hipStream_t s) {
+  return launch_round_any<float, D>(nt, a, w, ncomp, nb_new, nrec_old, s);
 }
 template <int D>
-bool DimKernels<D>::round(const RoundArgs<double>& a, const SweepArgs<double>& w, int ncomp, int nb_new,
+bool DimKernels<D>::round(int nt, const RoundArgs<double>& a, const SweepArgs<double>& w, int ncomp, int nb_new,
                           int nrec_old, hipStream_t s) {
-  return launch_round_any<double, D>(a, w, ncomp, nb_new, nrec_old, s);
+  return launch_round_any<double, D>(nt, a, w, ncomp, nb_new, nrec_old, s);
 }
 template <int D>
 void DimKernels<D>::target(const TargetArgs<float>& t, int64_t splits, hipStream_t s) {

@@ -236,10 +236,10 @@ int launch_sweep(int D, int nt, SweepArgs<T> a, hipStream_t s) {
 // Launches one fused AMIS round (k_round); false when the job's dimension
 // instantiation has none, and nothing was launched.
 template <class T>
-bool launch_round(int D, int ncomp, const RoundArgs<T>& a, const SweepArgs<T>& w, int nb_new, int nrec_old,
+bool launch_round(int D, int nt, int ncomp, const RoundArgs<T>& a, const SweepArgs<T>& w, int nb_new, int nrec_old,
                   hipStream_t s) {
   bool done = false;
-  with_dim(D, [&](auto k) { done = DimKernels<decltype(k)::value>::round(a, w, ncomp, nb_new, nrec_old, s); });
+  with_dim(D, [&](auto k) { done = DimKernels<decltype(k)::value>::round(nt, a, w, ncomp, nb_new, nrec_old, s); });
   return done;
 }
 
@@ -662,14 +662,29 @@ int run_rounds(is_job* j) {
     a.logmass = j->zerosT.as<T>();
     a.logtotal = T(0);
     a.record_nonfinite_after_exhaust = 0;
-    tick(j, 0, 0, s);
-    if (const int mrc = mix(a, 1)) return mrc;
-    tick(j, 0, 1, s);
     SweepArgs<T> w = sweep_args<T>(j, 0);
     w.lo = 0;
     w.hi = N;
     w.nv = 2;
-    const int nrec = launch_sweep<T>(D, 0, w, s);
+    int nrec = 0;
+    bool fused = false;
+    if (!batch_target) {
+      // one launch: the new-sample blocks write their own summary records
+      SweepArgs<T> wo = w;
+      wo.hi = 0;
+      wo.per_block = kRoundPerBlock;
+      const int nb_new = (int)cdiv(N, 64);
+      tick(j, 0, 0, s);
+      fused = launch_round<T>(D, 0, 1, a, wo, nb_new, 0, s);
+      tick(j, 0, 1, s);
+      nrec = nb_new;
+    }
+    if (!fused) {
+      tick(j, 0, 0, s);
+      if (const int mrc = mix(a, 1)) return mrc;
+      tick(j, 0, 1, s);
+      nrec = launch_sweep<T>(D, 0, w, s);
+    }
     FinishArgs f = base_finish(j, 0, counter);
     f.flags = kFinishSummary;
     f.summary_count = N;
@@ -718,7 +733,7 @@ int run_rounds(is_job* j) {
         const int nrec_old = first > 0 ? (int)cdiv(first, kRoundPerBlock) : 0;
         const int nb_new = (int)cdiv(N, 64);
         tick(j, t, 0, s);
-        fused = launch_round<T>(D, t + 1, a, wo, nb_new, nrec_old, s);
+        fused = launch_round<T>(D, 1, t + 1, a, wo, nb_new, nrec_old, s);
         tick(j, t, 1, s);
         nrec = nrec_old + nb_new;
       }

@@ -43,12 +43,13 @@ struct DimKernels {
   // k_sweep with `nt` moment tiles per side (0: summary only)
   static void sweep(int nt, const SweepArgs<float>& a, int blocks, int nw, size_t lds, hipStream_t s);
   static void sweep(int nt, const SweepArgs<double>& a, int blocks, int nw, size_t lds, hipStream_t s);
-  // k_round: one AMIS round (new samples beside the sweep of the retained ones);
-  // false when this instantiation has no fused round (d > 15)
-  static bool round(const RoundArgs<float>& a, const SweepArgs<float>& w, int ncomp, int nb_new, int nrec_old,
-                    hipStream_t s);
-  static bool round(const RoundArgs<double>& a, const SweepArgs<double>& w, int ncomp, int nb_new, int nrec_old,
-                    hipStream_t s);
+  // k_round: one round's new samples beside the sweep of the retained ones, with
+  // nt = 1 moment tiles (AMIS) or nt = 0 (summary only); false when this
+  // instantiation has no fused round (d > 15)
+  static bool round(int nt, const RoundArgs<float>& a, const SweepArgs<float>& w, int ncomp, int nb_new,
+                    int nrec_old, hipStream_t s);
+  static bool round(int nt, const RoundArgs<double>& a, const SweepArgs<double>& w, int ncomp, int nb_new,
+                    int nrec_old, hipStream_t s);
   // k_target_gemm over `splits` row slices
   static void target(const TargetArgs<float>& t, int64_t splits, hipStream_t s);
   static void target(const TargetArgs<double>& t, int64_t splits, hipStream_t s);
