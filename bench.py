@@ -591,9 +591,10 @@ def main_mis(args):
                 "fp64_plain_form": res["fp64"],
                 "speedup_vs_fp64_plain_form": res["fp32"]["ess_per_s"] / res["fp64"]["ess_per_s"],
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                             "frac": achieved / 8000.0, "traffic": _traffic(f"k_new_static_mis_d{D}_K{K}_N{n}_fp32"),
-                             "kernel": "k_new (assignment, draw, target, 16-term mixture log-sum-exp); "
-                                       "compute-bound: 16 whitened quadratic forms (~ 130 flop each) per sample",
+                             "frac": achieved / 8000.0, "traffic": _traffic(f"k_round_static_mis_d{D}_K{K}_N{n}_fp32"),
+                             "kernel": "k_round, summary only (assignment lookup, draw, target, 16-term mixture "
+                                       "log-sum-exp, a summary record per 64 samples); compute-bound: 16 whitened "
+                                       "quadratic forms (~ 130 flop each) per sample",
                              "mean_launch_ms": res["fp32"]["draw_mixture_ms"]}})
     if not args.no_cpu:
         import oracle

@@ -635,14 +635,29 @@ int run_rounds(is_job* j) {
     }
     a.k0 = 0;
     a.k1 = ncomp - 1;
-    tick(j, 0, 0, s);
-    if (const int mrc = mix(a, ncomp)) return mrc;
-    tick(j, 0, 1, s);
     SweepArgs<T> w = sweep_args<T>(j, 0);
     w.lo = 0;
     w.hi = N;
     w.nv = 2;
-    const int nrec = launch_sweep<T>(D, 0, w, s);
+    int nrec = 0;
+    bool fused = false;
+    if (!batch_target) {
+      // one launch: the new-sample blocks write their own summary records
+      SweepArgs<T> wo = w;
+      wo.hi = 0;
+      wo.per_block = kRoundPerBlock;
+      const int nb_new = (int)cdiv(N, 64);
+      tick(j, 0, 0, s);
+      fused = launch_round<T>(D, 0, ncomp, a, wo, nb_new, 0, s);
+      tick(j, 0, 1, s);
+      nrec = nb_new;
+    }
+    if (!fused) {
+      tick(j, 0, 0, s);
+      if (const int mrc = mix(a, ncomp)) return mrc;
+      tick(j, 0, 1, s);
+      nrec = launch_sweep<T>(D, 0, w, s);
+    }
     FinishArgs f = base_finish(j, 0, counter);
     f.flags = kFinishSummary;
     f.summary_count = N;
