@@ -393,6 +393,18 @@ __device__ __forceinline__ void sweep_load(const SweepArgs<T>& a, int64_t i, int
   }
 }
 
+// e^v for v <= 0 (or -inf): FP64 through exp; FP32 as 2^(v log2 e) from the
+// hardware base-2 exponential with log2 e split in two floats (the product's
+// rounding carried as a first-order correction), within a few ulp.
+__device__ __forceinline__ double exp_nonpos(double v) { return exp(v); }
+__device__ __forceinline__ float exp_nonpos(float v) {
+  const float L2E_HI = 1.44269502162933349609375f, L2E_LO = 1.925962989e-08f;
+  const float e_hi = v * L2E_HI;
+  const float e_lo = fmaf(v, L2E_HI, -e_hi) + v * L2E_LO;
+  const float t0 = __builtin_amdgcn_exp2f(e_hi);
+  return fmaf(t0 * 0.6931471805599453f, e_lo, t0);
+}
+
 // Per-wave running state of the summary monoid and the moment tiles.
 template <class T, int NT>
 struct SweepAcc {
@@ -431,7 +443,7 @@ __device__ __forceinline__ void sweep_accumulate(SweepAcc<T, NT>& st, T lw, cons
     for (int p = 0; p < (P > 0 ? P : 1); ++p) st.acc[p] *= f;
     st.mw = wm;
   }
-  const T se = (lw == ninf) ? T(0) : exp(T(0.5) * (lw - st.mw));
+  const T se = (lw == ninf) ? T(0) : exp_nonpos(T(0.5) * (lw - st.mw));
   const double sd = (double)se;
   const double e = sd * sd;
   st.s0 += e;
