@@ -40,6 +40,10 @@ from .api import (  # noqa: F401
     cov,
     BatchTarget,
     DeviceValues,
+    IdentityTransform,
+    IntervalTransform,
+    PositiveTransform,
+    SoftplusTransform,
     current_proposal,
     importance_sample,
     importance_sample_,

@@ -59,6 +59,8 @@ class JobDesc(ctypes.Structure):
         ("mis_group_count", ctypes.c_int32),
         ("mis_group_offsets", I32P),
         ("mis_group_members", I32P),
+        ("transform_kinds", I32P),
+        ("transform_bounds", F64P),
         ("target_batch", BATCH_TARGET_FN),
         ("target_user", VP),
     ]

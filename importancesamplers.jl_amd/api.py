@@ -503,7 +503,55 @@ def context(device=0):
     return _ctx
 
 
-def _job_desc(rng, target, algorithm, precision):
+@dataclass
+class IdentityTransform:
+    """x = z (src/transforms.jl:17)."""
+    code = 0
+    lower: float = -math.inf
+    upper: float = math.inf
+
+
+@dataclass
+class PositiveTransform:
+    """x = exp(z), log|J| = z (src/transforms.jl:30)."""
+    code = 1
+    lower: float = -math.inf
+    upper: float = math.inf
+
+
+@dataclass
+class SoftplusTransform:
+    """x = softplus(z), log|J| = z - softplus(z) (src/transforms.jl:43)."""
+    code = 2
+    lower: float = -math.inf
+    upper: float = math.inf
+
+
+@dataclass
+class IntervalTransform:
+    """x in (lower, upper), one side may be open (src/transforms.jl:45-48,249-270)."""
+    lower: float = -math.inf
+    upper: float = math.inf
+    code = 3
+
+    def __post_init__(self):
+        lo = -math.inf if self.lower is None else float(self.lower)
+        hi = math.inf if self.upper is None else float(self.upper)
+        if not lo < hi or (math.isinf(lo) and math.isinf(hi)):
+            raise ValueError("IntervalTransform needs lower < upper with a finite side")
+        self.lower, self.upper = lo, hi
+
+
+def _transform_arrays(transform, d):
+    ts = list(transform) if isinstance(transform, (list, tuple)) else [transform] * d
+    if len(ts) != d:
+        raise ValueError("one transform per coordinate")
+    kinds = np.asarray([t.code for t in ts], dtype=np.int32)
+    bounds = np.asarray([[t.lower, t.upper] for t in ts], dtype=np.float64).ravel()
+    return kinds, bounds
+
+
+def _job_desc(rng, target, algorithm, precision, transform=None):
     """Builds the C descriptor of one prepared sampler; returns (desc, keep, proposal,
     schedule) where `keep` holds the numpy buffers the descriptor points into."""
     desc = JobDesc()
@@ -565,6 +613,11 @@ def _job_desc(rng, target, algorithm, precision):
     if isinstance(target, BatchTarget):
         desc.target_batch = target._fn
         keep.append(target)
+    if transform is not None:
+        kinds, bounds = _transform_arrays(transform, p.dim)
+        keep.extend([kinds, bounds])
+        desc.transform_kinds = kinds.ctypes.data_as(I32P)
+        desc.transform_bounds = bounds.ctypes.data_as(F64P)
     return desc, keep, p, sched
 
 
@@ -572,13 +625,13 @@ def _job_desc(rng, target, algorithm, precision):
 class PreparedSampler:
     """`_PreparedImportanceSampler` on the MI355X device (one engine job)."""
 
-    def __init__(self, rng, target, algorithm, precision=FP64):
+    def __init__(self, rng, target, algorithm, precision=FP64, transform=None):
         self.algorithm = algorithm
         self.target = target
         self.precision = precision
         lib = _lib.load()
         ctx = context()
-        desc, keep, p, sched = _job_desc(rng, target, algorithm, precision)
+        desc, keep, p, sched = _job_desc(rng, target, algorithm, precision, transform)
         h = VP()
         _check(lib.is_job_create(ctx, ctypes.byref(desc), ctypes.byref(h)))
         self._adopt(h, desc, keep, p, sched)
@@ -765,10 +818,12 @@ class PreparedSampler:
         return a.reshape(T, K)
 
 
-def prepare_sampler(rng, target, algorithm, precision="fp64"):
-    """src/methods/importance_sampling.jl:357-404 (device-prepared)."""
+def prepare_sampler(rng, target, algorithm, precision="fp64", transform=None):
+    """src/methods/importance_sampling.jl:357-404 (device-prepared); transform: one
+    transform for every coordinate or one per coordinate (sampling stays in unconstrained
+    coordinates, the target sees the logical sample, results report it)."""
     prec = {"fp64": FP64, "fp32": FP32, FP64: FP64, FP32: FP32}[precision]
-    return PreparedSampler(rng, target, algorithm, prec)
+    return PreparedSampler(rng, target, algorithm, prec, transform)
 
 
 def importance_sample_(sampler: PreparedSampler):
@@ -777,9 +832,9 @@ def importance_sample_(sampler: PreparedSampler):
     return sampler.collect()
 
 
-def importance_sample(rng, target, algorithm, precision="fp64"):
+def importance_sample(rng, target, algorithm, precision="fp64", transform=None):
     """`importance_sample` (src/methods/importance_sampling.jl:857-874)."""
-    return importance_sample_(prepare_sampler(rng, target, algorithm, precision))
+    return importance_sample_(prepare_sampler(rng, target, algorithm, precision, transform))
 
 
 def current_proposal(sampler: PreparedSampler):

@@ -354,6 +354,7 @@ int is_batch_create(is_ctx* ctx, const is_job_desc* descs, int32_t njobs, int32_
     const is_job_desc& D = descs[j];
     BatchJobHost& H = b->jobs[j];
     if (D.method != IS_AMIS) return fail(IS_EUNSUPPORTED, "job %d: the batch runs AMIS jobs", j);
+    if (D.transform_kinds) return fail(IS_EUNSUPPORTED, "job %d: transforms are not supported in a batch", j);
     if (D.target_kind != IS_TARGET_LINREG && D.target_kind != IS_TARGET_LOGISTIC)
       return fail(IS_EUNSUPPORTED, "job %d: the batch runs regression targets", j);
     if (D.dim < 1 || D.dim > kBatchMaxDim) return fail(IS_EUNSUPPORTED, "job %d: dimension outside [1, %d]", j, kBatchMaxDim);

@@ -290,6 +290,65 @@ __host__ __device__ constexpr int exact_dim(int d) {
   return D <= kExactDims ? D : d;
 }
 
+// ---------------------------------------------------------------- scalar sample transforms
+// Forward map z -> x with log|dx/dz| (src/transforms.jl:160-235, helpers :295-333);
+// false when z, x or the log Jacobian is not finite or x leaves the support.
+// kind: 0 identity, 1 positive, 2 softplus, 3 interval (lo, hi; an infinite side is open).
+template <class T>
+__device__ __forceinline__ T tf_softplus(T z) {
+  return z > T(0) ? z + log1p(exp(-z)) : log1p(exp(z));
+}
+template <class T>
+__device__ __forceinline__ T tf_logistic(T z) {
+  if (z >= T(0)) return T(1) / (T(1) + exp(-z));
+  const T e = exp(z);
+  return e / (T(1) + e);
+}
+template <class T>
+__device__ __forceinline__ T tf_log_positive_difference(T upper, T lower) {
+  if (lower < T(0) && upper > T(0)) {
+    const T a = -lower, b = upper, larger = fmax(a, b), smaller = fmin(a, b);
+    return log(larger) + log1p(smaller / larger);
+  }
+  return log(upper - lower);
+}
+template <class T>
+__device__ __forceinline__ bool transform_forward(int kind, T lo, T hi, T z, T& x, T& lj) {
+  x = z;
+  lj = T(0);
+  if (!isfinite(z)) return false;
+  if (kind == 0) return true;
+  if (kind == 1) {
+    x = exp(z);
+    lj = z;
+    return isfinite(x) && x > T(0);
+  }
+  if (kind == 2) {
+    x = tf_softplus(z);
+    lj = z - x;
+    return isfinite(x) && isfinite(lj) && x > T(0);
+  }
+  const bool hl = isfinite(lo), hh = isfinite(hi);
+  if (hl && !hh) {
+    x = lo + exp(z);
+    lj = z;
+    return isfinite(x) && x > lo;
+  }
+  if (!hl && hh) {
+    x = hi - exp(z);
+    lj = z;
+    return isfinite(x) && x < hi;
+  }
+  if (!hl && !hh) return true;
+  const T p = tf_logistic(z);
+  if (lo < T(0) && hi > T(0))
+    x = (T(1) - p) * lo + p * hi;
+  else
+    x = p <= T(0.5) ? lo + p * (hi - lo) : hi - (T(1) - p) * (hi - lo);
+  lj = tf_log_positive_difference(hi, lo) - tf_softplus(-z) - tf_softplus(z);
+  return isfinite(x) && isfinite(lj) && x > lo && x < hi;
+}
+
 // ---------------------------------------------------------------- proposal slots
 // A slot packs one proposal for the density kernels:
 //   [ mu(d) | inv_diag(d) | L lower triangle incl. diagonal, row-major packed | lognorm | logmass ]

@@ -156,6 +156,9 @@ struct is_job {
   // mixture pass of every round, on the job's stream, outside any graph capture
   is_batch_target_fn tbatch = nullptr;
   void* tuser = nullptr;
+  // per-coordinate transforms: kinds and bounds on the device, logical samples XL
+  bool has_tf = false;
+  DevBuf tf_kinds_d, tf_bounds_d, XL;
   bool graph_dirty = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -454,6 +457,9 @@ RoundArgs<T> base_args(is_job* j, int t) {
   a.tkind = j->tkind;
   a.tslot = j->tslotT.as<T>();
   a.X = j->X.as<T>();
+  a.tf_kinds = j->has_tf ? j->tf_kinds_d.as<int32_t>() : nullptr;
+  a.tf_bounds = j->has_tf ? j->tf_bounds_d.as<double>() : nullptr;
+  a.XL = j->has_tf ? j->XL.as<T>() : nullptr;
   a.cap = j->total;
   a.logtargets = j->logtargets.as<T>();
   a.lognums = j->lognums.as<T>();
@@ -1001,6 +1007,7 @@ int alloc_job_buffers(is_job* j) {
   const int d = j->d;
   int rc;
   if ((rc = j->X.alloc(cap * d * sizeof(T)))) return rc;
+  if (j->has_tf && (rc = j->XL.alloc(cap * d * sizeof(T)))) return rc;
   if ((rc = j->logtargets.alloc(cap * sizeof(T)))) return rc;
   if ((rc = j->lognums.alloc(cap * sizeof(T)))) return rc;
   if ((rc = j->logweights.alloc(cap * sizeof(T)))) return rc;
@@ -1130,6 +1137,25 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   j->tkind = desc->target_kind;
   j->tbatch = desc->target_batch;
   j->tuser = desc->target_user;
+  if (desc->transform_kinds) {
+    if (desc->target_kind != IS_TARGET_STANDARD && desc->target_kind != IS_TARGET_GAUSSIAN)
+      return fail(IS_EUNSUPPORTED, "transforms are supported with the standard and Gaussian targets");
+    if (!desc->transform_bounds) return fail(IS_EINVAL, "transforms need bounds (2 per coordinate)");
+    const int dd = desc->dim;
+    for (int c = 0; c < dd; ++c) {
+      const int k = desc->transform_kinds[c];
+      const double lo = desc->transform_bounds[2 * c], hi = desc->transform_bounds[2 * c + 1];
+      if (k < IS_TRANSFORM_IDENTITY || k > IS_TRANSFORM_INTERVAL) return fail(IS_EINVAL, "unknown transform kind %d", k);
+      if (k == IS_TRANSFORM_INTERVAL && (lo != lo || hi != hi || !(lo < hi) || (std::isinf(lo) && std::isinf(hi))))
+        return fail(IS_EINVAL, "interval transform %d needs lower < upper with a finite side", c);
+    }
+    int rc;
+    if ((rc = j->tf_kinds_d.alloc((size_t)dd * sizeof(int32_t))) || (rc = j->tf_bounds_d.alloc((size_t)2 * dd * sizeof(double))))
+      return rc;
+    HIP_TRY(locked_memcpy(j->tf_kinds_d.p, desc->transform_kinds, (size_t)dd * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(locked_memcpy(j->tf_bounds_d.p, desc->transform_bounds, (size_t)2 * dd * sizeof(double), hipMemcpyHostToDevice));
+    j->has_tf = true;
+  }
   j->precision = desc->precision;
   j->key = RngKey{desc->seed, desc->stream};
   j->stride = slot_stride(d);
@@ -1342,6 +1368,7 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
   bytes += 2 * pcap * (1 + nv_mom) * 8 + 2 * (1 + nv_mom) * 8; // partial records, reduced records
   bytes += (int64_t)T * (5 * 8 + 3 * 8 + 2 * 4 + 3 * 8) + 4 * 64;
   if (desc->target_kind == IS_TARGET_CALLBACK) bytes += maxN * ts;
+  if (desc->transform_kinds) bytes += total * d * ts + (int64_t)d * 20;
   if (desc->target_kind == IS_TARGET_LOGISTIC || desc->target_kind == IS_TARGET_LINREG) {
     const int64_t n = desc->target_params && desc->target_params_len > 0 ? (int64_t)desc->target_params[0] : 0;
     bytes += n * (d + 1) * ts + maxN * ts;
@@ -1623,10 +1650,10 @@ int is_job_get_samples(is_job* job, double* out) {
   if (rc) return rc;
   if (job->precision == IS_FP32)
     hipLaunchKernelGGL((k_soa_to_aos<float>), dim3(cdiv(n * job->d, 256)), dim3(256), 0, job->stream,
-                       job->X.as<float>(), n, n, job->d, tmp.as<double>());
+                       (job->has_tf ? job->XL : job->X).as<float>(), n, n, job->d, tmp.as<double>());
   else
     hipLaunchKernelGGL((k_soa_to_aos<double>), dim3(cdiv(n * job->d, 256)), dim3(256), 0, job->stream,
-                       job->X.as<double>(), n, n, job->d, tmp.as<double>());
+                       (job->has_tf ? job->XL : job->X).as<double>(), n, n, job->d, tmp.as<double>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost, job->stream));
   HIP_TRY(hipStreamSynchronize(job->stream));
@@ -1751,7 +1778,7 @@ int is_job_get_ancestors(is_job* job, int64_t* out) {
 int is_job_device_results(is_job* job, const void** samples, int64_t* coord_stride, const void** logweights,
                           int32_t* is_float) {
   REQUIRE_RESULTS(job);
-  if (samples) *samples = job->X.p;
+  if (samples) *samples = job->has_tf ? job->XL.p : job->X.p;  // logical samples under a transform
   if (coord_stride) *coord_stride = job->total;
   if (logweights) *logweights = job->logweights.p;
   if (is_float) *is_float = job->precision == IS_FP32;
@@ -1812,13 +1839,21 @@ static int job_moments_typed(is_job* job, double* mean, double* var, double* cov
     c0 = job->committed.mu;
   else
     for (int c = 0; c < d; ++c) c0[c] = job->committed_locs[c];
-  HIP_TRY(locked_memcpy(center.p, c0.data(), d * sizeof(double), hipMemcpyHostToDevice));
+  if (job->has_tf) {
+    // logical samples: centre at the first sample's logical coordinates
+    hipLaunchKernelGGL((k_soa_to_aos<T>), dim3(1), dim3(256), 0, job->stream, job->XL.as<T>(), job->total, (int64_t)1,
+                       d, center.as<double>());
+    HIP_TRY(hipMemcpyAsync(c0.data(), center.p, d * sizeof(double), hipMemcpyDeviceToHost, job->stream));
+  } else {
+    HIP_TRY(locked_memcpy(center.p, c0.data(), d * sizeof(double), hipMemcpyHostToDevice));
+  }
   SweepArgs<T> w = sweep_args<T>(job, 0);
   w.lo = 0;
   w.hi = n;
   w.nv = nv;
   w.fail = nullptr;
   w.center = center.as<double>();
+  if (job->has_tf) w.X = job->XL.as<T>();
   w.partials = job->partials2.as<double>();
   const int blocks = launch_sweep<T>(job->D, sweep_nt(d, true), w, job->stream);
   FinishArgs f{};

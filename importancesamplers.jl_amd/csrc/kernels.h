@@ -61,6 +61,10 @@ struct RoundArgs {
   int ru_stride;
   int rs, us;         // radial normal / uniform counts per sample
   int record_nonfinite_after_exhaust;
+  // per-coordinate transforms (null: none) and the logical-sample store
+  const int32_t* tf_kinds;
+  const double* tf_bounds;
+  T* XL;
   // target
   int tkind;
   const T* tslot;           // Gaussian target slot

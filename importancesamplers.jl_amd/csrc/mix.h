@@ -210,6 +210,24 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
   if (wave == 0 && ok) {
     if (MODE == kPreTarget) {
       tv = a.pre_logtarget[j];
+    } else if (a.tf_kinds) {
+      // the target at the logical sample x = T(z) plus the forward log Jacobian; a
+      // transform failure makes the value NaN (a target-phase failure)
+      T xl[D];
+      T lj = T(0);
+      bool tok = true;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        xl[c] = T(0);
+        if (c < d) {
+          T xc, ljc;
+          tok &= transform_forward<T>(a.tf_kinds[c], (T)a.tf_bounds[2 * c], (T)a.tf_bounds[2 * c + 1], x[c], xc, ljc);
+          xl[c] = xc;
+          lj += ljc;
+          a.XL[(int64_t)c * a.cap + gi] = xc;
+        }
+      }
+      tv = tok ? inline_target<T, D>(a.tkind, d, tslot, xl, y) + lj : T(__builtin_nan(""));
     } else {
       tv = inline_target<T, D>(a.tkind, d, tslot, x, y);
     }

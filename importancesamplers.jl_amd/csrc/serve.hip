@@ -37,7 +37,8 @@ struct OwnedDesc {
   is_job_desc d{};
   std::vector<int64_t> round_size;
   std::vector<double> location, scale, bank_locations, bank_scales, bank_masses, target_params;
-  std::vector<int32_t> group_offsets, group_members;
+  std::vector<int32_t> group_offsets, group_members, transform_kinds;
+  std::vector<double> transform_bounds;
 
   int copy_from(const is_job_desc& s) {
     d = s;
@@ -75,6 +76,13 @@ struct OwnedDesc {
     d.bank_scales = bank_scales.empty() ? nullptr : bank_scales.data();
     d.bank_masses = bank_masses.empty() ? nullptr : bank_masses.data();
     d.target_params = target_params.empty() ? nullptr : target_params.data();
+    if (s.transform_kinds) {
+      if (!s.transform_bounds) return fail(IS_EINVAL, "transforms need bounds (2 per coordinate)");
+      transform_kinds.assign(s.transform_kinds, s.transform_kinds + dim);
+      transform_bounds.assign(s.transform_bounds, s.transform_bounds + 2 * dim);
+      d.transform_kinds = transform_kinds.data();
+      d.transform_bounds = transform_bounds.data();
+    }
     d.mis_group_offsets = group_offsets.empty() ? nullptr : group_offsets.data();
     d.mis_group_members = group_members.empty() ? nullptr : group_members.data();
     return IS_OK;

@@ -74,6 +74,12 @@ enum is_target_kind {
  * graph replay). */
 typedef int (*is_batch_target_fn)(void* user, const void* samples, int64_t coord_stride, int32_t dim,
                                   int64_t count, void* values, int32_t is_float, void* stream);
+enum is_transform_kind {   /* scalar sample transforms (src/transforms.jl:17-235)             */
+  IS_TRANSFORM_IDENTITY = 0,
+  IS_TRANSFORM_POSITIVE = 1, /* x = exp(z), log|J| = z                                         */
+  IS_TRANSFORM_SOFTPLUS = 2, /* x = softplus(z), log|J| = z - softplus(z)                      */
+  IS_TRANSFORM_INTERVAL = 3  /* bounds (l, u): l + exp(z), u - exp(z), or l + (u - l) logistic(z) */
+};
 enum is_phase {            /* AMISRoundError / NPMCRoundError / DMPMCRoundError phases       */
   IS_PHASE_NONE = 0,
   IS_PHASE_SAMPLING = 1,   /* :sampling / :proposal_draw                                     */
@@ -122,6 +128,13 @@ typedef struct is_job_desc {
   int32_t mis_group_count; /* partial mixture: number of groups G */
   const int32_t* mis_group_offsets; /* G + 1 offsets into the members, starting at 0 */
   const int32_t* mis_group_members; /* configured 1-based proposal ids, in group order, each id once */
+  /* per-coordinate transforms (prepare_sampler's transform=, src/methods/importance_sampling.jl:308-313):
+   * NULL, or d enum is_transform_kind values with 2d bounds (lower, upper, -inf / +inf for an
+   * open side).  Sampling, fits and mixtures stay in unconstrained coordinates z, the target
+   * sees x = T(z), the weights gain the forward log Jacobian and results report x.  Built-in
+   * standard and Gaussian targets only. */
+  const int32_t* transform_kinds;
+  const double* transform_bounds;
   /* caller's batch target (target_kind IS_TARGET_CALLBACK) */
   is_batch_target_fn target_batch;
   void* target_user;       /* passed back as the callback's first argument */

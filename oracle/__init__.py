@@ -40,6 +40,10 @@ Parity pins (what checks each oracle function; tests/test_oracle_golden.py):
 * ``logweight_summary`` / ``lognormalizer`` / ``moments`` --
   ``test/dm_pmc_kernel.jl:274-296`` and ``test/results.jl:9-75,357-388``.
 * ``cholesky`` -- numpy.linalg.cholesky on random SPD matrices.
+* ``transform_forward`` -- the scalar transforms of ``src/transforms.jl:160-235``
+  (numpy restatement): log Jacobian against a central finite difference of the
+  forward map (the check of ``test/transforms.jl:3-8``), supports, and the
+  closed forms of the positive and symmetric interval maps.
 * ``functional_moments`` -- the known answers of ``test/results.jl:9-38``
   (mean/var of abs2 and of x1 + 2 x2) and the variance range case of
   ``test/cuda_statistics.jl:27-38`` (exact rationals), plus the zero-weight
@@ -280,6 +284,36 @@ def moments(samples, lw):
     if lib().or_moments(_p(x), _p(lw), n, d, _p(mean), _p(var), _p(cov)):
         raise ZeroDivisionError("all weights are zero")
     return mean, var, cov.reshape(d, d).T.copy()
+
+
+def _softplus(z):
+    return np.where(z > 0, z + np.log1p(np.exp(-np.abs(z))), np.log1p(np.exp(np.minimum(z, 0))))
+
+
+def transform_forward(kind, lo, hi, z):
+    """src/transforms.jl:160-235 with the helpers of :295-333, elementwise over z:
+    returns (x, log|dx/dz|).  kind 0 identity, 1 positive, 2 softplus, 3 interval."""
+    z = _f64(z)
+    with np.errstate(over="ignore", invalid="ignore"):
+        if kind == 0:
+            return z.copy(), np.zeros_like(z)
+        if kind == 1:
+            return np.exp(z), z.copy()
+        if kind == 2:
+            x = _softplus(z)
+            return x, z - x
+        if math.isfinite(lo) and not math.isfinite(hi):
+            return lo + np.exp(z), z.copy()
+        if not math.isfinite(lo) and math.isfinite(hi):
+            return hi - np.exp(z), z.copy()
+        p = np.where(z >= 0, 1.0 / (1.0 + np.exp(-np.abs(z))), np.exp(-np.abs(z)) / (1.0 + np.exp(-np.abs(z))))
+        if lo < 0 < hi:
+            x = (1.0 - p) * lo + p * hi
+            span = math.log(max(-lo, hi)) + math.log1p(min(-lo, hi) / max(-lo, hi))
+        else:
+            x = np.where(p <= 0.5, lo + p * (hi - lo), hi - (1.0 - p) * (hi - lo))
+            span = math.log(hi - lo)
+        return x, span - _softplus(-z) - _softplus(z)
 
 
 def functional_moments(lw, values):
