@@ -2,8 +2,9 @@
 
 The host engine (csrc/engine.hip), the batched AMIS host side (csrc/batch.hip),
 the serving loop (csrc/serve.hip),
-one object per dimension instantiation (csrc/dim.hip with -DIS_DIM=D, D in the
-IS_DIMS list of csrc/launch.h) and one per batched-kernel shape (csrc/batch_k.hip
+four objects per dimension instantiation (csrc/dim.hip with -DIS_DIM=D, D in the
+IS_DIMS list of csrc/launch.h, and -DIS_DIM_PART=0..3: new-sample kernels and the
+sweep / round / target kernels, each in FP32 and FP64) and one per batched-kernel shape (csrc/batch_k.hip
 with -DIS_BATCH_FP and -DIS_BATCH_W) are compiled in parallel, then linked into
 one shared library."""
 
@@ -53,9 +54,8 @@ def _run(cmd):
     return res.returncode, err
 
 
-def build(verbose=False, jobs=None):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    os.makedirs(OBJDIR, exist_ok=True)
+def units():
+    """(source, object, extra flags) of every compilation unit of the library."""
     units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), []),
              (os.path.join(CSRC, "batch.hip"), os.path.join(OBJDIR, "batch.o"), []),
              (os.path.join(CSRC, "serve.hip"), os.path.join(OBJDIR, "serve.o"), [])]
@@ -64,12 +64,23 @@ def build(verbose=False, jobs=None):
             units.append((os.path.join(CSRC, "batch_k.hip"), os.path.join(OBJDIR, f"batch_k{fp}_w{w}.o"),
                           [f"-DIS_BATCH_FP={fp}", f"-DIS_BATCH_W={w}"]))
     for d in dims():
-        units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}.o"), [f"-DIS_DIM={d}"]))
-    jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)))
+        for part in range(4):
+            units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}_p{part}.o"),
+                          [f"-DIS_DIM={d}", f"-DIS_DIM_PART={part}"]))
+    return units
+
+
+def build(verbose=False, jobs=None, skip=None):
+    """Compiles every unit (or those `skip(unit)` does not exclude) and links the library."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJDIR, exist_ok=True)
+    all_units = units()
+    todo = [u for u in all_units if not (skip and skip(u))]
+    jobs = jobs or min(max(1, len(todo)), max(1, (os.cpu_count() or 4)))
     # the wide dimension units take the longest (their density loops unroll over up to
     # 64 coordinates): start them first so the pool stays full to the end
-    order = sorted(range(len(units)), key=lambda i: -_unit_cost(units[i][2]))
-    cmds = [[hipcc, *FLAGS, *units[i][2], "-c", units[i][0], "-o", units[i][1]] for i in order]
+    order = sorted(range(len(todo)), key=lambda i: -_unit_cost(todo[i][2]))
+    cmds = [[hipcc, *FLAGS, *todo[i][2], "-c", todo[i][0], "-o", todo[i][1]] for i in order]
     with ThreadPoolExecutor(max_workers=jobs) as pool:
         results = list(pool.map(_run, cmds))
     failed = [(cmd, err) for cmd, (rc, err) in zip(cmds, results) if rc != 0]
@@ -81,7 +92,7 @@ def build(verbose=False, jobs=None):
         for (rc, err) in results:
             if err.strip():
                 sys.stderr.write(err + "\n")
-    rc, err = _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *[o for _, o, _ in units]])
+    rc, err = _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *[o for _, o, _ in all_units]])
     if rc != 0:
         sys.stderr.write(err + "\n")
         raise RuntimeError("link failed for libissampler.so")

@@ -1,11 +1,29 @@
-// One dimension instantiation of the engine's kernels; compiled once per
-// IS_DIM value listed in IS_DIMS (launch.h).
+// One part of one dimension instantiation of the engine's kernels; compiled once
+// per IS_DIM value listed in IS_DIMS (launch.h) and per IS_DIM_PART:
+//   0: new-sample kernels, FP32     1: new-sample kernels, FP64
+//   2: sweep, fused round and target kernels, FP32
+//   3: sweep, fused round and target kernels, FP64
+// so that the wide instantiations compile in parallel.
 #include "dim_impl.h"
 
 #ifndef IS_DIM
 #error "compile dim.hip with -DIS_DIM=<dimension instantiation>"
 #endif
+#ifndef IS_DIM_PART
+#error "compile dim.hip with -DIS_DIM_PART=<0..3>"
+#endif
 
 namespace issampler {
-template struct DimKernels<IS_DIM>;
-}
+#if IS_DIM_PART == 0 || IS_DIM_PART == 2
+using T_ = float;
+#else
+using T_ = double;
+#endif
+#if IS_DIM_PART < 2
+template void DimKernels<IS_DIM>::new_samples(int, const RoundArgs<T_>&, int, int, int, hipStream_t);
+#else
+template void DimKernels<IS_DIM>::sweep(int, const SweepArgs<T_>&, int, int, size_t, hipStream_t);
+template bool DimKernels<IS_DIM>::round(int, const RoundArgs<T_>&, const SweepArgs<T_>&, int, int, int, hipStream_t);
+template void DimKernels<IS_DIM>::target(const TargetArgs<T_>&, int64_t, hipStream_t);
+#endif
+}  // namespace issampler
