@@ -132,6 +132,8 @@ SIGNATURES = {
     "is_job_functional_moments": (ctypes.c_int, [VP, VP, ctypes.c_int32, ctypes.c_int32, F64P, F64P]),
     "is_functional_moments_device": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32, ctypes.c_int64,
                                                     F64P, F64P, VP]),
+    "is_functional_moments_device_noalloc": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32,
+                                                            ctypes.c_int64, F64P, F64P, VP]),
     "is_job_resample": (ctypes.c_int, [VP, F64P, ctypes.c_int64, ctypes.c_int32, I64P]),
     "is_cdf_device": (ctypes.c_int, [VP, VP, ctypes.c_int64, VP, VP]),
     "is_select_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int32, VP, VP]),

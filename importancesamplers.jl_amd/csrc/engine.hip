@@ -1080,6 +1080,39 @@ int alloc_job_buffers(is_job* j) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free scratch
+// Device memory of is_functional_moments_device_noalloc: the column-major partial
+// records (4 columns of cdiv(n, kFuncPerBlock) doubles) followed by the two reduced
+// records (8 doubles).  Each GPU holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: the per-device
+// address comes from hipGetSymbolAddress at is_ctx_create and is the only one used.
+constexpr int64_t kScratchDoubles = 131072;  // 1 MiB
+constexpr int64_t kNoAllocMaxSamples = (kScratchDoubles - 8) / 4 * kFuncPerBlock;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: looks up this device's copy of `scratch` once.
+int bind_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return IS_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    return fail(IS_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return IS_OK;
+}
+
+double* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 // ===================================================================== C ABI
 extern "C" {
 
@@ -1096,6 +1129,8 @@ int is_ctx_create(int device, is_ctx** out) {
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(IS_EUNSUPPORTED, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  if (device >= kMaxDevices) return fail(IS_EUNSUPPORTED, "device %d: at most %d devices", device, kMaxDevices);
+  if (int rc = bind_scratch(device)) return rc;
   is_ctx* c = new is_ctx();
   c->device = device;
   *out = c;
@@ -1993,28 +2028,27 @@ int is_select_device(is_ctx* ctx, const double* cdf, const double* lw, int64_t n
 
 // Statistics.mean(f, r) / var(f, r) on device arrays: pass 0 records and their
 // merge give the mean; pass 1 (centre read from the pass-0 merge on the device)
-// gives the variance.  One host wait, for both results.
+// gives the variance.  One host wait, for both results.  `partials` holds
+// 4 * cdiv(n, kFuncPerBlock) doubles, `red` 8; both are written before they are
+// read, so neither needs clearing.
 extern "C++" {
 template <class T, class V>
-static int functional_moments(const void* lw, const void* val, int64_t n, double* mean, double* var,
-                              hipStream_t s) {
+static int functional_moments_in(const void* lw, const void* val, int64_t n, double* mean, double* var,
+                                 hipStream_t s, double* partials, double* red) {
   const int64_t blocks = cdiv(n, kFuncPerBlock);
   const int64_t pcap = blocks;
-  DevBuf partials, red;
-  int rc;
-  if ((rc = partials.alloc((size_t)pcap * 4 * sizeof(double))) || (rc = red.alloc(8 * sizeof(double)))) return rc;
-  double* r0 = red.as<double>();
+  double* r0 = red;
   double* r1 = r0 + 4;
   FinishArgs f{};
   hipLaunchKernelGGL((k_functional<T, V>), dim3((unsigned)blocks), dim3(kBlock), 0, s, (const T*)lw, (const V*)val, n,
-                     (const double*)nullptr, partials.as<double>(), pcap);
-  hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(3), dim3(kBlock), 0, s, partials.as<double>(), blocks, pcap, 3,
-                     r0, f);
+                     (const double*)nullptr, partials, pcap);
+  hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(3), dim3(kBlock), 0, s, (const double*)partials, blocks, pcap,
+                     3, r0, f);
   if (var) {
     hipLaunchKernelGGL((k_functional<T, V>), dim3((unsigned)blocks), dim3(kBlock), 0, s, (const T*)lw, (const V*)val,
-                       n, (const double*)r0, partials.as<double>(), pcap);
-    hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(3), dim3(kBlock), 0, s, partials.as<double>(), blocks, pcap,
-                       3, r1, f);
+                       n, (const double*)r0, partials, pcap);
+    hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(3), dim3(kBlock), 0, s, (const double*)partials, blocks,
+                       pcap, 3, r1, f);
   }
   HIP_TRY(hipGetLastError());
   double h[8];
@@ -2026,13 +2060,24 @@ static int functional_moments(const void* lw, const void* val, int64_t n, double
   return IS_OK;
 }
 
+static int functional_dispatch_in(const void* lw, int lw_fp64, const void* val, int val_fp64, int64_t n,
+                                  double* mean, double* var, hipStream_t s, double* partials, double* red) {
+  if (lw_fp64)
+    return val_fp64 ? functional_moments_in<double, double>(lw, val, n, mean, var, s, partials, red)
+                    : functional_moments_in<double, float>(lw, val, n, mean, var, s, partials, red);
+  return val_fp64 ? functional_moments_in<float, double>(lw, val, n, mean, var, s, partials, red)
+                  : functional_moments_in<float, float>(lw, val, n, mean, var, s, partials, red);
+}
+
 static int functional_dispatch(const void* lw, int lw_fp64, const void* val, int val_fp64, int64_t n, double* mean,
                                double* var, hipStream_t s) {
-  if (lw_fp64)
-    return val_fp64 ? functional_moments<double, double>(lw, val, n, mean, var, s)
-                    : functional_moments<double, float>(lw, val, n, mean, var, s);
-  return val_fp64 ? functional_moments<float, double>(lw, val, n, mean, var, s)
-                  : functional_moments<float, float>(lw, val, n, mean, var, s);
+  DevBuf partials, red;
+  int rc;
+  if ((rc = partials.alloc((size_t)cdiv(n, kFuncPerBlock) * 4 * sizeof(double))) ||
+      (rc = red.alloc(8 * sizeof(double))))
+    return rc;
+  return functional_dispatch_in(lw, lw_fp64, val, val_fp64, n, mean, var, s, partials.as<double>(),
+                                red.as<double>());
 }
 }  // extern "C++"
 
@@ -2043,6 +2088,21 @@ int is_functional_moments_device(is_ctx* ctx, const void* logweights_device, int
   HIP_TRY(hipSetDevice(ctx->device));
   return functional_dispatch(logweights_device, logweights_fp64, values_device, values_fp64, n, mean, var,
                              (hipStream_t)stream);
+}
+
+int is_functional_moments_device_noalloc(is_ctx* ctx, const void* logweights_device, int32_t logweights_fp64,
+                                         const void* values_device, int32_t values_fp64, int64_t n, double* mean,
+                                         double* var, void* stream) {
+  if (!ctx || !logweights_device || !values_device || n < 1) return fail(IS_EINVAL, "bad arguments");
+  if (n > kNoAllocMaxSamples)
+    return fail(IS_EINVAL, "n = %lld exceeds the allocation-free limit of %lld samples", (long long)n,
+                (long long)kNoAllocMaxSamples);
+  double* ptr = scratch_ptr(ctx->device);
+  if (!ptr) return fail(IS_ESTATE, "device %d has no scratch binding (create the context first)", ctx->device);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t records = 4 * cdiv(n, kFuncPerBlock);
+  return functional_dispatch_in(logweights_device, logweights_fp64, values_device, values_fp64, n, mean, var,
+                                (hipStream_t)stream, ptr, ptr + records);
 }
 
 int is_job_functional_moments(is_job* job, const void* values, int32_t values_fp64, int32_t values_on_device,

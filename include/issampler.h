@@ -266,6 +266,16 @@ int is_select_device(is_ctx* ctx, const double* cdf_device, const double* logwei
 int is_functional_moments_device(is_ctx* ctx, const void* logweights_device, int32_t logweights_fp64,
                                  const void* values_device, int32_t values_fp64, int64_t n, double* mean,
                                  double* var, void* stream);
+/* Allocation-free form of is_functional_moments_device, with the same arguments and results.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * calls no allocating routine: its partial records live in a fixed 1 MiB device array per GPU
+ * (bound at is_ctx_create), which limits n to 67,104,768 samples (IS_EINVAL beyond).  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (different streams or
+ * host threads) race; serialise them. */
+int is_functional_moments_device_noalloc(is_ctx* ctx, const void* logweights_device, int32_t logweights_fp64,
+                                         const void* values_device, int32_t values_fp64, int64_t n,
+                                         double* mean, double* var, void* stream);
 /* Host-buffer forms of the two primitives above (copy in, run, copy out); ancestors 1-based. */
 int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf);
 int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t n, const double* uniforms,
