@@ -48,6 +48,41 @@ def test_descriptor_layout_matches_header(pkg):
     assert names == [f[0] for f in pkg._lib.JobDesc._fields_]
 
 
+def _header_desc_fields():
+    """(name, C type) of is_job_desc, in order, from include/issampler.h."""
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct is_job_desc {") + len("typedef struct is_job_desc {"):
+                text.index("} is_job_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    out = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        m = re.match(r"(.*?)\s*(\w+)$", decl)
+        out.append((m.group(2), m.group(1).replace(" *", "*").replace("const ", "")))
+    return out
+
+
+# Julia field type for each C type of the descriptor
+_JULIA_OF_C = {"int32_t": "Int32", "int64_t": "Int64", "uint64_t": "UInt64", "double": "Float64",
+               "int64_t*": "Ptr{Int64}", "double*": "Ptr{Float64}", "int32_t*": "Ptr{Int32}",
+               "is_batch_target_fn": "Ptr{Cvoid}", "void*": "Ptr{Cvoid}"}
+
+
+def test_integration_julia_descriptor_matches_header():
+    """The Julia JobDesc that INTEGRATION.md documents mirrors every is_job_desc field,
+    in order and with the C type's Julia counterpart (a short struct would make
+    is_job_create read past the Julia object)."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = text[text.index("struct JobDesc"):]
+    body = body[body.index("\n") + 1: body.index("\nend")]
+    julia = [(m.group(1), m.group(2)) for m in re.finditer(r"^\s*(\w+)::([\w{}]+)", body, re.M)]
+    header = _header_desc_fields()
+    assert [n for n, _ in julia] == [n for n, _ in header]
+    assert [t for _, t in julia] == [_JULIA_OF_C[c] for _, c in header]
+
+
 def test_proposal_constructors_and_constants(pkg, oracle_mod):
     import math
 
