@@ -35,6 +35,7 @@ from .api import (  # noqa: F401
     StandardMIS,
     StratifiedMixture,
     SystematicResampling,
+    UnweightedSamples,
     WeightedSamples,
     context,
     cov,

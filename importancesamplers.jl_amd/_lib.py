@@ -135,6 +135,18 @@ SIGNATURES = {
     "is_functional_moments_device_noalloc": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32,
                                                             ctypes.c_int64, F64P, F64P, VP]),
     "is_job_resample": (ctypes.c_int, [VP, F64P, ctypes.c_int64, ctypes.c_int32, I64P]),
+    "is_job_result": (ctypes.c_int, [VP, ctypes.POINTER(VP)]),
+    "is_result_destroy": (ctypes.c_int, [VP]),
+    "is_result_count": (ctypes.c_int64, [VP]),
+    "is_result_is_weighted": (ctypes.c_int, [VP]),
+    "is_result_get_samples": (ctypes.c_int, [VP, F64P]),
+    "is_result_get_logweights": (ctypes.c_int, [VP, F64P]),
+    "is_result_device_arrays": (ctypes.c_int, [VP, ctypes.POINTER(VP), I64P, ctypes.POINTER(VP), I32P]),
+    "is_result_logweight_offset": (ctypes.c_int, [VP, F64P]),
+    "is_result_lognormalizer": (ctypes.c_int, [VP, F64P]),
+    "is_result_moments": (ctypes.c_int, [VP, F64P, F64P, F64P]),
+    "is_result_functional_moments": (ctypes.c_int, [VP, VP, ctypes.c_int32, ctypes.c_int32, F64P, F64P]),
+    "is_result_resample": (ctypes.c_int, [VP, F64P, ctypes.c_int64, ctypes.c_int32, I64P, ctypes.POINTER(VP)]),
     "is_cdf_device": (ctypes.c_int, [VP, VP, ctypes.c_int64, VP, VP]),
     "is_select_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int32, VP, VP]),
     "is_cdf": (ctypes.c_int, [VP, F64P, ctypes.c_int64, F64P]),

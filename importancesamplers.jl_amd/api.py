@@ -804,7 +804,7 @@ class PreparedSampler:
                 diag["proposal_evaluations"] = n * len(self.active_ids)
         if self.scalar:
             samples = samples[:, 0].copy()
-        return WeightedSamples(samples, logw, prov, diag, _sampler=self)
+        return WeightedSamples(samples, logw, prov, diag, _sampler=self, _result=_ResultHandle.of_job(self))
 
     def history(self):
         T, d = len(self.schedule), self.dim
@@ -862,6 +862,32 @@ def current_proposal(sampler: PreparedSampler):
 
 
 # ---------------------------------------------------------------- results (src/results.jl, src/statistics.jl)
+class _ResultHandle:
+    """An engine result (is_result): the run's samples and log-weights copied on the
+    device, which later runs of the sampler leave unchanged
+    (src/methods/importance_sampling.jl:904-906); destroyed with its owner."""
+
+    def __init__(self, handle, dim, scalar):
+        self.h = handle
+        self.dim = dim
+        self.scalar = scalar
+        self._lib = _lib.load()
+
+    @classmethod
+    def of_job(cls, sampler):
+        h = VP()
+        _check(sampler._lib.is_job_result(sampler._job, ctypes.byref(h)))
+        return cls(h, sampler.dim, sampler.scalar)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._lib.is_result_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 @dataclass
 class WeightedSamples:
     samples: np.ndarray
@@ -869,29 +895,65 @@ class WeightedSamples:
     provenance: dict = field(default_factory=dict)
     diagnostics: dict = field(default_factory=dict)
     _sampler: object = None
+    _result: object = None
 
     def __len__(self):
         return self.logweights.size
 
 
+class UnweightedSamples:
+    """`UnweightedSamples` (src/resampling.jl:24-48) of a device resample: the gathered
+    samples stay on the device (`device_arrays()`); `samples` copies them to the host."""
+
+    def __init__(self, handle: _ResultHandle, ancestors=None):
+        self._result = handle
+        self.ancestors = ancestors
+
+    def __len__(self):
+        return int(self._result._lib.is_result_count(self._result.h))
+
+    @property
+    def samples(self):
+        r = self._result
+        n, d = len(self), r.dim
+        out = np.zeros(n * d)
+        _check(r._lib.is_result_get_samples(r.h, _p(out)))
+        out = out.reshape(n, d)
+        return out[:, 0].copy() if r.scalar else out
+
+    def device_arrays(self):
+        """(samples device pointer, coordinate stride, is_float)."""
+        r = self._result
+        sp, stride, isf = VP(), ctypes.c_int64(), ctypes.c_int32()
+        _check(r._lib.is_result_device_arrays(r.h, ctypes.byref(sp), ctypes.byref(stride), None, ctypes.byref(isf)))
+        return sp.value, int(stride.value), bool(isf.value)
+
+
+def _handle(result):
+    h = getattr(result, "_result", None)
+    if h is None or not h.h:
+        raise ValueError("result is detached from its device storage")
+    return h
+
+
 def lognormalizer(result: WeightedSamples):
-    """LSE(logw) - log N (src/results.jl:560-567), reduced on the device when the
-    result still owns its device buffers."""
-    s = result._sampler
-    if s is not None and s._job is not None:
-        out = np.zeros(3)
-        _check(s._lib.is_job_lognormalizer(s._job, _p(out)))
-        return float(out[0])
-    if getattr(result, "_lognormalizer", None) is not None:
+    """LSE(logw) - log N (src/results.jl:560-567), reduced on the device over the
+    result's own device copy."""
+    if isinstance(result, UnweightedSamples):
+        raise ValueError("unweighted samples carry no weights")
+    if getattr(result, "_result", None) is None and getattr(result, "_lognormalizer", None) is not None:
         return result._lognormalizer  # batch jobs: the device's final-round reduction
-    raise ValueError("result is detached from its device sampler")
+    h = _handle(result)
+    out = np.zeros(3)
+    _check(h._lib.is_result_lognormalizer(h.h, _p(out)))
+    return float(out[0])
 
 
 def _device_moments(result):
-    s = result._sampler
-    d = s.dim
+    h = _handle(result)
+    d = h.dim
     mean, var, cov = np.zeros(d), np.zeros(d), np.zeros(d * d)
-    rc = s._lib.is_job_moments(s._job, _p(mean), _p(var), _p(cov))
+    rc = h._lib.is_result_moments(h.h, _p(mean), _p(var), _p(cov))
     if rc == _lib.IS_EFAILED:
         raise AllZeroWeightsError()
     _check(rc)
@@ -912,20 +974,18 @@ def _functional_moments(f, result, want_var):
     device-resident log-weights, reduced in FP64 on the device.  f is a DeviceValues, or
     a callable mapping the samples (n,) or (n, d) to one number per sample, evaluated on
     the result's host samples and uploaded."""
-    s = result._sampler
-    if s is None or s._job is None:
-        raise ValueError("result is detached from its device sampler")
+    h = _handle(result)
     m, v = np.zeros(1), np.zeros(1)
     if isinstance(f, DeviceValues):
-        rc = s._lib.is_job_functional_moments(s._job, VP(int(f.ptr)), int(bool(f.fp64)), 1, _p(m),
-                                              _p(v) if want_var else None)
+        rc = h._lib.is_result_functional_moments(h.h, VP(int(f.ptr)), int(bool(f.fp64)), 1, _p(m),
+                                                 _p(v) if want_var else None)
     else:
         vals = np.asarray(f(result.samples), dtype=np.float64)
         if vals.size != len(result) or (vals.ndim > 1 and vals.shape[0] != len(result)):
             raise ValueError("functionals must return one number per sample")
         vals = np.ascontiguousarray(vals.reshape(-1))
-        rc = s._lib.is_job_functional_moments(s._job, vals.ctypes.data_as(VP), 1, 0, _p(m),
-                                              _p(v) if want_var else None)
+        rc = h._lib.is_result_functional_moments(h.h, vals.ctypes.data_as(VP), 1, 0, _p(m),
+                                                 _p(v) if want_var else None)
     if rc == _lib.IS_EFAILED:
         raise AllZeroWeightsError()
     _check(rc)
@@ -964,16 +1024,21 @@ def cov(result: WeightedSamples, corrected=False):
 
 
 def resample(uniforms, result: WeightedSamples, count=None, method=None):
-    """src/resampling.jl:245-283: multinomial (count uniforms) or systematic (one uniform)."""
+    """resample (src/resampling.jl:245-283): multinomial (count uniforms) or systematic
+    (one uniform) ancestors of the normalised weights, the samples gathered on the
+    device into an UnweightedSamples (its `ancestors` are the 1-based indices)."""
     method = method or MultinomialResampling()
-    s = result._sampler
+    h = _handle(result)
     count = len(result) if count is None else int(count)
     if count <= 0:
         raise ValueError("resampling count must be positive")
     u = _f64(np.atleast_1d(uniforms))
+    if u.size < (1 if isinstance(method, SystematicResampling) else count):
+        raise ValueError("one uniform per draw (one for systematic resampling)")
     anc = np.zeros(count, dtype=np.int64)
-    rc = s._lib.is_job_resample(s._job, _p(u), count, method.code, _p(anc, I64P))
+    out = VP()
+    rc = h._lib.is_result_resample(h.h, _p(u), count, method.code, _p(anc, I64P), ctypes.byref(out))
     if rc == _lib.IS_EFAILED:
         raise AllZeroWeightsError()
     _check(rc)
-    return anc
+    return UnweightedSamples(_ResultHandle(out, h.dim, h.scalar), anc)

@@ -833,11 +833,7 @@ int run_rounds(is_job* j) {
       reduce(j->partials, nrec, 2, j->reduced, f);
       // global resampling (src/dm_pmc_execution.jl:267-295)
       const T* lw = j->logweights.as<T>() + first;
-      const int64_t ntiles = cdiv(N, (int64_t)kCdfThreads * kCdfPer);
-      hipLaunchKernelGGL((k_cdf_tiles<T>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, N, summ,
-                         j->tile_total.as<double>());
-      hipLaunchKernelGGL((k_cdf_write<T>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, N, summ,
-                         j->tile_total.as<double>(), ntiles, j->cdf.as<double>(), j->cdf_status.as<int>() + t);
+      launch_cdf<T>(lw, N, summ, j->tile_total.as<double>(), j->cdf.as<double>(), j->cdf_status.as<int>() + t, s);
       double* u = j->uniforms.as<double>() + (size_t)t * j->K;
       if (j->prefilled_uniforms[t]) {
         u = j->pre_uniforms.as<double>() + (size_t)t * j->K;
@@ -1071,7 +1067,9 @@ int alloc_job_buffers(is_job* j) {
   }
   if (j->method == IS_DMPMC) {
     if ((rc = j->cdf.alloc((size_t)j->maxN * sizeof(double)))) return rc;
-    if ((rc = j->tile_total.alloc((size_t)(cdiv(j->maxN, kCdfThreads * kCdfPer) + 1) * sizeof(double)))) return rc;
+    // tile totals, then the tile carries and the grand total
+    if ((rc = j->tile_total.alloc((size_t)(2 * cdf_tiles(j->maxN) + 2) * sizeof(double))))
+      return rc;
     if ((rc = j->uniforms.alloc((size_t)j->K * j->T * sizeof(double)))) return rc;
     if ((rc = j->ancestors.alloc((size_t)j->K * j->T * sizeof(int32_t)))) return rc;
   }
@@ -1411,7 +1409,7 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
   }
   if (desc->method == IS_DMPMC) {
     bytes += (int64_t)T * maxN * 4 + maxN * 8 + (int64_t)K * T * 12;
-    bytes += (cdiv(maxN, kCdfThreads * kCdfPer) + 1) * 8;
+    bytes += (2 * cdf_tiles(maxN) + 2) * 8;
   }
   if (desc->method == IS_MIS) bytes += maxN * 4 + (int64_t)K * (8 + 4 * 4 + ts) + 64;  // assignment, design tables
   return bytes;
@@ -1930,33 +1928,44 @@ int is_job_moments(is_job* job, double* mean, double* var, double* cov) {
 }
 
 extern "C++" {
+// resample (src/resampling.jl:245-283) of the job's device log-weights: the CDF in
+// the engine's fixed order, the ancestors of the host uniforms (count, or one for
+// systematic); `ancestors` (host, 1-based) and `gathered` (a results-only job with
+// `count` samples: the ancestors' samples and zero log-weights, the UnweightedSamples
+// of src/resampling.jl:263-283) are each optional.
 template <class T>
-static int job_resample_typed(is_job* job, const double* uniforms, int64_t count, int32_t method, int64_t* ancestors) {
+static int job_resample_typed(is_job* job, const double* uniforms, int64_t count, int32_t method, int64_t* ancestors,
+                              is_job* gathered) {
   const int64_t n = job->total;
   double lz[3];
   int rc = job_lognormalizer_typed<T>(job, lz);
   if (rc) return rc;
   if (lz[1] == -__builtin_huge_val()) return fail(IS_EFAILED, "all weights are zero");
   DevBuf cdf, tiles, u, anc, st;
-  const int64_t ntiles = cdiv(n, (int64_t)kCdfThreads * kCdfPer);
   const int64_t ucount = method == IS_SYSTEMATIC ? 1 : count;
-  if ((rc = cdf.alloc((size_t)n * sizeof(double))) || (rc = tiles.alloc((size_t)ntiles * sizeof(double))) ||
+  if ((rc = cdf.alloc((size_t)n * sizeof(double))) || (rc = tiles.alloc((size_t)(2 * cdf_tiles(n) + 2) * sizeof(double))) ||
       (rc = u.alloc((size_t)ucount * sizeof(double))) || (rc = anc.alloc((size_t)count * sizeof(int32_t))) ||
       (rc = st.alloc(sizeof(int))))
     return rc;
   HIP_TRY(locked_memcpy(u.p, uniforms, (size_t)ucount * sizeof(double), hipMemcpyHostToDevice));
-  hipLaunchKernelGGL((k_cdf_tiles<T>), dim3(ntiles), dim3(kCdfThreads), 0, job->stream, job->logweights.as<T>(), n,
-                     job->summary.as<double>(), tiles.as<double>());
-  hipLaunchKernelGGL((k_cdf_write<T>), dim3(ntiles), dim3(kCdfThreads), 0, job->stream, job->logweights.as<T>(), n,
-                     job->summary.as<double>(), tiles.as<double>(), ntiles, cdf.as<double>(), st.as<int>());
+  launch_cdf<T>(job->logweights.as<T>(), n, job->summary.as<double>(), tiles.as<double>(), cdf.as<double>(),
+                st.as<int>(), job->stream);
   hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(count, 256)), dim3(256), 0, job->stream, cdf.as<double>(),
                      job->logweights.as<T>(), n, u.as<double>(), count, method == IS_SYSTEMATIC ? 1 : 0,
                      anc.as<int32_t>());
+  if (gathered) {
+    const T* src = (job->has_tf ? job->XL : job->X).as<T>();
+    hipLaunchKernelGGL((k_gather_samples<T>), dim3(cdiv(count * job->d, 256)), dim3(256), 0, job->stream, src, n,
+                       job->d, anc.as<int32_t>(), count, gathered->X.as<T>());
+    hipLaunchKernelGGL((k_fill<T>), dim3(cdiv(count, 256)), dim3(256), 0, job->stream, gathered->logweights.as<T>(),
+                       count, T(0));
+  }
   HIP_TRY(hipGetLastError());
-  std::vector<int32_t> h((size_t)count);
-  HIP_TRY(hipMemcpyAsync(h.data(), anc.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, job->stream));
+  std::vector<int32_t> h(ancestors ? (size_t)count : 0);
+  if (ancestors)
+    HIP_TRY(hipMemcpyAsync(h.data(), anc.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, job->stream));
   HIP_TRY(hipStreamSynchronize(job->stream));
-  for (int64_t i = 0; i < count; ++i) ancestors[i] = h[i];
+  for (size_t i = 0; i < h.size(); ++i) ancestors[i] = h[i];
   return IS_OK;
 }
 }  // extern "C++"
@@ -1966,8 +1975,8 @@ int is_job_resample(is_job* job, const double* uniforms, int64_t count, int32_t 
   if (!uniforms || !ancestors || count < 1) return fail(IS_EINVAL, "resampling count must be positive");
   BusyGuard g(job);
   if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
-  return job->precision == IS_FP32 ? job_resample_typed<float>(job, uniforms, count, method, ancestors)
-                                   : job_resample_typed<double>(job, uniforms, count, method, ancestors);
+  return job->precision == IS_FP32 ? job_resample_typed<float>(job, uniforms, count, method, ancestors, nullptr)
+                                   : job_resample_typed<double>(job, uniforms, count, method, ancestors, nullptr);
 }
 
 int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* stream) {
@@ -1978,10 +1987,9 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   const int64_t pcap = cdiv(n, kSweepPerBlock) + 1;
   DevBuf partials, reduced, summary, counter, tiles, st;
   int rc;
-  const int64_t ntiles = cdiv(n, (int64_t)kCdfThreads * kCdfPer);
   if ((rc = partials.alloc((size_t)pcap * 3 * sizeof(double))) || (rc = reduced.alloc(4 * sizeof(double))) ||
       (rc = summary.alloc(5 * sizeof(double))) || (rc = counter.alloc(64)) ||
-      (rc = tiles.alloc((size_t)ntiles * sizeof(double))) || (rc = st.alloc(sizeof(int))))
+      (rc = tiles.alloc((size_t)(2 * cdf_tiles(n) + 2) * sizeof(double))) || (rc = st.alloc(sizeof(int))))
     return rc;
   HIP_TRY(hipMemsetAsync(counter.p, 0, 64, s));
   SweepArgs<double> w{};
@@ -2002,10 +2010,7 @@ int is_cdf_device(is_ctx* ctx, const double* lw, int64_t n, double* cdf, void* s
   hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(2), dim3(kBlock), 0, s, partials.as<double>(),
                      (int64_t)blocks, pcap, 2, reduced.as<double>(), f);
   launch_finish(1, reduced.as<double>(), f, s);
-  hipLaunchKernelGGL((k_cdf_tiles<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),
-                     tiles.as<double>());
-  hipLaunchKernelGGL((k_cdf_write<double>), dim3(ntiles), dim3(kCdfThreads), 0, s, lw, n, summary.as<double>(),
-                     tiles.as<double>(), ntiles, cdf, st.as<int>());
+  launch_cdf<double>(lw, n, summary.as<double>(), tiles.as<double>(), cdf, st.as<int>(), s);
   HIP_TRY(hipGetLastError());
   int status = 0;
   HIP_TRY(hipMemcpyAsync(&status, st.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2160,6 +2165,225 @@ int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t 
   HIP_TRY(locked_memcpy(h.data(), a.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < count; ++i) ancestors[i] = h[i];
   return IS_OK;
+}
+
+// ---------------------------------------------------------------- results with their own storage
+// A WeightedSamples of a run owns its storage, which later runs of the sampler do not
+// change (src/methods/importance_sampling.jl:904-906).  Here a result is a
+// results-only job: the run's logical samples and log-weights copied on the device,
+// the centre its moment pass uses, its own reduction buffers and stream.
+}  // extern "C"
+
+struct is_result {
+  is_job* j = nullptr;
+  bool unweighted = false;  // UnweightedSamples (src/resampling.jl:24-48): no weights
+};
+
+namespace {
+// A results-only job of n samples shaped like `src`; allocates the sample and
+// log-weight arrays (not filled) and the reduction buffers.
+template <class T>
+int result_shell(const is_job* src, int64_t n, std::unique_ptr<is_job>& out) {
+  std::unique_ptr<is_job> r(new is_job());
+  r->ctx = src->ctx;
+  r->method = IS_PLAIN;
+  r->family = src->family;
+  r->kind = src->kind;
+  r->scalar = src->scalar;
+  r->d = src->d;
+  r->D = src->D;
+  r->T = 1;
+  r->dof = src->dof;
+  r->precision = src->precision;
+  r->stride = src->stride;
+  r->total = n;
+  r->maxN = n;
+  r->schedule = {n};
+  r->use_graph = 0;
+  HIP_TRY(hipSetDevice(src->ctx->device));
+  HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+  const int d = r->d;
+  int rc;
+  if ((rc = r->X.alloc((size_t)n * d * sizeof(T))) || (rc = r->logweights.alloc((size_t)n * sizeof(T)))) {
+    is_job_destroy(r.release());
+    return rc;
+  }
+  r->pcap = cdiv(n, kSweepPerBlock) + 1;
+  const int nv_mom = 2 + d + d * (d + 1) / 2;
+  if ((rc = r->partials2.alloc((size_t)r->pcap * (1 + nv_mom) * sizeof(double))) ||
+      (rc = r->reduced2.alloc((size_t)(1 + nv_mom) * sizeof(double))) || (rc = r->summary.alloc(5 * sizeof(double))) ||
+      (rc = r->ctl.alloc(128))) {
+    is_job_destroy(r.release());
+    return rc;
+  }
+  if (locked_memset(r->ctl.p, 0, 128) != hipSuccess) {
+    is_job_destroy(r.release());
+    return fail(IS_EDEVICE, "hipMemset failed");
+  }
+  r->counter.p = r->ctl.as<char>() + 64;
+  r->counter.bytes = 64;
+  r->have_results = true;
+  out = std::move(r);
+  return IS_OK;
+}
+
+// The centre of the job's moment pass (as is_job_moments chooses it), FP64 on the host.
+template <class T>
+int moment_center(is_job* job, std::vector<double>& c0) {
+  const int d = job->d;
+  c0.assign(d, 0.0);
+  if (job->has_tf) {
+    DevBuf center;
+    int rc = center.alloc((size_t)d * sizeof(double));
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_soa_to_aos<T>), dim3(1), dim3(256), 0, job->stream, job->XL.as<T>(), job->total, (int64_t)1,
+                       d, center.as<double>());
+    HIP_TRY(hipMemcpyAsync(c0.data(), center.p, d * sizeof(double), hipMemcpyDeviceToHost, job->stream));
+    HIP_TRY(hipStreamSynchronize(job->stream));
+  } else if (job->method == IS_AMIS || job->method == IS_NPMC || job->method == IS_PLAIN) {
+    c0 = job->committed.mu;
+  } else {
+    for (int c = 0; c < d; ++c) c0[c] = job->committed_locs[c];
+  }
+  return IS_OK;
+}
+
+template <class T>
+int snapshot_typed(is_job* job, std::unique_ptr<is_job>& out) {
+  std::unique_ptr<is_job> r;
+  int rc = result_shell<T>(job, job->total, r);
+  if (rc) return rc;
+  if ((rc = moment_center<T>(job, r->committed.mu))) {
+    is_job_destroy(r.release());
+    return rc;
+  }
+  r->t_shift = job->t_shift;
+  const size_t n = (size_t)job->total;
+  const DevBuf& xs = job->has_tf ? job->XL : job->X;
+  if (hipMemcpyAsync(r->X.p, xs.p, n * job->d * sizeof(T), hipMemcpyDeviceToDevice, job->stream) != hipSuccess ||
+      hipMemcpyAsync(r->logweights.p, job->logweights.p, n * sizeof(T), hipMemcpyDeviceToDevice, job->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(job->stream) != hipSuccess) {
+    is_job_destroy(r.release());
+    return fail(IS_EDEVICE, "result copy failed");
+  }
+  out = std::move(r);
+  return IS_OK;
+}
+
+template <class T>
+int result_resample_typed(is_result* res, const double* uniforms, int64_t count, int32_t method, int64_t* ancestors,
+                          is_result** unweighted) {
+  is_job* j = res->j;
+  std::unique_ptr<is_job> g;
+  int rc;
+  if (unweighted) {
+    if ((rc = result_shell<T>(j, count, g))) return rc;
+    g->committed.mu = j->committed.mu;
+  }
+  rc = job_resample_typed<T>(j, uniforms, count, method, ancestors, g.get());
+  if (rc) {
+    if (g) is_job_destroy(g.release());
+    return rc;
+  }
+  if (unweighted) {
+    is_result* u = new is_result();
+    u->j = g.release();
+    u->unweighted = true;
+    *unweighted = u;
+  }
+  return IS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+#define REQUIRE_RESULT(res)                                   \
+  do {                                                        \
+    if (!(res) || !(res)->j) return fail(IS_EINVAL, "null result"); \
+  } while (0)
+
+int is_job_result(is_job* job, is_result** out) {
+  REQUIRE_RESULTS(job);
+  if (!out) return fail(IS_EINVAL, "null output pointer");
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  HIP_TRY(hipSetDevice(job->ctx->device));
+  std::unique_ptr<is_job> r;
+  const int rc = job->precision == IS_FP32 ? snapshot_typed<float>(job, r) : snapshot_typed<double>(job, r);
+  if (rc) return rc;
+  is_result* res = new is_result();
+  res->j = r.release();
+  *out = res;
+  return IS_OK;
+}
+
+int is_result_destroy(is_result* result) {
+  if (!result) return IS_OK;
+  is_job_destroy(result->j);
+  delete result;
+  return IS_OK;
+}
+
+int64_t is_result_count(const is_result* result) { return (result && result->j) ? result->j->total : 0; }
+
+int is_result_is_weighted(const is_result* result) { return (result && result->j && !result->unweighted) ? 1 : 0; }
+
+int is_result_get_samples(is_result* result, double* out) {
+  REQUIRE_RESULT(result);
+  if (!out) return fail(IS_EINVAL, "null output");
+  return is_job_get_samples(result->j, out);
+}
+
+int is_result_get_logweights(is_result* result, double* out) {
+  REQUIRE_RESULT(result);
+  if (result->unweighted) return fail(IS_ESTATE, "unweighted samples carry no log-weights");
+  if (!out) return fail(IS_EINVAL, "null output");
+  return is_job_get_logweights(result->j, out);
+}
+
+int is_result_device_arrays(is_result* result, const void** samples, int64_t* coord_stride, const void** logweights,
+                            int32_t* is_float) {
+  REQUIRE_RESULT(result);
+  if (logweights) *logweights = result->unweighted ? nullptr : result->j->logweights.p;
+  return is_job_device_results(result->j, samples, coord_stride, nullptr, is_float);
+}
+
+int is_result_logweight_offset(const is_result* result, double* out) {
+  REQUIRE_RESULT(result);
+  return is_job_logweight_offset(result->j, out);
+}
+
+int is_result_lognormalizer(is_result* result, double* out3) {
+  REQUIRE_RESULT(result);
+  if (result->unweighted) return fail(IS_ESTATE, "unweighted samples carry no log-weights");
+  if (!out3) return fail(IS_EINVAL, "null output");
+  return is_job_lognormalizer(result->j, out3);
+}
+
+int is_result_moments(is_result* result, double* mean, double* var, double* cov) {
+  REQUIRE_RESULT(result);
+  return is_job_moments(result->j, mean, var, cov);
+}
+
+int is_result_functional_moments(is_result* result, const void* values, int32_t values_fp64,
+                                 int32_t values_on_device, double* mean, double* var) {
+  REQUIRE_RESULT(result);
+  return is_job_functional_moments(result->j, values, values_fp64, values_on_device, mean, var);
+}
+
+int is_result_resample(is_result* result, const double* uniforms, int64_t count, int32_t method,
+                       int64_t* ancestors, is_result** unweighted) {
+  REQUIRE_RESULT(result);
+  if (result->unweighted) return fail(IS_ESTATE, "unweighted samples carry no weights to resample");
+  if (!uniforms || count < 1) return fail(IS_EINVAL, "resampling count must be positive");
+  if (count > INT32_MAX) return fail(IS_EINVAL, "resampling count exceeds 2^31 - 1");
+  BusyGuard g(result->j);
+  if (!g.ok) return fail(IS_EBUSY, "result is in use");
+  HIP_TRY(hipSetDevice(result->j->ctx->device));
+  return result->j->precision == IS_FP32
+             ? result_resample_typed<float>(result, uniforms, count, method, ancestors, unweighted)
+             : result_resample_typed<double>(result, uniforms, count, method, ancestors, unweighted);
 }
 
 int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3, int64_t n,

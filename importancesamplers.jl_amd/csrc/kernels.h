@@ -771,57 +771,132 @@ __device__ inline void cdf_thread_sums(const T* lw, int64_t n, double mx, int64_
   total = s;
 }
 
+// Wavefront scans in the fixed association the oracle restates (bit-exact CDF):
+// lane l of a wave ends with v_0 + ... + v_l from six shuffle-up steps
+// (Hillis-Steele: at step `off`, v_l <- v_{l-off} + v_l for l >= off).
+__device__ __forceinline__ double wave_incl_scan(double v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double up = __shfl_up(v, off, 64);
+    if (lane >= off) v = up + v;
+  }
+  return v;
+}
+
+// Exclusive prefix of one value per thread over a kCdfThreads-thread block and the
+// block total: within each wave the inclusive scan shifted by one lane (lane 0: 0);
+// across the waves W_0 + W_1 + ... in wave order, added to the lane's prefix.
+// `wtot` is kCdfThreads / 64 doubles of LDS.
+__device__ __forceinline__ void block_excl_scan(double tau, double* wtot, double& pre, double& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double incl = wave_incl_scan(tau, lane);
+  double ex = __shfl_up(incl, 1, 64);
+  if (lane == 0) ex = 0.0;
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  double run = 0.0, mine = 0.0;
+#pragma unroll
+  for (int w = 0; w < kCdfThreads / 64; ++w) {
+    if (w == wave) mine = run;
+    run = run + wtot[w];
+  }
+  pre = mine + ex;
+  total = run;
+}
+
+// Tile totals: tile b covers elements [b kCdfThreads kCdfPer, ...); each thread sums
+// its kCdfPer elements in order, the block scan adds the thread totals.  The thread
+// holding element n - 1 also writes its prefix plus its total, the part of the last
+// element's running value below the tile carry, to tile_total[ntiles].
 template <class T>
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_tiles(const T* __restrict__ lw, int64_t n,
                                                            const double* __restrict__ maxp,
                                                            double* __restrict__ tile_total) {
-  __shared__ double tau[kCdfThreads];
-  double local[kCdfPer], tot;
+  __shared__ double wtot[kCdfThreads / 64];
+  double local[kCdfPer], tot, pre, total;
   cdf_thread_sums(lw, n, *maxp, blockIdx.x, local, tot);
-  tau[threadIdx.x] = tot;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double run = 0.0;
-    for (int j = 0; j < kCdfThreads; ++j) run = run + tau[j];
-    tile_total[blockIdx.x] = run;
-  }
+  block_excl_scan(tot, wtot, pre, total);
+  if (threadIdx.x == 0) tile_total[blockIdx.x] = total;
+  const int64_t base = (int64_t)blockIdx.x * (kCdfThreads * kCdfPer) + (int64_t)threadIdx.x * kCdfPer;
+  if (base <= n - 1 && n - 1 < base + kCdfPer) tile_total[gridDim.x] = pre + tot;
 }
 
+// Tile carries (exclusive prefix of the tile totals) by the same scheme in one block:
+// thread j sums the totals of tiles [j C, (j + 1) C), C = ceil(ntiles / kCdfThreads),
+// in order; carry[b] = (block prefix of thread j) + (its running sum before b).
+// carry[ntiles] is the divisor, the last element's running value (its tile's carry
+// plus tile_total[ntiles]), so the last CDF entry is exactly 1 (src/resampling.jl:222);
+// status = 1 when the grand total is not finite and positive (every weight zero).
+static __global__ __launch_bounds__(kCdfThreads) void k_cdf_carry(const double* __restrict__ tile_total, int64_t ntiles,
+                                                           double* __restrict__ carry, int* __restrict__ status) {
+  __shared__ double wtot[kCdfThreads / 64];
+  const int64_t C = (ntiles + kCdfThreads - 1) / kCdfThreads;
+  const int64_t b0 = (int64_t)threadIdx.x * C;
+  const int64_t b1 = b0 + C < ntiles ? b0 + C : ntiles;
+  double s = 0.0;
+  for (int64_t b = b0; b < b1; ++b) s = s + tile_total[b];
+  double pre, total;
+  block_excl_scan(s, wtot, pre, total);
+  double run = 0.0;
+  for (int64_t b = b0; b < b1; ++b) {
+    const double c = pre + run;
+    carry[b] = c;
+    if (b == ntiles - 1) carry[ntiles] = c + tile_total[ntiles];
+    run = run + tile_total[b];
+  }
+  if (threadIdx.x == 0 && status) *status = (isfinite(total) && total > 0.0) ? 0 : 1;
+}
+
+// cdf[i] = (carry[b] + (thread prefix + running sum)) / carry[ntiles], the tile's sums
+// ordered as in k_cdf_tiles.
 template <class T>
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_write(const T* __restrict__ lw, int64_t n,
                                                            const double* __restrict__ maxp,
-                                                           const double* __restrict__ tile_total,
-                                                           int64_t ntiles, double* __restrict__ cdf,
-                                                           int* __restrict__ status) {
-  __shared__ double tau[kCdfThreads];
-  __shared__ double pre[kCdfThreads];
-  __shared__ double carry_s, total_s;
-  double local[kCdfPer], tot;
+                                                           const double* __restrict__ carry, int64_t ntiles,
+                                                           double* __restrict__ cdf) {
+  __shared__ double wtot[kCdfThreads / 64];
+  double local[kCdfPer], tot, p, tile_sum;
   cdf_thread_sums(lw, n, *maxp, blockIdx.x, local, tot);
-  tau[threadIdx.x] = tot;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double run = 0.0;
-    for (int j = 0; j < kCdfThreads; ++j) {
-      pre[j] = run;
-      run = run + tau[j];
-    }
-    double carry = 0.0;
-    for (int64_t t = 0; t < (int64_t)blockIdx.x; ++t) carry = carry + tile_total[t];
-    double all = 0.0;
-    for (int64_t t = 0; t < ntiles - 1; ++t) all = all + tile_total[t];
-    carry_s = carry;
-    total_s = all + tile_total[ntiles - 1];
-    if (blockIdx.x == 0 && status) *status = (isfinite(total_s) && total_s > 0.0) ? 0 : 1;
-  }
-  __syncthreads();
-  const double carry = carry_s, total = total_s, p = pre[threadIdx.x];
+  block_excl_scan(tot, wtot, p, tile_sum);
+  const double c = carry[blockIdx.x], last = carry[ntiles];
   const int64_t base = (int64_t)blockIdx.x * (kCdfThreads * kCdfPer) + (int64_t)threadIdx.x * kCdfPer;
 #pragma unroll
   for (int e = 0; e < kCdfPer; ++e) {
     const int64_t i = base + e;
-    if (i < n) cdf[i] = (carry + (p + local[e])) / total;
+    if (i < n) cdf[i] = (c + (p + local[e])) / last;
   }
+}
+
+// Launches the three CDF kernels on `s`; `work` holds 2 ntiles + 2 doubles (tile
+// totals, the last element's in-tile part, the carries and the divisor).
+inline int64_t cdf_tiles(int64_t n) { return (n + (int64_t)kCdfThreads * kCdfPer - 1) / ((int64_t)kCdfThreads * kCdfPer); }
+template <class T>
+inline void launch_cdf(const T* lw, int64_t n, const double* maxp, double* work, double* cdf, int* status,
+                       hipStream_t s) {
+  const int64_t ntiles = cdf_tiles(n);
+  double* carry = work + ntiles + 1;
+  hipLaunchKernelGGL((k_cdf_tiles<T>), dim3((unsigned)ntiles), dim3(kCdfThreads), 0, s, lw, n, maxp, work);
+  hipLaunchKernelGGL(k_cdf_carry, dim3(1), dim3(kCdfThreads), 0, s, (const double*)work, ntiles, carry, status);
+  hipLaunchKernelGGL((k_cdf_write<T>), dim3((unsigned)ntiles), dim3(kCdfThreads), 0, s, lw, n, maxp,
+                     (const double*)carry, ntiles, cdf);
+}
+
+// UnweightedSamples of a resample (src/resampling.jl:120-164): sample k of the output
+// is input sample ancestors[k] (1-based), coordinate-major with stride `count`.
+template <class T>
+__global__ void k_gather_samples(const T* __restrict__ X, int64_t cap, int d, const int32_t* __restrict__ ancestors,
+                                 int64_t count, T* __restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= count * d) return;
+  const int64_t k = idx % count;
+  const int c = (int)(idx / count);
+  out[idx] = X[(int64_t)c * cap + (ancestors[k] - 1)];
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ out, int64_t n, T value) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = value;
 }
 
 // ---------------------------------------------------------------- ancestor selection

@@ -249,8 +249,43 @@ int is_job_functional_moments(is_job* job, const void* values, int32_t values_fp
 int is_job_resample(is_job* job, const double* uniforms, int64_t count, int32_t method,
                     int64_t* ancestors);
 
+/* ---- results with their own device storage ---------------------------------------
+ * A WeightedSamples owns its storage: later runs of the sampler do not change it
+ * (src/methods/importance_sampling.jl:904-906, src/results.jl:79-98).  is_job_result
+ * copies the last run's logical samples and log-weights into a result on the device;
+ * the results API below works on that copy.  Destroy with is_result_destroy. */
+typedef struct is_result is_result;
+int is_job_result(is_job* job, is_result** out);
+int is_result_destroy(is_result* result);
+int64_t is_result_count(const is_result* result);
+/* 1 for WeightedSamples, 0 for the UnweightedSamples of is_result_resample */
+int is_result_is_weighted(const is_result* result);
+int is_result_get_samples(is_result* result, double* out);     /* d x count */
+int is_result_get_logweights(is_result* result, double* out);  /* count */
+/* device arrays (coordinate-major, stride = count; log-weights minus the offset, NULL
+ * for unweighted samples) -- valid until is_result_destroy */
+int is_result_device_arrays(is_result* result, const void** samples, int64_t* coord_stride,
+                            const void** logweights, int32_t* is_float);
+int is_result_logweight_offset(const is_result* result, double* out);
+/* lognormalizer, mean / var / cov and mean(f, r) / var(f, r) as the is_job_ forms above
+ * (src/results.jl:560-567, src/statistics.jl:11-103,144-199); unweighted samples have
+ * equal weights in the moments and no lognormalizer */
+int is_result_lognormalizer(is_result* result, double* out3);
+int is_result_moments(is_result* result, double* mean, double* var, double* cov);
+int is_result_functional_moments(is_result* result, const void* values, int32_t values_fp64,
+                                 int32_t values_on_device, double* mean, double* var);
+/* resample(rng, result, count; method) (src/resampling.jl:245-283): the CDF of the
+ * result's weights in the engine's fixed wavefront-scan order, the ancestors of the host
+ * uniforms (count of them, or one for systematic), optionally copied to `ancestors`
+ * (1-based), and optionally the UnweightedSamples gathered on the device
+ * (src/resampling.jl:120-164,263-283) as a new result in `*unweighted`. */
+int is_result_resample(is_result* result, const double* uniforms, int64_t count, int32_t method,
+                       int64_t* ancestors, is_result** unweighted);
+
 /* ---- primitives on device buffers (stream = hipStream_t or NULL) -------------------
- * _resampling_cdf! (src/resampling.jl:199-225) with the engine's blocked order,
+ * _resampling_cdf! (src/resampling.jl:199-225) with the engine's fixed wavefront-scan
+ * order (per-thread runs of 16, shuffle scans within waves, wave totals in order, tile
+ * carries by the same scheme),
  * and ancestor selection (src/resampling.jl:92-118). */
 int is_cdf_device(is_ctx* ctx, const double* logweights_device, int64_t n, double* cdf_device,
                   void* stream);

@@ -681,52 +681,92 @@ int or_cdf_sequential(const double* lw, int64_t n, double* cdf) {
   return 0;
 }
 
-// Blocked CDF: the engine's fixed summation order (see the CDF kernel in the
-// HIP library).  Tiles of kThreads*kPer elements; thread j of a tile sums its
-// kPer elements sequentially; thread prefixes are a sequential scan of the
-// thread totals; tile carries are a sequential scan of tile totals; element
-// value = carry + (thread_prefix + local_sum); finally divide by the last value.
+// Blocked CDF: the engine's fixed summation order (the wavefront-scan CDF kernels of
+// the HIP library).  Tiles of threads*per elements; thread j of a tile sums its `per`
+// elements in order; the block scan below gives each thread its exclusive prefix and
+// the tile total; tile carries come from the same block scan over per-thread runs of
+// C = ceil(ntiles / threads) consecutive tile totals; element value =
+// (carry + (thread_prefix + running_sum)) / last, where `last` is that value for the
+// final element, so the CDF ends at exactly 1 as the reference's does (cdf ./= cdf[end],
+// src/resampling.jl:199-225); AllZeroWeights is judged on the grand total of the tile
+// totals.  Only the association of the sums is the engine's.
+
+// Exclusive prefix of `tau` and its total in the device's block-scan association:
+// per 64-lane wave a Hillis-Steele inclusive scan (step `off`: x_l <- x_{l-off} + x_l
+// for l >= off, off = 1, 2, ..., 32), shifted by one lane (lane 0: 0); then the wave
+// totals in wave order, the running value added to each lane's prefix.
+static void block_excl_scan(const std::vector<double>& tau, std::vector<double>& pre, double& total) {
+  const int threads = (int)tau.size();
+  const int waves = threads / 64;
+  pre.assign(threads, 0.0);
+  std::vector<double> wt(waves);
+  for (int w = 0; w < waves; ++w) {
+    double x[64], y[64];
+    for (int l = 0; l < 64; ++l) x[l] = tau[64 * w + l];
+    for (int off = 1; off < 64; off <<= 1) {
+      for (int l = 0; l < 64; ++l) y[l] = (l >= off) ? x[l - off] + x[l] : x[l];
+      for (int l = 0; l < 64; ++l) x[l] = y[l];
+    }
+    for (int l = 0; l < 64; ++l) pre[64 * w + l] = l ? x[l - 1] : 0.0;
+    wt[w] = x[63];
+  }
+  double run = 0.0;
+  for (int w = 0; w < waves; ++w) {
+    for (int l = 0; l < 64; ++l) pre[64 * w + l] = run + pre[64 * w + l];
+    run = run + wt[w];
+  }
+  total = run;
+}
+
 int or_cdf_blocked(const double* lw, int64_t n, double* cdf, int threads, int per) {
+  if (threads % 64 != 0 || threads < 64) return 2;
   double mx = -kInf;
   for (int64_t i = 0; i < n; ++i) mx = std::max(mx, lw[i]);
   if (mx == -kInf) return 1;
   const int64_t tile = (int64_t)threads * per;
   const int64_t ntiles = (n + tile - 1) / tile;
-  std::vector<double> tile_total(ntiles);
-  std::vector<double> tau(threads), pre(threads);
-  for (int pass = 0; pass < 2; ++pass) {
-    double carry = 0.0;
-    for (int64_t t = 0; t < ntiles; ++t) {
-      for (int j = 0; j < threads; ++j) {
-        double s = 0.0;
-        for (int e = 0; e < per; ++e) {
-          int64_t i = t * tile + (int64_t)j * per + e;
-          double w = i < n ? det_exp(lw[i] - mx) : 0.0;
-          s = s + w;
-          if (pass == 1 && i < n) cdf[i] = s;  // local inclusive sum for now
-        }
-        tau[j] = s;
+  std::vector<double> tile_total(ntiles), tau(threads), pre;
+  std::vector<double> thread_pre((size_t)ntiles * threads);
+  for (int64_t t = 0; t < ntiles; ++t) {
+    for (int j = 0; j < threads; ++j) {
+      double s = 0.0;
+      for (int e = 0; e < per; ++e) {
+        const int64_t i = t * tile + (int64_t)j * per + e;
+        const double w = i < n ? det_exp(lw[i] - mx) : 0.0;
+        s = s + w;
+        if (i < n) cdf[i] = s;  // the thread's running sum for now
       }
-      double run = 0.0;
-      for (int j = 0; j < threads; ++j) {
-        pre[j] = run;
-        run = run + tau[j];
-      }
-      if (pass == 0) {
-        tile_total[t] = run;
-      } else {
-        for (int j = 0; j < threads; ++j)
-          for (int e = 0; e < per; ++e) {
-            int64_t i = t * tile + (int64_t)j * per + e;
-            if (i < n) cdf[i] = carry + (pre[j] + cdf[i]);
-          }
-        carry = carry + tile_total[t];
-      }
+      tau[j] = s;
+    }
+    block_excl_scan(tau, pre, tile_total[t]);
+    for (int j = 0; j < threads; ++j) thread_pre[(size_t)t * threads + j] = pre[j];
+  }
+  // tile carries: per-thread runs of C tile totals, then the block scan
+  const int64_t C = (ntiles + threads - 1) / threads;
+  std::vector<double> carry(ntiles);
+  for (int j = 0; j < threads; ++j) {
+    double s = 0.0;
+    for (int64_t b = (int64_t)j * C; b < std::min<int64_t>((int64_t)(j + 1) * C, ntiles); ++b) s = s + tile_total[b];
+    tau[j] = s;
+  }
+  double total;
+  block_excl_scan(tau, pre, total);
+  for (int j = 0; j < threads; ++j) {
+    double run = 0.0;
+    for (int64_t b = (int64_t)j * C; b < std::min<int64_t>((int64_t)(j + 1) * C, ntiles); ++b) {
+      carry[b] = pre[j] + run;
+      run = run + tile_total[b];
     }
   }
-  double total = cdf[n - 1];
   if (!(std::isfinite(total) && total > 0.0)) return 1;
-  for (int64_t i = 0; i < n; ++i) cdf[i] = cdf[i] / total;
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int j = 0; j < threads; ++j)
+      for (int e = 0; e < per; ++e) {
+        const int64_t i = t * tile + (int64_t)j * per + e;
+        if (i < n) cdf[i] = carry[t] + (thread_pre[(size_t)t * threads + j] + cdf[i]);
+      }
+  const double last = cdf[n - 1];
+  for (int64_t i = 0; i < n; ++i) cdf[i] = cdf[i] / last;
   return 0;
 }
 

@@ -408,7 +408,7 @@ def _host_select(eng, cdf, lw, u, method):
 def test_device_cdf_bit_exact_and_selection(eng, oracle_mod):
     _say("case: device CDF + selection")
     rng = np.random.default_rng(2)
-    for n in (1, 4095, 4096, 4097, 33000):
+    for n in (1, 4095, 4096, 4097, 33000, 1048577, 3000000):  # the last two: more tiles than scan threads
         lw = rng.normal(0, 3, n)
         lw[rng.random(n) < 0.05] = -np.inf
         if np.all(lw == -np.inf):
@@ -452,13 +452,50 @@ def test_results_resample_matches_oracle(eng, oracle_mod):
     res = eng.importance_sample_(s)
     x, lw, _ = oracle_mod.plain_is(q, 0, None, z)
     u = np.random.default_rng(1).random(500)
-    anc = eng.resample(u, res, 500)
+    us = eng.resample(u, res, 500)
     ref = oracle_mod.select_ancestors(oracle_mod.cdf_blocked(res.logweights), res.logweights, u)
-    assert np.array_equal(anc, ref)
-    anc_s = eng.resample([0.37], res, 1000, eng.SystematicResampling())
+    assert np.array_equal(us.ancestors, ref)
+    # UnweightedSamples gathered on the device (src/resampling.jl:120-164,263-283)
+    assert len(us) == 500 and np.array_equal(us.samples, res.samples[ref - 1])
+    assert np.allclose(eng.mean(us), res.samples[ref - 1].mean(axis=0), rtol=1e-12, atol=1e-12)
+    us_s = eng.resample([0.37], res, 1000, eng.SystematicResampling())
     ref_s = oracle_mod.select_ancestors(oracle_mod.cdf_blocked(res.logweights), res.logweights,
                                         oracle_mod.systematic_points(0.37, 1000))
-    assert np.array_equal(anc_s, ref_s)
+    assert np.array_equal(us_s.ancestors, ref_s)
+    assert np.array_equal(us_s.samples, res.samples[ref_s - 1])
+    with pytest.raises(ValueError):
+        eng.lognormalizer(us)  # unweighted samples carry no weights
+    # FP32 results gather their FP32 samples
+    s32 = eng.prepare_sampler(8, eng.StandardGaussianTarget(), eng.ImportanceSampling(q, 10000), precision="fp32")
+    r32 = eng.importance_sample_(s32)
+    us32 = eng.resample(u, r32, 500)
+    ref32 = oracle_mod.select_ancestors(oracle_mod.cdf_blocked(r32.logweights), r32.logweights, u)
+    assert np.array_equal(us32.ancestors, ref32)
+    assert np.array_equal(us32.samples, r32.samples[ref32 - 1])
+
+
+def test_results_keep_their_own_storage_across_runs(eng, oracle_mod):
+    """A WeightedSamples does not change when the sampler runs again
+    (src/methods/importance_sampling.jl:904-906)."""
+    _say("case: results own their storage")
+    rng = np.random.default_rng(5)
+    p, tgt = _amis_problem(eng, 3, "gauss", "factor", None, rng)
+    s = eng.prepare_sampler(12, tgt, eng.AMIS(p, rounds=3, round_size=400))
+    r1 = eng.importance_sample_(s)
+    lz1, m1, v1, c1 = eng.lognormalizer(r1), eng.mean(r1), eng.var(r1), eng.cov(r1)
+    f1 = eng.mean(lambda x: x[:, 0] ** 2, r1)
+    r2 = eng.importance_sample_(s)  # a second run (new counter-RNG draws, refitted proposals)
+    assert eng.lognormalizer(r2) != lz1
+    assert eng.lognormalizer(r1) == lz1
+    assert np.array_equal(eng.mean(r1), m1) and np.array_equal(eng.var(r1), v1) and np.array_equal(eng.cov(r1), c1)
+    assert eng.mean(lambda x: x[:, 0] ** 2, r1) == f1
+    # and they are the oracle's statistics of r1's own arrays
+    assert abs(lz1 - oracle_mod.lognormalizer(r1.logweights)) < 1e-12
+    om, ov, oc = oracle_mod.moments(r1.samples, r1.logweights)
+    assert np.max(np.abs(m1 - om)) < 1e-10 and np.max(np.abs(c1 - oc)) < 1e-10
+    # the kept result still resamples, from its own weights
+    us = eng.resample(rng.random(50), r1, 50)
+    assert len(us) == 50
 
 
 # ---------------------------------------------------------------- graph replay

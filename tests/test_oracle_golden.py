@@ -278,7 +278,7 @@ def test_resampling_zero_mass_support(oracle_mod):
 def test_blocked_cdf_close_to_sequential(oracle_mod):
     o = oracle_mod
     rng = np.random.default_rng(11)
-    for n in (1, 7, 4096, 4097, 32768, 100_001):
+    for n in (1, 7, 4096, 4097, 32768, 100_001, 1_048_577, 2_500_000):  # the last two: > 256 tiles
         lw = rng.normal(0, 3, n)
         a, b = o.cdf_sequential(lw), o.cdf_blocked(lw)
         assert b[-1] == 1.0
