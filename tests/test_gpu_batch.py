@@ -226,3 +226,63 @@ def test_batch_failure_is_isolated(eng):
     assert np.array_equal(cp.location, np.zeros(d))
     for jb, jg in ((0, 0), (1, 1), (3, 2), (4, 3)):
         assert np.array_equal(b.result(jb).logweights, good.result(jg).logweights), jb
+
+
+def _config4_jobs(eng, njobs, seed=4096, T=8, N=4096):
+    """Config-4 posteriors: n ~ U{20..2000}, d ~ U{2..32}, X ~ N(0,1), sigma = 0.5, prior
+    N(0, 4 I); AMIS 8 x 4,096 with Student-t (nu = 5) proposals from the conjugate
+    posterior with the scale inflated 1.5x."""
+    rng = np.random.default_rng(seed)
+    rngs, targets, algs = [], [], []
+    for j in range(njobs):
+        n = int(rng.integers(20, 2001))
+        d = int(rng.integers(2, 33))
+        X = rng.standard_normal((n, d))
+        y = X @ rng.normal(0.0, 2.0, d) + 0.5 * rng.standard_normal(n)
+        S = np.linalg.inv(X.T @ X / 0.25 + np.eye(d) / 4.0)
+        m = S @ (X.T @ y) / 0.25
+        L = np.linalg.cholesky(0.5 * (S + S.T))
+        rngs.append(1 + j)
+        targets.append(eng.LinearRegressionTarget(X, y, 0.5, 2.0))
+        algs.append(eng.AMIS(eng.FactorStudentT(5.0, m, 1.5 * L), rounds=T, round_size=N))
+    return rngs, targets, algs
+
+
+def test_batch_config4_512_posteriors_autotuned(eng, oracle_mod):
+    """512 config-4 posteriors in one persistent launch whose shape the autotuner picks;
+    two jobs of every padded dimension P = 4..32 replayed by the oracle (FP64,
+    log-weights within 1e-10), and the tuned result bit-identical to a run of the same
+    shape in another job order."""
+    _say("case: batch config 4, 512 posteriors, autotuned, oracle replay per P bucket")
+    rngs, targets, algs = _config4_jobs(eng, 512)
+    b = eng.prepare_batch(rngs, targets, algs, precision="fp64")
+    buckets = {}
+    for j, d in enumerate(b.dims):
+        buckets.setdefault((d + 3) // 4, [])
+        if len(buckets[(d + 3) // 4]) < 2:
+            buckets[(d + 3) // 4].append(j)
+    assert sorted(buckets) == list(range(1, 9))
+    picked = [j for js in buckets.values() for j in js]
+    exported = {j: _export(b, j) for j in picked}
+    choice = eng.autotune(b, candidates=[(4, 1, "lpt"), (8, 1, "lpt"), (4, 2, "input")], repeats=1)
+    assert choice["waves"] in (4, 8) and not choice["cached"]
+    st = b.run()
+    assert not st.any(), np.nonzero(st)
+    for j in picked:
+        normals, rn, ru = exported[j]
+        res = b.result(j)
+        means, facs, lns = b.history(j)
+        o = oracle_mod.adaptive(0, algs[j].proposal, algs[j].schedule, normals, targets[j].kind,
+                                targets[j].params(b.dims[j]), rn, ru, nthreads=16, replay=(means, facs, lns))
+        assert o["rc"] == 0, j
+        assert relerr(res.logweights, o["logweights"]) < RTOL, (j, b.dims[j])
+        assert relerr(res.diagnostics["round_lognormalizers"], o["round_lognormalizers"]) < RTOL, j
+        assert relerr(means[1:], o["fit_means"][1:]) < 1e-9, j
+    # the same batch in the reverse job order at the tuned shape: bit-identical (the
+    # FP64 kernel fits one workgroup per CU at 4 and 8 waves)
+    b2 = eng.prepare_batch(rngs, targets, algs, precision="fp64")
+    b2.set_order(np.arange(b2.njobs)[::-1].copy())
+    b2.set_tuning(choice["waves"], choice["blocks_per_cu"])
+    assert not b2.run().any()
+    for j in picked + [0, 511]:
+        assert np.array_equal(b2.result(j).logweights, b.result(j).logweights), j

@@ -176,3 +176,40 @@ def test_callback_request_served_beside_others(eng):
     srv.close()
     alone = eng.importance_sample(9, eng.BatchTarget(_HostGaussian(mean, L)), alg)
     assert np.array_equal(r1.logweights, alone.logweights)
+
+
+def test_callback_jobs_match_the_oracle(eng, oracle_mod):
+    """The callback path against the FP64 oracle fed the same counter-RNG normals, with the
+    oracle evaluating the same Gaussian log density itself: plain IS (samples and
+    log-weights), and AMIS replayed on the device's fitted proposals (log-weights, round
+    log Z, fits)."""
+    _say("case: batch-target callback vs oracle, plain IS d = 3 and AMIS d = 4")
+    mean, L = _problem(3, 11)
+    tgt = eng.GaussianTarget(mean, L)
+    q = eng.SphericalGaussian(np.zeros(3), 2.0)
+    s = eng.prepare_sampler(19, eng.BatchTarget(_HostGaussian(mean, L)), eng.ImportanceSampling(q, 12000))
+    z = s.export_randomness(0)["normals"]
+    res = eng.importance_sample_(s)
+    xo, lwo, fail = oracle_mod.plain_is(q, tgt.kind, tgt.params(3), z)
+    assert fail[0] == 0
+    assert np.max(np.abs(res.samples - xo)) < 1e-13
+    assert np.max(np.abs(res.logweights - lwo) / np.maximum(1.0, np.abs(lwo))) < 1e-10
+    assert abs(eng.lognormalizer(res) - oracle_mod.lognormalizer(lwo)) < 1e-10
+
+    mean, L = _problem(4, 12)
+    tgt = eng.GaussianTarget(mean, L)
+    sched = [3000, 3500, 4000, 2500]
+    q0 = eng.SphericalGaussian(np.zeros(4), 3.0)
+    s = eng.prepare_sampler(23, eng.BatchTarget(_HostGaussian(mean, L)), eng.AMIS(q0, rounds=4, round_size=sched))
+    normals = np.zeros((4, 4 * max(sched)))
+    for t, n in enumerate(sched):
+        normals[t, : 4 * n] = s.export_randomness(t)["normals"]
+    res = eng.importance_sample_(s)
+    means, facs, lns = s.history()
+    o = oracle_mod.adaptive(0, q0, sched, normals, tgt.kind, tgt.params(4), nthreads=4, replay=(means, facs, lns))
+    assert o["rc"] == 0
+    rel = np.abs(res.logweights - o["logweights"]) / np.maximum(1.0, np.abs(o["logweights"]))
+    assert np.max(rel) < 1e-10
+    assert np.max(np.abs(res.diagnostics["round_lognormalizers"] - o["round_lognormalizers"])) < 1e-10
+    assert np.max(np.abs(means[1:] - o["fit_means"][1:])) < 1e-9
+    assert np.max(np.abs(facs[1:] - o["fit_factors"][1:])) < 1e-9

@@ -194,6 +194,12 @@ AMIS_CASES = [
     ("d15-ragged", 15, [4001, 5537, 4777, 6048, 4640, 5999], "std", "factor", None),
     ("d16-factor", 16, [1500] * 6, "gauss", "factor", None),
     ("studentt3-d15", 15, [4700, 4900, 4650, 4800, 4700, 4900], "gauss", "factor", 3.0),
+    # d > 32: the wide buckets (48, 64) and the stand-alone fit kernel (k_finish);
+    # near-target proposals keep the wide fits well conditioned
+    ("d33-near", 33, [6000] * 4, "std", "near", None),
+    ("d48-near", 48, [7000] * 4, "std", "near", None),
+    ("d64-near", 64, [8000] * 4, "std", "near", None),
+    ("studentt5-d48-near", 48, [7000] * 3, "std", "near", 5.0),
 ]
 
 
@@ -210,6 +216,10 @@ def _amis_problem(eng, d, tname, kind, dof, rng):
         return eng.SphericalGaussian(0.3, 2.0), eng.GaussianTarget([0.75], [[1.1]])
     if kind == "spherical":
         p = eng.SphericalGaussian(loc, 2.5) if dof is None else eng.SphericalStudentT(dof, loc, 2.5)
+    elif kind == "near":
+        F = np.tril(rng.normal(0, 0.02, (d, d)), -1) + np.diag(np.linspace(1.0, 1.3, d))
+        loc = np.full(d, 0.05)
+        p = eng.FactorGaussian(loc, F) if dof is None else eng.FactorStudentT(dof, loc, F)
     elif kind == "diagonal":
         sc = np.linspace(1.5, 3.0, d)
         p = eng.DiagonalGaussian(loc, sc) if dof is None else eng.DiagonalStudentT(dof, loc, sc)
@@ -272,6 +282,59 @@ def test_amis_repeated_runs_start_from_committed_proposal(eng, oracle_mod):
     assert relerr(res2.logweights, o["logweights"], 1.0) < RTOL
 
 
+def _config2_problem(eng, seed=20260):
+    """AMIS on a 10-D correlated Gaussian: mean ~ N(0, 4 I), covariance Q diag(lambda) Q^T
+    with Q Haar and lambda log-uniform on [0.1, 10]; q0 = N(0, 25 I)."""
+    d = 10
+    rng = np.random.default_rng(seed)
+    mu = rng.normal(0.0, 2.0, d)
+    Q, R = np.linalg.qr(rng.standard_normal((d, d)))
+    Q = Q * np.sign(np.diag(R))
+    lam = np.exp(rng.uniform(np.log(0.1), np.log(10.0), d))
+    L = np.linalg.cholesky(Q @ np.diag(lam) @ Q.T)
+    return eng.GaussianTarget(mu, L), eng.FactorGaussian(np.zeros(d), 5.0 * np.eye(d))
+
+
+def test_amis_config2_full_shape_fp64_replay(eng, oracle_mod):
+    """The 50 x 20,000 workload at its own shape, FP64, each round replayed by the oracle
+    on the device's fitted proposals: log-weights, fits and round log Z."""
+    _say("case: AMIS config 2 full shape (T=50, N=20000, d=10) FP64 replay")
+    T, N, d = 50, 20000, 10
+    tgt, q0 = _config2_problem(eng)
+    s = eng.prepare_sampler(1, tgt, eng.AMIS(q0, rounds=T, round_size=N))
+    normals, _, _ = _export_batches(s, [N] * T, d)
+    res = eng.importance_sample_(s)
+    means, facs, lns = s.history()
+    o = oracle_mod.adaptive(0, q0, [N] * T, normals, tgt.kind, tgt.params(d), nthreads=16,
+                            replay=(means, facs, lns))
+    assert o["rc"] == 0
+    assert relerr(res.logweights, o["logweights"], 1.0) < RTOL
+    assert relerr(res.diagnostics["round_lognormalizers"], o["round_lognormalizers"], 1.0) < 1e-10
+    assert relerr(res.diagnostics["round_ess"], o["round_ess"], 1.0) < 1e-9
+    assert relerr(means[1:], o["fit_means"][1:], 1.0) < 1e-9
+    assert relerr(facs[1:], o["fit_factors"][1:], 1.0) < 1e-9
+    assert relerr(lns[1:], o["fit_lognormalizers"][1:], 1.0) < 1e-9
+
+
+def test_amis_config2_full_shape_fp32_against_fp64_oracle(eng, oracle_mod):
+    """FP32 storage and arithmetic (FP64 fit and log Z) at the 50 x 20,000 shape against
+    the oracle's own FP64 solve on the same normals: |delta log Z| < 1e-3 and moment error
+    < 1e-3 (the accuracy gate of the effective-samples metric)."""
+    _say("case: AMIS config 2 full shape FP32 vs FP64 oracle")
+    T, N, d = 50, 20000, 10
+    tgt, q0 = _config2_problem(eng)
+    s = eng.prepare_sampler(1, tgt, eng.AMIS(q0, rounds=T, round_size=N), precision="fp32")
+    normals, _, _ = _export_batches(s, [N] * T, d)
+    res = eng.importance_sample_(s)
+    o = oracle_mod.adaptive(0, q0, [N] * T, normals, tgt.kind, tgt.params(d), nthreads=16)
+    assert o["rc"] == 0
+    lz_o = oracle_mod.lognormalizer(o["logweights"])
+    assert abs(eng.lognormalizer(res) - lz_o) < 1e-3
+    om, _, oc = oracle_mod.moments(o["samples"].reshape(-1, d), o["logweights"])
+    assert np.max(np.abs(eng.mean(res) - om)) < 1e-3
+    assert np.max(np.abs(eng.cov(res) - oc)) < 1e-3
+
+
 def test_amis_fp32_matches_fp64_log_z(eng):
     _say("case: AMIS fp32 vs fp64")
     rng = np.random.default_rng(5)
@@ -317,11 +380,15 @@ def test_npmc_recurrence_known_answer(eng, oracle_mod):
     assert len(res) == 13 and res.diagnostics["proposal_evaluations"] == 13
 
 
-@pytest.mark.parametrize("d,schedule", [(2, [16, 16]), (5, [1000, 1500, 700]), (10, [4000] * 6)])
+@pytest.mark.parametrize("d,schedule", [(2, [16, 16]), (5, [1000, 1500, 700]), (10, [4000] * 6),
+                                        (33, [6000] * 3), (48, [7000] * 3), (64, [9000] * 3)])
 def test_npmc_parity(eng, oracle_mod, d, schedule):
     _say(f"case: NPMC parity d={d}")
     rng = np.random.default_rng(d)
-    p, tgt = _amis_problem(eng, d, "gauss", "factor", None, rng)
+    if d > 32:  # the wide buckets and k_finish, on a near-target start
+        p, tgt = _amis_problem(eng, d, "std", "near", None, rng)
+    else:
+        p, tgt = _amis_problem(eng, d, "gauss", "factor", None, rng)
     alg = eng.NPMC(p, rounds=len(schedule), round_size=schedule)
     s = eng.prepare_sampler(31, tgt, alg)
     normals, _, _ = _export_batches(s, schedule, d)

@@ -132,3 +132,39 @@ def test_serve_failing_request_in_mix(eng):
     st = srv.stats()
     srv.close()
     assert st["failed"] == 1 and st["completed"] == len(reqs) - 1
+
+
+def test_serve_d64_million_sample_request_matches_oracle(eng, oracle_mod):
+    """A served AMIS request at the largest dimension (d = 64, the k_finish fit) and
+    N = 1e6 samples per round, beside two smaller requests, replayed by the FP64 oracle
+    on the served job's own fitted proposals."""
+    _say("case: serving an AMIS request d = 64, N = 1e6 x 2 rounds, FP64, vs oracle replay")
+    d, N, T = 64, 1_000_000, 2
+    rng = np.random.default_rng(64)
+    F = np.tril(rng.normal(0, 0.02, (d, d)), -1) + np.diag(np.linspace(1.0, 1.3, d))
+    q0 = eng.FactorGaussian(np.full(d, 0.05), F)
+    tgt = eng.StandardGaussianTarget()
+    alg = eng.AMIS(q0, rounds=T, round_size=N)
+    ref = eng.prepare_sampler(640, tgt, alg)  # same descriptor and key: exports the normals it draws
+    normals = np.zeros((T, d * N))
+    for t in range(T):
+        normals[t] = ref.export_randomness(t)["normals"]
+    del ref
+    others = request_mix(eng, seed=9)[:2]
+    srv = eng.Server(memory_budget=64 << 30, max_active=3)
+    tk = [srv.submit(r, t, a, p) for r, t, a, p in others[:1]]
+    big = srv.submit(640, tgt, alg, "fp64")
+    tk += [srv.submit(r, t, a, p) for r, t, a, p in others[1:]]
+    res, timing = srv.result(big, timeout=110)
+    for t in tk:
+        srv.result(t, timeout=110)
+    srv.close()
+    means, facs, lns = res._sampler.history()
+    o = oracle_mod.adaptive(0, q0, [N] * T, normals, tgt.kind, tgt.params(d), nthreads=16,
+                            replay=(means, facs, lns))
+    assert o["rc"] == 0
+    rel = np.abs(res.logweights - o["logweights"]) / np.maximum(1.0, np.abs(o["logweights"]))
+    assert np.max(rel) < 1e-10
+    assert np.max(np.abs(res.diagnostics["round_lognormalizers"] - o["round_lognormalizers"])) < 1e-10
+    assert np.max(np.abs(means[1:] - o["fit_means"][1:])) < 1e-9
+    assert np.max(np.abs(facs[1:] - o["fit_factors"][1:])) < 1e-9
