@@ -19,6 +19,7 @@ from .api import (  # noqa: F401
     GaussianTarget,
     GlobalResampling,
     ImportanceSampling,
+    InvalidTransformError,
     LinearRegressionTarget,
     LogisticRegressionTarget,
     MultinomialResampling,

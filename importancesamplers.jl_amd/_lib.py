@@ -91,6 +91,7 @@ class Failure(ctypes.Structure):
         ("draw_first_logical_index", ctypes.c_int64),
         ("draw_reason_bits", ctypes.c_uint16),
         ("words", ctypes.c_uint64 * 3),
+        ("draw_first_block", ctypes.c_int32),
     ]
 
 

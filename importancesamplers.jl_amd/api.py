@@ -31,13 +31,15 @@ PLAIN_PHASES = {1: "proposal_draw", 2: "target", 3: "proposal_logdensity", 4: "l
 
 # ---------------------------------------------------------------- errors (reference types)
 class SamplerExecutionError(RuntimeError):
-    """src/methods/importance_sampling.jl:210-225: phase and first logical index."""
+    """src/methods/importance_sampling.jl:210-225: phase, first logical index and the
+    captured cause (an InvalidTransformError for transform failures)."""
 
-    def __init__(self, phase, index, reason_bits=0, message=""):
-        super().__init__(f"sampler failed during {phase} at sample {index}: {message}")
+    def __init__(self, phase, index, reason_bits=0, message="", cause=None):
+        super().__init__(f"sampler failed during {phase} at sample {index}: {message or cause or ''}")
         self.phase = phase
         self.index = index
         self.reason_bits = reason_bits
+        self.cause = cause
 
 
 class _RoundError(RuntimeError):
@@ -61,6 +63,28 @@ class NPMCRoundError(_RoundError):
 
 class DMPMCRoundError(_RoundError):
     method = "DM-PMC"
+
+
+class InvalidTransformError(ValueError):
+    """src/transforms.jl:79-95: a sample transform failed (`reason`: nonfinite_input,
+    nonfinite_output, nonfinite_logabsjac or outside_support) at `location` (the
+    1-based coordinate of a per-coordinate transform)."""
+
+    def __init__(self, reason, location=None):
+        super().__init__(f"invalid transform: {reason}" + ("" if location is None else f" at {location}"))
+        self.reason = reason
+        self.location = location
+
+
+def _transform_reason(bits):
+    """_native_transform_failure_reason (src/transforms.jl:282-287)."""
+    if bits & 0x0001:
+        return "nonfinite_input"
+    if bits & 0x0002:
+        return "nonfinite_output"
+    if bits & 0x0004:
+        return "nonfinite_logabsjac"
+    return "outside_support"
 
 
 class AllZeroWeightsError(ArithmeticError):
@@ -759,14 +783,19 @@ class PreparedSampler:
         else:
             index, bits = f.first_logical_index, f.reason_bits
         phase = PHASES.get(f.phase)
+        # transform reasons come with the failing coordinate in the block field
+        # (src/native_execution.jl:833-840)
+        block = f.draw_first_block if f.draw_first_logical_index else f.first_block
+        tcause = (InvalidTransformError(_transform_reason(bits), int(block) or None)
+                  if bits & 0x000F else None)
         if self.method == PLAIN:
-            raise SamplerExecutionError(PLAIN_PHASES.get(f.phase, phase), index, bits)
+            raise SamplerExecutionError(PLAIN_PHASES.get(f.phase, phase), index, bits, cause=tcause)
         if f.phase in (5, 7):
             cause = AllZeroWeightsError()
         elif f.phase == 6:
             cause = np.linalg.LinAlgError("fitted covariance is not positive definite")
         else:
-            cause = SamplerExecutionError(phase, index, bits)
+            cause = SamplerExecutionError(phase, index, bits, cause=tcause)
         err = {AMIS_M: AMISRoundError, NPMC_M: NPMCRoundError, DMPMC_M: DMPMCRoundError}[self.method]
         raise err(f.round, phase, cause, {"failure_count": int(f.count)})
 

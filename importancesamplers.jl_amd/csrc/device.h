@@ -312,41 +312,60 @@ __device__ __forceinline__ T tf_log_positive_difference(T upper, T lower) {
   }
   return log(upper - lower);
 }
+// Transform failure reasons (src/transforms.jl:106-109), part of kDrawReasons.
+constexpr uint16_t kTransformNonfiniteInput = 0x0001;
+constexpr uint16_t kTransformNonfiniteOutput = 0x0002;
+constexpr uint16_t kTransformNonfiniteLogjac = 0x0004;
+constexpr uint16_t kTransformOutsideSupport = 0x0008;
+
+// Returns the reason bits (0 on success) in the order of the reference's checks
+// (_native_transform_with_logjac, src/transforms.jl:160-235).  Interval bounds are
+// FP64 and, as the reference promotes to the bounds' type, the interval value,
+// its log Jacobian and the support check are computed in FP64 before x and lj are
+// rounded to T.
 template <class T>
-__device__ __forceinline__ bool transform_forward(int kind, T lo, T hi, T z, T& x, T& lj) {
+__device__ __forceinline__ uint16_t transform_forward(int kind, double lo, double hi, T z, T& x, T& lj) {
   x = z;
-  lj = T(0);
-  if (!isfinite(z)) return false;
-  if (kind == 0) return true;
+  lj = kind == 0 ? T(0) : z;
+  if (!isfinite(z)) return kTransformNonfiniteInput;
+  if (kind == 0) return 0;
   if (kind == 1) {
     x = exp(z);
-    lj = z;
-    return isfinite(x) && x > T(0);
+    if (!isfinite(x)) return kTransformNonfiniteOutput;
+    return x > T(0) ? 0 : kTransformOutsideSupport;
   }
   if (kind == 2) {
     x = tf_softplus(z);
     lj = z - x;
-    return isfinite(x) && isfinite(lj) && x > T(0);
+    if (!isfinite(x)) return kTransformNonfiniteOutput;
+    if (!isfinite(lj)) return kTransformNonfiniteLogjac;
+    return x > T(0) ? 0 : kTransformOutsideSupport;
   }
   const bool hl = isfinite(lo), hh = isfinite(hi);
-  if (hl && !hh) {
-    x = lo + exp(z);
-    lj = z;
-    return isfinite(x) && x > lo;
+  if (hl != hh) {
+    // one bound: the positive transform, then lower + distance or upper - distance
+    const T dist = exp(z);
+    x = dist;
+    if (!isfinite(dist)) return kTransformNonfiniteOutput;
+    if (!(dist > T(0))) return kTransformOutsideSupport;
+    const double xd = hl ? lo + (double)dist : hi - (double)dist;
+    x = (T)xd;
+    if (!isfinite(xd)) return kTransformNonfiniteOutput;
+    return (hl ? xd > lo : xd < hi) ? 0 : kTransformOutsideSupport;
   }
-  if (!hl && hh) {
-    x = hi - exp(z);
-    lj = z;
-    return isfinite(x) && x < hi;
-  }
-  if (!hl && !hh) return true;
+  if (!hl) return 0;  // no bound (refused at preparation)
   const T p = tf_logistic(z);
-  if (lo < T(0) && hi > T(0))
-    x = (T(1) - p) * lo + p * hi;
+  double xd;
+  if (lo < 0.0 && hi > 0.0)
+    xd = (double)(T(1) - p) * lo + (double)p * hi;
   else
-    x = p <= T(0.5) ? lo + p * (hi - lo) : hi - (T(1) - p) * (hi - lo);
-  lj = tf_log_positive_difference(hi, lo) - tf_softplus(-z) - tf_softplus(z);
-  return isfinite(x) && isfinite(lj) && x > lo && x < hi;
+    xd = p <= T(0.5) ? lo + (double)p * (hi - lo) : hi - (double)(T(1) - p) * (hi - lo);
+  const double ljd = (tf_log_positive_difference(hi, lo) + (double)(-tf_softplus(-z))) + (double)(-tf_softplus(z));
+  x = (T)xd;
+  lj = (T)ljd;
+  if (!isfinite(xd)) return kTransformNonfiniteOutput;
+  if (!isfinite(ljd)) return kTransformNonfiniteLogjac;
+  return (lo < xd && xd < hi) ? 0 : kTransformOutsideSupport;
 }
 
 // ---------------------------------------------------------------- proposal slots

@@ -562,7 +562,7 @@ int run_rounds(is_job* j) {
       launch_mix<T, kDrawOnly>(D, ncomp, a, s);
       HIP_TRY(hipMemsetAsync(j->tpre.p, 0xff, (size_t)a.n_new * sizeof(T), s));
       HIP_TRY(hipGetLastError());
-      const int cb = j->tbatch(j->tuser, j->X.as<T>() + a.first, j->total, d, a.n_new, j->tpre.p,
+      const int cb = j->tbatch(j->tuser, (j->has_tf ? j->XL : j->X).as<T>() + a.first, j->total, d, a.n_new, j->tpre.p,
                                sizeof(T) == 4 ? 1 : 0, (void*)s);
       if (cb != 0) {
         // the run fails in the target phase of this round and commits nothing
@@ -589,7 +589,7 @@ int run_rounds(is_job* j) {
     ta.Xd = j->Xd.as<T>();
     ta.yd = j->yd.as<T>();
     ta.inv_two_var = j->t_inv_two_var;
-    ta.beta = j->X.as<T>();
+    ta.beta = (j->has_tf ? j->XL : j->X).as<T>();  // the logical samples under a transform
     ta.cap = j->total;
     ta.first = a.first;
     ta.n_new = a.n_new;
@@ -958,10 +958,7 @@ int job_run_typed(is_job* j) {
       f.words[1] = w[1];
       f.words[2] = w[2];
       decode(w[1], f.first_logical_index, f.first_block, f.reason_bits);
-      if (w[2]) {
-        int32_t blk;
-        decode(w[2], f.draw_first_logical_index, blk, f.draw_reason_bits);
-      }
+      if (w[2]) decode(w[2], f.draw_first_logical_index, f.draw_first_block, f.draw_reason_bits);
       f.phase = phase_from_bits(w[1], w[2]);
       break;
     }
@@ -1171,8 +1168,6 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   j->tbatch = desc->target_batch;
   j->tuser = desc->target_user;
   if (desc->transform_kinds) {
-    if (desc->target_kind != IS_TARGET_STANDARD && desc->target_kind != IS_TARGET_GAUSSIAN)
-      return fail(IS_EUNSUPPORTED, "transforms are supported with the standard and Gaussian targets");
     if (!desc->transform_bounds) return fail(IS_EINVAL, "transforms need bounds (2 per coordinate)");
     const int dd = desc->dim;
     for (int c = 0; c < dd; ++c) {

@@ -146,10 +146,13 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
   bool ok = active;
   int gen = a.gen_slot;
   if (active && a.assign) gen = a.assign[j];
+  T tf_lj = T(0);  // pre-target pass under a transform: the draw pass's log Jacobian
   if (MODE == kPreTarget) {
-    // the draw-only pass marks an invalid draw (already recorded) with a NaN numerator
+    // the draw-only pass marks an invalid draw (already recorded) with a NaN numerator,
+    // and under a transform leaves the forward log Jacobian there
     if (active) {
       const T mark = a.lognums[gi];
+      if (a.tf_kinds) tf_lj = mark;
       if (mark != mark) {
         ok = false;
       } else {
@@ -191,7 +194,36 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
         good = valid ? 1 : 0;
       }
       drawn_f[lane] = good;
-      if (MODE == kDrawOnly) a.lognums[gi] = good ? T(0) : T(__builtin_nan(""));
+      if (MODE == kDrawOnly) {
+        T mark = good ? T(0) : T(__builtin_nan(""));
+        if (good && a.tf_kinds) {
+          // the batched target sees the logical sample x = T(z): write it and carry the
+          // forward log Jacobian to the pre-target pass in the numerator slot
+          T lj = T(0);
+          uint16_t treason = 0;
+          int tblock = 0;
+#pragma unroll
+          for (int c = 0; c < D; ++c)
+            if (c < d) {
+              T xc, ljc;
+              const uint16_t r =
+                  transform_forward<T>(a.tf_kinds[c], a.tf_bounds[2 * c], a.tf_bounds[2 * c + 1], x[c], xc, ljc);
+              if (r && !treason) {
+                treason = r;
+                tblock = c + 1;
+              }
+              lj += ljc;
+              a.XL[(int64_t)c * a.cap + gi] = xc;
+            }
+          if (treason) {
+            record_failure(a.fail, j + 1, tblock, treason);
+            mark = T(__builtin_nan(""));
+          } else {
+            mark = lj;
+          }
+        }
+        a.lognums[gi] = mark;
+      }
     }
     if (MODE == kDrawOnly) return;
     __syncthreads();
@@ -209,25 +241,37 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
   T tv = T(0);
   if (wave == 0 && ok) {
     if (MODE == kPreTarget) {
-      tv = a.pre_logtarget[j];
+      tv = a.pre_logtarget[j] + tf_lj;
     } else if (a.tf_kinds) {
-      // the target at the logical sample x = T(z) plus the forward log Jacobian; a
-      // transform failure makes the value NaN (a target-phase failure)
+      // the target at the logical sample x = T(z) plus the forward log Jacobian; the
+      // first failing coordinate (one transform block each) fails the draw with its
+      // transform reason (src/native_execution.jl:689-712,836-840)
       T xl[D];
       T lj = T(0);
-      bool tok = true;
+      uint16_t treason = 0;
+      int tblock = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         xl[c] = T(0);
         if (c < d) {
           T xc, ljc;
-          tok &= transform_forward<T>(a.tf_kinds[c], (T)a.tf_bounds[2 * c], (T)a.tf_bounds[2 * c + 1], x[c], xc, ljc);
+          const uint16_t r =
+              transform_forward<T>(a.tf_kinds[c], a.tf_bounds[2 * c], a.tf_bounds[2 * c + 1], x[c], xc, ljc);
+          if (r && !treason) {
+            treason = r;
+            tblock = c + 1;
+          }
           xl[c] = xc;
           lj += ljc;
           a.XL[(int64_t)c * a.cap + gi] = xc;
         }
       }
-      tv = tok ? inline_target<T, D>(a.tkind, d, tslot, xl, y) + lj : T(__builtin_nan(""));
+      if (treason) {
+        record_failure(a.fail, j + 1, tblock, treason);
+        ok = false;
+      } else {
+        tv = inline_target<T, D>(a.tkind, d, tslot, xl, y) + lj;
+      }
     } else {
       tv = inline_target<T, D>(a.tkind, d, tslot, x, y);
     }

@@ -131,8 +131,9 @@ typedef struct is_job_desc {
   /* per-coordinate transforms (prepare_sampler's transform=, src/methods/importance_sampling.jl:308-313):
    * NULL, or d enum is_transform_kind values with 2d bounds (lower, upper, -inf / +inf for an
    * open side).  Sampling, fits and mixtures stay in unconstrained coordinates z, the target
-   * sees x = T(z), the weights gain the forward log Jacobian and results report x.  Built-in
-   * standard and Gaussian targets only. */
+   * sees x = T(z), the weights gain the forward log Jacobian and results report x (every
+   * target kind, batched and callback targets receiving the logical samples).  A failing
+   * transform fails the draw with its reason bits (0x000f) and the coordinate as block. */
   const int32_t* transform_kinds;
   const double* transform_bounds;
   /* caller's batch target (target_kind IS_TARGET_CALLBACK) */
@@ -152,6 +153,8 @@ typedef struct is_failure {
   int64_t draw_first_logical_index; /* 0 when no draw failure */
   uint16_t draw_reason_bits;
   uint64_t words[3];       /* raw packed record */
+  int32_t draw_first_block; /* block of the first draw failure (a transform failure: the
+                               1-based coordinate of a per-coordinate transform) */
 } is_failure;
 
 /* ---- context ---------------------------------------------------------------------

@@ -99,6 +99,8 @@ def _declare(L):
     L.or_proposal_logdensity.argtypes = [i, d, i, i, F64P, F64P, d, F64P, i]
     L.or_target.restype = d
     L.or_target.argtypes = [i, F64P, i, F64P]
+    L.or_transform_forward.restype = i
+    L.or_transform_forward.argtypes = [i, d, d, d, F64P, F64P]
     L.or_student_t_multiplier.restype = i
     L.or_student_t_multiplier.argtypes = [d, F64P, F64P, F64P]
     L.or_plain_is.restype = None
@@ -192,6 +194,26 @@ def proposal_logdensity(p, x, packed=False):
     x = _f64(np.atleast_1d(x))
     return lib().or_proposal_logdensity(s.family, s.dof, s.kind, s.d, _p(s.location), _p(s.scale),
                                         s.lognormalizer, _p(x), int(bool(packed)))
+
+
+TRANSFORMED_TARGET = 5
+
+
+def transformed_target(inner_kind, inner_params, kinds, bounds):
+    """(kind, params) of the target inner(T(z)) + sum log|dT_c/dz_c| for per-coordinate
+    transforms (kinds: d codes, bounds: d (lower, upper) pairs), the engine's
+    transform= contract; a failing transform gives NaN."""
+    kinds = np.asarray(kinds, dtype=np.float64).ravel()
+    bounds = _f64(bounds).ravel()
+    inner = _f64(inner_params if inner_params is not None else [0.0]).ravel()
+    return TRANSFORMED_TARGET, np.concatenate([[float(inner_kind)], kinds, bounds, inner])
+
+
+def transform_forward_c(kind, lo, hi, z):
+    """The C restatement of the scalar forward transforms: (x, log|J|, reason bits)."""
+    x, lj = ctypes.c_double(), ctypes.c_double()
+    r = lib().or_transform_forward(int(kind), float(lo), float(hi), float(z), ctypes.byref(x), ctypes.byref(lj))
+    return x.value, lj.value, int(r)
 
 
 def target(kind, params, x):
@@ -314,6 +336,67 @@ def transform_forward(kind, lo, hi, z):
             x = np.where(p <= 0.5, lo + p * (hi - lo), hi - (1.0 - p) * (hi - lo))
             span = math.log(hi - lo)
         return x, span - _softplus(-z) - _softplus(z)
+
+
+TRANSFORM_NONFINITE_INPUT, TRANSFORM_NONFINITE_OUTPUT = 0x0001, 0x0002
+TRANSFORM_NONFINITE_LOGJAC, TRANSFORM_OUTSIDE_SUPPORT = 0x0004, 0x0008
+
+
+def transform_forward_checked(kind, lo, hi, z, dtype=np.float64):
+    """One scalar z through _native_transform_with_logjac (src/transforms.jl:160-235)
+    with its checks in the reference's order: returns (x, log|dx/dz|, reason bits).  z is
+    rounded to `dtype` (the storage type); the FP64 interval bounds promote the interval
+    arithmetic to FP64, as Julia promotes T to the bounds' type."""
+    T = np.dtype(dtype).type
+    z = T(z)
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        if not np.isfinite(z):
+            return z, (T(0) if kind == 0 else z), TRANSFORM_NONFINITE_INPUT
+        if kind == 0:
+            return z, T(0), 0
+        if kind == 1:
+            x = np.exp(z)
+            if not np.isfinite(x):
+                return x, z, TRANSFORM_NONFINITE_OUTPUT
+            return x, z, (0 if x > 0 else TRANSFORM_OUTSIDE_SUPPORT)
+        if kind == 2:
+            x = z + np.log1p(np.exp(-z)) if z > 0 else np.log1p(np.exp(z))
+            lj = z - x
+            if not np.isfinite(x):
+                return x, lj, TRANSFORM_NONFINITE_OUTPUT
+            if not np.isfinite(lj):
+                return x, lj, TRANSFORM_NONFINITE_LOGJAC
+            return x, lj, (0 if x > 0 else TRANSFORM_OUTSIDE_SUPPORT)
+        hl, hh = math.isfinite(lo), math.isfinite(hi)
+        if hl != hh:
+            dist = np.exp(z)
+            if not np.isfinite(dist):
+                return dist, z, TRANSFORM_NONFINITE_OUTPUT
+            if not dist > 0:
+                return dist, z, TRANSFORM_OUTSIDE_SUPPORT
+            x = lo + float(dist) if hl else hi - float(dist)
+            if not math.isfinite(x):
+                return x, z, TRANSFORM_NONFINITE_OUTPUT
+            return x, z, (0 if (x > lo if hl else x < hi) else TRANSFORM_OUTSIDE_SUPPORT)
+        p = T(1) / (T(1) + np.exp(-z)) if z >= 0 else np.exp(z) / (T(1) + np.exp(z))
+        if lo < 0 < hi:
+            x = float(T(1) - p) * lo + float(p) * hi
+            big, small = max(-lo, hi), min(-lo, hi)
+            span = math.log(big) + math.log1p(small / big)
+        else:
+            x = lo + float(p) * (hi - lo) if p <= T(0.5) else hi - float(T(1) - p) * (hi - lo)
+            span = math.log(hi - lo)
+
+        def lsig(v):  # -softplus(-v) in T
+            v = -v
+            return -(v + np.log1p(np.exp(-v)) if v > 0 else np.log1p(np.exp(v)))
+
+        lj = (span + float(lsig(z))) + float(lsig(-z))
+        if not math.isfinite(x):
+            return x, lj, TRANSFORM_NONFINITE_OUTPUT
+        if not math.isfinite(lj):
+            return x, lj, TRANSFORM_NONFINITE_LOGJAC
+        return x, lj, (0 if lo < x < hi else TRANSFORM_OUTSIDE_SUPPORT)
 
 
 def functional_moments(lw, values):

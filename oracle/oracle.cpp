@@ -301,10 +301,84 @@ inline int radial_uniform_stride(int family, double dof) {  // src/proposal_bank
 //   1  gaussian:  lognorm - r^2/2, r = L \ (x - m)                 params [m(d), L(d*d col-major), lognorm]
 //   2  logistic:  sum_i y_i*eta_i - log1pexp(eta_i) + prior         params [n, p, prior_var, X(n*p row-major), y(n)]
 //   3  linreg:    Gaussian likelihood + N(0, tau^2 I) prior         params [n, p, sigma, tau, X(n*p row-major), y(n)]
-enum TargetKind { kTStandard = 0, kTGaussian = 1, kTLogistic = 2, kTLinreg = 3 };
+//   5  transformed: inner(T(z)) + log|dT/dz| per coordinate       params [inner kind, kinds(d), bounds(2d), inner params]
+enum TargetKind { kTStandard = 0, kTGaussian = 1, kTLogistic = 2, kTLinreg = 3, kTTransformed = 5 };
+
+// Scalar forward transforms z -> x with log|dx/dz| (src/transforms.jl:160-235, helpers
+// :295-333), FP64; returns the reason bits (src/transforms.jl:106-109), 0 on success.
+// kind 0 identity, 1 positive, 2 softplus, 3 interval (an infinite bound is open).
+inline double tf_softplus(double z) { return z > 0.0 ? z + std::log1p(std::exp(-z)) : std::log1p(std::exp(z)); }
+inline double tf_log_positive_difference(double upper, double lower) {
+  if (lower < 0.0 && upper > 0.0) {
+    const double larger = std::max(-lower, upper), smaller = std::min(-lower, upper);
+    return std::log(larger) + std::log1p(smaller / larger);
+  }
+  return std::log(upper - lower);
+}
+int tf_forward(int kind, double lo, double hi, double z, double* x, double* lj) {
+  *x = z;
+  *lj = kind == 0 ? 0.0 : z;
+  if (!std::isfinite(z)) return 0x0001;
+  if (kind == 0) return 0;
+  if (kind == 1) {
+    *x = std::exp(z);
+    if (!std::isfinite(*x)) return 0x0002;
+    return *x > 0.0 ? 0 : 0x0008;
+  }
+  if (kind == 2) {
+    *x = tf_softplus(z);
+    *lj = z - *x;
+    if (!std::isfinite(*x)) return 0x0002;
+    if (!std::isfinite(*lj)) return 0x0004;
+    return *x > 0.0 ? 0 : 0x0008;
+  }
+  const bool hl = std::isfinite(lo), hh = std::isfinite(hi);
+  if (hl && !hh) {  // lower + PositiveTransform
+    const double dist = std::exp(z);
+    *x = dist;
+    if (!std::isfinite(dist)) return 0x0002;
+    if (!(dist > 0.0)) return 0x0008;
+    *x = lo + dist;
+    if (!std::isfinite(*x)) return 0x0002;
+    return *x > lo ? 0 : 0x0008;
+  }
+  if (!hl && hh) {  // upper - PositiveTransform
+    const double dist = std::exp(z);
+    *x = dist;
+    if (!std::isfinite(dist)) return 0x0002;
+    if (!(dist > 0.0)) return 0x0008;
+    *x = hi - dist;
+    if (!std::isfinite(*x)) return 0x0002;
+    return *x < hi ? 0 : 0x0008;
+  }
+  if (!hl) return 0;
+  // _logistic, _bounded_interval_value, and the log Jacobian
+  const double p = z >= 0.0 ? 1.0 / (1.0 + std::exp(-z)) : std::exp(z) / (1.0 + std::exp(z));
+  if (lo < 0.0 && hi > 0.0)
+    *x = (1.0 - p) * lo + p * hi;
+  else
+    *x = p <= 0.5 ? lo + p * (hi - lo) : hi - (1.0 - p) * (hi - lo);
+  *lj = (tf_log_positive_difference(hi, lo) + -tf_softplus(-z)) + -tf_softplus(z);
+  if (!std::isfinite(*x)) return 0x0002;
+  if (!std::isfinite(*lj)) return 0x0004;
+  return (lo < *x && *x < hi) ? 0 : 0x0008;
+}
 
 double target_eval(int kind, const double* tp, int d, const double* x, double* work) {
   switch (kind) {
+    case kTTransformed: {
+      const int inner = (int)tp[0];
+      const double* kinds = tp + 1;
+      const double* bounds = tp + 1 + d;
+      std::vector<double> xl(d);
+      double lj = 0.0;
+      for (int c = 0; c < d; ++c) {
+        double ljc;
+        if (tf_forward((int)kinds[c], bounds[2 * c], bounds[2 * c + 1], x[c], &xl[c], &ljc)) return kNaN;
+        lj += ljc;
+      }
+      return target_eval(inner, tp + 1 + 3 * d, d, xl.data(), work) + lj;
+    }
     case kTStandard: {
       double s = 0.0;
       for (int c = 0; c < d; ++c) s += x[c] * x[c];
@@ -462,6 +536,10 @@ double or_proposal_logdensity(int family, double dof, int kind, int d, const dou
   Prop p{family, dof, kind, d, mu, scale, lognorm};
   std::vector<double> work(d);
   return packed ? packed_logdensity(p, x, work.data()) : plain_logdensity(p, x, work.data());
+}
+
+int or_transform_forward(int kind, double lo, double hi, double z, double* x, double* lj) {
+  return tf_forward(kind, lo, hi, z, x, lj);
 }
 
 double or_target(int kind, const double* tp, int d, const double* x) {
