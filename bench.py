@@ -61,21 +61,39 @@ def final_summary(pkg, sampler):
     return float(ess[-1]), float(lz[-1])
 
 
-def cpu_baseline(pkg, target, proposal, threads):
+def cpu_threads():
+    """(threads, nproc): the CPU baseline uses the cores this process may run on
+    (sched_getaffinity, what `nproc` prints), at most 16, the host share of one GPU on
+    the MI355X boxes this runs on."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return min(16, nproc), nproc
+
+
+def oracle_solve(pkg, target, proposal, threads):
+    """The FP64 oracle (the CPU restatement of the reference's threaded CPU path) on the
+    workload with the counter-RNG normals of the first GPU run of seed 1: wall time,
+    final ESS, log Z and moments (the accuracy reference of the FP32 and FP64 solves)."""
     import oracle
 
-    # same workload and the same counter-RNG normals as the GPU solve
     s = pkg.prepare_sampler(1, target, pkg.AMIS(proposal, rounds=T, round_size=N))
     normals = np.zeros((T, D * N))
     for t in range(T):
         normals[t] = s.export_randomness(t)["normals"]
+    del s
     t0 = time.perf_counter()
     o = oracle.adaptive(0, proposal, [N] * T, normals, target.kind, target.params(D), nthreads=threads)
     dt = time.perf_counter() - t0
-    ess = float(o["round_ess"][-1])
-    return {"value": ess / dt, "unit": "effective samples/s", "cores": threads, "kind": "port",
-            "sample": f"the full workload (AMIS {T} rounds x {N}, d={D}), one solve, C++ FP64 oracle, "
-                      f"{threads} threads, {dt:.2f} s, ESS {ess:.0f}"}
+    m, _, c = oracle.moments(o["samples"].reshape(-1, D), o["logweights"])
+    return {"seconds": dt, "ess": float(o["round_ess"][-1]), "logz": oracle.lognormalizer(o["logweights"]),
+            "mean": m, "cov": c}
+
+
+def _accuracy(pkg, res, ref):
+    """|delta log Z| and the largest mean / covariance error of a result against a reference
+    dict (logz, mean, cov)."""
+    dlz = abs(pkg.lognormalizer(res) - ref["logz"])
+    dmom = float(max(np.max(np.abs(pkg.mean(res) - ref["mean"])), np.max(np.abs(pkg.cov(res) - ref["cov"]))))
+    return dlz, dmom
 
 
 # ---------------------------------------------------------------- batched posteriors
@@ -106,16 +124,19 @@ def batch_problem(pkg, njobs=BATCH_JOBS, seed=4096):
     return rngs, targets, algs, flops
 
 
-def batch_cpu_baseline(pkg, batch, targets, algs, threads, jobs=8):
+def batch_oracle_check(pkg, batch, targets, algs, threads, lz32, jobs=8):
+    """The first `jobs` posteriors through the FP64 oracle (each the full AMIS from its
+    initial proposal, on the normals the batch's next run draws): the CPU baseline, and
+    |delta log Z| of the batch's FP32 trial run of those jobs against the oracle."""
     import oracle
 
-    t_cpu, ess = 0.0, 0.0
+    t_cpu, ess, dlz = 0.0, 0.0, []
     for j in range(jobs):
         sched = algs[j].schedule
-        d, N = batch.dims[j], sched[0]
-        normals = np.zeros((len(sched), d * N))
-        rn = np.zeros((len(sched), 8 * N))
-        ru = np.zeros((len(sched), 8 * N))
+        d, N_ = batch.dims[j], sched[0]
+        normals = np.zeros((len(sched), d * N_))
+        rn = np.zeros((len(sched), 8 * N_))
+        ru = np.zeros((len(sched), 8 * N_))
         for t in range(len(sched)):
             r = batch.export_randomness(j, t)
             normals[t], rn[t], ru[t] = r["normals"], r["radial_normals"], r["radial_uniforms"]
@@ -124,42 +145,41 @@ def batch_cpu_baseline(pkg, batch, targets, algs, threads, jobs=8):
                             nthreads=threads)
         t_cpu += time.perf_counter() - t0
         ess += float(o["round_ess"][-1])
-    return {"value": ess / t_cpu, "unit": "effective samples/s", "cores": threads, "kind": "port",
-            "sample": f"the first {jobs} of the {len(algs)} posteriors (each the full {BATCH_T} x {BATCH_N} AMIS), "
-                      f"C++ FP64 oracle, {threads} threads, {t_cpu:.2f} s"}
+        dlz.append(abs(float(lz32[j]) - float(o["round_lognormalizers"][-1])))
+    cpu = {"value": ess / t_cpu, "unit": "effective samples/s", "cores": threads, "kind": "port",
+           "sample": f"the first {jobs} of the {len(algs)} posteriors (each the full {BATCH_T} x {BATCH_N} AMIS), "
+                     f"C++ FP64 oracle, {threads} threads, {t_cpu:.2f} s"}
+    return cpu, float(max(dlz))
 
 
-def main_batch(args):
-    import __graft_entry__
-
-    pkg = __graft_entry__.load_package()
-    pkg.context(0)
+def batch_summary(pkg, steps, warmup, threads, tuning=None, check=True, cpu=True):
     t0 = time.perf_counter()
     rngs, targets, algs, flops = batch_problem(pkg)
     b = pkg.prepare_batch(rngs, targets, algs, precision="fp32")
     t_prep = time.perf_counter() - t0
     print(f"batch prepared in {t_prep:.1f} s", file=sys.stderr, flush=True)
-    if args.tuning:
-        w, c = (int(v) for v in args.tuning.split(","))
+    if tuning:
+        w, c = (int(v) for v in tuning.split(","))
         b.set_tuning(w, c)
         tune = {"waves": w, "blocks_per_cu": c, "order": "lpt"}
     else:
         tune = pkg.autotune(b, candidates=[(w, c, "lpt") for w in (2, 4, 8) for c in (1, 2)], repeats=1)
     dlz = None
-    if not args.skip_fp64_check:
-        # FP64 reference solve and the FP32 solve on the same counter-RNG streams (no commit)
+    b.trial(*b.tuning)  # FP32 run on the next run's counter-RNG streams (no commit)
+    lz32 = b.lognormalizers()
+    if check:
+        # FP64 reference solve on the same counter-RNG streams (no commit)
         b64 = pkg.prepare_batch(rngs, targets, algs, precision="fp64")
         b64.trial(*b64.tuning)
-        b.trial(*b.tuning)
-        lz64, lz32 = b64.lognormalizers(), b.lognormalizers()
-        dlz = float(np.max(np.abs(lz32 - lz64)))
+        dlz = float(np.max(np.abs(lz32 - b64.lognormalizers())))
         del b64
-    print(f"tuning {tune}; fp64 check {dlz}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    cpu_line, dlz_oracle = (batch_oracle_check(pkg, b, targets, algs, threads, lz32) if cpu else (None, None))
+    print(f"tuning {tune}; fp64 check {dlz}; oracle check {dlz_oracle}", file=sys.stderr, flush=True)
+    for _ in range(warmup):
         b.run()
     lat, ess_total = [], 0.0
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         t1 = time.perf_counter()
         st = b.run()
         lat.append(time.perf_counter() - t1)
@@ -179,9 +199,9 @@ def main_batch(args):
         "value": ess_total / elapsed,
         "unit": "effective samples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -197,8 +217,9 @@ def main_batch(args):
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "dlogz_vs_fp64": dlz,
+        "dlogz_vs_oracle_max_of_8_jobs": dlz_oracle,
         "failed_jobs": failed,
-        "weighted_samples_per_s": BATCH_JOBS * BATCH_T * BATCH_N * args.steps / elapsed,
+        "weighted_samples_per_s": BATCH_JOBS * BATCH_T * BATCH_N * steps / elapsed,
         "prepare_s": t_prep,
         "roofline": {
             "bound": "mfma",
@@ -211,8 +232,18 @@ def main_batch(args):
             "mean_launch_ms": kernel_ms,
         },
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = batch_cpu_baseline(pkg, b, targets, algs, min(16, os.cpu_count() or 1))
+    if cpu_line is not None:
+        out["cpu_baseline"] = cpu_line
+    return out
+
+
+def main_batch(args):
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    out = batch_summary(pkg, args.steps, args.warmup, cpu_threads()[0], args.tuning, not args.skip_fp64_check,
+                        not args.no_cpu)
     print(json.dumps(out))
 
 
@@ -480,11 +511,7 @@ def dmpmc_problem(pkg, seed=51):
     return pkg.LogisticRegressionTarget(X, y, 10.0), alg, n, p
 
 
-def main_dmpmc(args):
-    import __graft_entry__
-
-    pkg = __graft_entry__.load_package()
-    pkg.context(0)
+def dmpmc_summary(pkg, steps, warmup):
     tgt, alg, n, p = dmpmc_problem(pkg)
     S, T_ = 64 * 512, 30
     flops = 2.0 * n * p * S * T_  # target X beta per solve (algorithmic)
@@ -495,45 +522,49 @@ def main_dmpmc(args):
         w = np.exp(lw - lw.max())
         return float(w.sum() ** 2 / (w * w).sum())
 
-    res, lz, rounds = {}, {}, {}
+    res = {}
     for prec in ("fp64", "fp32"):
         s = pkg.prepare_sampler(3, tgt, alg, precision=prec)
-        ess_s, elapsed, lat = _solve_stats(pkg, s, args.steps, args.warmup, full_ess)
-        out3 = np.zeros(3)
-        s._lib.is_job_lognormalizer(s._job, out3.ctypes.data_as(F64P_))
-        lz[prec] = float(out3[0])
+        ess_s, elapsed, lat = _solve_stats(pkg, s, steps, warmup, full_ess)
         s._lib.is_job_set_timing(s._job, 1)
         s.reset()
         s.run()
-        # a run from the initial bank: the precisions draw the same randomness
-        rlz = np.zeros(T_)
-        s._lib.is_job_get_round_summary(s._job, None, rlz.ctypes.data_as(F64P_), None)
-        rounds[prec] = (rlz, s.ancestors())
         ms = np.zeros(T_)
         s._lib.is_job_round_kernel_ms(s._job, ms.ctypes.data_as(F64P_))
         s._lib.is_job_set_timing(s._job, 0)
         res[prec] = {"ess_per_s": ess_s, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
-                     "p99_ms": 1e3 * float(np.percentile(lat, 99)), "ms_per_step": 1e3 * elapsed / args.steps,
+                     "p99_ms": 1e3 * float(np.percentile(lat, 99)), "ms_per_step": 1e3 * elapsed / steps,
                      "round_phase_ms": float(np.sum(ms))}
         del s
+    # the accuracy gate: a first FP32 run (from the initial bank) against a first FP64 run
+    # on the same counter-RNG draws that replays its bank trajectory (the FP32 run's
+    # ancestors every round, is_job_set_ancestors), so both precisions weigh samples of
+    # the same proposals in all 30 rounds
+    g32 = pkg.prepare_sampler(3, tgt, alg, precision="fp32")
+    g32.run()
+    rep = pkg.prepare_sampler(3, tgt, alg, precision="fp64")
+    rep.set_ancestors(g32.ancestors())
+    rep.run()
+    assert np.array_equal(rep.ancestors(), g32.ancestors())
+    lz32, lz64 = np.zeros(3), np.zeros(3)
+    g32._lib.is_job_lognormalizer(g32._job, lz32.ctypes.data_as(F64P_))
+    rep._lib.is_job_lognormalizer(rep._job, lz64.ctypes.data_as(F64P_))
+    rlz32, rlz64 = _round_ess(g32)[1], _round_ess(rep)[1]
+    gate = {"dlogz_all_rounds_max": float(np.max(np.abs(rlz32 - rlz64))),
+            "dlogz_pooled": float(abs(lz32[0] - lz64[0])), "rounds": T_,
+            "logz_fp32": float(lz32[0]), "logz_fp64_replay": float(lz64[0])}
+    del g32, rep
     peak = 157.3
     achieved = flops / (res["fp32"]["round_phase_ms"] * 1e-3) / 1e12
     out = _line("DM-PMC on a Bayesian logistic regression: X 100,000 x 51 (50 N(0,1) columns + intercept), "
                 "beta ~ N(0, I/50), prior N(0, 10I); K = 64 Gaussian proposals x 512 samples, 30 rounds, "
-                "systematic resampling; bank of 64 copies of the Laplace approximation", "f32", res["fp32"], args.steps,
-                args.warmup)
-    # rounds 0..t* draw from identical proposal banks, t* the first round whose
-    # resampled ancestors differ between the precisions (after it the runs
-    # follow different adaptive trajectories and differ by Monte Carlo error)
-    (rlz64, anc64), (rlz32, anc32) = rounds["fp64"], rounds["fp32"]
-    diff = [t for t in range(T_) if not np.array_equal(anc64[t], anc32[t])]
-    shared = (diff[0] + 1) if diff else T_
-    dlz_rounds = np.abs(rlz32 - rlz64)
-    out.update({"dlogz_vs_fp64": float(dlz_rounds[:shared].max()),
-                "dlogz_vs_fp64_definition": "max over rounds with identical proposal banks of |round log Z FP32 - "
-                                            "FP64|",
-                "rounds_with_identical_banks": shared,
-                "logz_fp32": lz["fp32"], "logz_fp64": lz["fp64"], "dlogz_final": abs(lz["fp32"] - lz["fp64"]),
+                "systematic resampling; bank of 64 copies of the Laplace approximation", "f32", res["fp32"], steps,
+                warmup)
+    out.update({"dlogz_vs_fp64": gate["dlogz_all_rounds_max"],
+                "dlogz_vs_fp64_definition": "max over all 30 rounds of |round log Z FP32 - FP64| with the FP64 run "
+                                            "replaying the FP32 run's ancestors (the same proposal banks)",
+                "fp64_replay_gate": gate,
+                "gate_passed": bool(gate["dlogz_all_rounds_max"] < 1e-3 and gate["dlogz_pooled"] < 1e-3),
                 "fp64_plain_form": res["fp64"],
                 "speedup_vs_fp64_plain_form": res["fp32"]["ess_per_s"] / res["fp64"]["ess_per_s"],
                 "solve_time_speedup_vs_fp64_plain_form": res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"],
@@ -543,7 +574,15 @@ def main_dmpmc(args):
                              "kernel": "per-round draw + target GEMM (X beta, 2 n p per sample) + mixture phase; "
                                        "achieved = GEMM flops / that phase's event time (a lower bound for the GEMM)",
                              "mean_launch_ms": res["fp32"]["round_phase_ms"] / T_}})
-    print(json.dumps(out))
+    return out
+
+
+def main_dmpmc(args):
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    print(json.dumps(dmpmc_summary(pkg, args.steps, args.warmup)))
 
 
 def mis_problem(pkg, seed=91):
@@ -620,6 +659,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="default workload: skip the config-3/4 figures")
     ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc", "mis", "copy"])
     ap.add_argument("--tuning", default=None, help="batch workload: 'waves,blocks_per_cu' instead of autotuning")
     ap.add_argument("--skip-fp64-check", action="store_true", help="batch workload: skip the FP64 reference solve")
@@ -648,23 +688,36 @@ def main():
     target, proposal, mu, S = problem(pkg)
     alg = pkg.AMIS(proposal, rounds=T, round_size=N)
 
-    # FP64 reference solve and the FP32 candidate on the same counter-RNG stream
+    # FP64 and FP32 first solves on the counter-RNG stream of seed 1 (run 0), then the CPU
+    # oracle on the same normals: the accuracy reference of both and the first CPU leg
+    # (A) of an A B B A interleave with the timed GPU solves (B)
+    threads, nproc = cpu_threads()
     ref = pkg.prepare_sampler(1, target, alg, precision="fp64")
     r64 = pkg.importance_sample_(ref)
     lz64 = pkg.lognormalizer(r64)
     m64, c64 = pkg.mean(r64), pkg.cov(r64)
-    chosen, dlz, dmom = "fp64", 0.0, 0.0
+    r32 = None
     if args.precision in ("auto", "fp32"):
         cand = pkg.prepare_sampler(1, target, alg, precision="fp32")
         r32 = pkg.importance_sample_(cand)
-        lz32 = pkg.lognormalizer(r32)
-        m32, c32 = pkg.mean(r32), pkg.cov(r32)
-        dlz = abs(lz32 - lz64)
-        dmom = float(max(np.max(np.abs(m32 - m64)), np.max(np.abs(c32 - c64))))
-        if args.precision == "fp32" or (dlz < 1e-3 and dmom < 1e-3):
-            chosen = "fp32"
-        del r32, cand
-    del r64
+        del cand
+    o1 = None if args.no_cpu else oracle_solve(pkg, target, proposal, threads)
+    acc = {}
+    if o1 is not None:
+        acc["fp64"] = _accuracy(pkg, r64, o1)
+        if r32 is not None:
+            acc["fp32"] = _accuracy(pkg, r32, o1)
+        gate_ref = "oracle"
+    elif r32 is not None:
+        acc["fp32"] = _accuracy(pkg, r32, {"logz": lz64, "mean": m64, "cov": c64})
+        gate_ref = "engine fp64 (oracle skipped: --no-cpu)"
+    chosen = "fp64"
+    if r32 is not None and (args.precision == "fp32" or (acc["fp32"][0] < 1e-3 and acc["fp32"][1] < 1e-3)):
+        chosen = "fp32"
+    dlz = abs(pkg.lognormalizer(r32) - lz64) if r32 is not None else 0.0
+    dmom = (float(max(np.max(np.abs(pkg.mean(r32) - m64)), np.max(np.abs(pkg.cov(r32) - c64))))
+            if r32 is not None else 0.0)
+    del r64, r32
 
     sampler = pkg.prepare_sampler(1, target, alg, precision=chosen)
     lib = sampler._lib
@@ -765,10 +818,41 @@ def main():
             "mean_launch_ms": float(np.mean(kms)),
         },
     }
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(pkg, target, proposal, threads)
+    out["accuracy_gate"] = {"reference": gate_ref if (o1 is not None or "fp32" in acc) else None,
+                            "tolerance": {"dlogz": 1e-3, "moments": 1e-3}}
+    for prec, (dz, dm) in acc.items():
+        out[f"dlogz_vs_oracle_{prec}" if o1 is not None else f"dlogz_vs_fp64_{prec}"] = dz
+        out[f"moment_err_vs_oracle_{prec}" if o1 is not None else f"moment_err_vs_fp64_{prec}"] = dm
+    if o1 is not None:
+        # the second CPU leg (A) after the GPU legs (B B): the baseline is the mean of the two
+        o2 = oracle_solve(pkg, target, proposal, threads)
+        sec = 0.5 * (o1["seconds"] + o2["seconds"])
+        out["cpu_baseline"] = {
+            "value": o1["ess"] / sec, "unit": "effective samples/s", "cores": threads, "kind": "port",
+            "sample": f"the full workload (AMIS {T} rounds x {N}, d={D}), C++ FP64 oracle on the GPU run's "
+                      f"normals, {threads} threads (nproc {nproc}), interleaved A B B A with the FP32 and FP64 GPU "
+                      f"legs: {o1['seconds']:.2f} s and {o2['seconds']:.2f} s, ESS {o1['ess']:.0f}"}
+    if not args.no_extras:
+        out["other_workloads"] = extra_workloads(pkg, args, threads)
     print(json.dumps(out))
+
+
+def extra_workloads(pkg, args, threads):
+    """The DM-PMC logistic (config 3) and batched-posterior (config 4) workloads at a few
+    steps each, every one with its own roofline and accuracy figures; a failure is
+    reported in place of the figures."""
+    out = {}
+    for name, fn in (("config3_dmpmc_logistic", lambda: dmpmc_summary(pkg, 3, 1)),
+                     ("config4_batched_linreg", lambda: batch_summary(pkg, 3, 1, threads, None, True,
+                                                                      not args.no_cpu))):
+        t0 = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as exc:  # reported, so the main line still prints
+            out[name] = {"error": repr(exc)[:300]}
+        out[name]["wall_s"] = time.perf_counter() - t0
+        print(f"{name} done in {out[name]['wall_s']:.1f} s", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -107,6 +107,7 @@ SIGNATURES = {
     "is_job_set_normals": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64]),
     "is_job_set_radial": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64, F64P, ctypes.c_int64]),
     "is_job_set_uniforms": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64]),
+    "is_job_set_ancestors": (ctypes.c_int, [VP, ctypes.c_int32, I64P, ctypes.c_int64]),
     "is_job_clear_prefilled": (ctypes.c_int, [VP]),
     "is_job_export_normals": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64]),
     "is_job_export_radial": (ctypes.c_int, [VP, ctypes.c_int32, F64P, ctypes.c_int64, F64P, ctypes.c_int64]),

@@ -758,6 +758,15 @@ class PreparedSampler:
             out["uniforms"] = u
         return out
 
+    def set_ancestors(self, ancestors):
+        """DM-PMC replay: round t resamples to ancestors[t] (K 1-based indices; a (T, K)
+        array, or a list with None for rounds left to the sampler)."""
+        for t, a in enumerate(ancestors):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+            _check(self._lib.is_job_set_ancestors(self._job, t, _p(a, I64P), a.size))
+
     def clear_prefilled(self):
         _check(self._lib.is_job_clear_prefilled(self._job))
 

@@ -142,7 +142,8 @@ struct is_job {
   HostPinned slot_up, slot_down;
   bool slots_resident = false;  // plain IS / static MIS: slot table already holds the committed state
   DevBuf pre_normals, pre_rn, pre_ru, pre_uniforms;
-  std::vector<int> prefilled, prefilled_radial, prefilled_uniforms;
+  std::vector<int> prefilled, prefilled_radial, prefilled_uniforms, prefilled_ancestors;
+  DevBuf pre_ancestors;  // DM-PMC: K x T given ancestors (1-based), replacing the resampling step
   // last run
   bool have_results = false;
   int last_status = 0;
@@ -833,17 +834,23 @@ int run_rounds(is_job* j) {
       reduce(j->partials, nrec, 2, j->reduced, f);
       // global resampling (src/dm_pmc_execution.jl:267-295)
       const T* lw = j->logweights.as<T>() + first;
-      launch_cdf<T>(lw, N, summ, j->tile_total.as<double>(), j->cdf.as<double>(), j->cdf_status.as<int>() + t, s);
-      double* u = j->uniforms.as<double>() + (size_t)t * j->K;
-      if (j->prefilled_uniforms[t]) {
-        u = j->pre_uniforms.as<double>() + (size_t)t * j->K;
-      } else {
-        hipLaunchKernelGGL(k_resampling_uniforms, dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->key,
-                           j->run_dev.as<unsigned long long>(), t, (int64_t)j->K, u);
-      }
       int32_t* anc = j->ancestors.as<int32_t>() + (size_t)t * j->K;
-      hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->cdf.as<double>(), lw, N,
-                         u, (int64_t)j->K, j->resampling == IS_SYSTEMATIC ? 1 : 0, anc);
+      if (j->prefilled_ancestors[t]) {
+        // replay seam: this round's ancestors are given (e.g. another run's trajectory)
+        hipLaunchKernelGGL((k_copy<int32_t>), dim3(cdiv(j->K, 64)), dim3(64), 0, s,
+                           (const int32_t*)(j->pre_ancestors.as<int32_t>() + (size_t)t * j->K), (int64_t)j->K, anc);
+      } else {
+        launch_cdf<T>(lw, N, summ, j->tile_total.as<double>(), j->cdf.as<double>(), j->cdf_status.as<int>() + t, s);
+        double* u = j->uniforms.as<double>() + (size_t)t * j->K;
+        if (j->prefilled_uniforms[t]) {
+          u = j->pre_uniforms.as<double>() + (size_t)t * j->K;
+        } else {
+          hipLaunchKernelGGL(k_resampling_uniforms, dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->key,
+                             j->run_dev.as<unsigned long long>(), t, (int64_t)j->K, u);
+        }
+        hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->cdf.as<double>(), lw,
+                           N, u, (int64_t)j->K, j->resampling == IS_SYSTEMATIC ? 1 : 0, anc);
+      }
       hipLaunchKernelGGL((k_gather_locations<T>), dim3(cdiv((int64_t)j->K * d, 64)), dim3(64), 0, s,
                          j->X.as<T>(), j->total, first, d, anc, j->K, j->slots64.as<double>(), j->slotsT.as<T>(),
                          j->stride);
@@ -1367,6 +1374,7 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   j->prefilled.assign(T, 0);
   j->prefilled_radial.assign(T, 0);
   j->prefilled_uniforms.assign(T, 0);
+  j->prefilled_ancestors.assign(T, 0);
   *out = guard.release();
   return IS_OK;
 }
@@ -1534,11 +1542,36 @@ int is_job_set_uniforms(is_job* job, int32_t round, const double* uniforms, int6
   return IS_OK;
 }
 
+int is_job_set_ancestors(is_job* job, int32_t round, const int64_t* ancestors, int64_t count) {
+  if (!job || !ancestors || round < 0 || round >= job->T || job->method != IS_DMPMC)
+    return fail(IS_EINVAL, "bad arguments");
+  if (count != job->K) return fail(IS_EINVAL, "one ancestor per active proposal (%d)", job->K);
+  const int64_t N = job->schedule[round];
+  std::vector<int32_t> h((size_t)count);
+  for (int64_t k = 0; k < count; ++k) {
+    if (ancestors[k] < 1 || ancestors[k] > N) return fail(IS_EINVAL, "ancestor %lld out of [1, %lld]",
+                                                          (long long)ancestors[k], (long long)N);
+    h[(size_t)k] = (int32_t)ancestors[k];
+  }
+  BusyGuard g(job);
+  if (!g.ok) return fail(IS_EBUSY, "sampler is already running");
+  if (!job->pre_ancestors.p) {
+    int rc = job->pre_ancestors.alloc((size_t)job->K * job->T * sizeof(int32_t));
+    if (rc) return rc;
+  }
+  HIP_TRY(locked_memcpy(job->pre_ancestors.as<int32_t>() + (size_t)round * job->K, h.data(), h.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+  job->prefilled_ancestors[round] = 1;
+  job->graph_dirty = true;
+  return IS_OK;
+}
+
 int is_job_clear_prefilled(is_job* job) {
   if (!job) return fail(IS_EINVAL, "null job");
   std::fill(job->prefilled.begin(), job->prefilled.end(), 0);
   std::fill(job->prefilled_radial.begin(), job->prefilled_radial.end(), 0);
   std::fill(job->prefilled_uniforms.begin(), job->prefilled_uniforms.end(), 0);
+  std::fill(job->prefilled_ancestors.begin(), job->prefilled_ancestors.end(), 0);
   job->graph_dirty = true;
   return IS_OK;
 }

@@ -894,6 +894,12 @@ __global__ void k_gather_samples(const T* __restrict__ X, int64_t cap, int d, co
 }
 
 template <class T>
+__global__ void k_copy(const T* __restrict__ src, int64_t n, T* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+template <class T>
 __global__ void k_fill(T* __restrict__ out, int64_t n, T value) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = value;

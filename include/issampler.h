@@ -187,6 +187,12 @@ int is_job_set_normals(is_job* job, int32_t round, const double* normals, int64_
 int is_job_set_radial(is_job* job, int32_t round, const double* normals, int64_t ncount,
                       const double* uniforms, int64_t ucount);
 int is_job_set_uniforms(is_job* job, int32_t round, const double* uniforms, int64_t count);
+/* DM-PMC replay seam: the given round resamples to these K ancestors (1-based indices into
+ * the round's samples, active-proposal order) instead of selecting them from its weights
+ * and uniforms, until is_job_clear_prefilled.  It replays another run's bank trajectory
+ * (is_job_get_ancestors), e.g. an FP64 run on an FP32 run's banks, so the per-round log Z
+ * of the two precisions is compared on the same proposals (src/dm_pmc_execution.jl:267-295). */
+int is_job_set_ancestors(is_job* job, int32_t round, const int64_t* ancestors, int64_t count);
 int is_job_clear_prefilled(is_job* job);
 /* The counter-RNG values the job consumes in `round` (for feeding the oracle). */
 int is_job_export_normals(is_job* job, int32_t round, double* out, int64_t count);

@@ -597,3 +597,33 @@ def test_graph_replays_match_eager_runs(eng):
             if rep == 1:
                 g.reset()
                 e.reset()
+
+
+def test_dmpmc_ancestor_replay_seam(eng, oracle_mod):
+    """is_job_set_ancestors: a run given another run's ancestors follows its bank
+    trajectory (FP64 with its own ancestors: bit-identical log-weights); an FP64 replay
+    of an FP32 run's trajectory matches it per round within the FP32 gate."""
+    _say("case: DM-PMC ancestor replay seam")
+    rng = np.random.default_rng(44)
+    d, K, sched = 4, 8, [1024] * 5
+    bank = _dm_problem(eng, d, K, "factor", rng)
+    tgt = eng.GaussianTarget(np.zeros(d), 2.0 * np.eye(d))
+    alg = eng.DeterministicMixturePMC(bank, rounds=len(sched), round_size=sched)
+    a = eng.prepare_sampler(3, tgt, alg)
+    ra = eng.importance_sample_(a)
+    b = eng.prepare_sampler(3, tgt, alg)
+    b.set_ancestors(a.ancestors())
+    rb = eng.importance_sample_(b)
+    assert np.array_equal(ra.logweights, rb.logweights)
+    assert np.array_equal(a.ancestors(), b.ancestors())
+    # the FP32 run's banks, replayed in FP64
+    s32 = eng.prepare_sampler(5, tgt, alg, precision="fp32")
+    r32 = eng.importance_sample_(s32)
+    s64 = eng.prepare_sampler(5, tgt, alg, precision="fp64")
+    s64.set_ancestors(s32.ancestors())
+    r64 = eng.importance_sample_(s64)
+    assert np.array_equal(s64.ancestors(), s32.ancestors())
+    dlz = np.abs(r32.diagnostics["round_lognormalizers"] - r64.diagnostics["round_lognormalizers"])
+    assert np.max(dlz) < 1e-4
+    with pytest.raises(ValueError):
+        b.set_ancestors([[0] * K])  # 1-based
