@@ -49,6 +49,7 @@ from .api import (  # noqa: F401
     current_proposal,
     importance_sample,
     importance_sample_,
+    logdensityof,
     lognormalizer,
     mean,
     prepare_sampler,

@@ -153,6 +153,8 @@ SIGNATURES = {
     "is_select_device": (ctypes.c_int, [VP, VP, VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int32, VP, VP]),
     "is_cdf": (ctypes.c_int, [VP, F64P, ctypes.c_int64, F64P]),
     "is_select": (ctypes.c_int, [VP, F64P, F64P, ctypes.c_int64, F64P, ctypes.c_int64, ctypes.c_int32, I64P]),
+    "is_proposal_logdensity": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                              F64P, F64P, F64P, ctypes.c_int64, ctypes.c_int32, F64P]),
     "is_rng_raw": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_uint64, ctypes.c_int64, U64P]),
     "is_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),

@@ -899,6 +899,24 @@ def current_proposal(sampler: PreparedSampler):
     return NativeProposal(p0.family, FACTOR, loc, L.ravel(), float(ln[0]), p0.dof, False)
 
 
+def logdensityof(proposal: NativeProposal, x, precision="fp64"):
+    """DensityInterface.logdensityof(proposal, x) (src/proposals.jl:24-55,603-669, with the
+    extended-range repair of src/student_t_range.jl:55-153), evaluated on the device: x is
+    one point (d,) or scalar, or points (n, d); returns a float or an (n,) array."""
+    lib = _lib.load()
+    d = proposal.dim
+    pts = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    single = pts.ndim == 1 and (pts.size == d)
+    pts = _f64(pts.reshape(-1, d))
+    scale = proposal.scale if proposal.kind != FACTOR else proposal.factor().T.ravel()
+    scale = _f64(scale)
+    out = np.zeros(pts.shape[0])
+    prec = {"fp64": FP64, "fp32": FP32, FP64: FP64, FP32: FP32}[precision]
+    _check(lib.is_proposal_logdensity(context(), proposal.family, proposal.dof, proposal.kind, d,
+                                      _p(_f64(proposal.location)), _p(scale), _p(pts), pts.shape[0], prec, _p(out)))
+    return float(out[0]) if single else out
+
+
 # ---------------------------------------------------------------- results (src/results.jl, src/statistics.jl)
 class _ResultHandle:
     """An engine result (is_result): the run's samples and log-weights copied on the

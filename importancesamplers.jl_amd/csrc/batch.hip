@@ -42,7 +42,7 @@ void pack_bslot(const HostProp& p, int d, int P, double* slot) {
     slot[P + r] = 1.0;
     if (r >= d) continue;
     slot[r] = p.mu[r];
-    slot[P + r] = 1.0 / p.L[r + (size_t)r * d];
+    slot[P + r] = slot_inv_entry<double>(p.L[r + (size_t)r * d]);
     for (int c = 0; c <= r; ++c) slot[2 * P + bslot_rowoff(r) + c] = p.L[r + (size_t)c * d];
   }
   slot[bslot_lognorm(P)] = p.lognorm;
@@ -206,6 +206,10 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
     J.fail = b->fail.as<unsigned long long>() + 3 * H.round_off;
   }
   std::vector<T> cT(c64.begin(), c64.end());
+  for (int j = 0; j < b->njobs; ++j) {  // inverse diagonals that are not normal in T: NaN
+    const BatchJobHost& H = b->jobs[j];
+    for (int r = 0; r < H.d; ++r) cT[H.com_off + H.P + r] = slot_inv_value<T>(c64[H.com_off + H.P + r]);
+  }
   HIP_TRY(locked_memcpy(b->xpack.p, xp.data(), nxp * sizeof(T), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(b->ypack.p, yp.data(), nyp * sizeof(T), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(b->com64.p, c64.data(), ncom * sizeof(double), hipMemcpyHostToDevice));

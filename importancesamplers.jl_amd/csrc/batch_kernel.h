@@ -408,11 +408,12 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
     }
     o64[lane] = mu;
     oT[lane] = (T)mu;
-    o64[P + lane] = invd;
-    oT[P + lane] = (T)invd;
+    // a reciprocal that is not normal in T: NaN, so the job fails with an invalid proposal
+    o64[P + lane] = slot_inv_value<double>(invd);
+    oT[P + lane] = slot_inv_value<T>(invd);
     if (oL) {
       oL[lane] = (T)mu;
-      oL[P + lane] = (T)invd;
+      oL[P + lane] = slot_inv_value<T>(invd);
     }
     const int base = 2 * P + bslot_rowoff(lane);
 #pragma unroll

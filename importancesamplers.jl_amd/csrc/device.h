@@ -434,11 +434,145 @@ __device__ __forceinline__ T radial_logdensity(const Family<T>& fam, T lognorm, 
   return lognorm - fam.half_dof_plus_d * radial;
 }
 
+// ---------------------------------------------------------------- extended-range densities
+// Values as (mantissa, binary exponent) pairs, so that a standardized coordinate
+// whose plain value over- or underflows keeps its magnitude
+// (src/student_t_range.jl:55-74: _range_subtract, _range_multiply, _range_divide).
+template <class T>
+struct RangeVal {
+  T m;
+  int e;
+};
+__device__ inline double rv_frexp(double v, int* e) { return frexp(v, e); }
+__device__ inline float rv_frexp(float v, int* e) { return frexpf(v, e); }
+__device__ inline double rv_ldexp(double v, int e) { return ldexp(v, e); }
+__device__ inline float rv_ldexp(float v, int e) { return ldexpf(v, e); }
+template <class T>
+__device__ inline RangeVal<T> range_of(T v) {
+  RangeVal<T> r;
+  r.m = rv_frexp(v, &r.e);
+  return r;
+}
+template <class T>
+__device__ inline RangeVal<T> range_subtract(RangeVal<T> a, RangeVal<T> b) {
+  if (a.m == T(0)) return RangeVal<T>{-b.m, b.e};
+  if (b.m == T(0)) return a;
+  const int ex = a.e > b.e ? a.e : b.e;
+  int sh;
+  const T m = rv_frexp(rv_ldexp(a.m, a.e - ex) - rv_ldexp(b.m, b.e - ex), &sh);
+  return RangeVal<T>{m, m == T(0) ? 0 : ex + sh};
+}
+template <class T>
+__device__ inline RangeVal<T> range_multiply(RangeVal<T> v, T factor) {
+  int ef, sh;
+  const T mf = rv_frexp(factor, &ef);
+  const T p = rv_frexp(v.m * mf, &sh);
+  return RangeVal<T>{p, p == T(0) ? 0 : v.e + ef + sh};
+}
+template <class T>
+__device__ inline RangeVal<T> range_divide(RangeVal<T> v, T divisor) {
+  int ed, sh;
+  const T md = rv_frexp(divisor, &ed);
+  const T q = rv_frexp(v.m / md, &sh);
+  return RangeVal<T>{q, q == T(0) ? 0 : v.e - ed + sh};
+}
+
+// The slot density where the plain forward substitution does not give it: the
+// standardized coordinates by division as the reference forms them, rows whose
+// plain value is not finite (from finite inputs) recomputed in extended range
+// (_repair_factor_squared_radius! / _factor_standardized_coordinate,
+// src/student_t_range.jl:76-116); Gaussian: lognorm - r^2/2, or - sum (y/2) y when
+// r^2 overflows (src/proposals.jl:24-34); Student-t: the log radius in extended range
+// (_factor_range_logradius!, src/student_t_range.jl:137-153) through
+// log1pexp(log r^2 - log nu) (src/proposals.jl:36-55).  Out of line: it runs only for
+// samples (or slots) at the edges of the floating-point range.
+template <class T, int D>
+__device__ __noinline__ T slot_logdensity_range(const T* s, int d, Family<T> fam, const T* x) {
+  const T* mu = s;
+  auto L = [&](int r, int c) { return s[2 * d + r * (r + 1) / 2 + c]; };
+  T y[D];
+  T r2 = T(0);
+  for (int r = 0; r < d; ++r) {
+    T v = x[r] - mu[r];
+    for (int c = 0; c < r; ++c) v -= L(r, c) * y[c];
+    v /= L(r, r);
+    y[r] = v;
+    r2 += v * v;
+  }
+  if (!isfinite(r2)) {
+    r2 = T(0);
+    for (int r = 0; r < d; ++r) {
+      T v = x[r] - mu[r];
+      for (int c = 0; c < r; ++c) v -= L(r, c) * y[c];
+      v /= L(r, r);
+      if (!isfinite(v) && isfinite(x[r]) && isfinite(mu[r])) {
+        RangeVal<T> val = range_subtract(range_of(x[r]), range_of(mu[r]));
+        bool done = false;
+        for (int c = 0; c < r && !done; ++c) {
+          if (!isfinite(y[c])) {
+            v = y[c];  // overflow of a Gaussian tail stays an overflow
+            done = true;
+          } else {
+            val = range_subtract(val, range_multiply(range_of(y[c]), L(r, c)));
+          }
+        }
+        if (!done) {
+          const RangeVal<T> q = range_divide(val, L(r, r));
+          v = rv_ldexp(q.m, q.e);
+        }
+      }
+      y[r] = v;
+      r2 += v * v;
+    }
+  }
+  const T lognorm = s[slot_lognorm_off(d)];
+  if (!fam.student) {
+    T half = T(0.5) * r2;
+    if (!isfinite(r2)) {
+      half = T(0);
+      for (int r = 0; r < d; ++r) half += (T(0.5) * y[r]) * y[r];
+    }
+    return lognorm - half;
+  }
+  T mant[D];
+  int expo[D];
+  T total = -T(__builtin_huge_val());
+  const T ln2 = T(0.6931471805599453);
+  for (int r = 0; r < d; ++r) {
+    RangeVal<T> val = range_subtract(range_of(x[r]), range_of(mu[r]));
+    for (int c = 0; c < r; ++c) val = range_subtract(val, range_multiply(RangeVal<T>{mant[c], expo[c]}, L(r, c)));
+    const RangeVal<T> q = range_divide(val, L(r, r));
+    mant[r] = q.m;
+    expo[r] = q.e;
+    const T logabs = log(fabs(q.m)) + T(q.e) * ln2;
+    total = logaddexp(total, T(2) * logabs);
+  }
+  return lognorm - fam.half_dof_plus_d * log1pexp(total - log(fam.dof));
+}
+
+// The inverse-diagonal entry of a packed slot: 1/L_rr when that is a normal number
+// in T, else NaN, which sends every density of the slot to slot_logdensity_range
+// (the multiply-by-reciprocal form would lose the value there).
+template <class T>
+__host__ __device__ inline T slot_inv_value(double inv) {
+  const T v = (T)inv;
+  const T a = v < T(0) ? -v : v;
+  const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+  const T big = sizeof(T) == 8 ? T(1.7976931348623157e308) : T(3.40282347e38f);
+  return (a >= tiny && a <= big) ? v : T(__builtin_nan(""));
+}
+template <class T>
+__host__ __device__ inline T slot_inv_entry(double diag) {
+  return slot_inv_value<T>(1.0 / diag);
+}
+
 // Whitened squared radius r^2 = |L^{-1}(x - mu)|^2 by forward substitution
 // (src/native_execution.jl:346-384), then the radial log density.
 // P is `const T*` for per-lane slots, or a constant-address-space pointer
 // (cptr<T>) for a slot shared by the whole wave, whose values then arrive by
-// scalar loads and feed the FMAs as SGPR operands.
+// scalar loads and feed the FMAs as SGPR operands.  Samples whose r^2 is not
+// finite (a Gaussian), or not finite, subnormal or with r^2/nu not finite (a
+// Student-t), take slot_logdensity_range.
 template <class T, int D, class P>
 __device__ __forceinline__ T slot_logdensity(P s, int d, const Family<T>& fam, const T (&x)[D], T (&y)[D]) {
   T r2 = T(0);
@@ -454,7 +588,18 @@ __device__ __forceinline__ T slot_logdensity(P s, int d, const Family<T>& fam, c
       r2 += v * v;
     }
   }
-  return radial_logdensity<T, D>(fam, s[slot_lognorm_off(d)], r2, y, d);
+  const T lognorm = s[slot_lognorm_off(d)];
+  if (!fam.student) {
+    if (__builtin_expect(isfinite(r2), 1)) return lognorm - T(0.5) * r2;
+  } else {
+    const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+    const T ratio = r2 / fam.dof;
+    if (__builtin_expect(r2 >= tiny && isfinite(ratio), 1)) return lognorm - fam.half_dof_plus_d * log1p(ratio);
+  }
+  T xc[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) xc[c] = x[c];
+  return slot_logdensity_range<T, D>((const T*)s, d, fam, xc);
 }
 
 // Packed affine draw x = mu + L z (src/adaptive_gaussian_execution.jl:49-62).

@@ -901,6 +901,12 @@ int job_run_typed(is_job* j) {
       pack_slot(j->committed, d, 0.0, h64);
     }
     for (size_t i = 0; i < n; ++i) hT[i] = (T)h64[i];
+    // inverse diagonals that are not normal in T mark the slot for the extended-range density
+    for (size_t k = 0; k < n / (size_t)j->stride; ++k)
+      for (int r = 0; r < d; ++r) {
+        const size_t at = k * j->stride + d + r;
+        hT[at] = slot_inv_value<T>(h64[at]);
+      }
     HIP_TRY(hipMemcpyAsync(j->slots64.p, h64, n * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(j->slotsT.p, hT, n * sizeof(T), hipMemcpyHostToDevice, s));
     j->slots_resident = fixed_state;
@@ -2412,6 +2418,63 @@ int is_result_resample(is_result* result, const double* uniforms, int64_t count,
   return result->j->precision == IS_FP32
              ? result_resample_typed<float>(result, uniforms, count, method, ancestors, unweighted)
              : result_resample_typed<double>(result, uniforms, count, method, ancestors, unweighted);
+}
+
+}  // extern "C"
+
+namespace {
+// logdensityof(proposal, x) for n points (d x n column-major FP64), evaluated in T by
+// the density kernels' own slot code (the 64-coordinate instantiation, any d <= 64).
+template <class T>
+__global__ void k_proposal_logdensity(const T* __restrict__ slot, int d, Family<T> fam, const double* __restrict__ pts,
+                                      int64_t n, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T x[kMaxDim], y[kMaxDim];
+#pragma unroll
+  for (int c = 0; c < kMaxDim; ++c) x[c] = c < d ? (T)pts[i * d + c] : T(0);
+  out[i] = (double)slot_logdensity<T, kMaxDim>(slot, d, fam, x, y);
+}
+
+template <class T>
+int proposal_logdensity_typed(is_ctx* ctx, const HostProp& p, int family, double dof, int d, const double* points,
+                              int64_t n, double* out) {
+  std::vector<double> s64((size_t)slot_stride(d));
+  pack_slot(p, d, 0.0, s64.data());
+  std::vector<T> sT(s64.begin(), s64.end());
+  for (int r = 0; r < d; ++r) sT[(size_t)d + r] = slot_inv_value<T>(s64[(size_t)d + r]);
+  DevBuf slot, pts, res;
+  int rc;
+  if ((rc = slot.alloc(sT.size() * sizeof(T))) || (rc = pts.alloc((size_t)n * d * sizeof(double))) ||
+      (rc = res.alloc((size_t)n * sizeof(double))))
+    return rc;
+  HIP_TRY(locked_memcpy(slot.p, sT.data(), sT.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(pts.p, points, (size_t)n * d * sizeof(double), hipMemcpyHostToDevice));
+  Family<T> fam;
+  fam.student = family == IS_STUDENT_T;
+  fam.dof = (T)dof;
+  fam.half_dof_plus_d = (T)((dof + (double)d) / 2.0);
+  hipLaunchKernelGGL((k_proposal_logdensity<T>), dim3((unsigned)cdiv(n, 256)), dim3(256), 0, 0, slot.as<T>(), d, fam,
+                     pts.as<double>(), n, res.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(locked_memcpy(out, res.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  return IS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int is_proposal_logdensity(is_ctx* ctx, int32_t family, double dof, int32_t scale_kind, int32_t dim,
+                           const double* location, const double* scale, const double* points, int64_t n,
+                           int32_t precision, double* out) {
+  if (!ctx || !location || !scale || !points || !out || n < 1 || dim < 1 || dim > kMaxDim)
+    return fail(IS_EINVAL, "bad arguments");
+  HostProp p;
+  const std::string err = make_prop(family, dof, scale_kind, dim, location, scale, p);
+  if (!err.empty()) return fail(IS_EINVAL, "%s", err.c_str());
+  HIP_TRY(hipSetDevice(ctx->device));
+  return precision == IS_FP32 ? proposal_logdensity_typed<float>(ctx, p, family, dof, dim, points, n, out)
+                              : proposal_logdensity_typed<double>(ctx, p, family, dof, dim, points, n, out);
 }
 
 int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3, int64_t n,

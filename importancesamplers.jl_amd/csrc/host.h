@@ -107,7 +107,7 @@ inline void pack_slot(const HostProp& p, int d, double logmass, double* slot) {
   std::fill(slot, slot + stride, 0.0);
   for (int r = 0; r < d; ++r) {
     slot[r] = p.mu[r];
-    slot[d + r] = 1.0 / p.L[r + (size_t)r * d];
+    slot[d + r] = slot_inv_entry<double>(p.L[r + (size_t)r * d]);
     for (int c = 0; c <= r; ++c) slot[2 * d + r * (r + 1) / 2 + c] = p.L[r + (size_t)c * d];
   }
   slot[slot_lognorm_off(d)] = p.lognorm;
@@ -160,8 +160,6 @@ inline std::string make_prop(int family, double dof, int kind, int d, const doub
   }
   if (family == IS_STUDENT_T) {
     if (!(std::isfinite(dof) && dof > 0.0)) return "degrees of freedom must be finite and positive";
-    if (factor_exponent_bound(out.L, d) >= 511)
-      return "factor magnitudes need the exponent-range repair path, which this engine does not provide";
     out.lognorm = student_t_logconstant(dof, d) - logabsdet;
   } else {
     out.lognorm = gaussian_lognormalizer(d, logabsdet);

@@ -468,13 +468,13 @@ static __global__ __launch_bounds__(64) void k_finish(const double* __restrict__
     ok = isfinite(mu) && diag > 0.0;
     lg = log(fabs(diag));
     out[lane] = mu;
-    out[d + lane] = 1.0 / diag;
+    out[d + lane] = slot_inv_entry<double>(diag);
     if (tf) {
       of[lane] = (float)mu;
-      of[d + lane] = (float)(1.0 / diag);
+      of[d + lane] = slot_inv_entry<float>(diag);
     } else {
       od[lane] = mu;
-      od[d + lane] = 1.0 / diag;
+      od[d + lane] = slot_inv_entry<double>(diag);
     }
     const int base = 2 * d + lane * (lane + 1) / 2;
     for (int c = 0; c <= lane; ++c) {
@@ -625,13 +625,13 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
     ok = isfinite(mu) && diag > 0.0;
     lg = log(fabs(diag));
     out[lane] = mu;
-    out[d + lane] = idiag;
+    out[d + lane] = slot_inv_value<double>(idiag);
     if (tf) {
       of[lane] = (float)mu;
-      of[d + lane] = (float)idiag;
+      of[d + lane] = slot_inv_value<float>(idiag);
     } else {
       od[lane] = mu;
-      od[d + lane] = idiag;
+      od[d + lane] = slot_inv_value<double>(idiag);
     }
     const int base = 2 * d + lane * (lane + 1) / 2;
 #pragma unroll

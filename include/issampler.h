@@ -324,6 +324,13 @@ int is_functional_moments_device_noalloc(is_ctx* ctx, const void* logweights_dev
 int is_cdf(is_ctx* ctx, const double* logweights, int64_t n, double* cdf);
 int is_select(is_ctx* ctx, const double* cdf, const double* logweights, int64_t n, const double* uniforms,
               int64_t count, int32_t method, int64_t* ancestors);
+/* logdensityof(proposal, x) (DensityInterface, src/proposals.jl:24-55,603-669 with the
+ * extended-range repair of src/student_t_range.jl:55-153) of a native radial proposal at n
+ * points (d x n column-major), evaluated on the device in FP64 or FP32 by the sampling
+ * kernels' own density code; d <= 64. */
+int is_proposal_logdensity(is_ctx* ctx, int32_t family, double dof, int32_t scale_kind, int32_t dim,
+                           const double* location, const double* scale, const double* points, int64_t n,
+                           int32_t precision, double* out);
 /* Raw Philox4x64-10 words for counters (i, c1, c2, c3), i = 0..n-1 (RNG known-answer seam). */
 int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3,
                int64_t n, uint64_t* out);

@@ -138,8 +138,10 @@ inline double standardized_difference(double sample, double location, double sca
   return sample / scale - location / scale;
 }
 
-// _radial_logdensity: src/proposals.jl:24-55 (range-risk branch excluded:
-// factors with binade exponent bound >= 511 are rejected before execution).
+// _radial_logdensity: src/proposals.jl:24-55 for the ordinary range; the range-risk
+// branch and the extended-range log radius (src/student_t_range.jl) are not restated
+// here: the engine's extended-range densities are pinned by closed forms evaluated in
+// mpmath (the reference's BigFloat known answers, test/native_proposals.jl:205-306).
 inline double radial_logdensity(const Prop& p, double r2, const double* coords) {
   const int d = p.d;
   if (p.family == 0) {
@@ -169,7 +171,7 @@ inline double radial_logdensity(const Prop& p, double r2, const double* coords) 
   } else {
     double s = 0.0;
     for (int c = 0; c < d; ++c) s = std::max(s, std::fabs(coords[c]));
-    if (!std::isfinite(s)) return kNaN;  // exponent-range path: out of scope
+    if (!std::isfinite(s)) return kNaN;  // the extended-range path (not restated)
     double scaled = 0.0;
     for (int c = 0; c < d; ++c) {
       double v = coords[c] / s;
