@@ -21,6 +21,7 @@ from .api import (  # noqa: F401
     ImportanceSampling,
     InvalidTransformError,
     LinearRegressionTarget,
+    LocalResampling,
     LogisticRegressionTarget,
     MultinomialResampling,
     NPMCRoundError,

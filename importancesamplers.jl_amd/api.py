@@ -476,7 +476,17 @@ class SystematicResampling:
 
 
 class GlobalResampling:
-    pass
+    """Every next-round location from the complete weighted round (src/methods/dm_pmc.jl:12-17)."""
+
+    name = "global"
+
+
+class LocalResampling:
+    """Each next-round location from its own proposal's weighted sample group
+    (src/methods/dm_pmc.jl:19-24, src/dm_pmc_execution.jl:80-265); one uniform per proposal."""
+
+    name = "local"
+    code = 2
 
 
 @dataclass
@@ -485,14 +495,18 @@ class DeterministicMixturePMC:
     rounds: int
     round_size: object
     resampling: object = None
-    method: object = None  # MultinomialResampling (default) or SystematicResampling
+    method: object = None  # GlobalResampling's selection: MultinomialResampling (default) or SystematicResampling
 
     def __post_init__(self):
         self.schedule = _schedule(self.rounds, self.round_size)
-        if self.resampling is not None and not isinstance(self.resampling, GlobalResampling):
-            raise NotImplementedError("only GlobalResampling is provided by this engine")
+        if self.resampling is None:
+            self.resampling = GlobalResampling()
+        if not isinstance(self.resampling, (GlobalResampling, LocalResampling)):
+            raise ValueError("resampling must be GlobalResampling() or LocalResampling()")
         if self.method is None:
             self.method = MultinomialResampling()
+        if isinstance(self.resampling, LocalResampling) and not isinstance(self.method, MultinomialResampling):
+            raise ValueError("LocalResampling draws one multinomial ancestor per proposal")
 
 
 # ---------------------------------------------------------------- RNG seams
@@ -615,7 +629,8 @@ def _job_desc(rng, target, algorithm, precision, transform=None):
                                                for q in bank.proposals]))
         desc.bank_masses = arr(bank.masses)
         if desc.method == DMPMC_M:
-            desc.resampling = algorithm.method.code
+            local = isinstance(algorithm.resampling, LocalResampling)
+            desc.resampling = LocalResampling.code if local else algorithm.method.code
         else:
             scheme = algorithm.mis_scheme
             desc.mis_scheme = scheme.code
@@ -834,7 +849,7 @@ class PreparedSampler:
             diag["clipped_counts"] = [math.isqrt(v) for v in self.schedule]
         if method == DMPMC_M:
             prov["proposal_id"] = self._get("is_job_get_proposal_ids", n, np.int64)
-            diag["resampling"] = "global"
+            diag["resampling"] = self.algorithm.resampling.name
         if method == MIS_M:
             prov = {"proposal_id": self._get("is_job_get_proposal_ids", n, np.int64)}
             diag["mis_scheme"] = self.algorithm.mis_scheme.name

@@ -64,7 +64,7 @@ def units():
             units.append((os.path.join(CSRC, "batch_k.hip"), os.path.join(OBJDIR, f"batch_k{fp}_w{w}.o"),
                           [f"-DIS_BATCH_FP={fp}", f"-DIS_BATCH_W={w}"]))
     for d in dims():
-        for part in range(4):
+        for part in range(6 if d == 64 else 4):  # parts 4, 5: the extended-range kernels
             units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}_p{part}.o"),
                           [f"-DIS_DIM={d}", f"-DIS_DIM_PART={part}"]))
     return units

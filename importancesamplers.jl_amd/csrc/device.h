@@ -484,16 +484,20 @@ __device__ inline RangeVal<T> range_divide(RangeVal<T> v, T divisor) {
 // src/student_t_range.jl:76-116); Gaussian: lognorm - r^2/2, or - sum (y/2) y when
 // r^2 overflows (src/proposals.jl:24-34); Student-t: the log radius in extended range
 // (_factor_range_logradius!, src/student_t_range.jl:137-153) through
-// log1pexp(log r^2 - log nu) (src/proposals.jl:36-55).  Out of line: it runs only for
-// samples (or slots) at the edges of the floating-point range.
+// log1pexp(log r^2 - log nu) (src/proposals.jl:36-55).  It runs only for samples (or
+// slots) at the edges of the floating-point range: its loops stay rolled and its arrays
+// live in private memory (volatile), so it adds few registers to the hot loops that
+// inline it.
 template <class T, int D>
-__device__ __noinline__ T slot_logdensity_range(const T* s, int d, Family<T> fam, const T* x) {
+__device__ __forceinline__ T slot_logdensity_range(const T* s, int d, const Family<T>& fam, const T (&x)[D]) {
   const T* mu = s;
   auto L = [&](int r, int c) { return s[2 * d + r * (r + 1) / 2 + c]; };
-  T y[D];
+  volatile T y[D];  // private memory: no registers taken from the hot loops
   T r2 = T(0);
+#pragma nounroll
   for (int r = 0; r < d; ++r) {
     T v = x[r] - mu[r];
+#pragma nounroll
     for (int c = 0; c < r; ++c) v -= L(r, c) * y[c];
     v /= L(r, r);
     y[r] = v;
@@ -501,13 +505,16 @@ __device__ __noinline__ T slot_logdensity_range(const T* s, int d, Family<T> fam
   }
   if (!isfinite(r2)) {
     r2 = T(0);
+#pragma nounroll
     for (int r = 0; r < d; ++r) {
       T v = x[r] - mu[r];
+#pragma nounroll
       for (int c = 0; c < r; ++c) v -= L(r, c) * y[c];
       v /= L(r, r);
       if (!isfinite(v) && isfinite(x[r]) && isfinite(mu[r])) {
         RangeVal<T> val = range_subtract(range_of(x[r]), range_of(mu[r]));
         bool done = false;
+#pragma nounroll
         for (int c = 0; c < r && !done; ++c) {
           if (!isfinite(y[c])) {
             v = y[c];  // overflow of a Gaussian tail stays an overflow
@@ -530,16 +537,19 @@ __device__ __noinline__ T slot_logdensity_range(const T* s, int d, Family<T> fam
     T half = T(0.5) * r2;
     if (!isfinite(r2)) {
       half = T(0);
+#pragma nounroll
       for (int r = 0; r < d; ++r) half += (T(0.5) * y[r]) * y[r];
     }
     return lognorm - half;
   }
-  T mant[D];
-  int expo[D];
+  volatile T mant[D];
+  volatile int expo[D];
   T total = -T(__builtin_huge_val());
   const T ln2 = T(0.6931471805599453);
+#pragma nounroll
   for (int r = 0; r < d; ++r) {
     RangeVal<T> val = range_subtract(range_of(x[r]), range_of(mu[r]));
+#pragma nounroll
     for (int c = 0; c < r; ++c) val = range_subtract(val, range_multiply(RangeVal<T>{mant[c], expo[c]}, L(r, c)));
     const RangeVal<T> q = range_divide(val, L(r, r));
     mant[r] = q.m;
@@ -572,8 +582,10 @@ __host__ __device__ inline T slot_inv_entry(double diag) {
 // (cptr<T>) for a slot shared by the whole wave, whose values then arrive by
 // scalar loads and feed the FMAs as SGPR operands.  Samples whose r^2 is not
 // finite (a Gaussian), or not finite, subnormal or with r^2/nu not finite (a
-// Student-t), take slot_logdensity_range.
-template <class T, int D, class P>
+// Student-t), need slot_logdensity_range: the RANGE instantiations compute it; the
+// others return NaN, which fails the run, and the host replays such a run on the
+// RANGE kernels (their private arrays would cost the hot kernels occupancy).
+template <class T, int D, bool RANGE = false, class P>
 __device__ __forceinline__ T slot_logdensity(P s, int d, const Family<T>& fam, const T (&x)[D], T (&y)[D]) {
   T r2 = T(0);
 #pragma unroll
@@ -596,10 +608,11 @@ __device__ __forceinline__ T slot_logdensity(P s, int d, const Family<T>& fam, c
     const T ratio = r2 / fam.dof;
     if (__builtin_expect(r2 >= tiny && isfinite(ratio), 1)) return lognorm - fam.half_dof_plus_d * log1p(ratio);
   }
-  T xc[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) xc[c] = x[c];
-  return slot_logdensity_range<T, D>((const T*)s, d, fam, xc);
+  if constexpr (RANGE) {
+    return slot_logdensity_range<T, D>((const T*)s, d, fam, x);
+  } else {
+    return T(__builtin_nan(""));
+  }
 }
 
 // Packed affine draw x = mu + L z (src/adaptive_gaussian_execution.jl:49-62).

@@ -7,7 +7,7 @@
 
 namespace issampler {
 
-template <class T, int D, int MODE>
+template <class T, int D, int MODE, bool RG = false>
 void launch_new_mode(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s) {
   const bool st = a.fam.student != 0;
   W = std::min(W, new_block_limit(D, (int)sizeof(T), st) / 64);
@@ -17,22 +17,40 @@ void launch_new_mode(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s
   a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / slot_bytes));
   const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
   if (MODE == kPreTarget)
-    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+    hipLaunchKernelGGL((k_new<T, D, MODE, false, RG>), dim3(blocks), dim3(64 * W), lds, s, a);
   else if (st)
-    hipLaunchKernelGGL((k_new<T, D, MODE, true>), dim3(blocks), dim3(64 * W), lds, s, a);
+    hipLaunchKernelGGL((k_new<T, D, MODE, true, RG>), dim3(blocks), dim3(64 * W), lds, s, a);
   else
-    hipLaunchKernelGGL((k_new<T, D, MODE, false>), dim3(blocks), dim3(64 * W), lds, s, a);
+    hipLaunchKernelGGL((k_new<T, D, MODE, false, RG>), dim3(blocks), dim3(64 * W), lds, s, a);
 }
 
-template <class T, int D>
+template <class T, int D, bool RG = false>
 void launch_new_any(int mode, const RoundArgs<T>& a, int ncomp, int blocks, int W, hipStream_t s) {
   if (mode == kDrawFused)
-    launch_new_mode<T, D, kDrawFused>(a, ncomp, blocks, W, s);
+    launch_new_mode<T, D, kDrawFused, RG>(a, ncomp, blocks, W, s);
   else if (mode == kDrawOnly)
-    launch_new_mode<T, D, kDrawOnly>(a, ncomp, blocks, W, s);
+    launch_new_mode<T, D, kDrawOnly, RG>(a, ncomp, blocks, W, s);
   else
-    launch_new_mode<T, D, kPreTarget>(a, ncomp, blocks, W, s);
+    launch_new_mode<T, D, kPreTarget, RG>(a, ncomp, blocks, W, s);
 }
+
+// The extended-range sweep: the 64-coordinate instantiation for any d <= 64, so
+// every moment-tile count 0..5 is instantiated.
+template <class T>
+void launch_sweep_range(int nt, const SweepArgs<T>& a, int blocks, int nw, size_t lds, hipStream_t s) {
+  switch (nt) {
+#define IS_RANGE_SWEEP(v)                                                                             \
+  case v:                                                                                           \
+    hipLaunchKernelGGL((k_sweep<T, kMaxDim, v, true>), dim3(blocks), dim3(64 * nw), lds, s, a); \
+    break;
+    IS_RANGE_SWEEP(0) IS_RANGE_SWEEP(1) IS_RANGE_SWEEP(2) IS_RANGE_SWEEP(3) IS_RANGE_SWEEP(4) IS_RANGE_SWEEP(5)
+#undef IS_RANGE_SWEEP
+    default:
+      break;
+  }
+}
+
+
 
 // moment tile counts an instantiation serves: d in the bucket of D, d + 1 values
 constexpr int nt_lo(int D) { return D <= kExactDims ? (D + 16) / 16 : (D == 24 ? 2 : D == 32 ? 2 : D == 48 ? 3 : 4); }

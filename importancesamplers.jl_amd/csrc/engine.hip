@@ -144,6 +144,7 @@ struct is_job {
   DevBuf pre_normals, pre_rn, pre_ru, pre_uniforms;
   std::vector<int> prefilled, prefilled_radial, prefilled_uniforms, prefilled_ancestors;
   DevBuf pre_ancestors;  // DM-PMC: K x T given ancestors (1-based), replacing the resampling step
+  DevBuf plan_group;     // DM-PMC: per round, the K + 1 group boundaries of the samples by proposal
   // last run
   bool have_results = false;
   int last_status = 0;
@@ -161,6 +162,9 @@ struct is_job {
   bool has_tf = false;
   DevBuf tf_kinds_d, tf_bounds_d, XL;
   bool graph_dirty = true;
+  // replay runs on the extended-range kernels (set after a run's densities left the
+  // floating-point range; kept while runs succeed there)
+  bool range_mode = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // optional per-round event timing of the round kernel
@@ -223,7 +227,10 @@ template <class T, int MODE>
 void launch_mix(int D, int ncomp, const RoundArgs<T>& a, hipStream_t s) {
   const int W = new_waves(a.d, ncomp);
   const int blocks = (int)std::max<int64_t>(1, cdiv(a.n_new, 64));
-  with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::new_samples(MODE, a, ncomp, blocks, W, s); });
+  if (D == kRangeDims)
+    RangeKernels::new_samples(MODE, a, ncomp, blocks, W, s);
+  else
+    with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::new_samples(MODE, a, ncomp, blocks, W, s); });
 }
 
 // Launches one sweep over [a.lo, a.hi); returns the number of partial records.
@@ -233,7 +240,10 @@ int launch_sweep(int D, int nt, SweepArgs<T> a, hipStream_t s) {
   const int blocks = (int)std::max<int64_t>(1, cdiv(a.hi - a.lo, kSweepPerBlock));
   a.per_block = kSweepPerBlock;
   const size_t lds = sweep_lds(nt, nw);
-  with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::sweep(nt, a, blocks, nw, lds, s); });
+  if (D == kRangeDims)
+    RangeKernels::sweep(nt, a, blocks, nw, lds, s);
+  else
+    with_dim(D, [&](auto k) { DimKernels<decltype(k)::value>::sweep(nt, a, blocks, nw, lds, s); });
   return blocks;
 }
 
@@ -545,7 +555,9 @@ template <class T>
 int run_rounds(is_job* j) {
   hipStream_t s = j->stream;
   const int d = j->d;
-  const int D = j->D;
+  // the extended-range kernels replay a run whose densities left the floating-point
+  // range (no fused round among them: the round falls back to new-sample + sweep)
+  const int D = j->range_mode ? kRangeDims : j->D;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
   const int nt_mom = sweep_nt(d, true);
   unsigned int* counter = j->counter.as<unsigned int>();
@@ -597,7 +609,7 @@ int run_rounds(is_job* j) {
     ta.rows_per_split = cdiv(cdiv(j->tn, splits), 64) * 64;
     ta.part = j->tpart.as<double>();
     const int64_t used = cdiv(j->tn, ta.rows_per_split);
-    launch_target<T>(D, ta, used, s);
+    launch_target<T>(j->D, ta, used, s);
     hipLaunchKernelGGL((k_target_combine<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, ta.part, (int)used,
                        a.n_new, ta.beta, ta.cap, ta.first, d, j->t_inv_two_prior, j->t_const - j->t_shift,
                        j->tpre.as<T>());
@@ -832,7 +844,8 @@ int run_rounds(is_job* j) {
       f.summary_count = N;
       f.summary_out = summ;
       reduce(j->partials, nrec, 2, j->reduced, f);
-      // global resampling (src/dm_pmc_execution.jl:267-295)
+      // resampling: global (src/dm_pmc_execution.jl:267-295) from the round's CDF, or
+      // local (:297-374), each proposal from its own group
       const T* lw = j->logweights.as<T>() + first;
       int32_t* anc = j->ancestors.as<int32_t>() + (size_t)t * j->K;
       if (j->prefilled_ancestors[t]) {
@@ -840,7 +853,8 @@ int run_rounds(is_job* j) {
         hipLaunchKernelGGL((k_copy<int32_t>), dim3(cdiv(j->K, 64)), dim3(64), 0, s,
                            (const int32_t*)(j->pre_ancestors.as<int32_t>() + (size_t)t * j->K), (int64_t)j->K, anc);
       } else {
-        launch_cdf<T>(lw, N, summ, j->tile_total.as<double>(), j->cdf.as<double>(), j->cdf_status.as<int>() + t, s);
+        if (j->resampling != IS_LOCAL)
+          launch_cdf<T>(lw, N, summ, j->tile_total.as<double>(), j->cdf.as<double>(), j->cdf_status.as<int>() + t, s);
         double* u = j->uniforms.as<double>() + (size_t)t * j->K;
         if (j->prefilled_uniforms[t]) {
           u = j->pre_uniforms.as<double>() + (size_t)t * j->K;
@@ -848,8 +862,13 @@ int run_rounds(is_job* j) {
           hipLaunchKernelGGL(k_resampling_uniforms, dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->key,
                              j->run_dev.as<unsigned long long>(), t, (int64_t)j->K, u);
         }
-        hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->cdf.as<double>(), lw,
-                           N, u, (int64_t)j->K, j->resampling == IS_SYSTEMATIC ? 1 : 0, anc);
+        if (j->resampling == IS_LOCAL)
+          hipLaunchKernelGGL((k_local_select<T>), dim3(j->K), dim3(kLocalLanes), 0, s, lw,
+                             (const int32_t*)(j->plan_group.as<int32_t>() + (size_t)t * (j->K + 1)), (const double*)u,
+                             anc, j->cdf_status.as<int>() + t);
+        else
+          hipLaunchKernelGGL((k_select_ancestors<T>), dim3(cdiv(j->K, 64)), dim3(64), 0, s, j->cdf.as<double>(),
+                             lw, N, u, (int64_t)j->K, j->resampling == IS_SYSTEMATIC ? 1 : 0, anc);
       }
       hipLaunchKernelGGL((k_gather_locations<T>), dim3(cdiv((int64_t)j->K * d, 64)), dim3(64), 0, s,
                          j->X.as<T>(), j->total, first, d, anc, j->K, j->slots64.as<double>(), j->slotsT.as<T>(),
@@ -988,6 +1007,21 @@ int job_run_typed(is_job* j) {
     }
   }
   j->failure = f;
+  if (f.round != 0 && !j->range_mode && d <= kMaxDim &&
+      (f.phase == IS_PHASE_DENOMINATOR || f.phase == IS_PHASE_WEIGHT || f.phase == IS_PHASE_TARGET)) {
+    // a density outside the ordinary floating-point range gives NaN in the regular
+    // kernels: replay the run (same counters, same committed state) on the
+    // extended-range kernels; the replay's outcome is the run's
+    j->range_mode = true;
+    j->graph_dirty = true;
+    j->run_index -= 1;
+    const int rc2 = job_run_typed<T>(j);
+    if (rc2 != IS_OK) {
+      j->range_mode = false;  // a failure of its own: later runs start on the regular kernels
+      j->graph_dirty = true;
+    }
+    return rc2;
+  }
   if (f.round != 0) {
     j->have_results = false;
     j->last_status = IS_EFAILED;
@@ -1177,6 +1211,8 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   j->T = T;
   j->dof = desc->dof;
   j->resampling = desc->resampling;
+  if (desc->method == IS_DMPMC && (desc->resampling < IS_MULTINOMIAL || desc->resampling > IS_LOCAL))
+    return fail(IS_EINVAL, "unknown resampling %d", desc->resampling);
   j->tkind = desc->target_kind;
   j->tbatch = desc->target_batch;
   j->tuser = desc->target_user;
@@ -1273,6 +1309,13 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
       int rc;
       if ((rc = j->assign.alloc(assign.size() * sizeof(int32_t)))) return rc;
       HIP_TRY(locked_memcpy(j->assign.p, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      std::vector<int32_t> groups((size_t)T * (K + 1), 0);
+      for (int t = 0; t < T; ++t)
+        for (int k = 0; k < K; ++k)
+          groups[(size_t)t * (K + 1) + k + 1] =
+              groups[(size_t)t * (K + 1) + k] + (int32_t)j->plan_counts[(size_t)t * K + k];
+      if ((rc = j->plan_group.alloc(groups.size() * sizeof(int32_t)))) return rc;
+      HIP_TRY(locked_memcpy(j->plan_group.p, groups.data(), groups.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     int rc;
     if ((rc = j->bank_ids_d.alloc(j->bank_ids.size() * sizeof(int32_t)))) return rc;
@@ -2433,7 +2476,7 @@ __global__ void k_proposal_logdensity(const T* __restrict__ slot, int d, Family<
   T x[kMaxDim], y[kMaxDim];
 #pragma unroll
   for (int c = 0; c < kMaxDim; ++c) x[c] = c < d ? (T)pts[i * d + c] : T(0);
-  out[i] = (double)slot_logdensity<T, kMaxDim>(slot, d, fam, x, y);
+  out[i] = (double)slot_logdensity<T, kMaxDim, true>(slot, d, fam, x, y);
 }
 
 template <class T>

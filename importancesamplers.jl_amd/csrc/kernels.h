@@ -93,7 +93,7 @@ __device__ inline void pair_index(int p, int& r, int& c) {
   c = p - r * (r + 1) / 2;
 }
 
-template <class T, int D, class P>
+template <class T, int D, bool RANGE = false, class P>
 __device__ __forceinline__ T inline_target(int tkind, int d, P tslot, const T (&x)[D], T (&y)[D]) {
   if (tkind == kTStandard) {
     T s = T(0);
@@ -107,7 +107,7 @@ __device__ __forceinline__ T inline_target(int tkind, int d, P tslot, const T (&
   g.student = 0;
   g.dof = T(0);
   g.half_dof_plus_d = T(0);
-  return slot_logdensity<T, D>(tslot, d, g, x, y);
+  return slot_logdensity<T, D, RANGE>(tslot, d, g, x, y);
 }
 
 #include "mix.h"
@@ -931,8 +931,92 @@ __global__ void k_select_ancestors(const double* __restrict__ cdf, const T* __re
   ancestors[k] = (int32_t)first;
 }
 
+// LocalResampling (src/dm_pmc_execution.jl:80-265): block k (kLocalLanes threads)
+// picks proposal k's ancestor from its own group of the round's samples,
+// [group[k], group[k + 1]).  The order is that of the reference's cooperative device
+// kernel (:173-246): the group maximum; lane l sums exp(lw - max) over its contiguous
+// chunk of ceil(count / kLocalLanes) samples; lane 0 adds the chunk totals in lane
+// order, draws u_k * total and finds the first chunk whose running total exceeds it
+// (else the last non-empty chunk, with its own total as remainder); that chunk's lane
+// returns the first sample whose running sum exceeds the remainder (else its last).
+// The exponentials are det_exp in FP64.  A group without positive finite weight gives
+// ancestor 0 and sets *status (AllZeroWeightsError in the resampling phase).
+constexpr int kLocalLanes = 256;
+template <class T>
+__global__ __launch_bounds__(kLocalLanes) void k_local_select(const T* __restrict__ lw,
+                                                              const int32_t* __restrict__ group,
+                                                              const double* __restrict__ uniforms,
+                                                              int32_t* __restrict__ ancestors,
+                                                              int* __restrict__ status) {
+  __shared__ double red[kLocalLanes];
+  __shared__ double chunk[kLocalLanes];
+  __shared__ int chosen_s;
+  __shared__ double thr_s;
+  const int k = blockIdx.x, l = threadIdx.x;
+  const int64_t g0 = group[k], g1 = group[k + 1];
+  const int64_t count = g1 - g0;
+  const double ninf = -__builtin_huge_val();
+  double m = ninf;
+  for (int64_t i = g0 + l; i < g1; i += kLocalLanes) m = fmax(m, (double)lw[i]);
+  red[l] = m;
+  __syncthreads();
+  for (int off = kLocalLanes / 2; off > 0; off >>= 1) {
+    if (l < off) red[l] = fmax(red[l], red[l + off]);
+    __syncthreads();
+  }
+  const double M = red[0];
+  const int64_t cs = (count + kLocalLanes - 1) / kLocalLanes;
+  const int64_t c0 = g0 + (int64_t)l * cs;
+  const int64_t c1 = c0 + cs < g1 ? c0 + cs : g1;
+  double tot = 0.0;
+  if (M != ninf)
+    for (int64_t i = c0; i < c1; ++i) tot = tot + det_exp((double)lw[i] - M);
+  chunk[l] = tot;
+  __syncthreads();
+  if (l == 0) {
+    double total = 0.0;
+    for (int j = 0; j < kLocalLanes; ++j) total = total + chunk[j];
+    int chosen = -1;
+    double rem = 0.0;
+    if (isfinite(total) && total > 0.0) {
+      const double draw = uniforms[k] * total;
+      chosen = (int)(count < kLocalLanes ? count : kLocalLanes) - 1;
+      rem = chunk[chosen];
+      double prefix = 0.0;
+      for (int j = 0; j < kLocalLanes; ++j) {
+        const double next = prefix + chunk[j];
+        if (draw < next) {
+          chosen = j;
+          rem = draw - prefix;
+          break;
+        }
+        prefix = next;
+      }
+    }
+    chosen_s = chosen;
+    thr_s = rem;
+    if (chosen < 0) {
+      ancestors[k] = 0;
+      *status = 1;
+    }
+  }
+  __syncthreads();
+  if (l == chosen_s) {
+    double cum = 0.0;
+    int64_t anc = c1 - 1;
+    for (int64_t i = c0; i < c1; ++i) {
+      cum = cum + det_exp((double)lw[i] - M);
+      if (thr_s < cum) {
+        anc = i;
+        break;
+      }
+    }
+    ancestors[k] = (int32_t)(anc + 1);
+  }
+}
+
 // DM-PMC: new locations are the ancestors' samples (src/resampling.jl:120-164);
-// writes both slot tables.
+// writes both slot tables.  An ancestor of 0 (a failed local group) gathers nothing.
 template <class T>
 __global__ void k_gather_locations(const T* __restrict__ X, int64_t cap, int64_t first, int d,
                                    const int32_t* __restrict__ ancestors, int K, double* __restrict__ slots64,
@@ -940,6 +1024,7 @@ __global__ void k_gather_locations(const T* __restrict__ X, int64_t cap, int64_t
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= K * d) return;
   const int k = idx / d, c = idx % d;
+  if (ancestors[k] < 1) return;
   const int64_t a = first + ancestors[k] - 1;
   const T v = X[(int64_t)c * cap + a];
   slots64[(int64_t)k * stride + c] = (double)v;

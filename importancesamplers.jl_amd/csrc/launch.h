@@ -55,4 +55,15 @@ struct DimKernels {
   static void target(const TargetArgs<double>& t, int64_t splits, hipStream_t s);
 };
 
+// The new-sample and sweep kernels with the extended-range densities
+// (slot_logdensity<..., RANGE = true>), 64-coordinate instantiation serving any d; a
+// run whose densities left the floating-point range is replayed on these.
+constexpr int kRangeDims = 1000;  // pseudo dimension instantiation selecting them
+struct RangeKernels {
+  static void new_samples(int mode, const RoundArgs<float>& a, int ncomp, int blocks, int W, hipStream_t s);
+  static void new_samples(int mode, const RoundArgs<double>& a, int ncomp, int blocks, int W, hipStream_t s);
+  static void sweep(int nt, const SweepArgs<float>& a, int blocks, int nw, size_t lds, hipStream_t s);
+  static void sweep(int nt, const SweepArgs<double>& a, int blocks, int nw, size_t lds, hipStream_t s);
+};
+
 }  // namespace issampler

@@ -50,7 +50,7 @@ __host__ __device__ constexpr size_t new_lds_fixed(int D, int W, int tsize) {
 // The body of one new-sample block `blk` (samples 64 blk .. 64 blk + 63).  On
 // return, wave 0's active lanes hold their sample in out_x and its log-weight
 // (-inf when it failed) in out_lw; other lanes leave both untouched.
-template <class T, int D, int MODE, bool ST>
+template <class T, int D, int MODE, bool ST, bool RG = false>
 __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T& out_lw, T (&out_x)[D]) {
   extern __shared__ double lds[];
   constexpr int ZS = D + 1;
@@ -270,10 +270,10 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
         record_failure(a.fail, j + 1, tblock, treason);
         ok = false;
       } else {
-        tv = inline_target<T, D>(a.tkind, d, tslot, xl, y) + lj;
+        tv = inline_target<T, D, RG>(a.tkind, d, tslot, xl, y) + lj;
       }
     } else {
-      tv = inline_target<T, D>(a.tkind, d, tslot, x, y);
+      tv = inline_target<T, D, RG>(a.tkind, d, tslot, x, y);
     }
     const uint16_t tr = target_reason(tv);
     if (tr) {
@@ -295,7 +295,7 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
       }
       for (int mi = mb; mi < me; ++mi) {
         const int k = a.mis_terms == 2 ? a.mis_gmem[mi] : gen;
-        const T q = slot_logdensity<T, D>(a.slots + (int64_t)k * a.stride, d, a.fam, x, y);
+        const T q = slot_logdensity<T, D, RG>(a.slots + (int64_t)k * a.stride, d, a.fam, x, y);
         if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
         const T v = (a.mis_terms == 2 ? a.mis_gcoef[mi] : T(0)) + q;
         if (v > m) {
@@ -320,7 +320,7 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
         int kk = base + ((wave + W - 1 - base % W) % W + W) % W;
         for (; kk < base + cnt; kk += W) {
           const int k = a.k0 + kk;
-          const T q = slot_logdensity<T, D>(comp + (kk - base) * a.stride, d, a.fam, x, y);
+          const T q = slot_logdensity<T, D, RG>(comp + (kk - base) * a.stride, d, a.fam, x, y);
           if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
           const T v = a.logmass[k] + q;
           if (v > m) {
@@ -376,11 +376,11 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
   for (int c = 0; c < D; ++c) out_x[c] = x[c];
 }
 
-template <class T, int D, int MODE, bool ST>
+template <class T, int D, int MODE, bool ST, bool RG = false>
 __global__ __launch_bounds__(new_block_limit(D, sizeof(T), ST)) void k_new(RoundArgs<T> a) {
   T lw;
   T x[D];
-  new_block<T, D, MODE, ST>(a, blockIdx.x, lw, x);
+  new_block<T, D, MODE, ST, RG>(a, blockIdx.x, lw, x);
 }
 
 // ---------------------------------------------------------------- streaming sweep
@@ -612,7 +612,7 @@ __device__ __forceinline__ void sweep_epilogue(SweepAcc<T, NT>& st, int d, int n
 }
 
 // The body of sweep block `blk` over [a.lo + blk * per_block, ...).
-template <class T, int D, int NT>
+template <class T, int D, int NT, bool RG = false>
 __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) {
   extern __shared__ double lds[];
   constexpr int RS = sweep_row_stride(NT);
@@ -644,7 +644,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) 
     if (i < bend) {
       if (a.append && i < a.append_hi) {
         T yv[D];
-        const T q = slot_logdensity<T, D>(slot, d, a.fam, cur.x, yv);
+        const T q = slot_logdensity<T, D, RG>(slot, d, a.fam, cur.x, yv);
         uint16_t reason = (q != q) ? kProposalInvalid : 0;
         T num = cur.num;
         if (!reason) {
@@ -677,9 +677,9 @@ __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) 
   sweep_epilogue<T, NT>(st, d, a.nv, a.partials + blk, a.pcap, lds);
 }
 
-template <class T, int D, int NT>
+template <class T, int D, int NT, bool RG = false>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
-  sweep_block<T, D, NT>(a, blockIdx.x);
+  sweep_block<T, D, NT, RG>(a, blockIdx.x);
 }
 
 // One AMIS round in one launch (samples are disjoint between the two roles):

@@ -49,7 +49,11 @@ enum is_method {
 enum is_family { IS_GAUSSIAN = 0, IS_STUDENT_T = 1 };             /* GaussianFamily / StudentTFamily */
 enum is_scale_kind { IS_SPHERICAL = 0, IS_DIAGONAL = 1, IS_FACTOR = 2 }; /* src/proposals.jl:158-169 */
 enum is_precision { IS_FP64 = 0, IS_FP32 = 1 };
-enum is_resampling { IS_MULTINOMIAL = 0, IS_SYSTEMATIC = 1 };     /* MultinomialResampling; systematic is new */
+enum is_resampling {        /* DM-PMC ancestor selection (src/methods/dm_pmc.jl:10-26, src/resampling.jl:13) */
+  IS_MULTINOMIAL = 0,       /* GlobalResampling with MultinomialResampling                              */
+  IS_SYSTEMATIC = 1,        /* GlobalResampling, systematic points (new)                                */
+  IS_LOCAL = 2              /* LocalResampling: each proposal's ancestor from its own group             */
+};
 enum is_mis_scheme {       /* static MIS designs (src/methods/static_mis.jl:100-147)             */
   IS_MIS_STRATIFIED = 0,   /* StratifiedMixture: stratified assignment, full mixture             */
   IS_MIS_RANDOM = 1,       /* RandomMixture: independent assignment, full mixture                */

@@ -125,6 +125,8 @@ def _declare(L):
     L.or_cdf_blocked.argtypes = [F64P, i64, F64P, i, i]
     L.or_select_ancestors.restype = None
     L.or_select_ancestors.argtypes = [F64P, F64P, i64, F64P, i64, I64P]
+    L.or_local_select.restype = i
+    L.or_local_select.argtypes = [F64P, I64P, i, F64P, I64P, i]
     L.or_systematic_points.restype = None
     L.or_systematic_points.argtypes = [d, i64, F64P]
     L.or_adaptive.restype = i
@@ -452,6 +454,20 @@ def select_ancestors(cdf, lw, u):
     cdf, lw, u = _f64(cdf), _f64(lw), _f64(np.atleast_1d(u))
     anc = np.zeros(u.size, dtype=np.int64)
     lib().or_select_ancestors(_p(cdf), _p(lw), cdf.size, _p(u), u.size, _p(anc, I64P))
+    return anc
+
+
+def local_select(lw, group_sizes, u, lanes=256):
+    """LocalResampling: each group's 1-based round index (lanes 256: the engine's
+    cooperative order; 0: the reference CPU kernel's); raises ZeroDivisionError when a
+    group has no positive weight."""
+    lw = _f64(lw)
+    group = np.ascontiguousarray(np.concatenate([[0], np.cumsum(group_sizes)]).astype(np.int64))
+    K = len(group_sizes)
+    u = _f64(np.atleast_1d(u))
+    anc = np.zeros(K, dtype=np.int64)
+    if lib().or_local_select(_p(lw), _p(group, I64P), K, _p(u), _p(anc, I64P), int(lanes)):
+        raise ZeroDivisionError("a local group has no positive weight")
     return anc
 
 

@@ -850,6 +850,76 @@ int or_cdf_blocked(const double* lw, int64_t n, double* cdf, int threads, int pe
   return 0;
 }
 
+// LocalResampling ancestor of each proposal from its own group [group[k], group[k+1])
+// of the round's log-weights, 1-based round indices; returns 1 when a group has no
+// positive finite weight (AllZeroWeightsError).  lanes == 0: the CPU kernel's order
+// (src/dm_pmc_execution.jl:80-125: the group maximum, the sum of exp(lw - max), the
+// first running sum above u * total, else the group's last sample).  lanes > 0: the
+// cooperative device order (:173-246) with `lanes` chunks, restated as the engine
+// runs it, exponentials by det_exp.
+int or_local_select(const double* lw, const int64_t* group, int K, const double* u, int64_t* anc, int lanes) {
+  for (int k = 0; k < K; ++k) {
+    const int64_t g0 = group[k], g1 = group[k + 1], count = g1 - g0;
+    double m = -kInf;
+    for (int64_t i = g0; i < g1; ++i) m = std::max(m, lw[i]);
+    anc[k] = 0;
+    if (lanes == 0) {
+      if (m == -kInf) return 1;
+      double total = 0.0;
+      for (int64_t i = g0; i < g1; ++i) total += std::exp(lw[i] - m);
+      if (!(std::isfinite(total) && total > 0.0)) return 1;
+      const double threshold = u[k] * total;
+      double cumulative = 0.0;
+      anc[k] = g1;
+      for (int64_t i = g0; i < g1; ++i) {
+        cumulative += std::exp(lw[i] - m);
+        if (threshold < cumulative) {
+          anc[k] = i + 1;
+          break;
+        }
+      }
+      continue;
+    }
+    const int64_t cs = (count + lanes - 1) / lanes;
+    std::vector<double> chunk(lanes, 0.0);
+    for (int l = 0; l < lanes; ++l) {
+      const int64_t c0 = g0 + (int64_t)l * cs, c1 = std::min(c0 + cs, g1);
+      double tot = 0.0;
+      if (m != -kInf)
+        for (int64_t i = c0; i < c1; ++i) tot = tot + det_exp(lw[i] - m);
+      chunk[l] = tot;
+    }
+    double total = 0.0;
+    for (int l = 0; l < lanes; ++l) total = total + chunk[l];
+    if (!(std::isfinite(total) && total > 0.0)) return 1;
+    const double draw = u[k] * total;
+    int chosen = (int)std::min<int64_t>(count, lanes) - 1;
+    double rem = chunk[chosen];
+    double prefix = 0.0;
+    for (int l = 0; l < lanes; ++l) {
+      const double next = prefix + chunk[l];
+      if (draw < next) {
+        chosen = l;
+        rem = draw - prefix;
+        break;
+      }
+      prefix = next;
+    }
+    const int64_t c0 = g0 + (int64_t)chosen * cs, c1 = std::min(c0 + cs, g1);
+    double cum = 0.0;
+    int64_t a = c1 - 1;
+    for (int64_t i = c0; i < c1; ++i) {
+      cum = cum + det_exp(lw[i] - m);
+      if (rem < cum) {
+        a = i;
+        break;
+      }
+    }
+    anc[k] = a + 1;
+  }
+  return 0;
+}
+
 // Ancestor selection: src/resampling.jl:92-118 (binary search + zero-mass plateau fix-up).
 // Returns 1-based ancestors.
 void or_select_ancestors(const double* cdf, const double* lw, int64_t n, const double* u,
@@ -1307,8 +1377,26 @@ int or_dmpmc(int family, double dof, int kind, int d, int K, const double* locs0
       fail_info[1] = 1;
       return 1;
     }
-    // global resampling: src/dm_pmc_execution.jl:267-295.
     const double* lw = logweights + offset;
+    if (resampling >= 4) {
+      // local resampling: src/dm_pmc_execution.jl:297-374, each proposal's ancestor from
+      // its own group (codes 4: the engine's cooperative order, 5: the CPU kernel's)
+      std::vector<int64_t> group(K + 1, 0), anc(K);
+      for (int k = 0; k < K; ++k) group[k + 1] = group[k] + counts[k + (int64_t)t * K];
+      if (or_local_select(lw, group.data(), K, uniforms + (int64_t)t * K, anc.data(), resampling == 4 ? 256 : 0)) {
+        fail_info[0] = t + 1;
+        fail_info[1] = 7;
+        return 2;
+      }
+      for (int k = 0; k < K; ++k) {
+        if (ancestors_out) ancestors_out[(int64_t)t * K + k] = anc[k];
+        for (int c = 0; c < d; ++c) locs[(int64_t)k * d + c] = samples[(offset + anc[k] - 1) * d + c];
+      }
+      or_logweight_summary(lw, N, round_ess + t, round_logz + t);
+      offset += N;
+      continue;
+    }
+    // global resampling: src/dm_pmc_execution.jl:267-295.
     int bad = (resampling >= 2) ? or_cdf_blocked(lw, N, cdf.data(), cdf_threads, cdf_per)
                                 : or_cdf_sequential(lw, N, cdf.data());
     if (bad) {

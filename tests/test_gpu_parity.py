@@ -627,3 +627,69 @@ def test_dmpmc_ancestor_replay_seam(eng, oracle_mod):
     assert np.max(dlz) < 1e-4
     with pytest.raises(ValueError):
         b.set_ancestors([[0] * K])  # 1-based
+
+
+@pytest.mark.parametrize("d,K,schedule,precision", [(3, 6, [600, 900, 750], "fp64"), (5, 16, [4096] * 4, "fp64"),
+                                                    (4, 8, [2000] * 3, "fp32")])
+def test_dmpmc_local_resampling_parity(eng, oracle_mod, d, K, schedule, precision):
+    """LocalResampling against the oracle fed the same draws and uniforms: bit-exact
+    ancestors (each from its own proposal's group), log-weights."""
+    _say(f"case: DM-PMC local resampling d={d} K={K} {precision}")
+    rng = np.random.default_rng(100 + K)
+    bank = _dm_problem(eng, d, K, "factor", rng)
+    alg = eng.DeterministicMixturePMC(bank, rounds=len(schedule), round_size=schedule,
+                                      resampling=eng.LocalResampling())
+    tgt = eng.GaussianTarget(np.zeros(d), 2.0 * np.eye(d))
+    s = eng.prepare_sampler(6, tgt, alg, precision=precision)
+    normals, _, _ = _export_batches(s, schedule, d)
+    uniforms = np.array([s.export_randomness(t)["uniforms"] for t in range(len(schedule))])
+    res = eng.importance_sample_(s)
+    assert res.diagnostics["resampling"] == "local"
+    order = oracle_mod.active_order(list(bank.masses))
+    counts, logc = oracle_mod.allocation_plan([bank.masses[i] for i in order], schedule)
+    P = [bank.proposals[i] for i in order]
+    o = oracle_mod.dmpmc(0, 0.0, 2, d, np.array([p.location for p in P]), np.array([p.factor() for p in P]),
+                         [p.lognormalizer for p in P], [i + 1 for i in order], schedule, counts, logc, normals,
+                         uniforms, tgt.kind, tgt.params(d), resampling=4)
+    assert o["rc"] == 0
+    if precision == "fp64":
+        assert relerr(res.logweights, o["logweights"], 1.0) < RTOL
+        assert np.array_equal(s.ancestors(), o["ancestors"])
+    # every ancestor lies in its own proposal's group of the round
+    Ka = len(order)  # active (positive-mass) proposals
+    starts = np.concatenate([[0], np.cumsum(np.asarray(counts).reshape(Ka, -1)[:, 0])])
+    anc = s.ancestors()
+    for k in range(Ka):
+        assert starts[k] < anc[0, k] <= starts[k + 1]
+
+
+def test_dmpmc_local_resampling_known_answer_and_empty_group(eng):
+    """test/dm_pmc.jl:798-817 and :885-911 on the device."""
+    _say("case: DM-PMC local resampling known answer")
+    bank = eng.ProposalBank([eng.SphericalGaussian(-2.0, 1.0), eng.SphericalGaussian(2.0, 1.0)])
+    alg = eng.DeterministicMixturePMC(bank, rounds=1, round_size=4, resampling=eng.LocalResampling())
+    rng = eng.PrefilledRNG(normals=[[-1.0, 1.0, -1.0, 1.0]], uniforms=[[0.5, 0.5]])
+    s = eng.prepare_sampler(rng, eng.StandardGaussianTarget(), alg)
+    res = eng.importance_sample_(s)
+    assert list(res.samples.reshape(-1)) == [-3.0, -1.0, 1.0, 3.0]
+    locs = [p.location[0] for p in eng.current_proposal(s).proposals]
+    assert locs == [-1.0, 1.0]
+
+    def left_empty(values, samples, stride, dim, count, is_float, stream):
+        h = ctypes.CDLL("libamdhip64.so.7")
+        h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipStreamSynchronize(stream)
+        x = np.empty(count)
+        h.hipMemcpy(x.ctypes.data, samples, count * 8, 2)
+        v = np.where(x < 0, -np.inf, -0.5 * x * x)
+        h.hipMemcpy(values, v.ctypes.data, count * 8, 1)
+        return 0
+
+    rng = eng.PrefilledRNG(normals=[[-1.0, 1.0, -1.0, 1.0]], uniforms=[[0.5, 0.5]])
+    s = eng.prepare_sampler(rng, eng.BatchTarget(left_empty), alg)
+    initial = [p.location[0] for p in eng.current_proposal(s).proposals]
+    with pytest.raises(eng.DMPMCRoundError) as ei:
+        eng.importance_sample_(s)
+    assert ei.value.phase == "resampling" and isinstance(ei.value.cause, eng.AllZeroWeightsError)
+    assert [p.location[0] for p in eng.current_proposal(s).proposals] == initial

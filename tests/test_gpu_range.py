@@ -142,10 +142,10 @@ def test_ordinary_densities_match_the_oracle(eng, oracle_mod):
 
 
 def test_sampling_with_subnormal_factors(eng):
-    """Plain IS and a static mixture whose Student-t proposal and Gaussian target factors
-    are subnormal (2^-1040 scale): the reciprocal diagonal is not representable, so the
-    sampling kernels take the extended-range densities; every log-weight matches mpmath
-    evaluated at the returned samples."""
+    """Plain IS whose Student-t proposal and Gaussian target factors are subnormal
+    (2^-1040 scale): the reciprocal diagonal is not representable, so the regular kernels
+    give NaN densities and the engine replays the run on the extended-range kernels;
+    every log-weight matches mpmath evaluated at the returned samples."""
     _say("case: sampling with subnormal Student-t and Gaussian factors")
     d, k = 2, 2.0 ** -1040
     L = np.array([[1.0, 0.0], [0.25, 0.75]])

@@ -687,3 +687,43 @@ def test_logistic_target_against_log_expit(oracle_mod):
         assert abs(tv - direct) < 1e-10 * max(1.0, abs(direct))
         if scale == 0.0:
             assert abs(ll + n * np.log(2.0)) < 1e-9
+
+
+def test_local_resampling_known_answer_and_reference_test_oracle(oracle_mod):
+    """LocalResampling (src/dm_pmc_execution.jl:80-265): the known answer of
+    test/dm_pmc.jl:798-817 (bank N(-2,1), N(2,1); samples [-3,-1,1,3] from normals
+    [-1,1,-1,1]; target -x^2/2; uniforms [0.5, 0.5] -> locations [-1, 1]), the empty-group
+    failure of :885-911, and the reference test oracle's per-group normalised cumsum
+    (test/support/dm_pmc.jl:205-218) on random groups, in both the engine's cooperative
+    order and the CPU kernel's."""
+    o = oracle_mod
+    x = np.array([-3.0, -1.0, 1.0, 3.0])
+    lq = np.logaddexp(np.log(0.5) - 0.5 * (x + 2) ** 2, np.log(0.5) - 0.5 * (x - 2) ** 2) - 0.5 * math.log(2 * math.pi)
+    lw = -0.5 * x * x - lq
+    for lanes in (256, 0):
+        anc = o.local_select(lw, [2, 2], [0.5, 0.5], lanes)
+        assert list(x[anc - 1]) == [-1.0, 1.0]
+    left_empty = np.where(x < 0, -np.inf, lw)
+    for lanes in (256, 0):
+        with pytest.raises(ZeroDivisionError):
+            o.local_select(left_empty, [2, 2], [0.5, 0.5], lanes)
+    rng = np.random.default_rng(21)
+    for trial in range(40):
+        K = int(rng.integers(1, 9))
+        sizes = rng.integers(1, 700, K)
+        lw = rng.normal(0, 2, int(sizes.sum()))
+        lw[rng.random(lw.size) < 0.1] = -np.inf
+        starts = np.concatenate([[0], np.cumsum(sizes)])
+        for k in range(K):  # every group keeps one finite weight
+            if np.all(np.isinf(lw[starts[k]:starts[k + 1]])):
+                lw[starts[k]] = 0.0
+        u = rng.random(K)
+        expected = []
+        for k in range(K):  # the reference test oracle: normalised cumsum, cdf[end] = 1
+            g = lw[starts[k]:starts[k + 1]]
+            w = np.exp(g - np.logaddexp.reduce(g))
+            cdf = np.cumsum(w)
+            cdf[-1] = 1.0
+            expected.append(starts[k] + int(np.argmax(u[k] < cdf)) + 1)
+        for lanes in (256, 0):
+            assert list(o.local_select(lw, sizes, u, lanes)) == expected, (trial, lanes)
