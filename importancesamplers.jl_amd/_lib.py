@@ -32,6 +32,11 @@ BATCH_TARGET_FN = ctypes.CFUNCTYPE(ctypes.c_int, VP, VP, ctypes.c_int64, ctypes.
                                    ctypes.c_int32, VP)
 
 
+class TransformBlock(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_int32), ("count", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("scalar", ctypes.c_int32), ("lower", ctypes.c_double), ("upper", ctypes.c_double)]
+
+
 class JobDesc(ctypes.Structure):
     _fields_ = [
         ("method", ctypes.c_int32),
@@ -63,6 +68,8 @@ class JobDesc(ctypes.Structure):
         ("transform_bounds", F64P),
         ("target_batch", BATCH_TARGET_FN),
         ("target_user", VP),
+        ("transform_block_count", ctypes.c_int32),
+        ("transform_blocks", ctypes.POINTER(TransformBlock)),
     ]
 
 
@@ -116,6 +123,7 @@ SIGNATURES = {
     "is_job_set_graph": (ctypes.c_int, [VP, ctypes.c_int32]),
     "is_job_round_kernel_ms": (ctypes.c_int, [VP, F64P]),
     "is_job_total": (ctypes.c_int64, [VP]),
+    "is_job_logical_dim": (ctypes.c_int32, [VP]),
     "is_job_failure": (ctypes.c_int, [VP, ctypes.POINTER(Failure)]),
     "is_job_get_samples": (ctypes.c_int, [VP, F64P]),
     "is_job_get_logweights": (ctypes.c_int, [VP, F64P]),

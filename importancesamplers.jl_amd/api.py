@@ -332,6 +332,64 @@ class BatchTarget:
         return np.zeros(1)
 
 
+_hip_runtime = None
+
+
+def _hip():
+    global _hip_runtime
+    if _hip_runtime is None:
+        for name in ("libamdhip64.so.7", "libamdhip64.so"):
+            try:
+                h = ctypes.CDLL(name)
+                break
+            except OSError:
+                h = None
+        if h is None:
+            raise EngineError("the HIP runtime library is not loadable")
+        h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip_runtime = h
+    return _hip_runtime
+
+
+class HostBatchTarget(BatchTarget):
+    """A batch target evaluated on the host (LogTarget(f; batch=batch!), src/targets.jl:17-43,
+    src/batched_targets.jl:92-114): batch(out, xs) fills the float64 array out (one value
+    per sample) from the logical samples xs, a (count, dim) array, or under a named layout
+    a dict of the named leaves ((count,) for an int selector, (count, len) for a range), as
+    the reference's batch callbacks receive them.  The job's stream is synchronised, the
+    samples are copied down and the values up through the HIP runtime."""
+
+    def __init__(self, batch):
+        self.host_batch = batch
+        self._layout = None
+
+        def dev(values, samples, stride, dim, count, is_float, stream):
+            h = _hip()
+            if h.hipStreamSynchronize(stream) != 0:
+                return -1
+            dt = np.float32 if is_float else np.float64
+            sz = np.dtype(dt).itemsize
+            X = np.empty((dim, count), dt)
+            for c in range(dim):
+                if h.hipMemcpy(X[c].ctypes.data, samples + c * stride * sz, count * sz, 2) != 0:
+                    return -1
+            xs = np.ascontiguousarray(X.T)
+            if self._layout is not None:
+                xs = self._layout.split(xs)
+            out = np.full(count, np.nan)
+            rc = self.host_batch(out, xs)
+            vals = np.ascontiguousarray(out, dtype=dt)
+            if h.hipMemcpy(values, vals.ctypes.data, count * sz, 1) != 0:
+                return -1
+            return int(rc or 0)
+
+        super().__init__(dev)
+
+    def _bind_layout(self, layout):
+        self._layout = layout
+
+
 @dataclass
 class LogisticRegressionTarget:
     """Bayesian logistic regression with prior N(0, prior_var I)."""
@@ -580,10 +638,145 @@ class IntervalTransform:
         self.lower, self.upper = lo, hi
 
 
+class DimensionMismatch(ValueError):
+    """Julia's DimensionMismatch: a transform given the wrong number of coordinates."""
+
+
+class SimplexTransform:
+    """SimplexTransform(K) (src/transforms.jl:51-70): K - 1 unconstrained coordinates to K
+    positive weights that sum to one, through the orthonormal embedding into the sum-zero
+    logit subspace; log|J| = log(K)/2 + sum(log, x), measured against the first K - 1
+    simplex coordinates.  Evaluated on the device in the sampler's precision."""
+
+    code = 4
+    lower = -math.inf
+    upper = math.inf
+
+    def __init__(self, dimension):
+        if type(dimension) is not int or dimension < 2:
+            raise ValueError("simplex dimension must be an Int of at least two")
+        self.dimension = dimension
+
+    def __eq__(self, other):
+        return isinstance(other, SimplexTransform) and other.dimension == self.dimension
+
+    def __hash__(self):
+        return hash(("simplex", self.dimension))
+
+    def __repr__(self):
+        return f"SimplexTransform({self.dimension})"
+
+
+_SCALAR_TRANSFORMS = (PositiveTransform, SoftplusTransform, IntervalTransform)
+_TRANSFORMS = (IdentityTransform,) + _SCALAR_TRANSFORMS + (SimplexTransform,)
+
+
+def _flat_selector(sel, d):
+    """_validated_flat_selector (src/proposal_composition.jl:179-201) with Python's 0-based
+    indices: an int, or a nonempty contiguous range (a range or slice with step 1)."""
+    if isinstance(sel, slice):
+        if sel.step not in (None, 1) or sel.start is None or sel.stop is None:
+            raise ValueError("flat transform selectors must be ints or contiguous ranges")
+        sel = range(sel.start, sel.stop)
+    if isinstance(sel, (int, np.integer)) and not isinstance(sel, (bool, np.bool_)):
+        sel = int(sel)
+        if not 0 <= sel < d:
+            raise ValueError(f"flat transform selector {sel} is out of bounds for dimension {d}")
+        return sel
+    if isinstance(sel, range):
+        if sel.step != 1:
+            raise ValueError("flat transform selectors must be ints or contiguous ranges")
+        if len(sel) == 0:
+            raise ValueError("flat transform ranges must be nonempty")
+        if sel.start < 0 or sel.stop > d:
+            raise ValueError(f"flat transform selector {sel} is out of bounds for dimension {d}")
+        return sel
+    raise ValueError("flat transform selectors must be ints or contiguous ranges")
+
+
+class _Layout:
+    """A flat selector layout (_FlatTransformLayout, src/proposal_composition.jl:11-14,
+    203-240) of named blocks over the d sampling coordinates, or one SimplexTransform over
+    all of them (`names` None: results stay one (n, K) array).  `blocks` is the C
+    is_transform_block array in name order; `dl` the logical dimension."""
+
+    def __init__(self, spec, d):
+        if isinstance(spec, SimplexTransform):
+            if d != spec.dimension - 1:
+                raise DimensionMismatch(f"{spec!r} requires {spec.dimension - 1} coordinates")
+            self.names = None
+            items = [("", (range(0, d), spec))]
+        else:
+            if not spec:
+                raise ValueError("flat selector transforms must contain blocks")
+            self.names = [str(k) for k in spec]
+            items = list(spec.items())
+        selected = np.zeros(d, dtype=bool)
+        self.selectors, self.transforms, self.lengths, self.scalars = [], [], [], []
+        blocks = (_lib.TransformBlock * len(items))()
+        row = 0
+        for b, (name, value) in enumerate(items):
+            sel, tf = value if isinstance(value, tuple) else (value, IdentityTransform())
+            if not isinstance(tf, _TRANSFORMS):
+                raise ValueError("flat transform selector values must be sample transforms")
+            sel = _flat_selector(sel, d)
+            scalar = isinstance(sel, int)
+            idx = range(sel, sel + 1) if scalar else sel
+            if isinstance(tf, _SCALAR_TRANSFORMS) and not scalar:
+                raise DimensionMismatch(f"{type(tf).__name__} requires one unconstrained scalar")
+            if isinstance(tf, SimplexTransform) and (scalar or len(idx) != tf.dimension - 1):
+                raise DimensionMismatch(f"{tf!r} requires {tf.dimension - 1} coordinates")
+            if selected[idx.start:idx.stop].any():
+                raise ValueError("flat transform selectors must be disjoint")
+            selected[idx.start:idx.stop] = True
+            n_out = tf.dimension if isinstance(tf, SimplexTransform) else len(idx)
+            blk = blocks[b]
+            blk.first, blk.count, blk.kind, blk.scalar = idx.start, len(idx), tf.code, int(scalar)
+            blk.lower, blk.upper = float(tf.lower), float(tf.upper)
+            self.selectors.append(sel)
+            self.transforms.append(tf)
+            self.lengths.append(n_out)
+            self.scalars.append(scalar)
+            row += n_out
+        if not selected.all():
+            raise ValueError("flat transform selectors must collectively cover the base dimension")
+        self.blocks = blocks
+        self.dl = row
+        self.offsets = np.concatenate([[0], np.cumsum(self.lengths)]).astype(int)
+
+    def location(self, block):
+        """_native_failure_location (src/native_execution.jl:826-830): the failing
+        block's selector."""
+        return self.selectors[block - 1] if 1 <= block <= len(self.selectors) else None
+
+    def split(self, flat):
+        """(n, dl) logical samples -> {name: (n,) for an int selector, (n, len) for a range}
+        (_allocate_flat_samples, src/proposal_composition.jl:70-76)."""
+        if self.names is None:
+            return flat
+        out = {}
+        for b, name in enumerate(self.names):
+            o, k = self.offsets[b], self.lengths[b]
+            out[name] = flat[:, o].copy() if self.scalars[b] else flat[:, o:o + k].copy()
+        return out
+
+
+def _layout_of(transform, d):
+    """A transform= argument that needs the layout path (a dict of named blocks or a
+    SimplexTransform), else None."""
+    if isinstance(transform, _Layout):
+        return transform
+    if isinstance(transform, (dict, SimplexTransform)):
+        return _Layout(transform, d)
+    return None
+
+
 def _transform_arrays(transform, d):
     ts = list(transform) if isinstance(transform, (list, tuple)) else [transform] * d
     if len(ts) != d:
         raise ValueError("one transform per coordinate")
+    if any(not isinstance(t, _TRANSFORMS[:-1]) for t in ts):
+        raise ValueError("per-coordinate transforms are scalar sample transforms")
     kinds = np.asarray([t.code for t in ts], dtype=np.int32)
     bounds = np.asarray([[t.lower, t.upper] for t in ts], dtype=np.float64).ravel()
     return kinds, bounds
@@ -652,7 +845,12 @@ def _job_desc(rng, target, algorithm, precision, transform=None):
     if isinstance(target, BatchTarget):
         desc.target_batch = target._fn
         keep.append(target)
-    if transform is not None:
+    layout = _layout_of(transform, p.dim)
+    if layout is not None:
+        keep.append(layout)
+        desc.transform_block_count = len(layout.blocks)
+        desc.transform_blocks = ctypes.cast(layout.blocks, ctypes.POINTER(_lib.TransformBlock))
+    elif transform is not None:
         kinds, bounds = _transform_arrays(transform, p.dim)
         keep.extend([kinds, bounds])
         desc.transform_kinds = kinds.ctypes.data_as(I32P)
@@ -670,6 +868,14 @@ class PreparedSampler:
         self.precision = precision
         lib = _lib.load()
         ctx = context()
+        dim = (algorithm.bank.proposals[0] if isinstance(algorithm, DeterministicMixturePMC) else
+               algorithm.proposal.proposals[0] if isinstance(getattr(algorithm, "proposal", None), ProposalBank)
+               else algorithm.proposal).dim
+        self.layout = _layout_of(transform, dim)
+        if self.layout is not None:
+            transform = self.layout
+            if hasattr(target, "_bind_layout"):
+                target._bind_layout(self.layout)
         desc, keep, p, sched = _job_desc(rng, target, algorithm, precision, transform)
         h = VP()
         _check(lib.is_job_create(ctx, ctypes.byref(desc), ctypes.byref(h)))
@@ -681,6 +887,7 @@ class PreparedSampler:
         self._keep = keep
         self.proposal0 = p
         self.dim = p.dim
+        self.logical_dim = int(_lib.load().is_job_logical_dim(handle))
         self.scalar = bool(p.scalar)
         self.schedule = list(sched)
         self._job = handle
@@ -696,6 +903,7 @@ class PreparedSampler:
         """Wraps a job created elsewhere (the serving loop) with the same descriptor."""
         self = cls.__new__(cls)
         self.algorithm, self.target, self.precision = algorithm, target, precision
+        self.layout = None
         desc, keep, p, sched = _job_desc(rng, target, algorithm, precision)
         self._adopt(handle, desc, keep, p, sched)
         return self
@@ -807,11 +1015,11 @@ class PreparedSampler:
         else:
             index, bits = f.first_logical_index, f.reason_bits
         phase = PHASES.get(f.phase)
-        # transform reasons come with the failing coordinate in the block field
-        # (src/native_execution.jl:833-840)
-        block = f.draw_first_block if f.draw_first_logical_index else f.first_block
-        tcause = (InvalidTransformError(_transform_reason(bits), int(block) or None)
-                  if bits & 0x000F else None)
+        # transform reasons come with the failing coordinate (or layout block) in the
+        # block field (src/native_execution.jl:826-840)
+        block = int(f.draw_first_block if f.draw_first_logical_index else f.first_block)
+        where = self.layout.location(block) if self.layout is not None else (block or None)
+        tcause = InvalidTransformError(_transform_reason(bits), where) if bits & 0x000F else None
         if self.method == PLAIN:
             raise SamplerExecutionError(PLAIN_PHASES.get(f.phase, phase), index, bits, cause=tcause)
         if f.phase in (5, 7):
@@ -830,7 +1038,7 @@ class PreparedSampler:
         return out
 
     def collect(self):
-        d, n, T = self.dim, self.total, len(self.schedule)
+        d, n, T = self.logical_dim, self.total, len(self.schedule)
         samples = self._get("is_job_get_samples", n * d).reshape(n, d)
         logw = self._get("is_job_get_logweights", n)
         prov = {"round": self._get("is_job_get_round_ids", n, np.int64)}
@@ -855,7 +1063,9 @@ class PreparedSampler:
             diag["mis_scheme"] = self.algorithm.mis_scheme.name
             if isinstance(self.algorithm.mis_scheme, (StratifiedMixture, RandomMixture)):
                 diag["proposal_evaluations"] = n * len(self.active_ids)
-        if self.scalar:
+        if self.layout is not None:
+            samples = self.layout.split(samples)
+        elif self.scalar:
             samples = samples[:, 0].copy()
         return WeightedSamples(samples, logw, prov, diag, _sampler=self, _result=_ResultHandle.of_job(self))
 
@@ -938,17 +1148,18 @@ class _ResultHandle:
     device, which later runs of the sampler leave unchanged
     (src/methods/importance_sampling.jl:904-906); destroyed with its owner."""
 
-    def __init__(self, handle, dim, scalar):
+    def __init__(self, handle, dim, scalar, layout=None):
         self.h = handle
         self.dim = dim
         self.scalar = scalar
+        self.layout = layout
         self._lib = _lib.load()
 
     @classmethod
     def of_job(cls, sampler):
         h = VP()
         _check(sampler._lib.is_job_result(sampler._job, ctypes.byref(h)))
-        return cls(h, sampler.dim, sampler.scalar)
+        return cls(h, sampler.logical_dim, sampler.scalar, sampler.layout)
 
     def __del__(self):
         try:
@@ -990,6 +1201,8 @@ class UnweightedSamples:
         out = np.zeros(n * d)
         _check(r._lib.is_result_get_samples(r.h, _p(out)))
         out = out.reshape(n, d)
+        if r.layout is not None:
+            return r.layout.split(out)
         return out[:, 0].copy() if r.scalar else out
 
     def device_arrays(self):
@@ -1063,13 +1276,53 @@ def _functional_moments(f, result, want_var):
     return float(m[0]), float(v[0])
 
 
+def _rowwise_moments(result, want_var):
+    """Per-coordinate weighted mean and variance of the logical samples: each coordinate's
+    device row through the functional reduction (src/statistics.jl:186-197,204-213)."""
+    h = _handle(result)
+    sp, stride, isf = VP(), ctypes.c_int64(), ctypes.c_int32()
+    _check(h._lib.is_result_device_arrays(h.h, ctypes.byref(sp), ctypes.byref(stride), None, ctypes.byref(isf)))
+    size = 4 if isf.value else 8
+    m, v = np.zeros(h.dim), np.zeros(h.dim)
+    mc, vc = np.zeros(1), np.zeros(1)
+    for c in range(h.dim):
+        row = VP(sp.value + c * stride.value * size)
+        rc = h._lib.is_result_functional_moments(h.h, row, 0 if isf.value else 1, 1, _p(mc),
+                                                 _p(vc) if want_var else None)
+        if rc == _lib.IS_EFAILED:
+            raise AllZeroWeightsError()
+        _check(rc)
+        m[c], v[c] = mc[0], vc[0]
+    return m, v
+
+
+def _named_summary(result, values):
+    """Per-block summaries of a named layout (_weighted_sum / _weighted_variance over a
+    NamedTuple, src/statistics.jl:191-194,215-223)."""
+    lay = _handle(result).layout
+    if lay.names is None:
+        return values
+    out = {}
+    for b, name in enumerate(lay.names):
+        o, k = lay.offsets[b], lay.lengths[b]
+        out[name] = float(values[o]) if lay.scalars[b] else values[o:o + k].copy()
+    return out
+
+
+def _layout_result(result):
+    h = getattr(result, "_result", None)
+    return h is not None and h.layout is not None and (h.layout.names is not None or h.dim > 64)
+
+
 def mean(f_or_result, result: WeightedSamples = None):
-    """mean(result): self-normalized weighted mean (src/statistics.jl:11-15);
-    mean(f, result): weighted expectation of the scalar functional f, evaluated on the
-    device samples (src/statistics.jl:17-31)."""
+    """mean(result): self-normalized weighted mean (src/statistics.jl:11-15), per name for
+    named samples; mean(f, result): weighted expectation of the scalar functional f,
+    evaluated on the device samples (src/statistics.jl:17-31)."""
     if result is not None:
         return _functional_moments(f_or_result, result, False)[0]
     result = f_or_result
+    if _layout_result(result):
+        return _named_summary(result, _rowwise_moments(result, False)[0])
     m, _, _ = _device_moments(result)
     return float(m[0]) if result.samples.ndim == 1 else m
 
@@ -1082,14 +1335,20 @@ def var(f_or_result, result: WeightedSamples = None, corrected=False):
     if result is not None:
         return _functional_moments(f_or_result, result, True)[1]
     result = f_or_result
+    if _layout_result(result):
+        return _named_summary(result, _rowwise_moments(result, True)[1])
     _, v, _ = _device_moments(result)
     return float(v[0]) if result.samples.ndim == 1 else v
 
 
 def cov(result: WeightedSamples, corrected=False):
-    """Uncorrected weighted covariance (src/statistics.jl:96-103)."""
+    """Uncorrected weighted covariance (src/statistics.jl:96-103); named samples have none
+    (src/statistics.jl:238-240)."""
     if corrected:
         raise ValueError("corrected=true is unsupported")
+    h = getattr(result, "_result", None)
+    if h is not None and h.layout is not None and h.layout.names is not None:
+        raise ValueError("covariance requires vector-valued samples")
     _, _, c = _device_moments(result)
     return c
 
@@ -1112,4 +1371,4 @@ def resample(uniforms, result: WeightedSamples, count=None, method=None):
     if rc == _lib.IS_EFAILED:
         raise AllZeroWeightsError()
     _check(rc)
-    return UnweightedSamples(_ResultHandle(out, h.dim, h.scalar), anc)
+    return UnweightedSamples(_ResultHandle(out, h.dim, h.scalar, h.layout), anc)

@@ -158,9 +158,14 @@ struct is_job {
   // mixture pass of every round, on the job's stream, outside any graph capture
   is_batch_target_fn tbatch = nullptr;
   void* tuser = nullptr;
-  // per-coordinate transforms: kinds and bounds on the device, logical samples XL
+  // per-coordinate transforms or a named layout: kinds and bounds (or the layout's blocks)
+  // on the device, logical samples XL (dl rows; dl = d without a layout)
   bool has_tf = false;
   DevBuf tf_kinds_d, tf_bounds_d, XL;
+  bool has_layout = false;
+  int dl = 0;
+  std::vector<is_transform_block> layout;
+  DevBuf layout_d;
   bool graph_dirty = true;
   // replay runs on the extended-range kernels (set after a run's densities left the
   // floating-point range; kept while runs succeed there)
@@ -323,7 +328,7 @@ int64_t target_splits(int64_t n_rows, int64_t n_new) {
 // sigma^2), plus the Gaussian log prior and the normalising constants.
 double regression_logtarget(const is_job* j, const double* beta) {
   const int64_t n = j->tn;
-  const int p = j->d;
+  const int p = j->dl;
   const double* X = j->tdata.data();
   const double* y = X + n * p;
   double ll = 0.0;
@@ -468,9 +473,13 @@ RoundArgs<T> base_args(is_job* j, int t) {
   a.tkind = j->tkind;
   a.tslot = j->tslotT.as<T>();
   a.X = j->X.as<T>();
-  a.tf_kinds = j->has_tf ? j->tf_kinds_d.as<int32_t>() : nullptr;
-  a.tf_bounds = j->has_tf ? j->tf_bounds_d.as<double>() : nullptr;
+  // a named layout maps the draws in its own pass (k_layout_forward), so the draw kernels
+  // see no per-coordinate transforms; both forms carry the log Jacobian to the pre-target pass
+  const bool per_coordinate = j->has_tf && !j->has_layout;
+  a.tf_kinds = per_coordinate ? j->tf_kinds_d.as<int32_t>() : nullptr;
+  a.tf_bounds = per_coordinate ? j->tf_bounds_d.as<double>() : nullptr;
   a.XL = j->has_tf ? j->XL.as<T>() : nullptr;
+  a.tf_carry = j->has_tf ? 1 : 0;
   a.cap = j->total;
   a.logtargets = j->logtargets.as<T>();
   a.lognums = j->lognums.as<T>();
@@ -561,21 +570,36 @@ int run_rounds(is_job* j) {
   const int nv_mom = 2 + d + d * (d + 1) / 2;
   const int nt_mom = sweep_nt(d, true);
   unsigned int* counter = j->counter.as<unsigned int>();
-  const bool batch_target =
-      j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG || j->tkind == IS_TARGET_CALLBACK;
+  // a named layout runs every target on the logical samples after the draw pass
+  const bool batch_target = j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG ||
+                            j->tkind == IS_TARGET_CALLBACK || j->has_layout;
+  const int dl = j->dl;
   auto mix = [&](RoundArgs<T>& a, int ncomp) -> int {
     if (!batch_target) {
       launch_mix<T, kDrawFused>(D, ncomp, a, s);
+      return IS_OK;
+    }
+    launch_mix<T, kDrawOnly>(D, ncomp, a, s);
+    if (j->has_layout && a.n_new > 0)
+      hipLaunchKernelGGL((k_layout_forward<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s,
+                         j->layout_d.as<LayoutBlock>(), (int)j->layout.size(), a.X, a.XL, a.cap, a.first, a.n_new,
+                         a.lognums, a.fail);
+    if (j->tkind == IS_TARGET_STANDARD || j->tkind == IS_TARGET_GAUSSIAN) {
+      // the built-in targets over the layout's logical rows
+      if (a.n_new > 0)
+        hipLaunchKernelGGL((k_target_logical<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, j->tkind, dl,
+                           j->tslotT.as<T>(), a.XL, a.cap, a.first, a.n_new, j->tpre.as<T>());
+      a.pre_logtarget = j->tpre.as<T>();
+      launch_mix<T, kPreTarget>(D, ncomp, a, s);
       return IS_OK;
     }
     if (j->tkind == IS_TARGET_CALLBACK) {
       // draw, the caller's batch target over the round's samples (values start as
       // NaN, src/batched_targets.jl:96-98), then the mixture with those values; the
       // mixture pass checks them (NaN / +Inf fail in the target phase)
-      launch_mix<T, kDrawOnly>(D, ncomp, a, s);
       HIP_TRY(hipMemsetAsync(j->tpre.p, 0xff, (size_t)a.n_new * sizeof(T), s));
       HIP_TRY(hipGetLastError());
-      const int cb = j->tbatch(j->tuser, (j->has_tf ? j->XL : j->X).as<T>() + a.first, j->total, d, a.n_new, j->tpre.p,
+      const int cb = j->tbatch(j->tuser, (j->has_tf ? j->XL : j->X).as<T>() + a.first, j->total, dl, a.n_new, j->tpre.p,
                                sizeof(T) == 4 ? 1 : 0, (void*)s);
       if (cb != 0) {
         // the run fails in the target phase of this round and commits nothing
@@ -593,11 +617,10 @@ int run_rounds(is_job* j) {
       return IS_OK;
     }
     // draw, batched target on MFMA, then the mixture with the precomputed target values
-    launch_mix<T, kDrawOnly>(D, ncomp, a, s);
     const int64_t splits = target_splits(j->tn, a.n_new);
     TargetArgs<T> ta{};
     ta.kind = j->tkind;
-    ta.p = d;
+    ta.p = dl;
     ta.n = j->tn;
     ta.Xd = j->Xd.as<T>();
     ta.yd = j->yd.as<T>();
@@ -609,9 +632,9 @@ int run_rounds(is_job* j) {
     ta.rows_per_split = cdiv(cdiv(j->tn, splits), 64) * 64;
     ta.part = j->tpart.as<double>();
     const int64_t used = cdiv(j->tn, ta.rows_per_split);
-    launch_target<T>(j->D, ta, used, s);
+    launch_target<T>(dim_bucket(dl), ta, used, s);
     hipLaunchKernelGGL((k_target_combine<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, ta.part, (int)used,
-                       a.n_new, ta.beta, ta.cap, ta.first, d, j->t_inv_two_prior, j->t_const - j->t_shift,
+                       a.n_new, ta.beta, ta.cap, ta.first, dl, j->t_inv_two_prior, j->t_const - j->t_shift,
                        j->tpre.as<T>());
     a.pre_logtarget = j->tpre.as<T>();
     launch_mix<T, kPreTarget>(D, ncomp, a, s);
@@ -1041,13 +1064,17 @@ int job_run_typed(is_job* j) {
   return IS_OK;
 }
 
+// Values per moment record of d coordinates (weights, means, packed second moments); the
+// weights alone past kMaxDim, where moments are refused.
+inline int moment_values(int d) { return d <= kMaxDim ? 2 + d + d * (d + 1) / 2 : 2; }
+
 template <class T>
 int alloc_job_buffers(is_job* j) {
   const size_t cap = (size_t)j->total;
   const int d = j->d;
   int rc;
   if ((rc = j->X.alloc(cap * d * sizeof(T)))) return rc;
-  if (j->has_tf && (rc = j->XL.alloc(cap * d * sizeof(T)))) return rc;
+  if (j->has_tf && (rc = j->XL.alloc(cap * j->dl * sizeof(T)))) return rc;
   if ((rc = j->logtargets.alloc(cap * sizeof(T)))) return rc;
   if ((rc = j->lognums.alloc(cap * sizeof(T)))) return rc;
   if ((rc = j->logweights.alloc(cap * sizeof(T)))) return rc;
@@ -1062,10 +1089,11 @@ int alloc_job_buffers(is_job* j) {
   // (a fused AMIS round adds one record per 64 new samples)
   j->pcap = cdiv((int64_t)cap, kSweepPerBlock) + cdiv(std::max<int64_t>(1, j->maxN), 64) + 1;
   const int nv_mom = 2 + d + d * (d + 1) / 2;
+  const int nv_res = std::max(nv_mom, moment_values(j->dl));  // results' moments over the logical samples
   if ((rc = j->partials.alloc((size_t)j->pcap * (1 + nv_mom) * sizeof(double)))) return rc;
-  if ((rc = j->partials2.alloc((size_t)j->pcap * (1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->partials2.alloc((size_t)j->pcap * (1 + nv_res) * sizeof(double)))) return rc;
   if ((rc = j->reduced.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
-  if ((rc = j->reduced2.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
+  if ((rc = j->reduced2.alloc((size_t)(1 + nv_res) * sizeof(double)))) return rc;
   if ((rc = j->summary.alloc((size_t)5 * std::max(1, j->T) * sizeof(double)))) return rc;
   if ((rc = j->clip.alloc(64))) return rc;
   {
@@ -1102,7 +1130,9 @@ int alloc_job_buffers(is_job* j) {
   }
   std::vector<double> zeros(std::max(j->nslots, 1), 0.0);
   if ((rc = upload_T<T>(j->zerosT, zeros))) return rc;
-  if (j->tkind == IS_TARGET_CALLBACK && (rc = j->tpre.alloc((size_t)j->maxN * sizeof(T)))) return rc;
+  if ((j->tkind == IS_TARGET_CALLBACK || (j->has_layout && (j->tkind == IS_TARGET_STANDARD || j->tkind == IS_TARGET_GAUSSIAN))) &&
+      (rc = j->tpre.alloc((size_t)j->maxN * sizeof(T))))
+    return rc;
   if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
     if ((rc = j->tpre.alloc((size_t)j->maxN * sizeof(T)))) return rc;
     int64_t part = 1;
@@ -1184,6 +1214,121 @@ int is_ctx_destroy(is_ctx* ctx) {
   return IS_OK;
 }
 
+// _prepare_flat_transform_layout (src/proposal_composition.jl:179-240): complete, disjoint
+// selectors; scalar transforms on Int selectors, simplices on ranges of K - 1 >= 1
+// coordinates (src/proposal_composition.jl:140-160); the logical dimension dl.
+int prepare_layout(is_job* j, const is_job_desc* desc) {
+  const int d = desc->dim, nb = desc->transform_block_count;
+  if (nb < 0 || !desc->transform_blocks) return fail(IS_EINVAL, "a transform layout needs its blocks");
+  if (desc->transform_kinds) return fail(IS_EINVAL, "give per-coordinate transforms or a layout, not both");
+  std::vector<char> selected(d, 0);
+  int dl = 0;
+  for (int b = 0; b < nb; ++b) {
+    const is_transform_block& B = desc->transform_blocks[b];
+    if (B.count < 1 || B.first < 0 || B.first > d - B.count)
+      return fail(IS_EINVAL, "transform block %d: selector is out of bounds for dimension %d", b + 1, d);
+    if (B.scalar != 0 && B.scalar != 1) return fail(IS_EINVAL, "transform block %d: scalar must be 0 or 1", b + 1);
+    if (B.scalar && B.count != 1) return fail(IS_EINVAL, "transform block %d: an Int selector holds one coordinate", b + 1);
+    if (B.kind < IS_TRANSFORM_IDENTITY || B.kind > IS_TRANSFORM_SIMPLEX)
+      return fail(IS_EINVAL, "transform block %d: unknown transform kind %d", b + 1, B.kind);
+    if (B.kind != IS_TRANSFORM_IDENTITY && B.kind != IS_TRANSFORM_SIMPLEX && !B.scalar)
+      return fail(IS_EINVAL, "transform block %d: a scalar transform requires one unconstrained scalar", b + 1);
+    if (B.kind == IS_TRANSFORM_SIMPLEX && B.scalar)
+      return fail(IS_EINVAL, "transform block %d: SimplexTransform(%d) requires %d coordinates", b + 1, B.count + 1,
+                  B.count);
+    if (B.kind == IS_TRANSFORM_INTERVAL &&
+        (B.lower != B.lower || B.upper != B.upper || !(B.lower < B.upper) || (std::isinf(B.lower) && std::isinf(B.upper))))
+      return fail(IS_EINVAL, "transform block %d: interval needs lower < upper with a finite side", b + 1);
+    for (int c = B.first; c < B.first + B.count; ++c) {
+      if (selected[c]) return fail(IS_EINVAL, "flat transform selectors must be disjoint");
+      selected[c] = 1;
+    }
+    dl += B.count + (B.kind == IS_TRANSFORM_SIMPLEX ? 1 : 0);
+  }
+  for (int c = 0; c < d; ++c)
+    if (!selected[c]) return fail(IS_EINVAL, "flat transform selectors must collectively cover the base dimension");
+  j->layout.assign(desc->transform_blocks, desc->transform_blocks + nb);
+  j->dl = dl;
+  j->has_tf = true;
+  j->has_layout = true;
+  return IS_OK;
+}
+
+extern "C++" {
+// The device blocks of a layout, with the simplex constants of src/transforms.jl:341-345
+// computed in the job's floating-point type T.
+template <class T>
+std::vector<LayoutBlock> layout_blocks(const std::vector<is_transform_block>& layout) {
+  std::vector<LayoutBlock> out(layout.size());
+  int row = 0;
+  for (size_t b = 0; b < layout.size(); ++b) {
+    const is_transform_block& B = layout[b];
+    LayoutBlock& L = out[b];
+    L.first = B.first;
+    L.count = B.count;
+    L.kind = B.kind;
+    L.out = row;
+    L.lower = B.lower;
+    L.upper = B.upper;
+    if (B.kind == IS_TRANSFORM_SIMPLEX) {
+      const T K = (T)(B.count + 1);
+      const T inv_root = T(1) / std::sqrt(K);
+      L.inv_root = (double)inv_root;
+      L.shared = (double)((T(1) + inv_root) / (T)B.count);
+      L.half_log_k = (double)(T(0.5) * std::log(K));
+    }
+    row += B.count + (B.kind == IS_TRANSFORM_SIMPLEX ? 1 : 0);
+  }
+  return out;
+}
+}  // extern "C++"
+
+// Logical image of one coordinate vector under the layout, in FP64 (the host's anchor
+// for the FP32 regression offset); false when a block fails.
+bool layout_forward_host(const std::vector<is_transform_block>& layout, const double* z, double* x) {
+  int row = 0;
+  for (const is_transform_block& B : layout) {
+    const double* zb = z + B.first;
+    double* xb = x + row;
+    if (B.kind == IS_TRANSFORM_SIMPLEX) {
+      const int K = B.count + 1;
+      double sum = 0.0;
+      for (int i = 0; i < B.count; ++i) sum += zb[i];
+      const double inv_root = 1.0 / std::sqrt((double)K), shared = (1.0 + inv_root) / (double)B.count;
+      double mx = inv_root * sum;
+      xb[B.count] = mx;
+      for (int i = 0; i < B.count; ++i) {
+        xb[i] = zb[i] - shared * sum;
+        mx = std::max(mx, xb[i]);
+      }
+      double es = 0.0;
+      for (int i = 0; i < K; ++i) es += (xb[i] = std::exp(xb[i] - mx));
+      for (int i = 0; i < K; ++i) xb[i] /= es;
+      row += K;
+    } else {
+      for (int i = 0; i < B.count; ++i) {
+        const double v = zb[i];
+        double o = v;
+        if (B.kind == IS_TRANSFORM_POSITIVE) {
+          o = std::exp(v);
+        } else if (B.kind == IS_TRANSFORM_SOFTPLUS) {
+          o = v > 0.0 ? v + std::log1p(std::exp(-v)) : std::log1p(std::exp(v));
+        } else if (B.kind == IS_TRANSFORM_INTERVAL) {
+          if (std::isfinite(B.lower) && std::isfinite(B.upper))
+            o = B.lower + (B.upper - B.lower) / (1.0 + std::exp(-v));
+          else
+            o = std::isfinite(B.lower) ? B.lower + std::exp(v) : B.upper - std::exp(v);
+        }
+        xb[i] = o;
+      }
+      row += B.count;
+    }
+  }
+  for (int c = 0; c < row; ++c)
+    if (!std::isfinite(x[c])) return false;
+  return true;
+}
+
 int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   if (!ctx || !desc || !out) return fail(IS_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(ctx->device));
@@ -1232,6 +1377,10 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
     HIP_TRY(locked_memcpy(j->tf_kinds_d.p, desc->transform_kinds, (size_t)dd * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(locked_memcpy(j->tf_bounds_d.p, desc->transform_bounds, (size_t)2 * dd * sizeof(double), hipMemcpyHostToDevice));
     j->has_tf = true;
+  }
+  j->dl = d;
+  if (desc->transform_block_count != 0) {
+    if (const int rc = prepare_layout(j, desc)) return rc;
   }
   j->precision = desc->precision;
   j->key = RngKey{desc->seed, desc->stream};
@@ -1327,45 +1476,47 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
     j->committed = j->initial;
     j->nslots = j->method == IS_PLAIN ? 1 : T + 1;
   }
-  // target
+  // target (over the logical dimension dl: d, or the layout's)
+  const int dl = j->dl;
   if (j->tkind == IS_TARGET_GAUSSIAN) {
-    if (!desc->target_params || desc->target_params_len < (int64_t)d + d * d + 1)
+    if (!desc->target_params || desc->target_params_len < (int64_t)dl + dl * dl + 1)
       return fail(IS_EINVAL, "Gaussian target needs mean, factor and lognormalizer");
     HostProp tp;
-    tp.mu.assign(desc->target_params, desc->target_params + d);
-    tp.L.assign(desc->target_params + d, desc->target_params + d + d * d);
-    tp.lognorm = desc->target_params[d + d * d];
-    for (int c = 0; c < d; ++c)
-      if (!(tp.L[c + (size_t)c * d] != 0.0)) return fail(IS_EINVAL, "target factor diagonal must be nonzero");
-    j->tslot.assign(j->stride, 0.0);
-    pack_slot(tp, d, 0.0, j->tslot.data());
+    tp.mu.assign(desc->target_params, desc->target_params + dl);
+    tp.L.assign(desc->target_params + dl, desc->target_params + dl + dl * dl);
+    tp.lognorm = desc->target_params[dl + dl * dl];
+    for (int c = 0; c < dl; ++c)
+      if (!(tp.L[c + (size_t)c * dl] != 0.0)) return fail(IS_EINVAL, "target factor diagonal must be nonzero");
+    j->tslot.assign(slot_stride(dl), 0.0);
+    pack_slot(tp, dl, 0.0, j->tslot.data());
   } else if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
     // [n, p, prior_var, X, y] (logistic) or [n, p, sigma, tau, X, y] (linear regression)
     const int head = j->tkind == IS_TARGET_LOGISTIC ? 3 : 4;
     const double* tp = desc->target_params;
     if (!tp || desc->target_params_len < head) return fail(IS_EINVAL, "regression target parameters are missing");
     const double nd = tp[0], pd = tp[1];
-    if (!(nd >= 1.0) || nd != std::floor(nd) || pd != (double)d)
+    if (!(nd >= 1.0) || nd != std::floor(nd) || pd != (double)dl)
       return fail(IS_EINVAL, "regression target needs n >= 1 rows and p == dim coefficients");
+    if (dl > kMaxDim) return fail(IS_EUNSUPPORTED, "regression targets take at most %d coefficients", kMaxDim);
     const int64_t n = (int64_t)nd;
-    if (desc->target_params_len < head + n * d + n) return fail(IS_EINVAL, "regression data is shorter than n x p + n");
+    if (desc->target_params_len < head + n * dl + n) return fail(IS_EINVAL, "regression data is shorter than n x p + n");
     j->tn = n;
     const double pi2 = 2.0 * 3.141592653589793;
     if (j->tkind == IS_TARGET_LOGISTIC) {
       const double pv = tp[2];
       if (!(pv > 0.0) || !std::isfinite(pv)) return fail(IS_EINVAL, "prior variance must be positive");
       j->t_inv_two_prior = 0.5 / pv;
-      j->t_const = -0.5 * (double)d * std::log(pi2 * pv);
+      j->t_const = -0.5 * (double)dl * std::log(pi2 * pv);
     } else {
       const double sigma = tp[2], tau = tp[3];
       if (!(sigma > 0.0) || !(tau > 0.0) || !std::isfinite(sigma) || !std::isfinite(tau))
         return fail(IS_EINVAL, "sigma and tau must be positive");
       j->t_inv_two_var = 0.5 / (sigma * sigma);
       j->t_inv_two_prior = 0.5 / (tau * tau);
-      j->t_const = -(double)n * std::log(sigma) - 0.5 * (double)n * std::log(pi2) - (double)d * std::log(tau) -
-                   0.5 * (double)d * std::log(pi2);
+      j->t_const = -(double)n * std::log(sigma) - 0.5 * (double)n * std::log(pi2) - (double)dl * std::log(tau) -
+                   0.5 * (double)dl * std::log(pi2);
     }
-    j->tdata.assign(tp + head, tp + head + n * d + n);
+    j->tdata.assign(tp + head, tp + head + n * dl + n);
     if (j->precision == IS_FP32) {
       // FP32 log-targets of a large data set sit near log pi(x0) ~ -n; storing them
       // minus that FP64 constant keeps their FP32 rounding at the scale of the
@@ -1377,6 +1528,12 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
           for (int c = 0; c < d; ++c) x0[c] += j->committed_locs[(size_t)k * d + c] / K;
       } else {
         x0 = j->initial.mu;
+      }
+      if (j->has_layout) {
+        // the offset's anchor is the logical image of that location
+        std::vector<double> xl(dl, 0.0);
+        if (!layout_forward_host(j->layout, x0.data(), xl.data())) std::fill(xl.begin(), xl.end(), 0.0);
+        x0.swap(xl);
       }
       const double c0 = regression_logtarget(j, x0.data());
       j->t_shift = std::isfinite(c0) ? c0 : 0.0;
@@ -1407,12 +1564,18 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
   }
   rc = j->precision == IS_FP32 ? upload_T<float>(j->logmassT, logmass) : upload_T<double>(j->logmassT, logmass);
   if (rc) return rc;
+  if (j->has_layout) {
+    const std::vector<LayoutBlock> lb =
+        j->precision == IS_FP32 ? layout_blocks<float>(j->layout) : layout_blocks<double>(j->layout);
+    if ((rc = j->layout_d.alloc(lb.size() * sizeof(LayoutBlock)))) return rc;
+    HIP_TRY(locked_memcpy(j->layout_d.p, lb.data(), lb.size() * sizeof(LayoutBlock), hipMemcpyHostToDevice));
+  }
   if (!j->tslot.empty()) {
     rc = j->precision == IS_FP32 ? upload_T<float>(j->tslotT, j->tslot) : upload_T<double>(j->tslotT, j->tslot);
     if (rc) return rc;
   }
   if (!j->tdata.empty()) {
-    const size_t nx = (size_t)j->tn * d;
+    const size_t nx = (size_t)j->tn * j->dl;
     std::vector<double> xpart(j->tdata.begin(), j->tdata.begin() + nx), ypart(j->tdata.begin() + nx, j->tdata.end());
     rc = j->precision == IS_FP32 ? upload_T<float>(j->Xd, xpart) : upload_T<double>(j->Xd, xpart);
     if (!rc) rc = j->precision == IS_FP32 ? upload_T<float>(j->yd, ypart) : upload_T<double>(j->yd, ypart);
@@ -1454,9 +1617,18 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
   bytes += (int64_t)T * (5 * 8 + 3 * 8 + 2 * 4 + 3 * 8) + 4 * 64;
   if (desc->target_kind == IS_TARGET_CALLBACK) bytes += maxN * ts;
   if (desc->transform_kinds) bytes += total * d * ts + (int64_t)d * 20;
+  int dl = d;
+  if (desc->transform_block_count > 0 && desc->transform_blocks) {
+    dl = 0;
+    for (int b = 0; b < desc->transform_block_count; ++b)
+      dl += std::max(0, desc->transform_blocks[b].count) + (desc->transform_blocks[b].kind == IS_TRANSFORM_SIMPLEX ? 1 : 0);
+    bytes += total * dl * ts + (int64_t)desc->transform_block_count * (int64_t)sizeof(LayoutBlock);
+    if (desc->target_kind == IS_TARGET_STANDARD || desc->target_kind == IS_TARGET_GAUSSIAN) bytes += maxN * ts;
+  }
+  if (desc->target_kind == IS_TARGET_GAUSSIAN) bytes += (int64_t)slot_stride(dl) * ts;
   if (desc->target_kind == IS_TARGET_LOGISTIC || desc->target_kind == IS_TARGET_LINREG) {
     const int64_t n = desc->target_params && desc->target_params_len > 0 ? (int64_t)desc->target_params[0] : 0;
-    bytes += n * (d + 1) * ts + maxN * ts;
+    bytes += n * (dl + 1) * ts + maxN * ts;
     bytes += std::max<int64_t>(1, target_splits(n, maxN) * maxN) * 8;
   }
   if (desc->method == IS_DMPMC) {
@@ -1738,6 +1910,8 @@ int is_job_round_kernel_ms(is_job* job, double* out) {
   return IS_OK;
 }
 
+int32_t is_job_logical_dim(const is_job* job) { return job ? job->dl : 0; }
+
 int64_t is_job_total(const is_job* job) { return job ? job->total : 0; }
 
 int is_job_failure(const is_job* job, is_failure* out) {
@@ -1755,15 +1929,16 @@ int is_job_failure(const is_job* job, is_failure* out) {
 int is_job_get_samples(is_job* job, double* out) {
   REQUIRE_RESULTS(job);
   const int64_t n = job->total;
+  const int dl = job->dl;
   DevBuf tmp;
-  int rc = tmp.alloc((size_t)n * job->d * sizeof(double));
+  int rc = tmp.alloc((size_t)n * dl * sizeof(double));
   if (rc) return rc;
   if (job->precision == IS_FP32)
-    hipLaunchKernelGGL((k_soa_to_aos<float>), dim3(cdiv(n * job->d, 256)), dim3(256), 0, job->stream,
-                       (job->has_tf ? job->XL : job->X).as<float>(), n, n, job->d, tmp.as<double>());
+    hipLaunchKernelGGL((k_soa_to_aos<float>), dim3(cdiv(n * dl, 256)), dim3(256), 0, job->stream,
+                       (job->has_tf ? job->XL : job->X).as<float>(), n, n, dl, tmp.as<double>());
   else
-    hipLaunchKernelGGL((k_soa_to_aos<double>), dim3(cdiv(n * job->d, 256)), dim3(256), 0, job->stream,
-                       (job->has_tf ? job->XL : job->X).as<double>(), n, n, job->d, tmp.as<double>());
+    hipLaunchKernelGGL((k_soa_to_aos<double>), dim3(cdiv(n * dl, 256)), dim3(256), 0, job->stream,
+                       (job->has_tf ? job->XL : job->X).as<double>(), n, n, dl, tmp.as<double>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost, job->stream));
   HIP_TRY(hipStreamSynchronize(job->stream));
@@ -1905,16 +2080,17 @@ static int job_lognormalizer_typed(is_job* job, double* out3) {
   w.nv = 2;
   w.fail = nullptr;
   w.partials = job->partials2.as<double>();
-  const int blocks = launch_sweep<T>(job->D, 0, w, job->stream);
+  if (job->D == 0) w.d = 1;  // results of more than 64 logical coordinates: the weights alone
+  const int blocks = launch_sweep<T>(job->D > 0 ? job->D : 1, 0, w, job->stream);
   FinishArgs f{};
-  f.d = job->d;
+  f.d = w.d;
   f.flags = kFinishSummary;
   f.summary_count = n;
   f.summary_out = job->summary.as<double>();
   f.counter = job->counter.as<unsigned int>() + 1;
   hipLaunchKernelGGL((k_reduce_finish<0, false>), dim3(2), dim3(kBlock), 0, job->stream,
                      job->partials2.as<double>(), (int64_t)blocks, job->pcap, 2, job->reduced2.as<double>(), f);
-  launch_finish(job->d, job->reduced2.as<double>(), f, job->stream);
+  launch_finish(w.d, job->reduced2.as<double>(), f, job->stream);
   HIP_TRY(hipGetLastError());
   double s[5];
   HIP_TRY(hipMemcpyAsync(s, job->summary.p, sizeof(s), hipMemcpyDeviceToHost, job->stream));
@@ -1938,7 +2114,8 @@ extern "C++" {
 template <class T>
 static int job_moments_typed(is_job* job, double* mean, double* var, double* cov) {
   const int64_t n = job->total;
-  const int d = job->d;
+  const int d = job->dl;  // the logical samples' dimension
+  if (d > kMaxDim) return fail(IS_EUNSUPPORTED, "moments of more than %d logical coordinates", kMaxDim);
   const int nv = 2 + d + d * (d + 1) / 2;
   // centre at the FP64 proposal mean of the first slot to keep the single pass well conditioned
   DevBuf center;
@@ -1964,8 +2141,9 @@ static int job_moments_typed(is_job* job, double* mean, double* var, double* cov
   w.fail = nullptr;
   w.center = center.as<double>();
   if (job->has_tf) w.X = job->XL.as<T>();
+  w.d = d;
   w.partials = job->partials2.as<double>();
-  const int blocks = launch_sweep<T>(job->D, sweep_nt(d, true), w, job->stream);
+  const int blocks = launch_sweep<T>(dim_bucket(d), sweep_nt(d, true), w, job->stream);
   FinishArgs f{};
   f.d = d;
   f.flags = 0;
@@ -2032,8 +2210,8 @@ static int job_resample_typed(is_job* job, const double* uniforms, int64_t count
                      anc.as<int32_t>());
   if (gathered) {
     const T* src = (job->has_tf ? job->XL : job->X).as<T>();
-    hipLaunchKernelGGL((k_gather_samples<T>), dim3(cdiv(count * job->d, 256)), dim3(256), 0, job->stream, src, n,
-                       job->d, anc.as<int32_t>(), count, gathered->X.as<T>());
+    hipLaunchKernelGGL((k_gather_samples<T>), dim3(cdiv(count * job->dl, 256)), dim3(256), 0, job->stream, src, n,
+                       job->dl, anc.as<int32_t>(), count, gathered->X.as<T>());
     hipLaunchKernelGGL((k_fill<T>), dim3(cdiv(count, 256)), dim3(256), 0, job->stream, gathered->logweights.as<T>(),
                        count, T(0));
   }
@@ -2267,8 +2445,9 @@ int result_shell(const is_job* src, int64_t n, std::unique_ptr<is_job>& out) {
   r->family = src->family;
   r->kind = src->kind;
   r->scalar = src->scalar;
-  r->d = src->d;
-  r->D = src->D;
+  r->d = src->dl;  // the logical samples
+  r->dl = src->dl;
+  r->D = src->dl <= kMaxDim ? dim_bucket(src->dl) : 0;
   r->T = 1;
   r->dof = src->dof;
   r->precision = src->precision;
@@ -2286,7 +2465,7 @@ int result_shell(const is_job* src, int64_t n, std::unique_ptr<is_job>& out) {
     return rc;
   }
   r->pcap = cdiv(n, kSweepPerBlock) + 1;
-  const int nv_mom = 2 + d + d * (d + 1) / 2;
+  const int nv_mom = moment_values(d);
   if ((rc = r->partials2.alloc((size_t)r->pcap * (1 + nv_mom) * sizeof(double))) ||
       (rc = r->reduced2.alloc((size_t)(1 + nv_mom) * sizeof(double))) || (rc = r->summary.alloc(5 * sizeof(double))) ||
       (rc = r->ctl.alloc(128))) {
@@ -2307,7 +2486,7 @@ int result_shell(const is_job* src, int64_t n, std::unique_ptr<is_job>& out) {
 // The centre of the job's moment pass (as is_job_moments chooses it), FP64 on the host.
 template <class T>
 int moment_center(is_job* job, std::vector<double>& c0) {
-  const int d = job->d;
+  const int d = job->dl;
   c0.assign(d, 0.0);
   if (job->has_tf) {
     DevBuf center;
@@ -2337,7 +2516,7 @@ int snapshot_typed(is_job* job, std::unique_ptr<is_job>& out) {
   r->t_shift = job->t_shift;
   const size_t n = (size_t)job->total;
   const DevBuf& xs = job->has_tf ? job->XL : job->X;
-  if (hipMemcpyAsync(r->X.p, xs.p, n * job->d * sizeof(T), hipMemcpyDeviceToDevice, job->stream) != hipSuccess ||
+  if (hipMemcpyAsync(r->X.p, xs.p, n * job->dl * sizeof(T), hipMemcpyDeviceToDevice, job->stream) != hipSuccess ||
       hipMemcpyAsync(r->logweights.p, job->logweights.p, n * sizeof(T), hipMemcpyDeviceToDevice, job->stream) !=
           hipSuccess ||
       hipStreamSynchronize(job->stream) != hipSuccess) {

@@ -65,6 +65,7 @@ struct RoundArgs {
   const int32_t* tf_kinds;
   const double* tf_bounds;
   T* XL;
+  int tf_carry;       // the pre-target pass reads the forward log Jacobian from lognums
   // target
   int tkind;
   const T* tslot;           // Gaussian target slot
@@ -112,6 +113,7 @@ __device__ __forceinline__ T inline_target(int tkind, int d, P tslot, const T (&
 
 #include "mix.h"
 #include "target.h"
+#include "layout.h"
 
 // ---------------------------------------------------------------- partial reduction + finish
 struct FinishArgs {

@@ -152,7 +152,7 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
     // and under a transform leaves the forward log Jacobian there
     if (active) {
       const T mark = a.lognums[gi];
-      if (a.tf_kinds) tf_lj = mark;
+      if (a.tf_carry) tf_lj = mark;
       if (mark != mark) {
         ok = false;
       } else {

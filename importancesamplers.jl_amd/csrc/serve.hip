@@ -39,6 +39,7 @@ struct OwnedDesc {
   std::vector<double> location, scale, bank_locations, bank_scales, bank_masses, target_params;
   std::vector<int32_t> group_offsets, group_members, transform_kinds;
   std::vector<double> transform_bounds;
+  std::vector<is_transform_block> transform_blocks;
 
   int copy_from(const is_job_desc& s) {
     d = s;
@@ -82,6 +83,11 @@ struct OwnedDesc {
       transform_bounds.assign(s.transform_bounds, s.transform_bounds + 2 * dim);
       d.transform_kinds = transform_kinds.data();
       d.transform_bounds = transform_bounds.data();
+    }
+    if (s.transform_block_count > 0) {
+      if (!s.transform_blocks) return fail(IS_EINVAL, "a transform layout needs its blocks");
+      transform_blocks.assign(s.transform_blocks, s.transform_blocks + s.transform_block_count);
+      d.transform_blocks = transform_blocks.data();
     }
     d.mis_group_offsets = group_offsets.empty() ? nullptr : group_offsets.data();
     d.mis_group_members = group_members.empty() ? nullptr : group_members.data();

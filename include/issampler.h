@@ -82,8 +82,23 @@ enum is_transform_kind {   /* scalar sample transforms (src/transforms.jl:17-235
   IS_TRANSFORM_IDENTITY = 0,
   IS_TRANSFORM_POSITIVE = 1, /* x = exp(z), log|J| = z                                         */
   IS_TRANSFORM_SOFTPLUS = 2, /* x = softplus(z), log|J| = z - softplus(z)                      */
-  IS_TRANSFORM_INTERVAL = 3  /* bounds (l, u): l + exp(z), u - exp(z), or l + (u - l) logistic(z) */
+  IS_TRANSFORM_INTERVAL = 3, /* bounds (l, u): l + exp(z), u - exp(z), or l + (u - l) logistic(z) */
+  IS_TRANSFORM_SIMPLEX = 4   /* SimplexTransform(K) (src/transforms.jl:51-70,402-456): K - 1
+                                coordinates to K positive weights summing to one, through the
+                                orthonormal sum-zero logit embedding; log|J| = log(K)/2 + sum(log, x) */
 };
+/* One block of a named flat transform layout (prepare_sampler's transform=(name = selector
+ * => transform, ...), src/proposal_composition.jl:179-240, src/native_execution.jl:556-712).
+ * Blocks are listed in name order, their selectors disjoint and together covering the d
+ * sampling coordinates; block b's logical values follow those of blocks 0..b-1 (a simplex
+ * block gives K = count + 1 of them, every other block count). */
+typedef struct is_transform_block {
+  int32_t first;           /* first sampling coordinate of the selector, 0-based */
+  int32_t count;           /* its sampling coordinates (a simplex: K - 1) */
+  int32_t kind;            /* enum is_transform_kind */
+  int32_t scalar;          /* 1: an Int selector (a scalar leaf, count 1), 0: a UnitRange */
+  double lower, upper;     /* IS_TRANSFORM_INTERVAL bounds (-inf / +inf for an open side) */
+} is_transform_block;
 enum is_phase {            /* AMISRoundError / NPMCRoundError / DMPMCRoundError phases       */
   IS_PHASE_NONE = 0,
   IS_PHASE_SAMPLING = 1,   /* :sampling / :proposal_draw                                     */
@@ -143,6 +158,12 @@ typedef struct is_job_desc {
   /* caller's batch target (target_kind IS_TARGET_CALLBACK) */
   is_batch_target_fn target_batch;
   void* target_user;       /* passed back as the callback's first argument */
+  /* named flat transform layout: 0, or that many blocks (transform_kinds then NULL).  The
+   * target sees the logical sample of dl = sum of the blocks' logical lengths values
+   * (is_job_logical_dim, the dimension of the target's parameters), results report it, and
+   * a failing block fails the draw with its reason bits and its 1-based block index. */
+  int32_t transform_block_count;
+  const is_transform_block* transform_blocks;
 } is_job_desc;
 
 /* Failure snapshot: _device_failure_snapshot (src/execution.jl:719-734) plus
@@ -158,7 +179,8 @@ typedef struct is_failure {
   uint16_t draw_reason_bits;
   uint64_t words[3];       /* raw packed record */
   int32_t draw_first_block; /* block of the first draw failure (a transform failure: the
-                               1-based coordinate of a per-coordinate transform) */
+                               1-based coordinate of a per-coordinate transform, or the
+                               1-based block of a named layout) */
 } is_failure;
 
 /* ---- context ---------------------------------------------------------------------
@@ -214,8 +236,10 @@ int is_job_set_graph(is_job* job, int32_t enable);
 
 /* ---- results (WeightedSamples, src/results.jl:79-98) -------------------------------- */
 int64_t is_job_total(const is_job* job);
+/* Logical dimension of the samples results report: d, or dl under a named transform layout. */
+int32_t is_job_logical_dim(const is_job* job);
 int is_job_failure(const is_job* job, is_failure* out);
-int is_job_get_samples(is_job* job, double* out);           /* d x total */
+int is_job_get_samples(is_job* job, double* out);           /* logical dim x total */
 int is_job_get_logweights(is_job* job, double* out);        /* total */
 int is_job_get_round_ids(is_job* job, int64_t* out);        /* provenance.round */
 int is_job_get_proposal_ids(is_job* job, int64_t* out);     /* provenance.proposal_id (DM-PMC) */
@@ -273,7 +297,7 @@ int is_result_destroy(is_result* result);
 int64_t is_result_count(const is_result* result);
 /* 1 for WeightedSamples, 0 for the UnweightedSamples of is_result_resample */
 int is_result_is_weighted(const is_result* result);
-int is_result_get_samples(is_result* result, double* out);     /* d x count */
+int is_result_get_samples(is_result* result, double* out);     /* logical dim x count */
 int is_result_get_logweights(is_result* result, double* out);  /* count */
 /* device arrays (coordinate-major, stride = count; log-weights minus the offset, NULL
  * for unweighted samples) -- valid until is_result_destroy */

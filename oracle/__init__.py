@@ -127,6 +127,11 @@ def _declare(L):
     L.or_select_ancestors.argtypes = [F64P, F64P, i64, F64P, i64, I64P]
     L.or_local_select.restype = i
     L.or_local_select.argtypes = [F64P, I64P, i, F64P, I64P, i]
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    L.or_simplex_forward.restype = i
+    L.or_simplex_forward.argtypes = [i, F64P, F64P, F64P]
+    L.or_layout_forward.restype = None
+    L.or_layout_forward.argtypes = [i, F64P, i, i64, F64P, F64P, F64P, I32P, I32P]
     L.or_systematic_points.restype = None
     L.or_systematic_points.argtypes = [d, i64, F64P]
     L.or_adaptive.restype = i
@@ -209,6 +214,146 @@ def transformed_target(inner_kind, inner_params, kinds, bounds):
     bounds = _f64(bounds).ravel()
     inner = _f64(inner_params if inner_params is not None else [0.0]).ravel()
     return TRANSFORMED_TARGET, np.concatenate([[float(inner_kind)], kinds, bounds, inner])
+
+
+LAYOUT_TARGET = 6
+
+
+def layout_blocks(layout):
+    """The (B, 5) block table (first, count, kind, lower, upper) of a flat selector layout
+    given as a list of (first, count, kind[, lower, upper]) or as the engine's C blocks."""
+    rows = []
+    for b in layout:
+        if hasattr(b, "first"):
+            rows.append([b.first, b.count, b.kind, b.lower, b.upper])
+        else:
+            b = list(b) + [-math.inf, math.inf][len(b) - 3:]
+            rows.append([float(v) for v in b[:5]])
+    return _f64(rows).reshape(-1, 5)
+
+
+def layout_dim(layout):
+    t = layout_blocks(layout)
+    return int(sum(int(c) + (1 if int(k) == 4 else 0) for c, k in zip(t[:, 1], t[:, 2])))
+
+
+def layout_target(inner_kind, inner_params, layout):
+    """(kind, params) of the target inner(T(z)) + log|J| for a flat selector layout (inner
+    over the logical dimension); a failing block gives NaN."""
+    t = layout_blocks(layout)
+    inner = _f64(inner_params if inner_params is not None else [0.0]).ravel()
+    return LAYOUT_TARGET, np.concatenate([[float(inner_kind), float(t.shape[0])], t.ravel(), inner])
+
+
+def layout_forward(layout, z):
+    """The C restatement of _native_transform_flat_blocks! over samples z (n, d): logical
+    samples (n, dl), log Jacobians (n,), reason bits (n,) and failing blocks (n,)."""
+    t = layout_blocks(layout)
+    z = _f64(np.atleast_2d(z))
+    n, d = z.shape
+    dl = layout_dim(layout)
+    x, lj = np.zeros((n, dl)), np.zeros(n)
+    reason, block = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    lib().or_layout_forward(t.shape[0], _p(t), d, n, _p(z), _p(x), _p(lj), _p(reason, I32P), _p(block, I32P))
+    return x, lj, reason, block
+
+
+def simplex_forward_c(z):
+    """The C restatement of the FP64 simplex forward map: (x, log|J|, reason bits)."""
+    z = _f64(np.atleast_1d(z))
+    x, lj = np.zeros(z.size + 1), ctypes.c_double()
+    r = lib().or_simplex_forward(z.size, _p(z), _p(x), ctypes.byref(lj))
+    return x, lj.value, int(r)
+
+
+def _cadd(total, correction, value):
+    # _compensated_add (src/transforms.jl:347-352)
+    corrected = value - correction
+    updated = total + corrected
+    return updated, (updated - total) - corrected
+
+
+def simplex_forward(z, dtype=np.float64):
+    """_transform_with_logjac(SimplexTransform(K), z) (src/transforms.jl:400-449) in the
+    scalar type `dtype` (numpy scalars round every operation to it): (x, log|J|, reason
+    bits), the checks in the reference's order."""
+    T = np.dtype(dtype).type
+    z = [T(v) for v in np.atleast_1d(z)]
+    K = len(z) + 1
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        total, corr = T(0), T(0)
+        for v in z:
+            if not np.isfinite(v):
+                return None, T(0), TRANSFORM_NONFINITE_INPUT
+            total, corr = _cadd(total, corr, v)
+        if not np.isfinite(total):
+            return None, T(0), TRANSFORM_NONFINITE_OUTPUT
+        inv_root = T(1) / np.sqrt(T(K))
+        shared = (T(1) + inv_root) / T(K - 1)
+        maxl = inv_root * total
+        x = [T(0)] * K
+        x[K - 1] = maxl
+        for i, v in enumerate(z):
+            logit = v - shared * total
+            if not np.isfinite(logit):
+                return None, T(0), TRANSFORM_NONFINITE_OUTPUT
+            x[i] = logit
+            maxl = max(maxl, logit)
+        es, ec = T(0), T(0)
+        for i in range(K):
+            x[i] = np.exp(x[i] - maxl)
+            es, ec = _cadd(es, ec, x[i])
+        if not np.isfinite(es):
+            return None, T(0), TRANSFORM_NONFINITE_OUTPUT
+        lj, lc = T(0.5) * np.log(T(K)), T(0)
+        for i in range(K):
+            w = x[i] / es
+            if not w > 0:
+                return None, lj, TRANSFORM_OUTSIDE_SUPPORT
+            x[i] = w
+            lj, lc = _cadd(lj, lc, np.log(w))
+        if not np.isfinite(lj):
+            return np.array(x, dtype=dtype), lj, TRANSFORM_NONFINITE_LOGJAC
+    return np.array(x, dtype=dtype), lj, 0
+
+
+def simplex_inverse(x, dtype=np.float64):
+    """_inverse_with_logjac(SimplexTransform(K), x) via _native_simplex_inverse!
+    (src/transforms.jl:354-394,451-456) in `dtype`: (z, log|J| of the forward map, reason
+    bits); weights must be positive, finite and sum to one within 8 eps."""
+    T = np.dtype(dtype).type
+    x = [T(v) for v in np.atleast_1d(x)]
+    K = len(x)
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        ws, wc, ls, lc = T(0), T(0), T(0), T(0)
+        z = [T(0)] * (K - 1)
+        last = T(0)
+        for i, w in enumerate(x):
+            if not np.isfinite(w):
+                return None, T(0), TRANSFORM_NONFINITE_INPUT
+            if not w > 0:
+                return None, T(0), TRANSFORM_OUTSIDE_SUPPORT
+            ws, wc = _cadd(ws, wc, w)
+            lw = np.log(w)
+            ls, lc = _cadd(ls, lc, lw)
+            if i < K - 1:
+                z[i] = lw
+            else:
+                last = lw
+        if not np.isfinite(ws):
+            return None, T(0), TRANSFORM_NONFINITE_INPUT
+        if not abs(ws - T(1)) <= T(8) * np.finfo(dtype).eps:
+            return None, T(0), TRANSFORM_OUTSIDE_SUPPORT
+        mean_lw = ls / T(K)
+        inv_root = T(1) / np.sqrt(T(K))
+        coef = inv_root / (T(1) - inv_root)
+        last_c = last - mean_lw
+        z = [zi - mean_lw + coef * last_c for zi in z]
+        lj = T(0.5) * np.log(T(K)) + ls
+        if not np.isfinite(lj):
+            return np.array(z, dtype=dtype), lj, TRANSFORM_NONFINITE_LOGJAC
+    return np.array(z, dtype=dtype), lj, 0
 
 
 def transform_forward_c(kind, lo, hi, z):

@@ -304,7 +304,9 @@ inline int radial_uniform_stride(int family, double dof) {  // src/proposal_bank
 //   2  logistic:  sum_i y_i*eta_i - log1pexp(eta_i) + prior         params [n, p, prior_var, X(n*p row-major), y(n)]
 //   3  linreg:    Gaussian likelihood + N(0, tau^2 I) prior         params [n, p, sigma, tau, X(n*p row-major), y(n)]
 //   5  transformed: inner(T(z)) + log|dT/dz| per coordinate       params [inner kind, kinds(d), bounds(2d), inner params]
-enum TargetKind { kTStandard = 0, kTGaussian = 1, kTLogistic = 2, kTLinreg = 3, kTTransformed = 5 };
+//   6  layout:     inner(T(z)) + log|J| of a flat selector layout  params [inner kind, B, B x (first, count, kind,
+//                  lower, upper), inner params of the logical dimension]
+enum TargetKind { kTStandard = 0, kTGaussian = 1, kTLogistic = 2, kTLinreg = 3, kTTransformed = 5, kTLayout = 6 };
 
 // Scalar forward transforms z -> x with log|dx/dz| (src/transforms.jl:160-235, helpers
 // :295-333), FP64; returns the reason bits (src/transforms.jl:106-109), 0 on success.
@@ -366,8 +368,111 @@ int tf_forward(int kind, double lo, double hi, double z, double* x, double* lj) 
   return (lo < *x && *x < hi) ? 0 : 0x0008;
 }
 
+// _compensated_add (src/transforms.jl:347-352)
+inline void compensated_add(double& total, double& correction, double value) {
+  const double corrected = value - correction;
+  const double updated = total + corrected;
+  correction = (updated - total) - corrected;
+  total = updated;
+}
+
+// _native_simplex_transform! (src/native_execution.jl:556-626) with the embedding constants
+// of src/transforms.jl:341-345, FP64: K - 1 coordinates z to K weights x and the forward
+// log Jacobian; the checks in the reference's order, the first failure's reason bit.
+uint16_t simplex_forward(int km1, const double* z, double* x, double* lj) {
+  const int K = km1 + 1;
+  double sum = 0.0, sum_c = 0.0;
+  *lj = 0.0;
+  for (int i = 0; i < km1; ++i) {
+    if (!std::isfinite(z[i])) return 0x0001;
+    x[i] = z[i];
+    compensated_add(sum, sum_c, z[i]);
+  }
+  if (!std::isfinite(sum)) return 0x0002;
+  const double inv_root = 1.0 / std::sqrt((double)K);
+  const double shared = (1.0 + inv_root) / (double)(K - 1);
+  double maxl = inv_root * sum;
+  x[km1] = maxl;
+  for (int i = 0; i < km1; ++i) {
+    const double logit = x[i] - shared * sum;
+    if (!std::isfinite(logit)) return 0x0002;
+    x[i] = logit;
+    maxl = std::max(maxl, logit);
+  }
+  double es = 0.0, es_c = 0.0;
+  for (int i = 0; i < K; ++i) {
+    x[i] = std::exp(x[i] - maxl);
+    compensated_add(es, es_c, x[i]);
+  }
+  if (!std::isfinite(es)) return 0x0002;
+  double acc = 0.5 * std::log((double)K), acc_c = 0.0;
+  for (int i = 0; i < K; ++i) {
+    const double w = x[i] / es;
+    if (!(w > 0.0)) return 0x0008;
+    x[i] = w;
+    compensated_add(acc, acc_c, std::log(w));
+  }
+  *lj = acc;
+  if (!std::isfinite(acc)) return 0x0004;
+  return 0;
+}
+
+// _native_transform_flat_blocks! (src/native_execution.jl:628-712): blocks (first, count,
+// kind, lower, upper) in name order map z to the logical x block by block; the blocks' log
+// Jacobians are summed as the reference's recursion does, block 1 + (block 2 + (... + 0)).
+// Returns the first failing block (1-based) with its reason in *reason, else 0.
+int layout_forward(int nb, const double* blocks, const double* z, double* x, double* lj, uint16_t* reason) {
+  std::vector<double> ljs(nb, 0.0);
+  int failed = 0;
+  *reason = 0;
+  int row = 0;
+  for (int b = 0; b < nb && !failed; ++b) {
+    const double* B = blocks + 5 * b;
+    const int first = (int)B[0], count = (int)B[1], kind = (int)B[2];
+    uint16_t r = 0;
+    if (kind == 4) {
+      r = simplex_forward(count, z + first, x + row, &ljs[b]);
+      row += count + 1;
+    } else if (kind == 0) {
+      for (int c = 0; c < count; ++c) {
+        x[row + c] = z[first + c];
+        if (!std::isfinite(z[first + c]) && !r) r = 0x0001;
+      }
+      row += count;
+    } else {
+      r = (uint16_t)tf_forward(kind, B[3], B[4], z[first], &x[row], &ljs[b]);
+      row += 1;
+    }
+    if (r) {
+      failed = b + 1;
+      *reason = r;
+    }
+  }
+  double acc = 0.0;
+  for (int b = nb - 1; b >= 0; --b) acc = ljs[b] + acc;
+  *lj = acc;
+  return failed;
+}
+
+int layout_dim(int nb, const double* blocks) {
+  int dl = 0;
+  for (int b = 0; b < nb; ++b) dl += (int)blocks[5 * b + 1] + ((int)blocks[5 * b + 2] == 4 ? 1 : 0);
+  return dl;
+}
+
 double target_eval(int kind, const double* tp, int d, const double* x, double* work) {
   switch (kind) {
+    case kTLayout: {
+      const int inner = (int)tp[0];
+      const int nb = (int)tp[1];
+      const double* blocks = tp + 2;
+      const int dl = layout_dim(nb, blocks);
+      std::vector<double> xl(dl), w(std::max(dl, 1));
+      double lj;
+      uint16_t reason;
+      if (layout_forward(nb, blocks, x, xl.data(), &lj, &reason)) return kNaN;
+      return target_eval(inner, tp + 2 + 5 * nb, dl, xl.data(), w.data()) + lj;
+    }
     case kTTransformed: {
       const int inner = (int)tp[0];
       const double* kinds = tp + 1;
@@ -542,6 +647,20 @@ double or_proposal_logdensity(int family, double dof, int kind, int d, const dou
 
 int or_transform_forward(int kind, double lo, double hi, double z, double* x, double* lj) {
   return tf_forward(kind, lo, hi, z, x, lj);
+}
+
+int or_simplex_forward(int km1, const double* z, double* x, double* lj) { return simplex_forward(km1, z, x, lj); }
+
+// n samples (d x n column-major z) through a layout: x (dl x n), lj (n), reason (n) and the
+// failing block (n, 0 when none).
+void or_layout_forward(int nb, const double* blocks, int d, int64_t n, const double* z, double* x, double* lj,
+                       int32_t* reason, int32_t* block) {
+  const int dl = layout_dim(nb, blocks);
+  for (int64_t i = 0; i < n; ++i) {
+    uint16_t r;
+    block[i] = layout_forward(nb, blocks, z + i * d, x + i * dl, &lj[i], &r);
+    reason[i] = r;
+  }
 }
 
 double or_target(int kind, const double* tp, int d, const double* x) {

@@ -15,7 +15,7 @@ LIB = os.path.join(ROOT, "importancesamplers.jl_amd", "libissampler.so")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(is_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|const char\*)\s+(is_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():
@@ -67,7 +67,8 @@ def _header_desc_fields():
 # Julia field type for each C type of the descriptor
 _JULIA_OF_C = {"int32_t": "Int32", "int64_t": "Int64", "uint64_t": "UInt64", "double": "Float64",
                "int64_t*": "Ptr{Int64}", "double*": "Ptr{Float64}", "int32_t*": "Ptr{Int32}",
-               "is_batch_target_fn": "Ptr{Cvoid}", "void*": "Ptr{Cvoid}"}
+               "is_batch_target_fn": "Ptr{Cvoid}", "void*": "Ptr{Cvoid}",
+               "is_transform_block*": "Ptr{TransformBlock}"}
 
 
 def test_integration_julia_descriptor_matches_header():
