@@ -124,6 +124,11 @@ SIGNATURES = {
     "is_job_round_kernel_ms": (ctypes.c_int, [VP, F64P]),
     "is_job_total": (ctypes.c_int64, [VP]),
     "is_job_logical_dim": (ctypes.c_int32, [VP]),
+    "is_group_create": (ctypes.c_int, [VP, ctypes.POINTER(JobDesc), ctypes.c_int32, ctypes.POINTER(VP)]),
+    "is_group_run": (ctypes.c_int, [VP]),
+    "is_group_size": (ctypes.c_int32, [VP]),
+    "is_group_job": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(VP)]),
+    "is_group_destroy": (ctypes.c_int, [VP]),
     "is_job_failure": (ctypes.c_int, [VP, ctypes.POINTER(Failure)]),
     "is_job_get_samples": (ctypes.c_int, [VP, F64P]),
     "is_job_get_logweights": (ctypes.c_int, [VP, F64P]),

@@ -899,18 +899,20 @@ class PreparedSampler:
             self.active_ids = [i + 1 for i in order]
 
     @classmethod
-    def from_handle(cls, handle, rng, target, algorithm, precision):
-        """Wraps a job created elsewhere (the serving loop) with the same descriptor."""
+    def from_handle(cls, handle, rng, target, algorithm, precision, owned_elsewhere=False):
+        """Wraps a job created elsewhere (the serving loop, a job group) with the same
+        descriptor; a job owned elsewhere (a group's member) is destroyed by its owner."""
         self = cls.__new__(cls)
         self.algorithm, self.target, self.precision = algorithm, target, precision
         self.layout = None
+        self._owned_elsewhere = owned_elsewhere
         desc, keep, p, sched = _job_desc(rng, target, algorithm, precision)
         self._adopt(handle, desc, keep, p, sched)
         return self
 
     def __del__(self):
         try:
-            if getattr(self, "_job", None):
+            if getattr(self, "_job", None) and not getattr(self, "_owned_elsewhere", False):
                 self._lib.is_job_destroy(self._job)
                 self._job = None
         except Exception:
@@ -1087,6 +1089,86 @@ def prepare_sampler(rng, target, algorithm, precision="fp64", transform=None):
     coordinates, the target sees the logical sample, results report it)."""
     prec = {"fp64": FP64, "fp32": FP32, FP64: FP64, FP32: FP32}[precision]
     return PreparedSampler(rng, target, algorithm, prec, transform)
+
+
+class PreparedGroup:
+    """Same-shape samplers advanced together (is_group): plain IS or AMIS with a built-in
+    standard or Gaussian target, d <= 15, one family, precision and round schedule;
+    proposals, targets' parameters and seeds may differ.  Every round is one launch over
+    all members; member m's results are those of its own prepared sampler run alone
+    (`members[m]` is that sampler, for the whole results API)."""
+
+    def __init__(self, specs, precision="fp64"):
+        prec = {"fp64": FP64, "fp32": FP32, FP64: FP64, FP32: FP32}[precision]
+        specs = list(specs)
+        if not specs:
+            raise ValueError("a group needs at least one sampler")
+        lib = _lib.load()
+        ctx = context()
+        descs = (JobDesc * len(specs))()
+        self._keep = []
+        for m, (rng, target, alg) in enumerate(specs):
+            desc, keep, _, _ = _job_desc(rng, target, alg, prec)
+            descs[m] = desc
+            self._keep.append(keep)
+        h = VP()
+        _check(lib.is_group_create(ctx, descs, len(specs), ctypes.byref(h)))
+        self._group, self._lib = h, lib
+        self.members = []
+        for m, (rng, target, alg) in enumerate(specs):
+            jh = VP()
+            _check(lib.is_group_job(h, m, ctypes.byref(jh)))
+            self.members.append(PreparedSampler.from_handle(jh, rng, target, alg, prec, owned_elsewhere=True))
+
+    def __len__(self):
+        return len(self.members)
+
+    def run(self):
+        """Runs every member once; returns the indices of the members that failed (each
+        member's failure is raised by `collect`)."""
+        rc = self._lib.is_group_run(self._group)
+        if rc not in (0, _lib.IS_EFAILED):
+            _check(rc)
+        if rc == 0:
+            return []
+        return [m for m, s in enumerate(self.members) if s.failure().round != 0]
+
+    def collect(self, m):
+        """Member m's WeightedSamples of the last run (raising its failure as a stand-alone
+        run would)."""
+        s = self.members[m]
+        f = s.failure()
+        if f.round != 0:
+            s._raise_failure()
+        return s.collect()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_group", None):
+                for s in self.members:
+                    s._job = None
+                self._lib.is_group_destroy(self._group)
+                self._group = None
+        except Exception:
+            pass
+
+
+def prepare_group(specs, precision="fp64"):
+    """A PreparedGroup of (rng, target, algorithm) triples."""
+    return PreparedGroup(specs, precision)
+
+
+def importance_sample_group_(group: PreparedGroup):
+    """Runs the group once and returns every member's WeightedSamples (in member order);
+    a failed member's entry is its exception."""
+    group.run()
+    out = []
+    for m in range(len(group)):
+        try:
+            out.append(group.collect(m))
+        except Exception as exc:  # the member's own failure, as its stand-alone run raises it
+            out.append(exc)
+    return out
 
 
 def importance_sample_(sampler: PreparedSampler):

@@ -64,7 +64,9 @@ def units():
             units.append((os.path.join(CSRC, "batch_k.hip"), os.path.join(OBJDIR, f"batch_k{fp}_w{w}.o"),
                           [f"-DIS_BATCH_FP={fp}", f"-DIS_BATCH_W={w}"]))
     for d in dims():
-        for part in range(6 if d == 64 else 4):  # parts 4, 5: the extended-range kernels
+        # parts 4, 5: the extended-range kernels; 6, 7: the grouped fused rounds (d <= 15)
+        parts = list(range(4)) + ([4, 5] if d == 64 else []) + ([6, 7] if d <= 15 else [])
+        for part in parts:
             units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}_p{part}.o"),
                           [f"-DIS_DIM={d}", f"-DIS_DIM_PART={part}"]))
     return units

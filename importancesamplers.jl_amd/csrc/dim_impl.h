@@ -68,7 +68,6 @@ void launch_sweep_any(int nt, const SweepArgs<T>& a, int blocks, int nw, size_t 
 
 // LDS of a fused round block: the larger of the sweep staging and the new-sample
 // layout, whose slot chunk may grow into kRoundLds.
-constexpr size_t kRoundLds = 40 * 1024;
 
 template <class T, int D>
 bool launch_round_any(int nt, const RoundArgs<T>& a0, const SweepArgs<T>& w, int ncomp, int nb_new, int nrec_old,
@@ -79,13 +78,8 @@ bool launch_round_any(int nt, const RoundArgs<T>& a0, const SweepArgs<T>& w, int
     if (nt != 0 && nt != 1) return false;
     RoundArgs<T> a = a0;
     const bool st = a.fam.student != 0;
-    const size_t slot_bytes = (size_t)a.stride * sizeof(T);
-    const size_t tslot_bytes = a.tkind == kTGaussian ? slot_bytes : 0;
-    const size_t fixed = new_lds_fixed(D, 4, (int)sizeof(T)) + tslot_bytes;
-    const size_t cap = std::max(sweep_lds(nt, 4), kRoundLds);
-    const size_t room = cap > fixed ? cap - fixed : 0;
-    a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), room / slot_bytes));
-    const size_t lds = std::max(sweep_lds(nt, 4), fixed + (size_t)a.slot_chunk * slot_bytes);
+    size_t lds = 0;
+    a.slot_chunk = round_slot_chunk(D, nt, a.stride, (int)sizeof(T), a.tkind == kTGaussian, ncomp, &lds);
     const int blocks = nb_new + nrec_old;
     if (nt == 1) {
       if (st)
@@ -97,6 +91,29 @@ bool launch_round_any(int nt, const RoundArgs<T>& a0, const SweepArgs<T>& w, int
         hipLaunchKernelGGL((k_round<T, D, 0, true>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
       else
         hipLaunchKernelGGL((k_round<T, D, 0, false>), dim3(blocks), dim3(256), lds, s, a, w, nb_new, nrec_old);
+    }
+    return true;
+  }
+}
+
+template <class T, int D>
+bool launch_round_group_any(int nt, bool st, const RoundArgs<T>* ga, const SweepArgs<T>* gw, int jobs, int nb_new,
+                            int nrec_old, size_t lds, hipStream_t s) {
+  if constexpr (D > kExactDims || nt_lo(D) != 1) {
+    return false;
+  } else {
+    if (nt != 0 && nt != 1) return false;
+    const dim3 grid((unsigned)(nb_new + nrec_old), (unsigned)jobs);
+    if (nt == 1) {
+      if (st)
+        hipLaunchKernelGGL((k_round_group<T, D, 1, true>), grid, dim3(256), lds, s, ga, gw, nb_new, nrec_old);
+      else
+        hipLaunchKernelGGL((k_round_group<T, D, 1, false>), grid, dim3(256), lds, s, ga, gw, nb_new, nrec_old);
+    } else {
+      if (st)
+        hipLaunchKernelGGL((k_round_group<T, D, 0, true>), grid, dim3(256), lds, s, ga, gw, nb_new, nrec_old);
+      else
+        hipLaunchKernelGGL((k_round_group<T, D, 0, false>), grid, dim3(256), lds, s, ga, gw, nb_new, nrec_old);
     }
     return true;
   }
@@ -133,6 +150,16 @@ template <int D>
 bool DimKernels<D>::round(int nt, const RoundArgs<double>& a, const SweepArgs<double>& w, int ncomp, int nb_new,
                           int nrec_old, hipStream_t s) {
   return launch_round_any<double, D>(nt, a, w, ncomp, nb_new, nrec_old, s);
+}
+template <int D>
+bool DimKernels<D>::round_group(int nt, bool student, const RoundArgs<float>* ga, const SweepArgs<float>* gw, int jobs,
+                                int nb_new, int nrec_old, size_t lds, hipStream_t s) {
+  return launch_round_group_any<float, D>(nt, student, ga, gw, jobs, nb_new, nrec_old, lds, s);
+}
+template <int D>
+bool DimKernels<D>::round_group(int nt, bool student, const RoundArgs<double>* ga, const SweepArgs<double>* gw,
+                                int jobs, int nb_new, int nrec_old, size_t lds, hipStream_t s) {
+  return launch_round_group_any<double, D>(nt, student, ga, gw, jobs, nb_new, nrec_old, lds, s);
 }
 template <int D>
 void DimKernels<D>::target(const TargetArgs<float>& t, int64_t splits, hipStream_t s) {

@@ -560,6 +560,89 @@ SweepArgs<T> sweep_args(is_job* j, int t) {
   return w;
 }
 
+// The launch arguments of one round of plain IS (t = 0) or AMIS (round t) as the fused
+// form issues them: the new samples' blocks beside the sweep share wo of the retained
+// samples [0, first), then the reduction and finish f.  w is the sweep over every
+// retained sample (the unfused form).  Stand-alone runs and job groups share it.
+template <class T>
+struct RoundPlan {
+  RoundArgs<T> a;
+  SweepArgs<T> w, wo;
+  FinishArgs f;
+  int nt = 0, ncomp = 1, nv = 2, nb_new = 0, nrec_old = 0;
+};
+
+template <class T>
+RoundPlan<T> round_plan(is_job* j, int t, unsigned int* counter) {
+  RoundPlan<T> p;
+  const int d = j->d;
+  const int64_t N = j->schedule[t];
+  RoundArgs<T>& a = p.a;
+  a = base_args<T>(j, t);
+  set_prefilled(j, t, a);
+  a.n_new = N;
+  p.nb_new = (int)cdiv(N, 64);
+  if (j->method == IS_PLAIN) {
+    a.first = 0;
+    a.k0 = a.k1 = 0;
+    a.gen_slot = 0;
+    a.logmass = j->zerosT.as<T>();
+    a.logtotal = T(0);
+    a.record_nonfinite_after_exhaust = 0;
+    p.w = sweep_args<T>(j, 0);
+    p.w.lo = 0;
+    p.w.hi = N;
+    p.w.nv = 2;
+    p.wo = p.w;
+    p.wo.hi = 0;
+    p.wo.per_block = kRoundPerBlock;
+    p.f = base_finish(j, 0, counter);
+    p.f.flags = kFinishSummary;
+    p.f.summary_count = N;
+    p.f.summary_out = j->summary.as<double>();
+    return p;
+  }
+  // AMIS: new samples against the (t + 1)-term temporal mixture, then one sweep over
+  // every retained sample: append of the newest component, weights, summary and moments
+  const int64_t first = j->offsets[t];
+  const int64_t last = j->offsets[t + 1];
+  a.first = first;
+  a.record_nonfinite_after_exhaust = 1;
+  a.k0 = 0;
+  a.k1 = t;
+  a.gen_slot = t;
+  a.logmass = j->logmassT.as<T>();
+  a.logtotal = (T)std::log((double)last);
+  double* slot64_t = j->slots64.as<double>() + (size_t)t * j->stride;
+  SweepArgs<T>& w = p.w;
+  w = sweep_args<T>(j, t);
+  w.lo = 0;
+  w.hi = last;
+  w.append = first > 0;
+  w.append_hi = first;
+  w.slot = j->slotsT.as<T>() + (size_t)t * j->stride;
+  w.logmass = (T)std::log((double)N);
+  w.logtotal = a.logtotal;
+  w.center = slot64_t;
+  w.nv = 2 + d + d * (d + 1) / 2;
+  p.wo = w;
+  p.wo.hi = first;
+  p.wo.per_block = kRoundPerBlock;
+  p.nt = 1;
+  p.ncomp = t + 1;
+  p.nv = w.nv;
+  p.nrec_old = first > 0 ? (int)cdiv(first, kRoundPerBlock) : 0;
+  FinishArgs& f = p.f;
+  f = base_finish(j, t, counter);
+  f.flags = kFinishSummary | kFinishFit;
+  f.summary_count = last;
+  f.center = slot64_t;
+  f.prev_slot = slot64_t;
+  f.next_slot64 = j->slots64.as<double>() + (size_t)(t + 1) * j->stride;
+  f.next_slot_t = j->slotsT.as<T>() + (size_t)(t + 1) * j->stride;
+  return p;
+}
+
 template <class T>
 int run_rounds(is_job* j) {
   hipStream_t s = j->stream;
@@ -709,44 +792,23 @@ int run_rounds(is_job* j) {
     return IS_OK;
   }
   if (j->method == IS_PLAIN) {
-    const int64_t N = j->schedule[0];
-    RoundArgs<T> a = base_args<T>(j, 0);
-    set_prefilled(j, 0, a);
-    a.first = 0;
-    a.n_new = N;
-    a.k0 = a.k1 = 0;
-    a.gen_slot = 0;
-    a.logmass = j->zerosT.as<T>();
-    a.logtotal = T(0);
-    a.record_nonfinite_after_exhaust = 0;
-    SweepArgs<T> w = sweep_args<T>(j, 0);
-    w.lo = 0;
-    w.hi = N;
-    w.nv = 2;
+    RoundPlan<T> p = round_plan<T>(j, 0, counter);
     int nrec = 0;
     bool fused = false;
     if (!batch_target) {
       // one launch: the new-sample blocks write their own summary records
-      SweepArgs<T> wo = w;
-      wo.hi = 0;
-      wo.per_block = kRoundPerBlock;
-      const int nb_new = (int)cdiv(N, 64);
       tick(j, 0, 0, s);
-      fused = launch_round<T>(D, 0, 1, a, wo, nb_new, 0, s);
+      fused = launch_round<T>(D, 0, 1, p.a, p.wo, p.nb_new, 0, s);
       tick(j, 0, 1, s);
-      nrec = nb_new;
+      nrec = p.nb_new;
     }
     if (!fused) {
       tick(j, 0, 0, s);
-      if (const int mrc = mix(a, 1)) return mrc;
+      if (const int mrc = mix(p.a, 1)) return mrc;
       tick(j, 0, 1, s);
-      nrec = launch_sweep<T>(D, 0, w, s);
+      nrec = launch_sweep<T>(D, 0, p.w, s);
     }
-    FinishArgs f = base_finish(j, 0, counter);
-    f.flags = kFinishSummary;
-    f.summary_count = N;
-    f.summary_out = j->summary.as<double>();
-    reduce(j->partials, nrec, 2, j->reduced, f);
+    reduce(j->partials, nrec, 2, j->reduced, p.f);
     HIP_TRY(hipGetLastError());
     return IS_OK;
   }
@@ -762,52 +824,24 @@ int run_rounds(is_job* j) {
     a.record_nonfinite_after_exhaust = 1;
     double* slot64_t = j->slots64.as<double>() + (size_t)t * j->stride;
     if (j->method == IS_AMIS) {
-      // new samples against the t-term temporal mixture, then one sweep over every
-      // retained sample: append of the newest component, weights, summary and moments
-      a.k0 = 0;
-      a.k1 = t;
-      a.gen_slot = t;
-      a.logmass = j->logmassT.as<T>();
-      a.logtotal = (T)std::log((double)last);
-      SweepArgs<T> w = sweep_args<T>(j, t);
-      w.lo = 0;
-      w.hi = last;
-      w.append = first > 0;
-      w.append_hi = first;
-      w.slot = j->slotsT.as<T>() + (size_t)t * j->stride;
-      w.logmass = (T)std::log((double)N);
-      w.logtotal = a.logtotal;
-      w.center = slot64_t;
-      w.nv = nv_mom;
+      RoundPlan<T> p = round_plan<T>(j, t, counter);
       int nrec = 0;
       bool fused = false;
-      if (!batch_target && !w.clip) {
+      if (!batch_target && !p.w.clip) {
         // one launch: the new samples' blocks (with their own moment records) beside
         // the sweep of the retained samples [0, first)
-        SweepArgs<T> wo = w;
-        wo.hi = first;
-        wo.per_block = kRoundPerBlock;
-        const int nrec_old = first > 0 ? (int)cdiv(first, kRoundPerBlock) : 0;
-        const int nb_new = (int)cdiv(N, 64);
         tick(j, t, 0, s);
-        fused = launch_round<T>(D, 1, t + 1, a, wo, nb_new, nrec_old, s);
+        fused = launch_round<T>(D, 1, t + 1, p.a, p.wo, p.nb_new, p.nrec_old, s);
         tick(j, t, 1, s);
-        nrec = nrec_old + nb_new;
+        nrec = p.nrec_old + p.nb_new;
       }
       if (!fused) {
-        if (const int mrc = mix(a, t + 1)) return mrc;
+        if (const int mrc = mix(p.a, t + 1)) return mrc;
         tick(j, t, 0, s);
-        nrec = launch_sweep<T>(D, nt_mom, w, s);
+        nrec = launch_sweep<T>(D, nt_mom, p.w, s);
         tick(j, t, 1, s);
       }
-      FinishArgs f = base_finish(j, t, counter);
-      f.flags = kFinishSummary | kFinishFit;
-      f.summary_count = last;
-      f.center = slot64_t;
-      f.prev_slot = slot64_t;
-      f.next_slot64 = j->slots64.as<double>() + (size_t)(t + 1) * j->stride;
-      f.next_slot_t = j->slotsT.as<T>() + (size_t)(t + 1) * j->stride;
-      reduce(j->partials, nrec, nv_mom, j->reduced, f);
+      reduce(j->partials, nrec, nv_mom, j->reduced, p.f);
     } else if (j->method == IS_NPMC) {
       a.k0 = a.k1 = t;
       a.gen_slot = t;
@@ -920,6 +954,36 @@ void decode(uint64_t packed, int64_t& index, int32_t& block, uint16_t& bits) {
 }
 
 template <class T>
+int run_epilogue(is_job* j);
+
+// Slot-table entries of the committed state: the bank's K slots (DM-PMC, static MIS) or slot 0.
+size_t committed_slots(const is_job* j) {
+  const bool bank = j->method == IS_DMPMC || j->method == IS_MIS;
+  return (size_t)(bank ? j->K : 1) * j->stride;
+}
+
+// The committed state as the run's first slots, FP64 (h64) and in T (hT, the inverse
+// diagonals that are not normal in T marked for the extended-range density).
+template <class T>
+void pack_committed(is_job* j, double* h64, T* hT) {
+  const int d = j->d;
+  const size_t n = committed_slots(j);
+  if (j->method == IS_DMPMC || j->method == IS_MIS) {
+    std::copy(j->bank_slots_init.begin(), j->bank_slots_init.begin() + n, h64);
+    for (int k = 0; k < j->K; ++k)
+      for (int c = 0; c < d; ++c) h64[(size_t)k * j->stride + c] = j->committed_locs[(size_t)k * d + c];
+  } else {
+    pack_slot(j->committed, d, 0.0, h64);
+  }
+  for (size_t i = 0; i < n; ++i) hT[i] = (T)h64[i];
+  for (size_t k = 0; k < n / (size_t)j->stride; ++k)
+    for (int r = 0; r < d; ++r) {
+      const size_t at = k * j->stride + d + r;
+      hT[at] = slot_inv_value<T>(h64[at]);
+    }
+}
+
+template <class T>
 int job_run_typed(is_job* j) {
   hipStream_t s = j->stream;
   const int T_ = j->T;
@@ -931,24 +995,10 @@ int job_run_typed(is_job* j) {
   const int d = j->d;
   const bool fixed_state = j->method == IS_PLAIN || j->method == IS_MIS;
   if (!(fixed_state && j->slots_resident)) {
-    const bool bank = j->method == IS_DMPMC || j->method == IS_MIS;
-    const size_t n = (size_t)(bank ? j->K : 1) * j->stride;
     double* h64 = j->slot_up.as<double>();
+    const size_t n = committed_slots(j);
     T* hT = reinterpret_cast<T*>(h64 + n);
-    if (bank) {
-      std::copy(j->bank_slots_init.begin(), j->bank_slots_init.begin() + n, h64);
-      for (int k = 0; k < j->K; ++k)
-        for (int c = 0; c < d; ++c) h64[(size_t)k * j->stride + c] = j->committed_locs[(size_t)k * d + c];
-    } else {
-      pack_slot(j->committed, d, 0.0, h64);
-    }
-    for (size_t i = 0; i < n; ++i) hT[i] = (T)h64[i];
-    // inverse diagonals that are not normal in T mark the slot for the extended-range density
-    for (size_t k = 0; k < n / (size_t)j->stride; ++k)
-      for (int r = 0; r < d; ++r) {
-        const size_t at = k * j->stride + d + r;
-        hT[at] = slot_inv_value<T>(h64[at]);
-      }
+    pack_committed<T>(j, h64, hT);
     HIP_TRY(hipMemcpyAsync(j->slots64.p, h64, n * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(j->slotsT.p, hT, n * sizeof(T), hipMemcpyHostToDevice, s));
     j->slots_resident = fixed_state;
@@ -991,6 +1041,16 @@ int job_run_typed(is_job* j) {
     HIP_TRY(hipMemcpyAsync(j->slot_down.p, j->slots64.p, (size_t)j->K * j->stride * sizeof(double),
                            hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return run_epilogue<T>(j);
+}
+
+// A run's outcome from its status image in j->ctl_down (and, for AMIS / NPMC / DM-PMC, the
+// fitted state in j->slot_down): the failure snapshot, the extended-range replay of a run
+// whose densities left the floating-point range, and the transactional commit.
+template <class T>
+int run_epilogue(is_job* j) {
+  const int T_ = j->T;
+  const int d = j->d;
   const char* down = j->ctl_down.as<const char>();
   const uint64_t* rec = reinterpret_cast<const uint64_t*>(down + ((const char*)j->fail_rec.p - j->ctl.as<char>()));
   const int* fit = reinterpret_cast<const int*>(down + ((const char*)j->fit_status.p - j->ctl.as<char>()));
@@ -2710,6 +2770,332 @@ int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t 
                      tmp.as<unsigned long long>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(locked_memcpy(out, tmp.p, tmp.bytes, hipMemcpyDeviceToHost));
+  return IS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- job groups
+// A group advances `count` same-shape jobs (plain IS or AMIS with a built-in standard or
+// Gaussian target, d <= 15, no transforms) together: each round is one k_round_group
+// launch over every member (blockIdx.y = member) and one k_reduce_finish_group, the
+// whole run one captured graph between a staging upload and one status download.  The
+// members' launch arguments are the stand-alone runs' own (round_plan), so every member's
+// results equal its stand-alone run's.
+namespace {
+
+// Per member: its control block and slot tables.
+struct GroupMember {
+  uint64_t* ctl;
+  double* slot0_64;
+  void* slot0_t;
+  const double* slotT_64;
+};
+
+// Run start for every member (block = member): the run word and the zeroed status words
+// [1, up_words) of the control block, and the committed slot images staged by the host
+// (n64 FP64 values, then the T copy of tbytes bytes).
+__global__ __launch_bounds__(256) void k_group_begin(const GroupMember* __restrict__ mem, const uint64_t* __restrict__ stage,
+                                                     int members, int up_words, int n64, int tbytes) {
+  const int m = blockIdx.x;
+  const GroupMember g = mem[m];
+  const uint64_t* img64 = stage + members + (size_t)m * n64;
+  const int twords = (tbytes + 7) / 8;
+  const uint64_t* imgT = stage + members + (size_t)members * n64 + (size_t)m * twords;
+  for (int q = threadIdx.x; q < up_words; q += blockDim.x) g.ctl[q] = q == 0 ? stage[m] : 0ull;
+  uint64_t* s64 = reinterpret_cast<uint64_t*>(g.slot0_64);
+  for (int q = threadIdx.x; q < n64; q += blockDim.x) s64[q] = img64[q];
+  unsigned char* sT = reinterpret_cast<unsigned char*>(g.slot0_t);
+  const unsigned char* iT = reinterpret_cast<const unsigned char*>(imgT);
+  for (int q = threadIdx.x; q < tbytes; q += blockDim.x) sT[q] = iT[q];
+}
+
+// Run end for every member: its status words [down_word, total_words) and, for AMIS, the
+// fitted final slot (n64 FP64 values) into its row of the gather buffer.
+__global__ __launch_bounds__(256) void k_group_end(const GroupMember* __restrict__ mem, uint64_t* __restrict__ gather,
+                                                   int down_word, int total_words, int n64, int row_words) {
+  const int m = blockIdx.x;
+  const GroupMember g = mem[m];
+  uint64_t* out = gather + (size_t)m * row_words;
+  const int nd = total_words - down_word;
+  for (int q = threadIdx.x; q < nd; q += blockDim.x) out[q] = g.ctl[down_word + q];
+  if (g.slotT_64) {
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(g.slotT_64);
+    for (int q = threadIdx.x; q < n64; q += blockDim.x) out[nd + q] = src[q];
+  }
+}
+
+struct GroupRound {
+  int nb_new = 0, nrec_old = 0, nt = 0, nv = 2;
+  int64_t nrec = 0;
+  size_t lds = 0;
+};
+
+}  // namespace
+
+struct is_group {
+  is_ctx* ctx = nullptr;
+  std::vector<is_job*> jobs;
+  int method = IS_PLAIN, precision = IS_FP64, d = 0, D = 0, T = 1, student = 0;
+  hipStream_t stream = nullptr;
+  std::vector<GroupRound> rounds;
+  DevBuf round_args, reduce_args, members, stage, gather;
+  HostPinned stage_up, gather_down;
+  size_t round_bytes = 0, sweep_bytes = 0;
+  int up_words = 0, down_word = 0, total_words = 0, n64 = 0, tbytes = 0, row_words = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::atomic<bool> busy{false};
+  ~is_group() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    CaptureGuard cg(capture_lock());
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+template <class T>
+int group_prepare(is_group* g) {
+  const int M = (int)g->jobs.size();
+  is_job* j0 = g->jobs[0];
+  g->round_bytes = sizeof(RoundArgs<T>);
+  g->sweep_bytes = sizeof(SweepArgs<T>);
+  g->rounds.assign(g->T, GroupRound{});
+  std::vector<char> rargs((size_t)g->T * M * (g->round_bytes + g->sweep_bytes));
+  std::vector<GroupReduce> red((size_t)g->T * M);
+  for (int t = 0; t < g->T; ++t) {
+    for (int m = 0; m < M; ++m) {
+      is_job* j = g->jobs[m];
+      RoundPlan<T> p = round_plan<T>(j, t, j->counter.as<unsigned int>());
+      GroupRound& r = g->rounds[t];
+      p.a.slot_chunk = round_slot_chunk(g->D, p.nt, p.a.stride, (int)sizeof(T), p.a.tkind == kTGaussian, p.ncomp, &r.lds);
+      r.nb_new = p.nb_new;
+      r.nrec_old = p.nrec_old;
+      r.nt = p.nt;
+      r.nv = p.nv;
+      r.nrec = (int64_t)p.nrec_old + p.nb_new;
+      char* base = rargs.data() + (size_t)t * M * (g->round_bytes + g->sweep_bytes);
+      std::memcpy(base + (size_t)m * g->round_bytes, &p.a, g->round_bytes);
+      std::memcpy(base + (size_t)M * g->round_bytes + (size_t)m * g->sweep_bytes, &p.wo, g->sweep_bytes);
+      GroupReduce& gr = red[(size_t)t * M + m];
+      gr.partials = j->partials.as<double>();
+      gr.reduced = j->reduced.as<double>();
+      gr.f = p.f;
+    }
+  }
+  int rc;
+  if ((rc = g->round_args.alloc(rargs.size())) || (rc = g->reduce_args.alloc(red.size() * sizeof(GroupReduce))))
+    return rc;
+  HIP_TRY(locked_memcpy(g->round_args.p, rargs.data(), rargs.size(), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(g->reduce_args.p, red.data(), red.size() * sizeof(GroupReduce), hipMemcpyHostToDevice));
+  // member table, staging and gather layouts
+  std::vector<GroupMember> mem(M);
+  for (int m = 0; m < M; ++m) {
+    is_job* j = g->jobs[m];
+    mem[m].ctl = j->ctl.as<uint64_t>();
+    mem[m].slot0_64 = j->slots64.as<double>();
+    mem[m].slot0_t = j->slotsT.p;
+    mem[m].slotT_64 = j->method == IS_AMIS ? j->slots64.as<double>() + (size_t)j->T * j->stride : nullptr;
+  }
+  if ((rc = g->members.alloc(mem.size() * sizeof(GroupMember)))) return rc;
+  HIP_TRY(locked_memcpy(g->members.p, mem.data(), mem.size() * sizeof(GroupMember), hipMemcpyHostToDevice));
+  g->up_words = (int)(j0->ctl_up_bytes / 8);
+  g->down_word = (int)(j0->ctl_down_off / 8);
+  g->total_words = (int)(j0->ctl.bytes / 8);
+  g->n64 = j0->stride;
+  g->tbytes = j0->stride * (int)sizeof(T);
+  g->row_words = (g->total_words - g->down_word) + (j0->method == IS_AMIS ? g->n64 : 0);
+  const size_t stage_words = (size_t)M + (size_t)M * g->n64 + (size_t)M * ((g->tbytes + 7) / 8);
+  if ((rc = g->stage.alloc(stage_words * 8)) || (rc = g->stage_up.alloc(stage_words * 8)) ||
+      (rc = g->gather.alloc((size_t)M * g->row_words * 8)) || (rc = g->gather_down.alloc((size_t)M * g->row_words * 8)))
+    return rc;
+  std::memset(g->stage_up.p, 0, stage_words * 8);
+  return IS_OK;
+}
+
+template <class T>
+int group_issue(is_group* g) {
+  const int M = (int)g->jobs.size();
+  hipStream_t s = g->stream;
+  hipLaunchKernelGGL(k_group_begin, dim3(M), dim3(256), 0, s, g->members.as<GroupMember>(), g->stage.as<uint64_t>(), M,
+                     g->up_words, g->n64, g->tbytes);
+  const size_t per_round = (size_t)M * (g->round_bytes + g->sweep_bytes);
+  for (int t = 0; t < g->T; ++t) {
+    const GroupRound& r = g->rounds[t];
+    const char* base = g->round_args.as<char>() + (size_t)t * per_round;
+    const RoundArgs<T>* ga = reinterpret_cast<const RoundArgs<T>*>(base);
+    const SweepArgs<T>* gw = reinterpret_cast<const SweepArgs<T>*>(base + (size_t)M * g->round_bytes);
+    bool done = false;
+    with_dim(g->D, [&](auto k) {
+      done = DimKernels<decltype(k)::value>::round_group(r.nt, g->student != 0, ga, gw, M, r.nb_new, r.nrec_old, r.lds,
+                                                          s);
+    });
+    if (!done) return fail(IS_EUNSUPPORTED, "no fused round for dimension %d", g->d);
+    const GroupReduce* gr = g->reduce_args.as<GroupReduce>() + (size_t)t * M;
+    const int64_t pcap = g->jobs[0]->pcap;
+    switch (g->d) {
+#define IS_GROUP_REDUCE(v)                                                                                     \
+  case v:                                                                                                      \
+    hipLaunchKernelGGL((k_reduce_finish_group<v, true>), dim3(r.nv, M), dim3(kBlock), 0, s, gr, r.nrec, pcap, r.nv); \
+    break;
+      IS_EXACT_DIMS(IS_GROUP_REDUCE)
+#undef IS_GROUP_REDUCE
+      default:
+        return fail(IS_EUNSUPPORTED, "no grouped reduction for dimension %d", g->d);
+    }
+  }
+  hipLaunchKernelGGL(k_group_end, dim3(M), dim3(256), 0, s, g->members.as<GroupMember>(), g->gather.as<uint64_t>(),
+                     g->down_word, g->total_words, g->n64, g->row_words);
+  HIP_TRY(hipGetLastError());
+  return IS_OK;
+}
+
+template <class T>
+int group_run_typed(is_group* g) {
+  const int M = (int)g->jobs.size();
+  hipStream_t s = g->stream;
+  // stage the members' run words and committed slot images
+  uint64_t* st = g->stage_up.as<uint64_t>();
+  const int twords = (g->tbytes + 7) / 8;
+  for (int m = 0; m < M; ++m) {
+    is_job* j = g->jobs[m];
+    st[m] = (uint64_t)j->run_index << 32;
+    double* h64 = reinterpret_cast<double*>(st + M + (size_t)m * g->n64);
+    T* hT = reinterpret_cast<T*>(st + M + (size_t)M * g->n64 + (size_t)m * twords);
+    pack_committed<T>(j, h64, hT);
+  }
+  HIP_TRY(hipMemcpyAsync(g->stage.p, g->stage_up.p, g->stage_up.bytes, hipMemcpyHostToDevice, s));
+  if (!g->graph_exec) {
+    CaptureGuard cg(capture_lock());
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = group_issue<T>(g);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(IS_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    g->graph = gr;
+    HIP_TRY(hipGraphInstantiate(&g->graph_exec, gr, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipGraphLaunch(g->graph_exec, s));
+  HIP_TRY(hipMemcpyAsync(g->gather_down.p, g->gather.p, g->gather_down.bytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  // each member's outcome from its status image, as a stand-alone run's
+  int result = IS_OK;
+  const size_t down_bytes = (size_t)(g->total_words - g->down_word) * 8;
+  for (int m = 0; m < M; ++m) {
+    is_job* j = g->jobs[m];
+    const char* row = g->gather_down.as<const char>() + (size_t)m * g->row_words * 8;
+    std::memcpy(j->ctl_down.as<char>() + j->ctl_down_off, row, down_bytes);
+    if (j->method == IS_AMIS) std::memcpy(j->slot_down.p, row + down_bytes, (size_t)g->n64 * 8);
+    j->slots_resident = false;
+    const int rc = run_epilogue<T>(j);
+    if (rc == IS_EFAILED)
+      result = IS_EFAILED;
+    else if (rc != IS_OK)
+      return rc;
+  }
+  return result;
+}
+
+}  // namespace
+
+extern "C" {
+
+int is_group_create(is_ctx* ctx, const is_job_desc* descs, int32_t count, is_group** out) {
+  if (!ctx || !descs || !out || count < 1) return fail(IS_EINVAL, "bad arguments");
+  const is_job_desc& a = descs[0];
+  for (int32_t m = 0; m < count; ++m) {
+    const is_job_desc& b = descs[m];
+    if (b.method != IS_PLAIN && b.method != IS_AMIS)
+      return fail(IS_EUNSUPPORTED, "group member %d: groups run plain IS and AMIS", m);
+    if (b.target_kind != IS_TARGET_STANDARD && b.target_kind != IS_TARGET_GAUSSIAN)
+      return fail(IS_EUNSUPPORTED, "group member %d: groups take the built-in standard and Gaussian targets", m);
+    if (b.transform_kinds || b.transform_block_count)
+      return fail(IS_EUNSUPPORTED, "group member %d: groups run without transforms", m);
+    if (b.dim < 1 || b.dim > 15) return fail(IS_EUNSUPPORTED, "group member %d: groups take d <= 15", m);
+    const int Tb = b.method == IS_PLAIN ? 1 : b.rounds;
+    const int Ta = a.method == IS_PLAIN ? 1 : a.rounds;
+    if (b.method != a.method || b.dim != a.dim || b.family != a.family || b.precision != a.precision || Tb != Ta ||
+        !b.round_size)
+      return fail(IS_EINVAL, "group member %d: members share method, dimension, family, precision and rounds", m);
+    for (int t = 0; t < Tb; ++t)
+      if (b.round_size[t] != a.round_size[t]) return fail(IS_EINVAL, "group member %d: round sizes differ", m);
+  }
+  std::unique_ptr<is_group> g(new is_group());
+  g->ctx = ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  for (int32_t m = 0; m < count; ++m) {
+    is_job* j = nullptr;
+    const int rc = is_job_create(ctx, &descs[m], &j);
+    if (rc) {
+      for (is_job* q : g->jobs) is_job_destroy(q);
+      return rc;
+    }
+    g->jobs.push_back(j);
+  }
+  is_job* j0 = g->jobs[0];
+  g->method = j0->method;
+  g->precision = j0->precision;
+  g->d = j0->d;
+  g->D = j0->D;
+  g->T = j0->T;
+  g->student = j0->family == IS_STUDENT_T;
+  {
+    CaptureGuard cg(capture_lock());
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+      for (is_job* q : g->jobs) is_job_destroy(q);
+      return fail(IS_EDEVICE, "hipStreamCreate failed");
+    }
+  }
+  const int rc = g->precision == IS_FP32 ? group_prepare<float>(g.get()) : group_prepare<double>(g.get());
+  if (rc) {
+    for (is_job* q : g->jobs) is_job_destroy(q);
+    return rc;
+  }
+  *out = g.release();
+  return IS_OK;
+}
+
+int is_group_run(is_group* group) {
+  if (!group) return fail(IS_EINVAL, "null group");
+  bool expected = false;
+  if (!group->busy.compare_exchange_strong(expected, true)) return fail(IS_EBUSY, "group is already running");
+  std::vector<std::unique_ptr<BusyGuard>> guards;
+  for (is_job* j : group->jobs) {
+    guards.emplace_back(new BusyGuard(j));
+    if (!guards.back()->ok) {
+      guards.clear();
+      group->busy = false;
+      return fail(IS_EBUSY, "a member is running on its own");
+    }
+  }
+  (void)hipSetDevice(group->ctx->device);
+  const int rc = group->precision == IS_FP32 ? group_run_typed<float>(group) : group_run_typed<double>(group);
+  guards.clear();
+  group->busy = false;
+  return rc;
+}
+
+int32_t is_group_size(const is_group* group) { return group ? (int32_t)group->jobs.size() : 0; }
+
+int is_group_job(is_group* group, int32_t index, is_job** out) {
+  if (!group || !out || index < 0 || index >= (int32_t)group->jobs.size()) return fail(IS_EINVAL, "bad arguments");
+  *out = group->jobs[index];
+  return IS_OK;
+}
+
+int is_group_destroy(is_group* group) {
+  if (!group) return IS_OK;
+  if (group->stream) (void)hipStreamSynchronize(group->stream);
+  for (is_job* j : group->jobs) is_job_destroy(j);
+  group->jobs.clear();
+  delete group;
   return IS_OK;
 }
 

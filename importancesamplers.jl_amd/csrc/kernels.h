@@ -163,9 +163,8 @@ __device__ __forceinline__ void fit_small(const double* __restrict__ red, const 
                                           const FitPrefetch<DR>& pf, bool summary);
 
 template <int DR, bool EXACT>
-static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
-                                                          int64_t nrec, int64_t pcap, int nv,
-                                                          double* __restrict__ reduced, FinishArgs f) {
+__device__ __forceinline__ void reduce_finish_block(const double* __restrict__ partials, int64_t nrec, int64_t pcap,
+                                                    int nv, double* __restrict__ reduced, const FinishArgs& f) {
   __shared__ double wm[kBlock / 64], ws[kBlock / 64];
   const int tid = threadIdx.x;
   const int k = blockIdx.x;
@@ -244,6 +243,26 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* _
   } else {
     (void)f;
   }
+}
+
+template <int DR, bool EXACT>
+static __global__ __launch_bounds__(kBlock) void k_reduce_finish(const double* __restrict__ partials,
+                                                          int64_t nrec, int64_t pcap, int nv,
+                                                          double* __restrict__ reduced, FinishArgs f) {
+  reduce_finish_block<DR, EXACT>(partials, nrec, pcap, nv, reduced, f);
+}
+
+// One job's reduction in a group launch (grid nv x jobs).
+struct GroupReduce {
+  const double* partials;
+  double* reduced;
+  FinishArgs f;
+};
+template <int DR, bool EXACT>
+static __global__ __launch_bounds__(kBlock) void k_reduce_finish_group(const GroupReduce* __restrict__ g, int64_t nrec,
+                                                                int64_t pcap, int nv) {
+  const GroupReduce r = g[blockIdx.y];
+  reduce_finish_block<DR, EXACT>(r.partials, nrec, pcap, nv, r.reduced, r.f);
 }
 
 // Weighted functional partial records for Statistics.mean(f, r) and var(f, r)

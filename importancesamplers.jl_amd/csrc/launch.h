@@ -5,6 +5,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 
 #include "kernels.h"
@@ -35,6 +36,20 @@ inline size_t sweep_lds(int nt, int nw) {
 
 constexpr size_t kNewSlotBudget = 48 * 1024;  // LDS bytes for staged slots per new-sample block
 
+// Fused round launches (k_round, k_round_group; D <= 15): the mixture slots staged per
+// chunk and the dynamic LDS bytes, for nt moment tiles and ncomp mixture components.
+constexpr size_t kRoundLds = 40 * 1024;
+inline int round_slot_chunk(int D, int nt, int stride, int tsize, bool gaussian_target, int ncomp, size_t* lds) {
+  const size_t slot_bytes = (size_t)stride * tsize;
+  const size_t tslot_bytes = gaussian_target ? slot_bytes : 0;
+  const size_t fixed = new_lds_fixed(D, 4, tsize) + tslot_bytes;
+  const size_t cap = std::max(sweep_lds(nt, 4), kRoundLds);
+  const size_t room = cap > fixed ? cap - fixed : 0;
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), room / slot_bytes));
+  *lds = std::max(sweep_lds(nt, 4), fixed + (size_t)chunk * slot_bytes);
+  return chunk;
+}
+
 template <int D>
 struct DimKernels {
   // k_new in mode `mode` (kDrawFused / kDrawOnly / kPreTarget), 64 samples per block of W waves
@@ -50,6 +65,12 @@ struct DimKernels {
                     int nrec_old, hipStream_t s);
   static bool round(int nt, const RoundArgs<double>& a, const SweepArgs<double>& w, int ncomp, int nb_new,
                     int nrec_old, hipStream_t s);
+  // k_round_group: the same round for `jobs` same-shape jobs (device argument tables, the
+  // slot chunk already set by round_slot_chunk); false without a fused round
+  static bool round_group(int nt, bool student, const RoundArgs<float>* ga, const SweepArgs<float>* gw, int jobs,
+                          int nb_new, int nrec_old, size_t lds, hipStream_t s);
+  static bool round_group(int nt, bool student, const RoundArgs<double>* ga, const SweepArgs<double>* gw, int jobs,
+                          int nb_new, int nrec_old, size_t lds, hipStream_t s);
   // k_target_gemm over `splits` row slices
   static void target(const TargetArgs<float>& t, int64_t splits, hipStream_t s);
   static void target(const TargetArgs<double>& t, int64_t splits, hipStream_t s);

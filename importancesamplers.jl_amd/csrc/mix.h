@@ -692,7 +692,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
 // The new-sample blocks (few, and bound by their mixture latency) then run
 // beside the sweep instead of before it.
 template <class T, int D, int NT, bool ST>
-__global__ __launch_bounds__(256) void k_round(RoundArgs<T> a, SweepArgs<T> w, int nb_new, int nrec_old) {
+__device__ __forceinline__ void round_block(const RoundArgs<T>& a, const SweepArgs<T>& w, int nb_new, int nrec_old) {
   extern __shared__ double lds[];
   if ((int)blockIdx.x < nb_new) {
     constexpr int RS = sweep_row_stride(NT);
@@ -715,4 +715,20 @@ __global__ __launch_bounds__(256) void k_round(RoundArgs<T> a, SweepArgs<T> w, i
   } else {
     sweep_block<T, D, NT>(w, (int64_t)blockIdx.x - nb_new);
   }
+}
+
+template <class T, int D, int NT, bool ST>
+__global__ __launch_bounds__(256) void k_round(RoundArgs<T> a, SweepArgs<T> w, int nb_new, int nrec_old) {
+  round_block<T, D, NT, ST>(a, w, nb_new, nrec_old);
+}
+
+// The same round for a group of same-shape jobs: blockIdx.y selects the job, whose
+// arguments are read (scalar loads, before any store) from the group's argument tables.
+template <class T, int D, int NT, bool ST>
+__global__ __launch_bounds__(256) void k_round_group(const RoundArgs<T>* __restrict__ ga,
+                                                     const SweepArgs<T>* __restrict__ gw, int nb_new,
+                                                     int nrec_old) {
+  const RoundArgs<T> a = ga[blockIdx.y];
+  const SweepArgs<T> w = gw[blockIdx.y];
+  round_block<T, D, NT, ST>(a, w, nb_new, nrec_old);
 }

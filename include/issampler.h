@@ -363,6 +363,26 @@ int is_proposal_logdensity(is_ctx* ctx, int32_t family, double dof, int32_t scal
 int is_rng_raw(is_ctx* ctx, uint64_t key0, uint64_t key1, uint64_t c1, uint64_t c2, uint64_t c3,
                int64_t n, uint64_t* out);
 
+/* ---- job groups: same-shape samplers advanced together ---------------------------
+ * A group holds `count` prepared samplers of one shape -- plain IS or AMIS, one family,
+ * dimension d <= 15, precision and round sizes; a built-in standard or Gaussian target;
+ * no transforms -- whose proposals, target parameters and seeds are free.  is_group_run
+ * runs every member once: each round is one launch over all members (the stand-alone
+ * fused round with the member in blockIdx.y) and one grouped reduction, the whole run
+ * one captured graph between one staging upload and one status download.  Member m's
+ * outcome and results are those of importance_sample! on its own descriptor (the same
+ * launch arguments and counter-RNG draws): is_group_job returns it for every is_job_
+ * results call and its is_job_failure.  is_group_run returns IS_EFAILED when a member
+ * failed.  The group owns its members (destroyed with it); a member may also run alone,
+ * not while its group runs (IS_EBUSY).  No reference counterpart: the reference runs
+ * such samplers one at a time (src/methods/importance_sampling.jl:912-953). */
+typedef struct is_group is_group;
+int is_group_create(is_ctx* ctx, const is_job_desc* descs, int32_t count, is_group** out);
+int is_group_run(is_group* group);
+int32_t is_group_size(const is_group* group);
+int is_group_job(is_group* group, int32_t index, is_job** out);
+int is_group_destroy(is_group* group);
+
 /* ---- batched AMIS: many independent posteriors in one persistent launch ----------
  * A batch is a vector of prepared AMIS samplers run together; job j means
  * importance_sample!(prepare_sampler(rng_j, target_j, AMIS(q_j; rounds, round_size)))
