@@ -652,6 +652,139 @@ def main_mis(args):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------- grouped solves
+GROUP_PLAIN_MEMBERS, GROUP_AMIS_MEMBERS = 4096, 32
+
+
+def _group_steps(group, steps, warmup):
+    """Times `steps` grouped solves (every member reset to its initial proposal, one
+    is_group_run, the members' final-round ESS read back): (ESS/s, elapsed, latencies)."""
+    for _ in range(warmup):
+        group.reset()
+        group.run()
+    lat, ess_total = [], 0.0
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        group.reset()
+        failed = group.run()
+        lat.append(time.perf_counter() - t0)
+        assert not failed, f"group members failed: {failed[:8]}"
+        ess_total += float(np.sum(group.round_summary(-1)[0]))
+    elapsed = time.perf_counter() - t_start
+    return ess_total / elapsed, elapsed, lat
+
+
+def grouped_plain_summary(pkg, steps, warmup, check=True):
+    """Config 1 as a grouped solve: GROUP_PLAIN_MEMBERS independent plain-IS problems
+    (d = 3, -|x|^2/2 under SphericalGaussian(0, 1.5), N = 10,000, seeds 42, 43, ...) in
+    one launch per step, FP32, against the plain FP64 form (one importance_sample! of
+    the same problem per step); members 0..7 checked against the FP64 oracle on their
+    own counter-RNG normals."""
+    import oracle
+
+    q = pkg.SphericalGaussian(np.zeros(3), 1.5)
+    alg = pkg.ImportanceSampling(q, 10_000)
+    tgt = pkg.StandardGaussianTarget()
+    M = GROUP_PLAIN_MEMBERS
+    # the plain FP64 form
+    s64 = pkg.prepare_sampler(42, tgt, alg, precision="fp64")
+    ess64, el64, lat64 = _solve_stats(pkg, s64, 20 * steps, warmup, lambda x: float(_round_ess(x)[0][0]))
+    del s64
+    g = pkg.prepare_group([(42 + m, tgt, alg) for m in range(M)], "fp32")
+    acc = None
+    if check:
+        # the first run, against the oracle on the normals it draws (exported before it)
+        zs = [g.members[m].export_randomness(0)["normals"] for m in range(8)]
+        assert not g.run()
+        dz, dm = 0.0, 0.0
+        for m in range(8):
+            xo, lwo, fail = oracle.plain_is(q, 0, None, zs[m])
+            assert fail[0] == 0
+            mo, _, co = oracle.moments(xo.reshape(-1, 3), lwo)
+            r = g.collect(m)
+            dz = max(dz, abs(pkg.lognormalizer(r) - oracle.lognormalizer(lwo)))
+            dm = max(dm, float(max(np.max(np.abs(pkg.mean(r) - mo)), np.max(np.abs(pkg.cov(r) - co)))))
+        acc = (dz, dm)
+    ess_s, elapsed, lat = _group_steps(g, steps, warmup)
+    # algorithmic bytes per member: each sample's x, log-target, log-numerator, log-weight
+    # (FP32) and round id written once
+    bytes_step = M * 10_000 * (3 * 4 + 3 * 4 + 4)
+    del g
+    r = {"ess_per_s": ess_s, "ms_per_step": 1e3 * elapsed / steps, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+         "p99_ms": 1e3 * float(np.percentile(lat, 99))}
+    out = _line(f"config 1 grouped: {M} independent plain-IS problems (d=3, -|x|^2/2 under SphericalGaussian(0, 1.5), "
+                "N=10,000 each) in one fused launch per step", "f32", r, steps, warmup)
+    out.update({
+        "members": M,
+        "fp64_plain_form": {"ess_per_s": ess64, "ms_per_step": 1e3 * el64 / (20 * steps),
+                            "p50_ms": 1e3 * float(np.percentile(lat64, 50)),
+                            "p99_ms": 1e3 * float(np.percentile(lat64, 99))},
+        "speedup_vs_fp64_plain_form": ess_s / ess64,
+        "weighted_samples_per_s": M * 10_000 * steps / elapsed,
+        "whole_step_hbm_rate": {"achieved": bytes_step * steps / elapsed / 1e9, "unit": "GB/s",
+                                "frac": bytes_step * steps / elapsed / 1e9 / HBM_PEAK_GBS,
+                                "note": "algorithmic bytes (28 B per sample) over the whole step's wall time "
+                                        "(host reset and read-back included): a lower bound for the kernel"},
+    })
+    if acc is not None:
+        out.update({"dlogz_vs_oracle_max_of_8": acc[0], "moment_err_vs_oracle_max_of_8": acc[1],
+                    "gate_passed": bool(acc[0] < 1e-3 and acc[1] < 1e-3)})
+    return out
+
+
+def grouped_amis_summary(pkg, steps, warmup, fp64_form, oracle_ref):
+    """Config 2 as a grouped solve: GROUP_AMIS_MEMBERS solves of the AMIS workload
+    (seeds 1, 2, ...) advanced together, FP32, one grouped round launch and one grouped
+    reduction per round; member 0 (seed 1) is checked against the FP64 oracle solve of
+    seed 1 (`oracle_ref`: log Z, mean, covariance)."""
+    target, proposal, _, _ = problem(pkg)
+    alg = pkg.AMIS(proposal, rounds=T, round_size=N)
+    M = GROUP_AMIS_MEMBERS
+    g = pkg.prepare_group([(1 + m, target, alg) for m in range(M)], "fp32")
+    acc = None
+    if oracle_ref is not None:
+        # the first run: member 0 draws what the oracle solve of seed 1 was fed
+        assert not g.run()
+        acc = _accuracy(pkg, g.collect(0), oracle_ref)
+    ess_s, elapsed, lat = _group_steps(g, steps, warmup)
+    ess_last = g.round_summary(-1)[0]
+    del g
+    old_bytes, new_bytes = D * 4 + 4 * 4 + 4, D * 4 + 3 * 4 + 4
+    bytes_solve = float(np.sum(old_bytes * N * np.arange(T) + new_bytes * N))
+    r = {"ess_per_s": ess_s, "ms_per_step": 1e3 * elapsed / steps, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+         "p99_ms": 1e3 * float(np.percentile(lat, 99))}
+    out = _line(f"config 2 grouped: {M} AMIS solves (d={D} correlated Gaussian target, q0=N(0,25I), T={T} x N={N}, "
+                "seeds 1..M) advanced together, one grouped round launch and one grouped reduction per round",
+                "f32", r, steps, warmup)
+    out.update({
+        "members": M,
+        "ms_per_member_solve": 1e3 * elapsed / steps / M,
+        "fp64_plain_form": fp64_form,
+        "speedup_vs_fp64_plain_form": ess_s / fp64_form["ess_per_s"] if fp64_form else None,
+        "weighted_samples_per_s": M * T * N * steps / elapsed,
+        "min_member_final_ess": float(np.min(ess_last)),
+        "whole_step_hbm_rate": {"achieved": M * bytes_solve * steps / elapsed / 1e9, "unit": "GB/s",
+                                "frac": M * bytes_solve * steps / elapsed / 1e9 / HBM_PEAK_GBS,
+                                "note": "the round kernel's algorithmic bytes (60 B per retained, 56 B per new "
+                                        "sample) over the whole step's wall time (reductions, fits, host reset "
+                                        "and read-back included): a lower bound for the kernel"},
+    })
+    if acc is not None:
+        out.update({"member0_dlogz_vs_oracle": acc[0], "member0_moment_err_vs_oracle": acc[1],
+                    "gate_passed": bool(acc[0] < 1e-3 and acc[1] < 1e-3)})
+    return out
+
+
+def main_grouped(args):
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    pkg.context(0)
+    print(json.dumps({"config1_grouped": grouped_plain_summary(pkg, args.steps, args.warmup),
+                      "config2_grouped": grouped_amis_summary(pkg, max(2, args.steps // 4), 1, None, None)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -660,7 +793,7 @@ def main():
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="default workload: skip the config-3/4 figures")
-    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc", "mis", "copy"])
+    ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc", "mis", "copy", "grouped"])
     ap.add_argument("--tuning", default=None, help="batch workload: 'waves,blocks_per_cu' instead of autotuning")
     ap.add_argument("--skip-fp64-check", action="store_true", help="batch workload: skip the FP64 reference solve")
     args = ap.parse_args()
@@ -680,6 +813,8 @@ def main():
         return main_copy(args)
     if args.workload == "mis":
         return main_mis(args)
+    if args.workload == "grouped":
+        return main_grouped(args)
 
     import __graft_entry__
 
@@ -833,16 +968,23 @@ def main():
                       f"normals, {threads} threads (nproc {nproc}), interleaved A B B A with the FP32 and FP64 GPU "
                       f"legs: {o1['seconds']:.2f} s and {o2['seconds']:.2f} s, ESS {o1['ess']:.0f}"}
     if not args.no_extras:
-        out["other_workloads"] = extra_workloads(pkg, args, threads)
+        oref = None if o1 is None else {"logz": o1["logz"], "mean": o1["mean"], "cov": o1["cov"]}
+        out["other_workloads"] = extra_workloads(pkg, args, threads, fp64_form, oref)
+        out["north_star_grouped_vs_fp64_plain_form"] = {
+            k: out["other_workloads"][k].get("speedup_vs_fp64_plain_form")
+            for k in ("config1_grouped", "config2_grouped", "config3_dmpmc_logistic")}
     print(json.dumps(out))
 
 
-def extra_workloads(pkg, args, threads):
-    """The DM-PMC logistic (config 3) and batched-posterior (config 4) workloads at a few
-    steps each, every one with its own roofline and accuracy figures; a failure is
+def extra_workloads(pkg, args, threads, fp64_form=None, oracle_ref=None):
+    """The grouped forms of configs 1 and 2 (many solves per launch, FP32, against the
+    plain FP64 form), the DM-PMC logistic (config 3) and batched-posterior (config 4)
+    workloads at a few steps each, every one with its own accuracy figures; a failure is
     reported in place of the figures."""
     out = {}
-    for name, fn in (("config3_dmpmc_logistic", lambda: dmpmc_summary(pkg, 3, 1)),
+    for name, fn in (("config1_grouped", lambda: grouped_plain_summary(pkg, 10, 2)),
+                     ("config2_grouped", lambda: grouped_amis_summary(pkg, 4, 1, fp64_form, oracle_ref)),
+                     ("config3_dmpmc_logistic", lambda: dmpmc_summary(pkg, 3, 1)),
                      ("config4_batched_linreg", lambda: batch_summary(pkg, 3, 1, threads, None, True,
                                                                       not args.no_cpu))):
         t0 = time.perf_counter()

@@ -128,6 +128,8 @@ SIGNATURES = {
     "is_group_run": (ctypes.c_int, [VP]),
     "is_group_size": (ctypes.c_int32, [VP]),
     "is_group_job": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(VP)]),
+    "is_group_reset": (ctypes.c_int, [VP]),
+    "is_group_round_summary": (ctypes.c_int, [VP, ctypes.c_int32, F64P, F64P]),
     "is_group_destroy": (ctypes.c_int, [VP]),
     "is_job_failure": (ctypes.c_int, [VP, ctypes.POINTER(Failure)]),
     "is_job_get_samples": (ctypes.c_int, [VP, F64P]),

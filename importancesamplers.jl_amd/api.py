@@ -1133,6 +1133,20 @@ class PreparedGroup:
             return []
         return [m for m, s in enumerate(self.members) if s.failure().round != 0]
 
+    def reset(self):
+        """Every member's next run starts from its initial proposal (is_group_reset)."""
+        _check(self._lib.is_group_reset(self._group))
+
+    def round_summary(self, round_index=-1):
+        """(ess, lognormalizers): every member's round ESS and log normaliser of the last
+        run, member order (is_group_round_summary)."""
+        T_ = len(self.members[0].schedule)
+        t = round_index % T_
+        ess = np.zeros(len(self.members))
+        lz = np.zeros(len(self.members))
+        _check(self._lib.is_group_round_summary(self._group, t, ess.ctypes.data_as(F64P), lz.ctypes.data_as(F64P)))
+        return ess, lz
+
     def collect(self, m):
         """Member m's WeightedSamples of the last run (raising its failure as a stand-alone
         run would)."""

@@ -2846,6 +2846,7 @@ struct is_group {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::atomic<bool> busy{false};
+  bool ran = false;  // a run has completed (status images hold its values)
   ~is_group() {
     if (stream) (void)hipStreamSynchronize(stream);
     CaptureGuard cg(capture_lock());
@@ -2986,6 +2987,7 @@ int group_run_typed(is_group* g) {
   HIP_TRY(hipMemcpyAsync(g->gather_down.p, g->gather.p, g->gather_down.bytes, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   // each member's outcome from its status image, as a stand-alone run's
+  g->ran = true;
   int result = IS_OK;
   const size_t down_bytes = (size_t)(g->total_words - g->down_word) * 8;
   for (int m = 0; m < M; ++m) {
@@ -3083,6 +3085,32 @@ int is_group_run(is_group* group) {
 }
 
 int32_t is_group_size(const is_group* group) { return group ? (int32_t)group->jobs.size() : 0; }
+
+int is_group_reset(is_group* group) {
+  if (!group) return fail(IS_EINVAL, "null group");
+  if (group->busy) return fail(IS_EBUSY, "group is already running");
+  for (is_job* j : group->jobs) {
+    const int rc = is_job_reset(j);
+    if (rc) return rc;
+  }
+  return IS_OK;
+}
+
+int is_group_round_summary(is_group* group, int32_t round, double* ess, double* lognormalizers) {
+  if (!group || round < 0 || round >= group->T) return fail(IS_EINVAL, "bad arguments");
+  if (!group->ran) return fail(IS_ESTATE, "the group has not run yet");
+  for (size_t m = 0; m < group->jobs.size(); ++m) {
+    const is_job* j = group->jobs[m];
+    // the member's status image of the group's last run (copied in by group_run_typed)
+    const char* down = j->ctl_down.as<const char>();
+    auto at = [&](const DevView& v) {
+      return reinterpret_cast<const double*>(down + ((const char*)v.p - j->ctl.as<char>())) + round;
+    };
+    if (ess) ess[m] = *at(j->round_ess);
+    if (lognormalizers) lognormalizers[m] = *at(j->round_logz) + j->t_shift;
+  }
+  return IS_OK;
+}
 
 int is_group_job(is_group* group, int32_t index, is_job** out) {
   if (!group || !out || index < 0 || index >= (int32_t)group->jobs.size()) return fail(IS_EINVAL, "bad arguments");

@@ -381,6 +381,13 @@ int is_group_create(is_ctx* ctx, const is_job_desc* descs, int32_t count, is_gro
 int is_group_run(is_group* group);
 int32_t is_group_size(const is_group* group);
 int is_group_job(is_group* group, int32_t index, is_job** out);
+/* is_job_reset on every member: each next run starts from its initial proposal. */
+int is_group_reset(is_group* group);
+/* Every member's ESS and log normaliser of round `round` (0-based) of the last run, in
+ * member order (`count` = group size entries each; either pointer may be null): the
+ * values is_job_get_round_summary returns member by member.  A failed member's entries
+ * are those its own status holds (zero ESS, -inf log normaliser for the failed round). */
+int is_group_round_summary(is_group* group, int32_t round, double* ess, double* lognormalizers);
 int is_group_destroy(is_group* group);
 
 /* ---- batched AMIS: many independent posteriors in one persistent launch ----------

@@ -118,3 +118,28 @@ def test_group_shape_checks(eng):
                             eng.NPMC(eng.SphericalGaussian(np.zeros(3), 1.0), rounds=2, round_size=256))])
     one = eng.prepare_group([(5, eng.StandardGaussianTarget(), alg)])
     _same(eng.importance_sample_group_(one)[0], eng.importance_sample(5, eng.StandardGaussianTarget(), alg))
+
+
+def test_group_reset_and_round_summary(eng):
+    _say("case: group reset and gathered round summaries (4 AMIS members, d = 10, T = 4 x N = 1,500), fp32")
+    specs = _amis_specs(eng, 4, T=4, N=1500)
+    g = eng.prepare_group(specs, "fp32")
+    first = eng.importance_sample_group_(g)
+    for t in range(4):
+        ess, lz = g.round_summary(t)
+        for m in range(4):
+            assert ess[m] == first[m].diagnostics["round_ess"][t]
+            assert lz[m] == first[m].diagnostics["round_lognormalizers"][t]
+    # the third run after a reset starts from the initial proposal again, as a stand-alone
+    # sampler's run, run, reset, run does (the counter-RNG draws follow the run index)
+    eng.importance_sample_group_(g)
+    g.reset()
+    third = eng.importance_sample_group_(g)
+    for m in (0, 3):
+        rng, tgt, alg = specs[m]
+        s = eng.prepare_sampler(rng, tgt, alg, "fp32")
+        eng.importance_sample_(s)
+        eng.importance_sample_(s)
+        s.reset()
+        _same(third[m], eng.importance_sample_(s))
+        assert g.round_summary(-1)[1][m] == third[m].diagnostics["round_lognormalizers"][-1]
