@@ -30,6 +30,10 @@ FLAGS = [
     "-Wno-unused-variable",
 ]
 NOISE = "packed-fp32-ops"
+# MFMA accumulators in VGPRs (one register file on gfx950): no copies between the
+# accumulation registers and the VGPRs that rescale or read the tiles.  Not for the
+# FP64 extended-range unit (dim64 part 5), which this compiler fails to build with it.
+VGPR_MFMA = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 BATCH_WAVES = (2, 4, 8)  # waves per workgroup of the batched AMIS kernel (batch_k.hip)
 
 
@@ -67,8 +71,9 @@ def units():
         # parts 4, 5: the extended-range kernels; 6, 7: the grouped fused rounds (d <= 15)
         parts = list(range(4)) + ([4, 5] if d == 64 else []) + ([6, 7] if d <= 15 else [])
         for part in parts:
+            vgpr = [] if (d == 64 and part == 5) else VGPR_MFMA
             units.append((os.path.join(CSRC, "dim.hip"), os.path.join(OBJDIR, f"dim{d}_p{part}.o"),
-                          [f"-DIS_DIM={d}", f"-DIS_DIM_PART={part}"]))
+                          [f"-DIS_DIM={d}", f"-DIS_DIM_PART={part}", *vgpr]))
     return units
 
 

@@ -615,6 +615,65 @@ __device__ __forceinline__ T slot_logdensity(P s, int d, const Family<T>& fam, c
   }
 }
 
+// slot_logdensity for a slot staged in LDS, at an exact dimension (D == d): the
+// same operations in the same order, with the slot's values read into registers in
+// a few batches (centre and inverse diagonal, then blocks of L's rows) issued ahead
+// of their use, so the substitution waits on a few LDS round trips rather than one
+// per pair of values.
+__host__ __device__ constexpr int lds_row_batch_end(int r, int D, int batch) {
+  int r1 = r + 1;
+  while (r1 < D && (r1 + 1) * (r1 + 2) / 2 - r * (r + 1) / 2 <= batch) ++r1;
+  return r1;
+}
+
+template <class T, int D, int R>
+__device__ __forceinline__ void lds_rows(const T* __restrict__ s, const T (&x)[D], const T (&mu)[D],
+                                         const T (&inv)[D], T (&y)[D], T& r2) {
+  constexpr int R1 = lds_row_batch_end(R, D, sizeof(T) == 4 ? 48 : 24);
+  constexpr int B0 = R * (R + 1) / 2, B1 = R1 * (R1 + 1) / 2;
+  T lv[B1 - B0];
+#pragma unroll
+  for (int i = 0; i < B1 - B0; ++i) lv[i] = s[2 * D + B0 + i];
+#pragma unroll
+  for (int rr = R; rr < R1; ++rr) {
+    T v = x[rr] - mu[rr];
+    const int row = rr * (rr + 1) / 2 - B0;
+#pragma unroll
+    for (int c = 0; c < rr; ++c) v -= lv[row + c] * y[c];
+    v *= inv[rr];
+    y[rr] = v;
+    r2 += v * v;
+  }
+  if constexpr (R1 < D) lds_rows<T, D, R1>(s, x, mu, inv, y, r2);
+}
+
+template <class T, int D, bool RANGE = false>
+__device__ __forceinline__ T slot_logdensity_lds(const T* __restrict__ s, const Family<T>& fam, const T (&x)[D],
+                                                 T (&y)[D]) {
+  static_assert(D <= kExactDims, "exact dimensions only");
+  T mu[D], inv[D];
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    mu[r] = s[r];
+    inv[r] = s[D + r];
+  }
+  T r2 = T(0);
+  lds_rows<T, D, 0>(s, x, mu, inv, y, r2);
+  const T lognorm = s[slot_lognorm_off(D)];
+  if (!fam.student) {
+    if (__builtin_expect(isfinite(r2), 1)) return lognorm - T(0.5) * r2;
+  } else {
+    const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+    const T ratio = r2 / fam.dof;
+    if (__builtin_expect(r2 >= tiny && isfinite(ratio), 1)) return lognorm - fam.half_dof_plus_d * log1p(ratio);
+  }
+  if constexpr (RANGE) {
+    return slot_logdensity_range<T, D>(s, D, fam, x);
+  } else {
+    return T(__builtin_nan(""));
+  }
+}
+
 // Packed affine draw x = mu + L z (src/adaptive_gaussian_execution.jl:49-62).
 template <class T, int D, class P>
 __device__ __forceinline__ bool slot_draw(P s, int d, const T (&z)[D], T (&x)[D]) {

@@ -14,8 +14,10 @@ void launch_new_mode(RoundArgs<T> a, int ncomp, int blocks, int W, hipStream_t s
   const size_t slot_bytes = (size_t)a.stride * sizeof(T);
   const size_t tslot_bytes = a.tkind == kTGaussian ? slot_bytes : 0;
   const size_t avail = kNewSlotBudget > tslot_bytes ? kNewSlotBudget - tslot_bytes : 0;
-  a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / slot_bytes));
-  const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * slot_bytes;
+  // each staged component: its slot and its log mixture mass
+  const size_t comp_bytes = slot_bytes + sizeof(T);
+  a.slot_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), avail / comp_bytes));
+  const size_t lds = new_lds_fixed(D, W, (int)sizeof(T)) + tslot_bytes + (size_t)a.slot_chunk * comp_bytes;
   if (MODE == kPreTarget)
     hipLaunchKernelGGL((k_new<T, D, MODE, false, RG>), dim3(blocks), dim3(64 * W), lds, s, a);
   else if (st)

@@ -200,8 +200,8 @@ void tick(is_job* j, int t, int which, hipStream_t s) {
 
 // ---------------------------------------------------------------- launch dispatch
 constexpr int64_t kSweepPerBlock = 512;  // samples per sweep block (one partial record)
-// samples per sweep block in the fused AMIS round: more 64-sample steps per wave
-// amortise the block's start and its reduction
+// samples per sweep block of the fused plain-IS / static-MIS round (it has no sweep
+// blocks; the AMIS rounds size theirs with round_sweep_per_block)
 constexpr int64_t kRoundPerBlock = 1024;
 
 // Calls f(std::integral_constant<int, D>) for the job's dimension instantiation.
@@ -560,6 +560,20 @@ SweepArgs<T> sweep_args(is_job* j, int t) {
   return w;
 }
 
+// Samples per sweep block of a fused round over `first` retained samples: the
+// fewest whole 64-sample steps per wave (4 waves per block) that keep the sweep
+// blocks and the round's nb_new new-sample blocks within one pass of the chip
+// (kRoundBlocksPerCu resident blocks per CU), so that small rounds spread over many
+// short blocks; the records stay within the partial-record columns.
+constexpr int kRoundBlocksPerCu = 4;
+int64_t round_sweep_per_block(const is_job* j, int64_t first, int nb_new) {
+  const int cus = j->ctx ? j->ctx->cus : 256;
+  const int64_t budget = std::max<int64_t>(cus, (int64_t)kRoundBlocksPerCu * cus - nb_new);
+  int64_t per_block = std::max<int64_t>(256, cdiv(cdiv(first, budget), 256) * 256);
+  while (per_block < first && cdiv(first, per_block) + nb_new > j->pcap) per_block += 256;
+  return per_block;
+}
+
 // The launch arguments of one round of plain IS (t = 0) or AMIS (round t) as the fused
 // form issues them: the new samples' blocks beside the sweep share wo of the retained
 // samples [0, first), then the reduction and finish f.  w is the sweep over every
@@ -627,11 +641,11 @@ RoundPlan<T> round_plan(is_job* j, int t, unsigned int* counter) {
   w.nv = 2 + d + d * (d + 1) / 2;
   p.wo = w;
   p.wo.hi = first;
-  p.wo.per_block = kRoundPerBlock;
+  p.wo.per_block = round_sweep_per_block(j, first, p.nb_new);
   p.nt = 1;
   p.ncomp = t + 1;
   p.nv = w.nv;
-  p.nrec_old = first > 0 ? (int)cdiv(first, kRoundPerBlock) : 0;
+  p.nrec_old = first > 0 ? (int)cdiv(first, p.wo.per_block) : 0;
   FinishArgs& f = p.f;
   f = base_finish(j, t, counter);
   f.flags = kFinishSummary | kFinishFit;
@@ -1265,6 +1279,7 @@ int is_ctx_create(int device, is_ctx** out) {
   if (int rc = bind_scratch(device)) return rc;
   is_ctx* c = new is_ctx();
   c->device = device;
+  c->cus = std::max(1, prop.multiProcessorCount);
   *out = c;
   return IS_OK;
 }

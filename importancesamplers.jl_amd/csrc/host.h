@@ -18,6 +18,7 @@
 
 struct is_ctx {
   int device = 0;
+  int cus = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   std::mutex mu;
 };

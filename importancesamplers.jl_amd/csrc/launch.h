@@ -45,8 +45,9 @@ inline int round_slot_chunk(int D, int nt, int stride, int tsize, bool gaussian_
   const size_t fixed = new_lds_fixed(D, 4, tsize) + tslot_bytes;
   const size_t cap = std::max(sweep_lds(nt, 4), kRoundLds);
   const size_t room = cap > fixed ? cap - fixed : 0;
-  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), room / slot_bytes));
-  *lds = std::max(sweep_lds(nt, 4), fixed + (size_t)chunk * slot_bytes);
+  const size_t comp_bytes = slot_bytes + tsize;  // a staged slot and its log mixture mass
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, ncomp), room / comp_bytes));
+  *lds = std::max(sweep_lds(nt, 4), fixed + (size_t)chunk * comp_bytes);
   return chunk;
 }
 

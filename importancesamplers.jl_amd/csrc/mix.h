@@ -24,10 +24,11 @@ __device__ inline void lse_merge(T& m, T& s, T m2, T s2) {
 // the deterministic-mixture log-sum-exp over slots [k0, k1] and the weight
 // (src/mis_execution.jl:601-771; plain IS src/native_execution.jl:741-812).
 // A block of W waves owns 64 consecutive samples, one per lane:
-//   A  the ceil(d/2) Box-Muller pairs of the 64 samples (and their Student-t
-//      radial multipliers) are spread over all 64 W threads and staged in LDS,
-//      while the target slot and the first chunk of mixture slots are copied
-//      into LDS (one memory round trip, overlapped with the Philox work);
+//   A  the normals of the 64 samples, one Philox block (two Box-Muller pairs)
+//      per thread, and their Student-t radial multipliers are spread over all
+//      64 W threads and staged in LDS, while the target slot and the first chunk
+//      of mixture slots with their log masses are copied into LDS (one memory
+//      round trip, overlapped with the Philox work);
 //   B  wave 0 forms x = mu + L z, stores X and shares x through LDS;
 //   C  wave w evaluates the components kk with (kk + 1) mod W == w, so every
 //      slot read is a wave-uniform LDS broadcast; wave 0 also evaluates the
@@ -35,7 +36,8 @@ __device__ inline void lse_merge(T& m, T& s, T m2, T s2) {
 //   D  wave 0 merges the W partial log-sum-exp states in wave order and writes
 //      the sample's outputs.
 // LDS: normals 64 x (D + 1) doubles | multipliers 64 | x 64 x (D + 1) T |
-//      W x 64 partial (m, s, flag) | 64 draw flags | target slot | slot chunk.
+//      W x 64 partial (m, s, flag) | 64 draw flags | target slot | slot chunk |
+//      the chunk's log masses.
 // Threads per block the register budget allows: x, y and z of D values each.
 __host__ __device__ constexpr int new_block_limit(int D, int tsize, bool st) {
   return (!st && D * tsize <= 64) ? 1024 : (D * tsize <= 128 ? 512 : 256);
@@ -73,6 +75,8 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
   T* comp = tslot + (staged_target ? a.stride : 0);
   const int ncomp = a.k1 - a.k0 + 1;
   const int chunk = min(ncomp, a.slot_chunk);
+  // the chunk's log mixture masses follow its slots
+  T* cmass = comp + chunk * a.stride;
 
   // ---- A: slot staging, normals and radial multipliers
   if (staged_target)
@@ -80,38 +84,50 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
   {
     const T* src = a.slots + (int64_t)a.k0 * a.stride;
     for (int q = tid; q < chunk * a.stride; q += blockDim.x) comp[q] = src[q];
+    if (a.mis_terms == 0)
+      for (int q = tid; q < chunk; q += blockDim.x) cmass[q] = a.logmass[a.k0 + q];
   }
   if (MODE != kPreTarget) {
     const uint64_t rng_word = *a.run_hi | (uint64_t)a.round;
     const int np = (d + 1) >> 1;
-    const int njobs = 64 * np + (ST ? 64 : 0);
+    // one job per Philox block: pairs 2b and 2b + 1 of a sample take words (0, 1) and
+    // (2, 3) of its block b
+    const int nb = (np + 1) >> 1;
+    const int njobs = 64 * nb + (ST ? 64 : 0);
     for (int q = tid; q < njobs; q += blockDim.x) {
-      if (q < 64 * np) {
-        const int s = q / np, p = q - s * np;
+      if (q < 64 * nb) {
+        const int s = q / nb, b = q - s * nb;
         const int64_t js = j0 + s;
-        double z0 = 0.0, z1 = 0.0;
+        const bool second = 2 * b + 1 < np;
+        double z[4] = {0.0, 0.0, 0.0, 0.0};
         if (js < a.n_new) {
           if (a.pre_normals) {
             const double* src = a.pre_normals + js * a.nstride;
-            z0 = src[2 * p];
-            if (2 * p + 1 < d) z1 = src[2 * p + 1];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (4 * b + u < d) z[u] = src[4 * b + u];
           } else {
-            // pair p uses words 2(p mod 2), 2(p mod 2) + 1 of Philox block p / 2
-            const U64x4 w = philox4x64((uint64_t)js, rng_word, kPurposeNormals, (uint64_t)(p >> 1), a.key.k0,
-                                       a.key.k1);
-            const uint64_t wa = (p & 1) ? w.v[2] : w.v[0];
-            const uint64_t wb = (p & 1) ? w.v[3] : w.v[1];
-            const double r = sqrt(-2.0 * log(u53_open(wa)));
-            double sn, cs;
-            sincospi(2.0 * u53_open(wb), &sn, &cs);
-            z0 = r * cs;
-            z1 = r * sn;
+            const U64x4 w = philox4x64((uint64_t)js, rng_word, kPurposeNormals, (uint64_t)b, a.key.k0, a.key.k1);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if (h == 0 || second) {
+                const double r = sqrt(-2.0 * log(u53_open(w.v[2 * h])));
+                double sn, cs;
+                sincospi(2.0 * u53_open(w.v[2 * h + 1]), &sn, &cs);
+                z[2 * h] = r * cs;
+                z[2 * h + 1] = r * sn;
+              }
+            }
           }
         }
-        zs[s * ZS + 2 * p] = z0;
-        zs[s * ZS + 2 * p + 1] = z1;
+        zs[s * ZS + 4 * b] = z[0];
+        zs[s * ZS + 4 * b + 1] = z[1];
+        if (second) {
+          zs[s * ZS + 4 * b + 2] = z[2];
+          zs[s * ZS + 4 * b + 3] = z[3];
+        }
       } else if (ST) {
-        const int s = q - 64 * np;
+        const int s = q - 64 * nb;
         const int64_t js = j0 + s;
         double mult = 1.0;
         int drawn = 1;
@@ -313,16 +329,21 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
         __syncthreads();
         const T* src = a.slots + (int64_t)(a.k0 + base) * a.stride;
         for (int q = tid; q < cnt * a.stride; q += blockDim.x) comp[q] = src[q];
+        for (int q = tid; q < cnt; q += blockDim.x) cmass[q] = a.logmass[a.k0 + base + q];
         __syncthreads();
       }
       if (ok) {
         // first kk >= base with (kk + 1) mod W == wave
         int kk = base + ((wave + W - 1 - base % W) % W + W) % W;
         for (; kk < base + cnt; kk += W) {
-          const int k = a.k0 + kk;
-          const T q = slot_logdensity<T, D, RG>(comp + (kk - base) * a.stride, d, a.fam, x, y);
-          if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
-          const T v = a.logmass[k] + q;
+          const T* sl = comp + (kk - base) * a.stride;
+          T q;
+          if constexpr (D <= kExactDims)
+            q = slot_logdensity_lds<T, D, RG>(sl, a.fam, x, y);
+          else
+            q = slot_logdensity<T, D, RG>(sl, d, a.fam, x, y);
+          if (a.k0 + kk == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
+          const T v = cmass[kk - base] + q;
           if (v > m) {
             sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
             m = v;
@@ -428,30 +449,27 @@ struct SweepLoad {
   int32_t rid;
 };
 
+// Loads sample i's inputs; a lane past the block's end (i >= bend) reads the block's
+// last sample instead (finite values, so its zero weight contributes exact zeros),
+// which keeps every load unconditional.
 template <class T, int D, int NT>
 __device__ __forceinline__ void sweep_load(const SweepArgs<T>& a, int64_t i, int64_t bend, SweepLoad<T, D>& L) {
   const int d = exact_dim<D>(a.d);
-  const bool valid = i < bend;
-  const bool app = a.append && i < a.append_hi;
+  const int64_t ic = i < bend ? i : bend - 1;
+  const bool app = a.append && ic < a.append_hi;
 #pragma unroll
   for (int c = 0; c < D; ++c) L.x[c] = T(0);
-  L.num = T(0);
-  L.tgt = T(0);
-  L.lw = -T(__builtin_huge_val());
-  L.rid = 0;
-  if (valid) {
-    if (NT > 0 || app) {
+  if (NT > 0 || app) {
 #pragma unroll
-      for (int c = 0; c < D; ++c)
-        if (c < d) L.x[c] = a.X[(int64_t)c * a.cap + i];
-    }
-    if (app) {
-      L.num = a.lognums[i];
-      L.tgt = a.logtargets[i];
-      L.rid = a.round_ids[i];
-    } else {
-      L.lw = a.logweights[i];
-    }
+    for (int c = 0; c < D; ++c)
+      if (c < d) L.x[c] = a.X[(int64_t)c * a.cap + ic];
+  }
+  if (app) {
+    L.num = a.lognums[ic];
+    L.tgt = a.logtargets[ic];
+    L.rid = a.round_ids[ic];
+  } else {
+    L.lw = a.logweights[ic];
   }
 }
 
@@ -468,12 +486,16 @@ __device__ __forceinline__ float exp_nonpos(float v) {
 }
 
 // Per-wave running state of the summary monoid and the moment tiles.
+// With one moment tile (NT = 1) the tile accumulates in two independent MFMA chains
+// (even and odd 4-sample steps of each 64-sample step), summed when the wave's state
+// is read out; with more tiles, their own chains are independent already.
 template <class T, int NT>
 struct SweepAcc {
   static constexpr int P = NT * (NT + 1) / 2;
   T mw;
   double s0, s2;
   double4_t acc[P > 0 ? P : 1];
+  double4_t acc2[1];
 };
 
 template <class T, int NT>
@@ -482,8 +504,8 @@ __device__ __forceinline__ void sweep_acc_init(SweepAcc<T, NT>& st) {
   st.s0 = 0.0;
   st.s2 = 0.0;
 #pragma unroll
-  for (int p = 0; p < (SweepAcc<T, NT>::P > 0 ? SweepAcc<T, NT>::P : 1); ++p)
-    st.acc[p] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int p = 0; p < (SweepAcc<T, NT>::P > 0 ? SweepAcc<T, NT>::P : 1); ++p) st.acc[p] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  st.acc2[0] = (double4_t){0.0, 0.0, 0.0, 0.0};
 }
 
 // One 64-sample step of a wave: lane `lane` contributes log-weight lw (-inf: no
@@ -503,6 +525,7 @@ __device__ __forceinline__ void sweep_accumulate(SweepAcc<T, NT>& st, T lw, cons
     st.s2 *= f * f;
 #pragma unroll
     for (int p = 0; p < (P > 0 ? P : 1); ++p) st.acc[p] *= f;
+    if (P == 1) st.acc2[0] *= f;
     st.mw = wm;
   }
   const T se = (lw == ninf) ? T(0) : exp_nonpos(T(0.5) * (lw - st.mw));
@@ -530,7 +553,10 @@ __device__ __forceinline__ void sweep_accumulate(SweepAcc<T, NT>& st, T lw, cons
       for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
         for (int tj = 0; tj <= ti; ++tj) {
-          st.acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], av[tj], st.acc[p], 0, 0, 0);
+          if (P == 1 && (stp & 1))
+            st.acc2[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], av[tj], st.acc2[0], 0, 0, 0);
+          else
+            st.acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], av[tj], st.acc[p], 0, 0, 0);
           ++p;
         }
     }
@@ -564,7 +590,8 @@ __device__ __forceinline__ void sweep_epilogue(SweepAcc<T, NT>& st, int d, int n
 #pragma unroll
   for (int p = 0; p < P; ++p)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) tiles[((size_t)wave * P + p) * 256 + lane * 4 + r] = st.acc[p][r];
+    for (int r = 0; r < 4; ++r)
+      tiles[((size_t)wave * P + p) * 256 + lane * 4 + r] = P == 1 ? st.acc[p][r] + st.acc2[0][r] : st.acc[p][r];
   __syncthreads();
   double M = wmax[0];
   for (int w = 1; w < nw; ++w) M = fmax(M, wmax[w]);
@@ -611,6 +638,35 @@ __device__ __forceinline__ void sweep_epilogue(SweepAcc<T, NT>& st, int d, int n
   }
 }
 
+// One wave's state as the column-major partial record at `out` (the layout
+// sweep_epilogue writes), when no other wave of the block holds samples: its
+// values are the block's own, so each lane stores the tile entries it holds.
+template <class T, int NT>
+__device__ __forceinline__ void wave_record(const SweepAcc<T, NT>& st, int d, double* out, int64_t pcap, int lane) {
+  static_assert(NT <= 1, "one moment tile per side");
+  const double s0 = wave_sum(st.s0);
+  const double s2 = wave_sum(st.s2);
+  if (lane == 0) {
+    out[0] = (double)st.mw;
+    out[2 * pcap] = s2;
+    if (NT == 0) out[pcap] = s0;
+  }
+  if (NT > 0) {
+    // v_mfma_f64_16x16x4f64 result layout: register i of lane l holds entry
+    // (row 4 i + l / 16, column l % 16); value 0 is C[d][d], 2 + c is C[d][c]
+    // (c < d) and 2 + d + r (r + 1) / 2 + c is C[r][c] (c <= r < d)
+    const int c = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 4 * i + (lane >> 4);
+      if (r <= d && c <= r) {
+        const int v = r == d ? (c == d ? 0 : 2 + c) : 2 + d + r * (r + 1) / 2 + c;
+        out[(int64_t)(1 + v) * pcap] = st.acc[0][i] + st.acc2[0][i];
+      }
+    }
+  }
+}
+
 // The body of sweep block `blk` over [a.lo + blk * per_block, ...).
 template <class T, int D, int NT, bool RG = false>
 __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) {
@@ -634,19 +690,16 @@ __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) 
   const int64_t bstart = a.lo + blk * a.per_block;
   const int64_t bend = min(a.hi, bstart + a.per_block);
   const int64_t step = (int64_t)nw * 64;
-  SweepLoad<T, D> cur, nxt;
-  int64_t base = bstart + (int64_t)wave * 64;
-  if (base < bend) sweep_load<T, D, NT>(a, base + lane, bend, cur);
-  for (; base < bend; base += step) {
-    if (base + step < bend) sweep_load<T, D, NT>(a, base + step + lane, bend, nxt);
+  // one 64-sample step of the wave on the loaded inputs L of samples base + lane
+  auto step_on = [&](const SweepLoad<T, D>& L, int64_t base) {
     const int64_t i = base + lane;
     T lw = ninf;
     if (i < bend) {
       if (a.append && i < a.append_hi) {
         T yv[D];
-        const T q = slot_logdensity<T, D, RG>(slot, d, a.fam, cur.x, yv);
+        const T q = slot_logdensity<T, D, RG>(slot, d, a.fam, L.x, yv);
         uint16_t reason = (q != q) ? kProposalInvalid : 0;
-        T num = cur.num;
+        T num = L.num;
         if (!reason) {
           const T value = logaddexp<T>(num, a.logmass + q);
           reason = proposal_reason(value);
@@ -656,8 +709,8 @@ __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) 
           }
         }
         if (reason) record_failure(a.fail, i + 1, 0, reason);
-        if (cur.rid != 0) {
-          const T w = cur.tgt - (num - a.logtotal);
+        if (L.rid != 0) {
+          const T w = L.tgt - (num - a.logtotal);
           const uint16_t wr = logweight_reason(w);
           if (wr) {
             record_failure(a.fail, i + 1, 0, wr);
@@ -667,12 +720,24 @@ __device__ __forceinline__ void sweep_block(const SweepArgs<T>& a, int64_t blk) 
           }
         }
       } else {
-        lw = cur.lw;
+        lw = L.lw;
       }
       if (a.clip) lw = fmin(lw, (T)clip);
     }
-    sweep_accumulate<T, D, NT>(st, lw, cur.x, d, cen, rows, lane);
-    cur = nxt;
+    sweep_accumulate<T, D, NT>(st, lw, L.x, d, cen, rows, lane);
+  };
+  // two load buffers in turn: the next step's inputs are in flight while a step runs
+  SweepLoad<T, D> la, lb;
+  int64_t base = bstart + (int64_t)wave * 64;
+  if (base < bend) sweep_load<T, D, NT>(a, base + lane, bend, la);
+  while (base < bend) {
+    if (base + step < bend) sweep_load<T, D, NT>(a, base + step + lane, bend, lb);
+    step_on(la, base);
+    base += step;
+    if (base >= bend) break;
+    if (base + step < bend) sweep_load<T, D, NT>(a, base + step + lane, bend, la);
+    step_on(lb, base);
+    base += step;
   }
   sweep_epilogue<T, NT>(st, d, a.nv, a.partials + blk, a.pcap, lds);
 }
@@ -684,8 +749,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
 
 // One AMIS round in one launch (samples are disjoint between the two roles):
 //   blocks [0, nb_new)   draw the round's new samples (new_block, 64 per block,
-//                        4 waves) and write the moment partial record of their
-//                        64 samples at nrec_old + block;
+//                        4 waves); wave 0, which holds them, accumulates and
+//                        writes the moment partial record of the 64 samples at
+//                        nrec_old + block;
 //   blocks [nb_new, ...) sweep the retained samples [w.lo, w.hi) (append of the
 //                        newest component, weights, summary, moments), records
 //                        [0, nrec_old).
@@ -701,24 +767,28 @@ __device__ __forceinline__ void round_block(const RoundArgs<T>& a, const SweepAr
 #pragma unroll
     for (int c = 0; c < D; ++c) x[c] = T(0);
     new_block<T, D, kDrawFused, ST>(a, blockIdx.x, lw, x);
-    __syncthreads();
+    // after the mixture's barrier only wave 0 touches LDS: it alone holds the
+    // block's samples, accumulates them and writes the record
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave != 0) return;
     const int d = exact_dim<D>(a.d);
     SweepAcc<T, NT> st;
     sweep_acc_init(st);
-    if (wave == 0) {
-      if (NT > 0)
-        for (int q = lane; q < 64 * RS; q += 64) lds[q] = 0.0;
-      sweep_accumulate<T, D, NT>(st, lw, x, d, as_const(w.center), lds, lane);
-    }
-    sweep_epilogue<T, NT>(st, d, w.nv, w.partials + nrec_old + blockIdx.x, w.pcap, lds);
+    if (NT > 0)
+      for (int q = lane; q < 64 * RS; q += 64) lds[q] = 0.0;
+    sweep_accumulate<T, D, NT>(st, lw, x, d, as_const(w.center), lds, lane);
+    wave_record<T, NT>(st, d, w.partials + nrec_old + blockIdx.x, w.pcap, lane);
   } else {
     sweep_block<T, D, NT>(w, (int64_t)blockIdx.x - nb_new);
   }
 }
 
+// Four waves per SIMD (at most 128 VGPRs): the LDS of four blocks per CU allows no more,
+// and the new-sample role's interleaved densities would otherwise take more registers.
 template <class T, int D, int NT, bool ST>
-__global__ __launch_bounds__(256) void k_round(RoundArgs<T> a, SweepArgs<T> w, int nb_new, int nrec_old) {
+__global__ __launch_bounds__(256) void k_round(RoundArgs<T> a,
+                                                                                        SweepArgs<T> w, int nb_new,
+                                                                                        int nrec_old) {
   round_block<T, D, NT, ST>(a, w, nb_new, nrec_old);
 }
 
