@@ -49,7 +49,7 @@ def _unit_cost(extra):
     for flag in extra:
         if flag.startswith("-DIS_DIM="):
             return int(flag.split("=")[1])
-    return 20 if extra else 10  # batch kernels, then the host-side units
+    return 20 if any(f.startswith("-DIS_BATCH") for f in extra) else 10  # batch kernels, then the host-side units
 
 
 def _run(cmd):
@@ -60,13 +60,13 @@ def _run(cmd):
 
 def units():
     """(source, object, extra flags) of every compilation unit of the library."""
-    units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), []),
-             (os.path.join(CSRC, "batch.hip"), os.path.join(OBJDIR, "batch.o"), []),
-             (os.path.join(CSRC, "serve.hip"), os.path.join(OBJDIR, "serve.o"), [])]
+    units = [(os.path.join(CSRC, "engine.hip"), os.path.join(OBJDIR, "engine.o"), list(VGPR_MFMA)),
+             (os.path.join(CSRC, "batch.hip"), os.path.join(OBJDIR, "batch.o"), list(VGPR_MFMA)),
+             (os.path.join(CSRC, "serve.hip"), os.path.join(OBJDIR, "serve.o"), list(VGPR_MFMA))]
     for fp in (32, 64):
         for w in BATCH_WAVES:
             units.append((os.path.join(CSRC, "batch_k.hip"), os.path.join(OBJDIR, f"batch_k{fp}_w{w}.o"),
-                          [f"-DIS_BATCH_FP={fp}", f"-DIS_BATCH_W={w}"]))
+                          [f"-DIS_BATCH_FP={fp}", f"-DIS_BATCH_W={w}", *VGPR_MFMA]))
     for d in dims():
         # parts 4, 5: the extended-range kernels; 6, 7: the grouped fused rounds (d <= 15)
         parts = list(range(4)) + ([4, 5] if d == 64 else []) + ([6, 7] if d <= 15 else [])

@@ -126,6 +126,12 @@ struct is_job {
   double t_shift = 0.0;                  // log-target offset kept off the device values (FP32 regression)
   std::vector<double> tdata;             // X (n x p row-major) then y (n), FP64, until upload
   DevBuf Xd, yd, tpre, tpart;
+  // FP32 logistic target, p <= 64: the centred BF16 form (k_logistic_bf16): the rows'
+  // BF16 split fragments, x_i . x0 and the responses (FP32, padded), the centre x0
+  std::vector<double> t_x0;              // the centre: the log-target offset's anchor
+  bool t_bf16 = false;
+  int t_ks2 = 0;                          // 32-wide K steps
+  DevBuf t_xfrag, t_base, t_y, t_x0d;
   // device buffers
   DevBuf X, logtargets, lognums, logweights, round_ids, proposal_ids;
   DevBuf slots64, slotsT, logmassT, zerosT, tslotT, assign, bank_ids_d;
@@ -715,6 +721,35 @@ int run_rounds(is_job* j) {
     }
     // draw, batched target on MFMA, then the mixture with the precomputed target values
     const int64_t splits = target_splits(j->tn, a.n_new);
+    if (j->t_bf16) {
+      // FP32 logistic: the centred BF16 form
+      LogisticBf16Args ba{};
+      ba.p = dl;
+      ba.n = j->tn;
+      ba.xfrag = j->t_xfrag.as<u32x4_t>();
+      ba.base = j->t_base.as<float>();
+      ba.yv = j->t_y.as<float>();
+      ba.x0 = j->t_x0d.as<double>();
+      ba.beta = (j->has_tf ? j->XL : j->X).as<float>();
+      ba.cap = j->total;
+      ba.first = a.first;
+      ba.n_new = a.n_new;
+      ba.ntiles = cdiv(j->tn, 16);
+      ba.tiles_per_split = cdiv(cdiv(ba.ntiles, splits), 4) * 4;
+      ba.part = j->tpart.as<double>();
+      const int64_t used = cdiv(ba.ntiles, ba.tiles_per_split);
+      const dim3 grid((unsigned)cdiv(a.n_new, 64), (unsigned)used);
+      if (j->t_ks2 == 1)
+        hipLaunchKernelGGL(k_logistic_bf16<1>, grid, dim3(256), 0, s, ba);
+      else
+        hipLaunchKernelGGL(k_logistic_bf16<2>, grid, dim3(256), 0, s, ba);
+      hipLaunchKernelGGL((k_target_combine<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, ba.part,
+                         (int)used, a.n_new, (const T*)ba.beta, ba.cap, ba.first, dl, j->t_inv_two_prior,
+                         j->t_const - j->t_shift, j->tpre.as<T>());
+      a.pre_logtarget = j->tpre.as<T>();
+      launch_mix<T, kPreTarget>(D, ncomp, a, s);
+      return IS_OK;
+    }
     TargetArgs<T> ta{};
     ta.kind = j->tkind;
     ta.p = dl;
@@ -1226,6 +1261,66 @@ int alloc_job_buffers(is_job* j) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- centred BF16 logistic data
+// Round-to-nearest-even FP32 -> BF16 (the hardware conversion's rounding).
+static uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static double bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+// The data of the centred BF16 logistic form (k_logistic_bf16), from the FP64 rows in
+// j->tdata: each row split as x = xh + xl (BF16 each, xl the rounding of x - xh) in the
+// MFMA A-operand order, x_i . x0 in FP64 rounded to FP32, the responses, the centre.
+int upload_logistic_bf16(is_job* j, int p) {
+  const int64_t n = j->tn;
+  const int ks2 = p <= 32 ? 1 : 2;
+  const int64_t ntiles = cdiv(n, 16);
+  const double* X = j->tdata.data();
+  const double* y = X + (size_t)n * p;
+  std::vector<uint16_t> frag((size_t)ntiles * ks2 * 2 * 64 * 8, 0);
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int ks = 0; ks < ks2; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int64_t row = 16 * t + (l & 15);
+        for (int e = 0; e < 8; ++e) {
+          const int c = 32 * ks + 8 * (l >> 4) + e;
+          if (row >= n || c >= p) continue;
+          const double x = X[(size_t)row * p + c];
+          const uint16_t h = bf16_bits((float)x);
+          const uint16_t lo = bf16_bits((float)(x - bf16_value(h)));
+          frag[((((size_t)t * ks2 + ks) * 2 + 0) * 64 + l) * 8 + e] = h;
+          frag[((((size_t)t * ks2 + ks) * 2 + 1) * 64 + l) * 8 + e] = lo;
+        }
+      }
+  std::vector<float> base((size_t)ntiles * 16, 0.0f), yv((size_t)ntiles * 16, 0.0f);
+  for (int64_t i = 0; i < n; ++i) {
+    double v = 0.0;
+    for (int c = 0; c < p; ++c) v += X[(size_t)i * p + c] * j->t_x0[c];
+    base[i] = (float)v;
+    yv[i] = (float)y[i];
+  }
+  int rc;
+  if ((rc = j->t_xfrag.alloc(frag.size() * sizeof(uint16_t))) || (rc = j->t_base.alloc(base.size() * sizeof(float))) ||
+      (rc = j->t_y.alloc(yv.size() * sizeof(float))) || (rc = j->t_x0d.alloc((size_t)p * sizeof(double))))
+    return rc;
+  HIP_TRY(locked_memcpy(j->t_xfrag.p, frag.data(), frag.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(j->t_base.p, base.data(), base.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(j->t_y.p, yv.data(), yv.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(locked_memcpy(j->t_x0d.p, j->t_x0.data(), (size_t)p * sizeof(double), hipMemcpyHostToDevice));
+  j->t_bf16 = true;
+  j->t_ks2 = ks2;
+  return IS_OK;
+}
+
 // ---------------------------------------------------------------- allocation-free scratch
 // Device memory of is_functional_moments_device_noalloc: the column-major partial
 // records (4 columns of cdiv(n, kFuncPerBlock) doubles) followed by the two reduced
@@ -1612,6 +1707,7 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
       }
       const double c0 = regression_logtarget(j, x0.data());
       j->t_shift = std::isfinite(c0) ? c0 : 0.0;
+      j->t_x0 = x0;
     }
   }
   {
@@ -1655,6 +1751,9 @@ int is_job_create(is_ctx* ctx, const is_job_desc* desc, is_job** out) {
     rc = j->precision == IS_FP32 ? upload_T<float>(j->Xd, xpart) : upload_T<double>(j->Xd, xpart);
     if (!rc) rc = j->precision == IS_FP32 ? upload_T<float>(j->yd, ypart) : upload_T<double>(j->yd, ypart);
     if (rc) return rc;
+    if (j->precision == IS_FP32 && j->tkind == IS_TARGET_LOGISTIC && dl <= 64 && (int)j->t_x0.size() == dl) {
+      if ((rc = upload_logistic_bf16(j, dl))) return rc;
+    }
     j->tdata.clear();
     j->tdata.shrink_to_fit();
   }
@@ -1705,6 +1804,8 @@ int64_t is_job_device_bytes(const is_job_desc* desc) {
     const int64_t n = desc->target_params && desc->target_params_len > 0 ? (int64_t)desc->target_params[0] : 0;
     bytes += n * (dl + 1) * ts + maxN * ts;
     bytes += std::max<int64_t>(1, target_splits(n, maxN) * maxN) * 8;
+    if (desc->target_kind == IS_TARGET_LOGISTIC && desc->precision == IS_FP32 && dl <= 64)
+      bytes += cdiv(n, 16) * (dl <= 32 ? 1 : 2) * 2 * 64 * 16 + cdiv(n, 16) * 16 * 8 + (int64_t)dl * 8;
   }
   if (desc->method == IS_DMPMC) {
     bytes += (int64_t)T * maxN * 4 + maxN * 8 + (int64_t)K * T * 12;

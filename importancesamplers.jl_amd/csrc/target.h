@@ -282,6 +282,186 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
   }
 }
 
+// ---------------------------------------------------------------- centred BF16 logistic target
+// FP32 jobs with a logistic target and p <= 64: eta = X beta as
+//   eta_i = x_i . x0 + x_i . (beta - x0),
+// with x_i . x0 (the centre x0: the job's initial location) computed once per job in
+// FP64, and the centred product on the BF16 matrix cores as three products of split
+// operands, X = Xh + Xl and delta = beta - x0 = dh + dl (each part BF16):
+//   X delta ~ Xh dh + Xh dl + Xl dh, FP32 accumulation,
+// whose dropped terms are about 2^-16 |x_i| |delta|: small because delta is a
+// draw's distance from the centre, not the coefficient itself.  Each row's term
+// y eta - max(eta, 0) - log1p(e^-|eta|) uses the one eta, so an error in eta enters
+// the sum weighted by the gradient y - sigmoid(eta), small near the posterior mode.
+// The log1p terms of a lane's rows are summed as one logarithm of the product of
+// u = 1 + t over eight rows, t = e^-|eta|, with each u's rounding t - (u - 1) added
+// exactly (a t below half an FP32 ulp of 1 is kept that way), and the logarithm by an
+// unbiased atanh series after range reduction to [sqrt(1/2), sqrt(2)).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+struct LogisticBf16Args {
+  int p;                    // coefficients, <= 32 KS2
+  int64_t n;                // data rows
+  const u32x4_t* xfrag;     // [row tile][K step][Xh, Xl][lane]: 8 BF16 of row l % 16, columns 8 (l / 16) + e
+  const float* base;        // ntiles * 16: x_i . x0 (zero past n)
+  const float* yv;          // ntiles * 16: responses (zero past n)
+  const double* x0;         // p
+  const float* beta;        // samples, SoA: beta[c * cap + first + s]
+  int64_t cap, first, n_new;
+  int64_t ntiles;           // row tiles of 16 rows
+  int64_t tiles_per_split;
+  double* part;             // [split][n_new]
+};
+
+// log(v) for v in [1, 2^24): v = m 2^e with m in [sqrt(1/2), sqrt(2)), then
+// 2 atanh(s), s = (m - 1) / (m + 1) in [-0.172, 0.172], through the s^12 term (the
+// first omitted term is below 2e-11); s by one reciprocal and an exact-residual step,
+// e ln 2 with ln 2 split in two floats.
+__device__ __forceinline__ float log_unbiased_f32(float v) {
+  int e;
+  float m = frexpf(v, &e);  // m in [0.5, 1)
+  if (m < 0.70710678f) {
+    m *= 2.0f;
+    e -= 1;
+  }
+  const float num = m - 1.0f, den = m + 1.0f;
+  const float r = __builtin_amdgcn_rcpf(den);
+  const float s0 = num * r;
+  const float sv = fmaf(fmaf(-s0, den, num), r, s0);
+  const float s2 = sv * sv;
+  float q = 1.0f / 13.0f;
+  q = fmaf(q, s2, 1.0f / 11.0f);
+  q = fmaf(q, s2, 1.0f / 9.0f);
+  q = fmaf(q, s2, 1.0f / 7.0f);
+  q = fmaf(q, s2, 1.0f / 5.0f);
+  q = fmaf(q, s2, 1.0f / 3.0f);
+  const float s2v = 2.0f * sv;
+  const float lm = fmaf(s2v * s2, q, s2v);
+  const float fe = (float)e;
+  return fmaf(fe, 0.693145751953125f, fmaf(fe, 1.428606765330187e-06f, lm));
+}
+
+template <int KS2>
+__global__ __launch_bounds__(256) void k_logistic_bf16(LogisticBf16Args a) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t s0 = (int64_t)blockIdx.x * 64;
+  const int64_t t_lo = (int64_t)blockIdx.y * a.tiles_per_split;
+  const int64_t t_hi = min(a.ntiles, t_lo + a.tiles_per_split);
+  // B operands: lane l holds delta[8 (l / 16) + e][sample s0 + 16 q + l % 16], split
+  bf16x8_t bh[4][KS2], bl[4][KS2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t s = s0 + 16 * q + (lane & 15);
+#pragma unroll
+    for (int ks = 0; ks < KS2; ++ks) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int c = 32 * ks + 8 * (lane >> 4) + e;
+        float dv = 0.0f;
+        if (c < a.p && s < a.n_new) dv = (float)((double)a.beta[(int64_t)c * a.cap + a.first + s] - a.x0[c]);
+        const __bf16 h = (__bf16)dv;
+        bh[q][ks][e] = h;
+        bl[q][ks][e] = (__bf16)(dv - (float)h);
+      }
+    }
+  }
+  const float L2E_HI = 1.44269502162933349609375f, L2E_LO = 1.925962989e-08f;
+  double ll[4] = {0.0, 0.0, 0.0, 0.0};
+  // per sample column: the product of u, the roundings of u, the sum of y eta - max(eta, 0)
+  float pm[4] = {1.0f, 1.0f, 1.0f, 1.0f}, perr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pmax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int pending = 0;  // tiles folded into pm since the last logarithm
+  // this lane's output rows of a tile: 4 (l / 16) + r
+  const int rbase = 4 * (lane >> 4);
+  u32x4_t ah[KS2], al[KS2];
+  float4_t bs, ys;
+  auto fetch = [&](int64_t t) {
+    const int64_t tc = t < t_hi ? t : t_hi - 1;
+#pragma unroll
+    for (int ks = 0; ks < KS2; ++ks) {
+      ah[ks] = a.xfrag[((tc * KS2 + ks) * 2 + 0) * 64 + lane];
+      al[ks] = a.xfrag[((tc * KS2 + ks) * 2 + 1) * 64 + lane];
+    }
+    bs = *reinterpret_cast<const float4_t*>(a.base + tc * 16 + rbase);
+    ys = *reinterpret_cast<const float4_t*>(a.yv + tc * 16 + rbase);
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ll[q] += (double)(pmax[q] - (log_unbiased_f32(pm[q]) + perr[q]));
+      pm[q] = 1.0f;
+      perr[q] = 0.0f;
+      pmax[q] = 0.0f;
+    }
+    pending = 0;
+  };
+  int64_t t = t_lo + wave;
+  if (t < t_hi) fetch(t);
+  for (; t < t_hi; t += 4) {
+    u32x4_t ch[KS2], cl[KS2];
+    float4_t cb = bs, cy = ys;
+#pragma unroll
+    for (int ks = 0; ks < KS2; ++ks) {
+      ch[ks] = ah[ks];
+      cl[ks] = al[ks];
+    }
+    if (t + 4 < t_hi) fetch(t + 4);
+    float4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS2; ++ks) {
+      const bf16x8_t xh = __builtin_bit_cast(bf16x8_t, ch[ks]);
+      const bf16x8_t xl = __builtin_bit_cast(bf16x8_t, cl[ks]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bh[q][ks], acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, bl[q][ks], acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, bh[q][ks], acc[q], 0, 0, 0);
+      }
+    }
+    const int64_t row0 = t * 16 + rbase;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool valid = row0 + r < a.n;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float eta = cb[r] + acc[q][r];
+        const float av = fabsf(eta);
+        const float e_hi = -av * L2E_HI;
+        const float e_lo = fmaf(-av, L2E_HI, -e_hi) - av * L2E_LO;
+        const float t0 = __builtin_amdgcn_exp2f(e_hi);
+        const float tv = fmaf(t0 * 0.6931471805599453f, e_lo, t0);
+        const float u = 1.0f + tv;
+        if (valid) {
+          // y eta - max(eta, 0) is exact for y in {0, 1}
+          pmax[q] += fmaf(cy[r], eta, -fmaxf(eta, 0.0f));
+          perr[q] += tv - (u - 1.0f);
+          pm[q] *= u;
+        }
+      }
+    }
+    if (++pending == 2) flush();
+  }
+  if (pending) flush();
+  // sum the four row groups of each column (lanes l, l^16, l^32, l^48), then the waves
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ll[q] += __shfl_xor(ll[q], 16, 64);
+    ll[q] += __shfl_xor(ll[q], 32, 64);
+  }
+  __shared__ double red[4 * 64];
+  if (lane < 16) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 64 + 16 * q + lane] = ll[q];
+  }
+  __syncthreads();
+  if (tid < 64 && s0 + tid < a.n_new) {
+    const double v = ((red[tid] + red[64 + tid]) + red[128 + tid]) + red[192 + tid];
+    a.part[(int64_t)blockIdx.y * a.n_new + s0 + tid] = v;
+  }
+}
+
 // Per new sample: the row-slice partials in slice order, plus the log prior
 // and normalising constants, into the target value the mixture kernel reads.
 template <class T>
