@@ -132,6 +132,10 @@ struct is_job {
   bool t_bf16 = false;
   int t_ks2 = 0;                          // 32-wide K steps
   DevBuf t_xfrag, t_base, t_y, t_x0d;
+  // FP32 jobs with a two-pass (batch-target) round, 16 <= d <= 64: the mixture pass on
+  // the matrix cores (k_mix_prep, k_mix_mfma): inverse-factor fragments, centres and
+  // log normalisers of up to nslots components, and the samples' mixture values
+  DevBuf mix_frag, mix_mu, mix_lognorm, mix_out;
   // device buffers
   DevBuf X, logtargets, lognums, logweights, round_ids, proposal_ids;
   DevBuf slots64, slotsT, logmassT, zerosT, tslotT, assign, bank_ids_d;
@@ -566,6 +570,66 @@ SweepArgs<T> sweep_args(is_job* j, int t) {
   return w;
 }
 
+// FP32 jobs whose rounds draw their samples before the mixture pass (a batch target,
+// or a named layout), at 16 <= d <= 64: that pass's component densities run on the
+// matrix cores (k_mix_prep, k_mix_mfma) for rounds of at least kMixMfmaComponents
+// components, except on the extended-range replay kernels.
+constexpr int kMixMfmaComponents = 8;
+inline bool mix_on_mfma(const is_job* j) {
+  const bool two_pass = j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG ||
+                        j->tkind == IS_TARGET_CALLBACK || j->has_layout;
+  return j->precision == IS_FP32 && two_pass && j->d >= 16 && j->d <= 64 && j->method != IS_NPMC &&
+         j->method != IS_PLAIN;
+}
+
+// The mixture values of a round's new samples on the matrix cores: each component's
+// inverse factor, then the samples' log-sum-exp over the components [a.k0, a.k1].
+template <class T>
+void launch_mix_mfma(is_job* j, const RoundArgs<T>& a, int ncomp, hipStream_t s) {
+  const int d = j->d, kt = cdiv(d, 16);
+  MixPrepArgs pa{};
+  pa.d = d;
+  pa.kp = 16 * kt;
+  pa.k0 = a.k0;
+  pa.ncomp = ncomp;
+  pa.slots64 = j->slots64.as<double>();
+  pa.stride = j->stride;
+  pa.frag = j->mix_frag.as<float>();
+  pa.mu = j->mix_mu.as<float>();
+  pa.lognorm = j->mix_lognorm.as<float>();
+  MixMfmaArgs ma{};
+  ma.d = d;
+  ma.kp = 16 * kt;
+  ma.ncomp = ncomp;
+  ma.k0 = a.k0;
+  ma.frag = pa.frag;
+  ma.mu = pa.mu;
+  ma.lognorm = pa.lognorm;
+  ma.logmass = reinterpret_cast<const float*>(a.logmass);
+  ma.fam.student = a.fam.student;
+  ma.fam.dof = (float)a.fam.dof;
+  ma.fam.half_dof_plus_d = (float)a.fam.half_dof_plus_d;
+  ma.X = reinterpret_cast<const float*>(a.X);
+  ma.cap = a.cap;
+  ma.first = a.first;
+  ma.n_new = a.n_new;
+  ma.assign = a.assign;
+  ma.gen_slot = a.gen_slot;
+  ma.out = j->mix_out.as<float>();
+  const dim3 gp((unsigned)ncomp), gm((unsigned)cdiv(a.n_new, 64));
+  switch (kt) {
+#define IS_MIX_CASE(v)                                                                     \
+  case v:                                                                                  \
+    hipLaunchKernelGGL(k_mix_prep<v>, gp, dim3(64), 0, s, pa);                             \
+    hipLaunchKernelGGL(k_mix_mfma<v>, gm, dim3(256), 0, s, ma);                            \
+    break;
+    IS_MIX_CASE(1) IS_MIX_CASE(2) IS_MIX_CASE(3) IS_MIX_CASE(4)
+#undef IS_MIX_CASE
+    default:
+      break;
+  }
+}
+
 // Samples per sweep block of a fused round over `first` retained samples: the
 // fewest whole 64-sample steps per wave (4 waves per block) that keep the sweep
 // blocks and the round's nb_new new-sample blocks within one pass of the chip
@@ -677,6 +741,18 @@ int run_rounds(is_job* j) {
   const bool batch_target = j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG ||
                             j->tkind == IS_TARGET_CALLBACK || j->has_layout;
   const int dl = j->dl;
+  // the mixture pass after the target values: on the matrix cores where mix_on_mfma
+  // allows, then the new-sample kernel in pre-target mode
+  auto pretarget = [&](const RoundArgs<T>& a, int ncomp) {
+    RoundArgs<T> b = a;
+    if constexpr (sizeof(T) == 4) {
+      if (mix_on_mfma(j) && !j->range_mode && b.mis_terms == 0 && ncomp >= kMixMfmaComponents && b.n_new > 0) {
+        launch_mix_mfma<T>(j, b, ncomp, s);
+        b.pre_mix = j->mix_out.as<T>();
+      }
+    }
+    launch_mix<T, kPreTarget>(D, ncomp, b, s);
+  };
   auto mix = [&](RoundArgs<T>& a, int ncomp) -> int {
     if (!batch_target) {
       launch_mix<T, kDrawFused>(D, ncomp, a, s);
@@ -693,7 +769,7 @@ int run_rounds(is_job* j) {
         hipLaunchKernelGGL((k_target_logical<T>), dim3((unsigned)cdiv(a.n_new, 256)), dim3(256), 0, s, j->tkind, dl,
                            j->tslotT.as<T>(), a.XL, a.cap, a.first, a.n_new, j->tpre.as<T>());
       a.pre_logtarget = j->tpre.as<T>();
-      launch_mix<T, kPreTarget>(D, ncomp, a, s);
+      pretarget(a, ncomp);
       return IS_OK;
     }
     if (j->tkind == IS_TARGET_CALLBACK) {
@@ -716,7 +792,7 @@ int run_rounds(is_job* j) {
         return fail(IS_EFAILED, "batch target callback returned %d in round %d", cb, a.round + 1);
       }
       a.pre_logtarget = j->tpre.as<T>();
-      launch_mix<T, kPreTarget>(D, ncomp, a, s);
+      pretarget(a, ncomp);
       return IS_OK;
     }
     // draw, batched target on MFMA, then the mixture with the precomputed target values
@@ -747,7 +823,7 @@ int run_rounds(is_job* j) {
                          (int)used, a.n_new, (const T*)ba.beta, ba.cap, ba.first, dl, j->t_inv_two_prior,
                          j->t_const - j->t_shift, j->tpre.as<T>());
       a.pre_logtarget = j->tpre.as<T>();
-      launch_mix<T, kPreTarget>(D, ncomp, a, s);
+      pretarget(a, ncomp);
       return IS_OK;
     }
     TargetArgs<T> ta{};
@@ -769,7 +845,7 @@ int run_rounds(is_job* j) {
                        a.n_new, ta.beta, ta.cap, ta.first, dl, j->t_inv_two_prior, j->t_const - j->t_shift,
                        j->tpre.as<T>());
     a.pre_logtarget = j->tpre.as<T>();
-    launch_mix<T, kPreTarget>(D, ncomp, a, s);
+    pretarget(a, ncomp);
     return IS_OK;
   };
   auto reduce = [&](const DevBuf& partials, int nrec, int nv, DevBuf& reduced, const FinishArgs& f) {
@@ -1242,6 +1318,14 @@ int alloc_job_buffers(is_job* j) {
   if ((j->tkind == IS_TARGET_CALLBACK || (j->has_layout && (j->tkind == IS_TARGET_STANDARD || j->tkind == IS_TARGET_GAUSSIAN))) &&
       (rc = j->tpre.alloc((size_t)j->maxN * sizeof(T))))
     return rc;
+  if (mix_on_mfma(j)) {
+    const int kt = cdiv(d, 16);
+    const size_t ncomp = (size_t)std::max(j->nslots, 1);
+    if ((rc = j->mix_frag.alloc(ncomp * mix_pairs(kt) * 64 * sizeof(float))) ||
+        (rc = j->mix_mu.alloc(ncomp * 16 * kt * sizeof(float))) || (rc = j->mix_lognorm.alloc(ncomp * sizeof(float))) ||
+        (rc = j->mix_out.alloc((size_t)j->maxN * sizeof(float))))
+      return rc;
+  }
   if (j->tkind == IS_TARGET_LOGISTIC || j->tkind == IS_TARGET_LINREG) {
     if ((rc = j->tpre.alloc((size_t)j->maxN * sizeof(T)))) return rc;
     int64_t part = 1;

@@ -70,6 +70,7 @@ struct RoundArgs {
   int tkind;
   const T* tslot;           // Gaussian target slot
   const T* pre_logtarget;   // batch-target values for new samples (kPreTarget)
+  const T* pre_mix;         // mixture log densities of the new samples (k_mix_mfma), or null
   // storage
   T* X;
   int64_t cap;
@@ -113,6 +114,7 @@ __device__ __forceinline__ T inline_target(int tkind, int d, P tslot, const T (&
 
 #include "mix.h"
 #include "target.h"
+#include "mix_mfma.h"
 #include "layout.h"
 
 // ---------------------------------------------------------------- partial reduction + finish

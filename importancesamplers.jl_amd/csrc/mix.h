@@ -322,6 +322,18 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
         }
       }
     }
+  } else if (a.pre_mix) {
+    // the mixture log density from the matrix-core pass (NaN: the generating
+    // component's density is not valid)
+    if (wave == 0 && ok) {
+      const T v = a.pre_mix[j];
+      if (v != v) {
+        genbad = 1;
+      } else {
+        m = v;
+        sum = T(1);
+      }
+    }
   } else {
     for (int base = 0; base < ncomp; base += chunk) {
       const int cnt = min(chunk, ncomp - base);
