@@ -31,6 +31,18 @@ __host__ __device__ constexpr int bslot_pad(int d) { return (d + 3) & ~3; }
 __host__ __device__ constexpr int bslot_lognorm(int P) { return 2 * P + bslot_rowoff(P); }
 __host__ __device__ constexpr int bslot_size(int P) { return (bslot_lognorm(P) + 2 + 3) & ~3; }
 
+// FP32 jobs keep, per LDS slot, the inverse factor M = L^{-1} (FP64 solve, FP32 values)
+// as v_mfma_f32_16x16x4f32 A operands for the whitened coordinates y = M (x - mu):
+// row tiles i < ceil(P / 16), K steps kk < min(4 (i + 1), P / 4) (M is lower
+// triangular and coordinates past P are not read), 64 lanes per (i, kk) pair, lane l
+// holding M[16 i + l % 16][4 kk + l / 16] (rows past P zero).
+__host__ __device__ constexpr int bfrag_pairs(int P) {
+  int n = 0;
+  for (int i = 0; i < (P + 15) / 16; ++i) n += (4 * (i + 1) < P / 4) ? 4 * (i + 1) : P / 4;
+  return n;
+}
+__host__ __device__ constexpr int bfrag_elems(int P) { return 64 * bfrag_pairs(P); }
+
 // Samples are stored in blocks of 64: coordinate c of sample i sits at
 // bx_base(i, P) + 64 c, so a wave's 64 samples of one coordinate are one
 // contiguous 256-byte (FP32) segment and every coordinate is a constant offset.
@@ -70,7 +82,8 @@ struct BatchArgs {
   int slots_off;                // element offset of the slot area in LDS (after the wave stages)
 };
 
-// LDS bytes of one launch: W wave stages plus the largest job's rounds x slot.
+// LDS bytes of one launch: W wave stages plus the largest job's rounds x slot
+// (for FP32 jobs, with the slots' factor fragments after them).
 inline size_t batch_lds_bytes(int W, int tsize, int max_slot_area) {
   return (size_t)W * 64 * kBatchStage * tsize + (size_t)max_slot_area * tsize;
 }

@@ -416,7 +416,9 @@ int is_batch_create(is_ctx* ctx, const is_job_desc* descs, int32_t njobs, int32_
     }
     Xs[j] = tp + head;
     ys[j] = tp + head + H.n_rows * H.d;
-    b->max_slot_area = std::max(b->max_slot_area, H.rounds * H.SS);
+    // FP32: each LDS slot's factor fragments after the slots (element offset rounds x SS)
+    const int frag_area = precision == IS_FP32 ? H.rounds * bfrag_elems(H.P) : 0;
+    b->max_slot_area = std::max(b->max_slot_area, H.rounds * H.SS + frag_area);
   }
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));

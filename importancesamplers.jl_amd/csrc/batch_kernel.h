@@ -52,37 +52,54 @@ struct Vec16<double> {
   static constexpr int E = 2;
 };
 
-// Student-t radial randomness of one sample: the 8 buffered gamma normals and
-// uniforms of _buffered_unit_gamma (src/native_execution.jl:147-180), four per
-// Philox block, one block at a time (the scheduler barriers keep the blocks
-// from being interleaved, which would multiply the live registers).
-__device__ inline void batch_radial(const RngKey& key, uint64_t sample, uint64_t round, int rs, int us, double* rn,
-                                    double* ru) {
+// Student-t radial multiplier of one sample (the 8 buffered gamma attempts of
+// _buffered_unit_gamma, src/native_execution.jl:147-180) with the attempts generated
+// one Philox block (four attempts) at a time, the second block only for lanes that
+// reject all of the first: the values and the result of the buffered form, without
+// holding all 17 values in registers.
+__device__ inline void batch_radial_normals(const RngKey& key, uint64_t sample, uint64_t round, int blk,
+                                            double (&rn)[4]) {
+  const U64x4 w = philox4x64(sample, round, kPurposeRadialNormals, (uint64_t)blk, key.k0, key.k1);
 #pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
+    double sn, cs;
+    sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
+    rn[2 * p] = r * cs;
+    rn[2 * p + 1] = r * sn;
+  }
+}
+
+__device__ inline bool batch_multiplier(const RngKey& key, uint64_t sample, uint64_t round, int us, double dof,
+                                        double* out) {
+  double rn[4];
+  if (us == 0 || dof == 1.0) {
+    batch_radial_normals(key, sample, round, 0, rn);
+    const double den = fabs(rn[0]);
+    if (!(isfinite(den) && den > 0.0)) return false;
+    *out = 1.0 / den;
+    return true;
+  }
+  const GammaSetup g = gamma_setup(dof);
+#pragma unroll 1
   for (int blk = 0; blk < kGammaAttempts / 4; ++blk) {
-    if (blk * 4 < rs) {
-      const U64x4 w = philox4x64(sample, round, kPurposeRadialNormals, (uint64_t)blk, key.k0, key.k1);
+    batch_radial_normals(key, sample, round, blk, rn);
+    const U64x4 wu = philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)blk, key.k0, key.k1);
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
-        double sn, cs;
-        sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
-        rn[4 * blk + 2 * p] = r * cs;
-        rn[4 * blk + 2 * p + 1] = r * sn;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const double root = 1.0 + g.coefficient * rn[q];
+      if (!(root > 0.0)) continue;
+      if (!gamma_accepts(g, rn[q], root, u53(wu.v[q]))) continue;
+      return gamma_multiplier(
+          g, dof, root,
+          [&] {
+            const int b = kGammaAttempts / 4;  // the extra uniform: block 2, word 0
+            return u53(philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)b, key.k0, key.k1).v[0]);
+          },
+          out);
     }
-    __builtin_amdgcn_sched_barrier(0);
   }
-#pragma unroll
-  for (int blk = 0; blk < (kGammaAttempts + 4) / 4; ++blk) {
-    if (blk * 4 < us) {
-      const U64x4 w = philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)blk, key.k0, key.k1);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (blk * 4 + q <= kGammaAttempts) ru[blk * 4 + q] = u53(w.v[q]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
+  return false;
 }
 
 // Whitened density of a batch slot (layout in batch.h, padded dimension P)
@@ -179,23 +196,30 @@ __device__ inline void batch_accumulate(T lw, const T (&x)[P], const T* centre, 
       A.acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, A.acc[2], 0, 0, 0);
       A.acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, A.acc[4], 0, 0, 0);
     }
+    // four steps' operands in flight at a time, not all sixteen
+    if ((st & 3) == 3) __builtin_amdgcn_sched_barrier(0);
   }
   wave_sync();
 }
 
-// Regression log-likelihood of the wave's 64 staged samples (lane = sample):
-// eta = X_data beta on MFMA over 16-row tiles, epilogue per element, FP32
-// partial sums folded into FP64 every 8 tiles.
+// The wave's 64 staged samples as MFMA B operands: breg[q][k] = coordinate 4 k + l / 16
+// of sample 16 q + l % 16 (lane l).
 template <class T, int P>
-__device__ inline double batch_target(const BatchJob<T>& J, const T* stage, int lane) {
-  typedef MfmaTile<T> M;
-  typedef T V4 __attribute__((ext_vector_type(4)));
-  constexpr int KS = P / 4;
-  T breg[4][KS];
+__device__ inline void batch_operands(const T* stage, int lane, T (&breg)[4][P / 4]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int k = 0; k < KS; ++k) breg[q][k] = stage[(16 * q + (lane & 15)) * kBatchStage + 4 * k + (lane >> 4)];
+    for (int k = 0; k < P / 4; ++k) breg[q][k] = stage[(16 * q + (lane & 15)) * kBatchStage + 4 * k + (lane >> 4)];
+}
+
+// Regression log-likelihood of the wave's 64 samples (lane = sample), from their B
+// operands: eta = X_data beta on MFMA over 16-row tiles, epilogue per element, FP32
+// partial sums folded into FP64 every 8 tiles.
+template <class T, int P>
+__device__ inline double batch_target(const BatchJob<T>& J, const T (&breg)[4][P / 4], int lane) {
+  typedef MfmaTile<T> M;
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  constexpr int KS = P / 4;
   const T* xp = J.xpack + lane;
   const V4* yp = reinterpret_cast<const V4*>(J.ypack) + lane;
   const bool logistic = J.tkind == kTLogistic;
@@ -262,13 +286,110 @@ __device__ inline double batch_target(const BatchJob<T>& J, const T* stage, int 
   return logistic ? mine : -J.inv_two_var * mine;
 }
 
-// Wave 0: merge the W wave records, write the round summary and fit slot t + 1.
-// rec(w) = [mw, s0, s2, s1(32), tiles 3 x 256] as doubles in wave w's stage.
-template <class T, int W, int P>
-__device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, int rec_stride, T* slots_lds,
-                                    int lane) {
-  const int d = J.d;
-  const int SS = J.slot_elems;
+// FP32 jobs: r^2 = |M (x - mu)|^2 of the wave's 64 samples under one slot on
+// v_mfma_f32_16x16x4f32 (A = the slot's factor fragments, B = the samples' centred
+// coordinates, four 16-sample tiles q), the squares summed over each lane's tile rows
+// and then over the four lanes of a sample by a two-step exchange that leaves lane l
+// with the sum of its own sample (16 q + l % 16 with q = l / 16).
+template <int P>
+__device__ inline float batch_mfma_r2(const float* frag, const float* mu, const float (&bx)[4][P / 4], int lane) {
+  constexpr int KS = P / 4, KT = (P + 15) / 16;
+  float4_t acc[4][KT];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < KT; ++i) acc[q][i] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  float bm[KS];
+#pragma unroll
+  for (int kk = 0; kk < KS; ++kk) bm[kk] = mu[4 * kk + (lane >> 4)];
+  int pidx = 0;
+#pragma unroll
+  for (int i = 0; i < KT; ++i)
+#pragma unroll
+    for (int kk = 0; kk < (4 * (i + 1) < KS ? 4 * (i + 1) : KS); ++kk) {
+      const float a = frag[pidx * 64 + lane];
+      ++pidx;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc[q][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bx[q][kk] - bm[kk], acc[q][i], 0, 0, 0);
+    }
+  float r2q[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    float v = 0.0f;
+#pragma unroll
+    for (int i = 0; i < KT; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v = fmaf(acc[q][i][r], acc[q][i][r], v);
+    r2q[q] = v;
+  }
+  // lanes l and l ^ 32 swap halves (keep tiles q with q / 2 = l / 32), then l and l ^ 16
+  const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
+  float k0 = h5 ? r2q[2] : r2q[0], k1 = h5 ? r2q[3] : r2q[1];
+  k0 += __shfl_xor(h5 ? r2q[0] : r2q[2], 32, 64);
+  k1 += __shfl_xor(h5 ? r2q[1] : r2q[3], 32, 64);
+  const float keep = h4 ? k1 : k0;
+  return keep + __shfl_xor(h4 ? k0 : k1, 16, 64);
+}
+
+// Log density of a slot from r^2 (the finite-range values of radial_logdensity); a
+// non-finite r^2 gives NaN (the sample is then failed as an invalid proposal).  For
+// r^2 below the smallest normal, log1p of the ratio is as far below one unit in the
+// last place of the log normaliser as the density kernels' rescaled sum.
+__device__ inline float batch_radial_r2(const Family<float>& fam, float lognorm, float r2) {
+  if (!fam.student) return isfinite(r2) ? lognorm - 0.5f * r2 : __builtin_nanf("");
+  const float ratio = r2 / fam.dof;
+  float radial;
+  if (isfinite(ratio))
+    radial = log1p(ratio);
+  else if (isfinite(r2))
+    radial = log(r2) - log(fam.dof);
+  else
+    return __builtin_nanf("");
+  return lognorm - fam.half_dof_plus_d * radial;
+}
+
+// Wave 0 (FP32 jobs): the factor fragments of one slot from its FP64 factor, row r in
+// lane r's Lrow (entries past the diagonal unread) and 1 / L_rr in lane r's inv.  Lane c
+// solves L m = e_c by forward substitution in FP64, m_r = (delta_rc - sum_{k<r} L_rk m_k)
+// / L_rr (the order of the density kernels' solve), reading L_rk from lane r with
+// v_readlane (no memory operations, so nothing can hoist a row's reads ahead of the
+// previous rows' arithmetic); the columns are transposed through LDS (Ms, P x P floats)
+// into the fragments.
+template <int P>
+__device__ inline void batch_frags(const double (&Lrow)[P], double inv, float* Ms, float* frag, int lane) {
+  double m[P];
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+    double v = (r == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < r; ++k) v -= readlane_f64(Lrow[k], r) * m[k];
+    m[r] = v * readlane_f64(inv, r);
+  }
+  if (lane < P) {
+#pragma unroll
+    for (int r = 0; r < P; ++r) Ms[r * P + lane] = (float)m[r];
+  }
+  wave_sync();
+  int pidx = 0;
+#pragma unroll
+  for (int i = 0; i < (P + 15) / 16; ++i) {
+    const int row = 16 * i + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < (4 * (i + 1) < P / 4 ? 4 * (i + 1) : P / 4); ++kk) {
+      frag[pidx * 64 + lane] = row < P ? Ms[row * P + 4 * kk + (lane >> 4)] : 0.0f;
+      ++pidx;
+    }
+  }
+  wave_sync();
+}
+
+// All waves, round end: the W wave records rec(w) = [mw, s0, s2, s1(32), tiles 3 x 256]
+// (doubles in wave w's stage) merged entry by entry into rec(0), each entry of the
+// first-order sums and moment tiles scaled by exp(mw - M) and summed in wave order;
+// thread tid owns entries tid, tid + 64 W, ...  The summary words stay for batch_finish.
+template <int W>
+__device__ inline void batch_merge(double* rec0, int rec_stride, int tid) {
   const double ninf = -__builtin_huge_val();
   double M = rec0[0];
 #pragma unroll
@@ -279,35 +400,34 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
     const double mw = rec0[w * rec_stride];
     f[w] = (mw == ninf) ? 0.0 : exp(mw - M);
   }
+  for (int e = 3 + tid; e < 35 + 3 * 256; e += 64 * W) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v += f[w] * rec0[w * rec_stride + e];
+    rec0[e] = v;
+  }
+}
+
+// Wave 0, after batch_merge: the round summary and the fit of slot t + 1.
+// FP32 jobs also get slot t + 1's factor fragments (frags: the LDS fragment area;
+// scratch: P x P floats of LDS, free at this point).
+template <class T, int W, int P>
+__device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, int rec_stride, T* slots_lds,
+                                    float* frags, float* scratch, int lane) {
+  const int d = J.d;
+  const int SS = J.slot_elems;
+  const double ninf = -__builtin_huge_val();
+  double M = rec0[0];
+#pragma unroll
+  for (int w = 1; w < W; ++w) M = fmax(M, rec0[w * rec_stride]);
   double s0 = 0.0, s2 = 0.0;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    s0 += f[w] * rec0[w * rec_stride + 1];
-    s2 += f[w] * f[w] * rec0[w * rec_stride + 2];
+    const double mw = rec0[w * rec_stride];
+    const double f = (mw == ninf) ? 0.0 : exp(mw - M);
+    s0 += f * rec0[w * rec_stride + 1];
+    s2 += f * f * rec0[w * rec_stride + 2];
   }
-  // first-order sums (lane c) and the moment tiles (4 x 3 entries per lane), merged in place
-  double s1 = 0.0;
-  if (lane < kBatchMaxDim) {
-#pragma unroll
-    for (int w = 0; w < W; ++w) s1 += f[w] * rec0[w * rec_stride + 3 + lane];
-  }
-  double tv[12];
-#pragma unroll
-  for (int p = 0; p < 3; ++p)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      double v = 0.0;
-#pragma unroll
-      for (int w = 0; w < W; ++w) v += f[w] * rec0[w * rec_stride + 35 + p * 256 + lane * 4 + i];
-      tv[p * 4 + i] = v;
-    }
-  wave_sync();
-  if (lane < kBatchMaxDim) rec0[3 + lane] = s1;
-#pragma unroll
-  for (int p = 0; p < 3; ++p)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) rec0[35 + p * 256 + lane * 4 + i] = tv[p * 4 + i];
-  wave_sync();
   // summary (src/execution.jl:176-186)
   const int64_t count = (int64_t)(t + 1) * J.N;
   double ess, logz;
@@ -394,6 +514,9 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
   T* oL = (t + 1 < J.rounds) ? slots_lds + (t + 1) * SS : nullptr;
   bool ok = true;
   double lg = 0.0;
+  double Lrow[P], inv_out = 1.0;  // the new factor's row `lane` and 1 / L_rr (FP32 fragments)
+#pragma unroll
+  for (int c = 0; c < P; ++c) Lrow[c] = 0.0;
   if (lane < P) {
     double mu = 0.0, invd = 1.0;
     if (row) {
@@ -414,6 +537,7 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
     if (oL) {
       oL[lane] = (T)mu;
       oL[P + lane] = slot_inv_value<T>(invd);
+      inv_out = slot_inv_value<double>(invd);
     }
     const int base = 2 * P + bslot_rowoff(lane);
 #pragma unroll
@@ -424,6 +548,7 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
         o64[base + c] = v;
         oT[base + c] = (T)v;
         if (oL) oL[base + c] = (T)v;
+        Lrow[c] = v;
       }
     }
   }
@@ -440,6 +565,7 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
     if (oL) oL[off] = (T)lognorm;
     J.fit_status[t] = st;
   }
+  if (oL && frags) batch_frags<P>(Lrow, inv_out, scratch, frags + (t + 1) * bfrag_elems(P), lane);
 }
 
 // One job, all rounds, by the whole workgroup; P = the job's padded dimension.
@@ -455,8 +581,27 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
   fam.student = J.student;
   fam.dof = J.fdof;
   fam.half_dof_plus_d = J.fhalf;
+  // FP32: the slots' factor fragments after the slots; wave 1's stage is the LDS
+  // scratch of their transposes (free whenever wave 0 computes them)
+  constexpr bool kMfma = sizeof(T) == 4;
+  constexpr int FE = bfrag_elems(P);
+  float* frags = kMfma ? reinterpret_cast<float*>(slots + J.rounds * SS) : nullptr;
+  float* scratch = reinterpret_cast<float*>(lds_t + 64 * kBatchStage);
   // slot 0 (set from the committed proposal by k_batch_begin)
   for (int q = tid; q < SS; q += 64 * W) slots[q] = J.slotsT[q];
+  if (kMfma && wave == 0) {
+    const double* s64 = J.slots64;
+    double Lrow[P], inv = 1.0;
+#pragma unroll
+    for (int c = 0; c < P; ++c) Lrow[c] = 0.0;
+    if (lane < P) {
+#pragma unroll
+      for (int c = 0; c < P; ++c)
+        if (c <= lane) Lrow[c] = s64[2 * P + bslot_rowoff(lane) + c];
+      inv = s64[P + lane];
+    }
+    batch_frags<P>(Lrow, inv, scratch, frags, lane);
+  }
   __syncthreads();
   for (int t = 0; t < J.rounds; ++t) {
     const uint64_t rng_word = run_hi | (uint64_t)t;
@@ -480,9 +625,7 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       {
         double mult = 1.0;
         if (J.student) {
-          double rn[kGammaAttempts], ru[kGammaAttempts + 1];
-          batch_radial(key, (uint64_t)j, rng_word, J.rs, J.us, rn, ru);
-          if (!student_t_multiplier(J.dof, rn, ru, J.us > 0, &mult)) {
+          if (!batch_multiplier(key, (uint64_t)j, rng_word, J.us, J.dof, &mult)) {
             if (active) record_failure(fail_t, j + 1, 0, kDrawExhausted);
             mult = __builtin_nan("");
             ok = false;
@@ -506,11 +649,22 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
             zq[2 * p] = T(r * cs * mult);
             zq[2 * p + 1] = T(r * sn * mult);
           }
+          // rows r >= 4 blk: the row's four entries of this column block in one
+          // 16-byte-aligned LDS read (entries past the diagonal are not added),
+          // a few rows' reads in flight at a time
+          typedef typename Vec16<T>::type V;
+          constexpr int E = Vec16<T>::E;
+          constexpr int RG = 16 / E;  // rows per scheduling group
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int c = 4 * blk + e;
+          for (int r = 4 * blk; r < P; ++r) {
+            V ch[4 / E];
 #pragma unroll
-            for (int r = c; r < P; ++r) x[r] += Ls[bslot_rowoff(r) + c] * zq[e];
+            for (int h = 0; h < 4 / E; ++h)
+              ch[h] = *reinterpret_cast<const V*>(Ls + bslot_rowoff(r) + 4 * blk + E * h);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (4 * blk + e <= r) x[r] += ch[e / E][e % E] * zq[e];
+            if ((r - 4 * blk) % RG == RG - 1) __builtin_amdgcn_sched_barrier(0);
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -534,8 +688,10 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       }
       // target on MFMA for the wave's 64 samples (coordinates staged in LDS)
       wave_sync();
-      const double ll = batch_target<T, P>(J, stage, lane);
+      T bx[4][P / 4];
+      batch_operands<T, P>(stage, lane, bx);
       wave_sync();
+      const double ll = batch_target<T, P>(J, bx, lane);
       double b2 = 0.0;
 #pragma unroll
       for (int c = 0; c < P; ++c) b2 += (double)x[c] * (double)x[c];
@@ -547,10 +703,25 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
           ok = false;
         }
       }
-      // deterministic mixture over slots 0..t (src/mis_execution.jl:30-95)
+      // deterministic mixture over slots 0..t (src/mis_execution.jl:30-95); FP32:
+      // the whitened coordinates of the whole wave on MFMA, per slot
       T m = ninf, sum = T(0);
       bool genbad = false;
-      if (ok) {
+      if constexpr (kMfma) {
+        for (int k = 0; k <= t; ++k) {
+          const T* sk = slots + k * SS;
+          const T r2 = batch_mfma_r2<P>(frags + k * FE, sk, bx, lane);
+          const T q = batch_radial_r2(fam, sk[bslot_lognorm(P)], r2);
+          if (k == t && (q != q || q == ninf)) genbad = true;
+          const T v = J.logN + q;
+          if (v > m) {
+            sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
+            m = v;
+          } else {
+            sum += LseAcc<T>::fast_exp(v - m);
+          }
+        }
+      } else if (ok) {
         T y[P];
         for (int k = 0; k <= t; ++k) {
           const T q = bslot_logdensity<T, P>(slots + k * SS, fam, x, y);
@@ -601,10 +772,28 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       }
       const T cln = v ? J.ln[i] : T(0);
       const T clt = v ? J.lt[i] : ninf;
+      T qm = T(0);
+      if constexpr (kMfma) {
+        // B operands straight from the sample pool: base is a multiple of 64, so the
+        // wave's samples are one 64-sample block (lanes past `first` read samples of
+        // round t, whose columns are ignored)
+        const T* xblk = J.X + bx_base(base, P);
+        T bb[4][P / 4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int k = 0; k < P / 4; ++k) bb[q][k] = xblk[64 * (4 * k + (lane >> 4)) + 16 * q + (lane & 15)];
+        qm = batch_radial_r2(fam, slot_t[bslot_lognorm(P)], batch_mfma_r2<P>(frags + t * FE, slot_t, bb, lane));
+      }
       T lwv = ninf;
       if (v) {
-        T y[P];
-        const T q = bslot_logdensity<T, P>(slot_t, fam, cx, y);
+        T q;
+        if constexpr (kMfma) {
+          q = qm;
+        } else {
+          T y[P];
+          q = bslot_logdensity<T, P>(slot_t, fam, cx, y);
+        }
         uint16_t reason = (q != q) ? kProposalInvalid : 0;
         T num = cln;
         if (!reason) {
@@ -652,7 +841,10 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
 #pragma unroll
       for (int i = 0; i < 4; ++i) rec[35 + p * 256 + lane * 4 + i] = A.acc[p][i];
     __syncthreads();
-    if (wave == 0) batch_finish<T, W, P>(J, t, reinterpret_cast<double*>(lds_t), rec_stride, slots, lane);
+    batch_merge<W>(reinterpret_cast<double*>(lds_t), rec_stride, tid);
+    __syncthreads();
+    if (wave == 0)
+      batch_finish<T, W, P>(J, t, reinterpret_cast<double*>(lds_t), rec_stride, slots, frags, scratch, lane);
     __syncthreads();
   }
 }

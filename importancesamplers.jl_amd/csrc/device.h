@@ -694,7 +694,37 @@ __device__ __forceinline__ bool slot_draw(P s, int d, const T (&z)[D], T (&x)[D]
 
 // ---------------------------------------------------------------- Student-t radial multiplier
 // _buffered_unit_gamma / _student_t_radial_multiplier (src/native_execution.jl:147-180)
-// with the gamma candidate of src/proposals.jl:738-751.
+// with the gamma candidate of src/proposals.jl:738-751, in three steps shared by the
+// buffered form below and the batched kernel's form (which generates its attempts
+// block by block): the constants of the shape dof / 2, the acceptance test of one
+// attempt whose root 1 + c normal is positive, and the multiplier of the accepted one.
+struct GammaSetup {
+  double shape, offset, coefficient;
+};
+__device__ inline GammaSetup gamma_setup(double dof) {
+  GammaSetup g;
+  g.shape = dof / 2.0;
+  const double adjusted = g.shape < 1.0 ? g.shape + 1.0 : g.shape;
+  g.offset = adjusted - 1.0 / 3.0;
+  g.coefficient = (1.0 / sqrt(g.offset)) / 3.0;
+  return g;
+}
+__device__ inline bool gamma_accepts(const GammaSetup& g, double normal, double root, double uniform) {
+  const double cube = root * root * root;
+  const double n2 = normal * normal;
+  return uniform < 1.0 - 0.0331 * (n2 * n2) || log(uniform) < 0.5 * n2 + g.offset * (1.0 - cube + log(cube));
+}
+// boost(): the extra uniform, read only for a shape below 1
+template <class Boost>
+__device__ inline bool gamma_multiplier(const GammaSetup& g, double dof, double root, Boost boost, double* out) {
+  double gamma = g.offset * (root * root * root);
+  if (g.shape < 1.0) gamma *= pow(boost(), 1.0 / g.shape);
+  if (!(isfinite(gamma) && gamma > 0.0)) return false;
+  *out = sqrt(dof / (2.0 * gamma));
+  return true;
+}
+
+// Over buffered attempts: rn[kGammaAttempts], ru[kGammaAttempts + 1].
 __device__ inline bool student_t_multiplier(double dof, const double* rn, const double* ru,
                                             bool have_uniforms, double* out) {
   if (!have_uniforms || dof == 1.0) {
@@ -703,25 +733,12 @@ __device__ inline bool student_t_multiplier(double dof, const double* rn, const 
     *out = 1.0 / den;
     return true;
   }
-  const double shape = dof / 2.0;
-  const double adjusted = shape < 1.0 ? shape + 1.0 : shape;
-  const double offset = adjusted - 1.0 / 3.0;
-  const double coefficient = (1.0 / sqrt(offset)) / 3.0;
+  const GammaSetup g = gamma_setup(dof);
   for (int a = 0; a < kGammaAttempts; ++a) {
-    const double normal = rn[a];
-    const double uniform = ru[a];
-    const double root = 1.0 + coefficient * normal;
+    const double root = 1.0 + g.coefficient * rn[a];
     if (!(root > 0.0)) continue;
-    const double cube = root * root * root;
-    const double n2 = normal * normal;
-    const bool accepted = uniform < 1.0 - 0.0331 * (n2 * n2) ||
-                          log(uniform) < 0.5 * n2 + offset * (1.0 - cube + log(cube));
-    if (!accepted) continue;
-    double gamma = offset * cube;
-    if (shape < 1.0) gamma *= pow(ru[kGammaAttempts], 1.0 / shape);
-    if (!(isfinite(gamma) && gamma > 0.0)) break;
-    *out = sqrt(dof / (2.0 * gamma));
-    return true;
+    if (!gamma_accepts(g, rn[a], root, ru[a])) continue;
+    return gamma_multiplier(g, dof, root, [&] { return ru[kGammaAttempts]; }, out);
   }
   return false;
 }
