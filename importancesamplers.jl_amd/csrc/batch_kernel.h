@@ -145,6 +145,11 @@ __device__ inline T bslot_logdensity(const T* s, const Family<T>& fam, const T (
 // acc[3..4]: first-order sums sum e (x - c) for coordinates 0..15 and 16..31
 // (every column of these tiles holds the same sums).
 constexpr int kBatchTiles = 5;
+// FP32 jobs accumulate each 64-sample step's moments on FP32 MFMA (FP64 across steps)
+template <class T>
+__host__ __device__ constexpr bool batch_fp32_moments() {
+  return sizeof(T) == 4;
+}
 struct BatchAcc {
   double s0, s2;
   double4_t acc[kBatchTiles];
@@ -172,6 +177,49 @@ __device__ inline void batch_accumulate(T lw, const T (&x)[P], const T* centre, 
   A.s0 += e;
   A.s2 += e * e;
   T* row = stage + lane * kBatchStage;
+  if constexpr (batch_fp32_moments<T>()) {
+    // FP32 jobs: the staged products se (x - c) in FP32 and, when P is not a multiple
+    // of 16, se itself as coordinate P, so the first-order sums are row P of the
+    // second-moment tiles; v_mfma_f32_16x16x4f32 over the 64 samples, then the tiles
+    // added into the FP64 accumulators (FP32 tile layout: register r of lane l holds
+    // row 4 (l / 16) + r, column l % 16)
+    constexpr bool embed = P % 16 != 0;
+#pragma unroll
+    for (int c = 0; c < 16 * NT; ++c) {
+      T v = T(0);
+      if (c < P) v = (se != T(0)) ? se * (x[c] - centre[c]) : T(0);
+      if (embed && c == P) v = se;
+      row[c] = v;
+    }
+    if (!embed) row[kBatchMaxDim] = se;
+    wave_sync();
+    float4_t f[kBatchTiles];
+#pragma unroll
+    for (int p = 0; p < kBatchTiles; ++p) f[p] = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < 16; ++st) {
+      const T* rs = stage + (4 * st + (lane >> 4)) * kBatchStage;
+      const float a0 = rs[lane & 15];
+      f[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, f[0], 0, 0, 0);
+      if (!embed) f[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, rs[kBatchMaxDim], f[3], 0, 0, 0);
+      if (NT > 1) {
+        const float a1 = rs[16 + (lane & 15)];
+        f[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a0, f[1], 0, 0, 0);
+        f[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a1, f[2], 0, 0, 0);
+        if (!embed) f[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, rs[kBatchMaxDim], f[4], 0, 0, 0);
+      }
+      if ((st & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    wave_sync();
+#pragma unroll
+    for (int p = 0; p < kBatchTiles; ++p) {
+      const bool used = (p == 0) || (NT > 1 && p < 3) || (!embed && (p == 3 || (NT > 1 && p == 4)));
+      if (used)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) A.acc[p][i] += (double)f[p][i];
+    }
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < 16 * NT; ++c) {
     double v = 0.0;
@@ -482,7 +530,9 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
       const int r = lane;
       const int ti = r >> 4, tj = c >> 4, ra = r & 15, cb = c & 15;
       const int p = ti * (ti + 1) / 2 + tj;
-      const int ln = cb + 16 * (ra & 3), reg_i = ra >> 2;
+      // FP64 tiles: row 4 r + l / 16 in register r; FP32 tiles: row 4 (l / 16) + r
+      const int ln = batch_fp32_moments<T>() ? cb + 16 * (ra >> 2) : cb + 16 * (ra & 3);
+      const int reg_i = batch_fp32_moments<T>() ? (ra & 3) : (ra >> 2);
       const double S = rec0[35 + p * 256 + ln * 4 + reg_i];
       double v = S / s0 - mean * (rec0[3 + c] / s0);
       if (c == lane) v += ridge;
@@ -828,8 +878,31 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       rec[1] = s0;
       rec[2] = s2;
     }
-    // first-order sums: tile entry (row i, column 0) sits in lane 16 (i & 3), register i >> 2
-    if ((lane & 15) == 0) {
+    if constexpr (batch_fp32_moments<T>()) {
+      // FP32 tile layout: entry (row i, column j) in lane j + 16 (i / 4), register i % 4
+      if constexpr (P % 16 != 0) {
+        // first-order sums: row P % 16 of tile (0, 0) (P < 16) or of tiles (1, 0), (1, 1)
+        constexpr int ra = P & 15;
+        if ((lane >> 4) == (ra >> 2)) {
+          const int cb = lane & 15;
+          if (P < 16) {
+            rec[3 + cb] = cb < P ? A.acc[0][ra & 3] : 0.0;
+            rec[3 + 16 + cb] = 0.0;
+          } else {
+            rec[3 + cb] = A.acc[1][ra & 3];
+            rec[3 + 16 + cb] = 16 + cb < P ? A.acc[2][ra & 3] : 0.0;
+          }
+        }
+      } else if ((lane & 15) == 0) {
+        // first-order sums: column 0 of tiles 3 and 4
+        const int g = lane >> 4;
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) rec[3 + 16 * ti + 4 * g + q] = A.acc[3 + ti][q];
+      }
+    } else if ((lane & 15) == 0) {
+      // first-order sums: tile entry (row i, column 0) sits in lane 16 (i & 3), register i >> 2
       const int g = lane >> 4;
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)
