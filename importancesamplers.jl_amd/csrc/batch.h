@@ -52,6 +52,7 @@ __host__ __device__ inline int64_t bx_base(int64_t i, int P) { return (i >> 6) *
 template <class T>
 struct BatchJob {
   int d, dp, rounds, student, tkind, slot_elems, rs, us;  // dp: padded dimension P
+  int tbf16;            // FP32 linear regression, P >= 8: the centred BF16 target
   int64_t N, cap, row_tiles, n_rows;
   double dof, logconstant, inv_two_var, inv_two_prior, tconst;
   T logN, fdof, fhalf, pad_;
@@ -61,7 +62,11 @@ struct BatchJob {
   T* ln;                // AMIS log numerators
   T* lw;                // log weights
   const T* xpack;       // data rows in MFMA A-fragment order (row_tiles x P/4 x 64)
-  const T* ypack;       // responses in MFMA output order (row_tiles x 64 x 4)
+  const T* ypack;       // responses in MFMA output order (row_tiles x 64 x 4); with tbf16 the
+                        // centred residuals y - X x0 (FP64, rounded to T)
+  const uint32_t* xbf;  // tbf16: rows split x = xh + xl in BF16 as 16x16x32 A operands,
+                        // [row tile][hi, lo][lane][8 BF16]
+  const double* x0;     // tbf16: the centre (P entries, zero past d)
   double* slots64;      // rounds + 1 slots, FP64
   T* slotsT;            // the same slots in T
   double* committed64;  // committed proposal (slot 0 of the next run)

@@ -30,8 +30,11 @@ struct BatchJobHost {
   RngKey key{0, 0};
   // element offsets into the pools
   size_t x_off = 0, log_off = 0, xpack_off = 0, ypack_off = 0, slot_off = 0, com_off = 0, round_off = 0;
+  size_t xbf_off = 0, x0_off = 0;
+  bool bf16 = false;  // the centred BF16 target (FP32 linear regression, P >= 8)
   double cost = 0.0;
   std::vector<double> slot0;  // initial proposal, batch layout (FP64)
+  std::vector<double> x0;     // the BF16 target's centre: the initial proposal's location
 };
 
 // Packs a proposal into the batch slot layout for padded dimension P
@@ -72,6 +75,7 @@ struct is_batch {
   size_t tsize = 4;
   DevBuf jobs_dev, order_dev, queue, run_dev, status_dev;
   DevBuf X, lt, ln, lw, xpack, ypack, slots64, slotsT, com64, comT, ress, rlogz, fit, fail;
+  DevBuf xbf, x0d;  // the centred BF16 target's split rows and centres
   uint64_t run_index = 0;
   bool graph_dirty = true;
   hipGraph_t graph = nullptr;
@@ -118,16 +122,53 @@ void pack_data(const BatchJobHost& J, const double* X, const double* y, T* xpack
       }
 }
 
+// The centred BF16 linear-regression target's data: each row split as x = xh + xl
+// (BF16 each, xl the rounding of x - xh) in the 16x16x32 A-operand order (lane l:
+// row 16 rt + l % 16, coordinates 8 (l / 16) .. + 7), and the residuals at the centre,
+// y - X x0 in FP64 rounded to FP32, in the MFMA output order.
+void pack_bf16(const BatchJobHost& J, const double* X, const double* y, uint16_t* xbf, float* ypack) {
+  for (int64_t rt = 0; rt < J.row_tiles; ++rt)
+    for (int l = 0; l < 64; ++l) {
+      const int64_t r = 16 * rt + (l & 15);
+      for (int e = 0; e < 8; ++e) {
+        const int c = 8 * (l >> 4) + e;
+        uint16_t h = 0, lo = 0;
+        if (r < J.n_rows && c < J.d) {
+          const double x = X[r * J.d + c];
+          h = bf16_bits((float)x);
+          lo = bf16_bits((float)(x - bf16_value(h)));
+        }
+        xbf[((rt * 2 + 0) * 64 + l) * 8 + e] = h;
+        xbf[((rt * 2 + 1) * 64 + l) * 8 + e] = lo;
+      }
+    }
+  for (int64_t rt = 0; rt < J.row_tiles; ++rt)
+    for (int l = 0; l < 64; ++l)
+      for (int q = 0; q < 4; ++q) {
+        const int64_t r = 16 * rt + 4 * (l >> 4) + q;
+        double v = 0.0;
+        if (r < J.n_rows) {
+          v = y[r];
+          for (int c = 0; c < J.d; ++c) v -= X[r * J.d + c] * J.x0[c];
+        }
+        ypack[(rt * 64 + l) * 4 + q] = (float)v;
+      }
+}
+
 template <class T>
 int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::vector<const double*>& ys) {
-  size_t nx = 0, nlog = 0, nxp = 0, nyp = 0, nslot = 0, ncom = 0, nround = 0;
+  size_t nx = 0, nlog = 0, nxp = 0, nyp = 0, nslot = 0, ncom = 0, nround = 0, nbf = 0, nx0 = 0;
   for (auto& J : b->jobs) {
     J.x_off = nx;
     nx += (size_t)(J.cap / 64) * 64 * J.P;
     J.log_off = nlog;
     nlog += (size_t)J.cap;
     J.xpack_off = nxp;
-    nxp += (size_t)J.row_tiles * (J.P / 4) * 64;
+    if (!J.bf16) nxp += (size_t)J.row_tiles * (J.P / 4) * 64;
+    J.xbf_off = nbf;
+    if (J.bf16) nbf += (size_t)J.row_tiles * 2 * 64 * 8;
+    J.x0_off = nx0;
+    if (J.bf16) nx0 += (size_t)J.P;
     J.ypack_off = nyp;
     nyp += (size_t)J.row_tiles * 64 * 4;
     J.slot_off = nslot;
@@ -144,6 +185,8 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
   if ((rc = b->lw.alloc(nlog * sizeof(T)))) return rc;
   if ((rc = b->xpack.alloc(nxp * sizeof(T)))) return rc;
   if ((rc = b->ypack.alloc(nyp * sizeof(T)))) return rc;
+  if ((rc = b->xbf.alloc(std::max<size_t>(nbf, 1) * sizeof(uint16_t)))) return rc;
+  if ((rc = b->x0d.alloc(std::max<size_t>(nx0, 1) * sizeof(double)))) return rc;
   if ((rc = b->slots64.alloc(nslot * sizeof(double)))) return rc;
   if ((rc = b->slotsT.alloc(nslot * sizeof(T)))) return rc;
   if ((rc = b->com64.alloc(ncom * sizeof(double)))) return rc;
@@ -160,11 +203,21 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
   HIP_TRY(locked_memset(b->slotsT.p, 0, b->slotsT.bytes));
   // data in MFMA fragment order, committed slots, job table
   std::vector<T> xp(nxp), yp(nyp);
+  std::vector<uint16_t> xb(nbf);
+  std::vector<double> x0s(nx0, 0.0);
   std::vector<double> c64(ncom);
   std::vector<BatchJob<T>> table(b->njobs);
   for (int j = 0; j < b->njobs; ++j) {
     const BatchJobHost& H = b->jobs[j];
-    pack_data<T>(H, Xs[j], ys[j], xp.data() + H.xpack_off, yp.data() + H.ypack_off);
+    if constexpr (sizeof(T) == 4) {
+      if (H.bf16) {
+        pack_bf16(H, Xs[j], ys[j], xb.data() + H.xbf_off, yp.data() + H.ypack_off);
+        std::copy(H.x0.begin(), H.x0.end(), x0s.begin() + H.x0_off);
+      }
+    }
+    if (!H.bf16) {
+      pack_data<T>(H, Xs[j], ys[j], xp.data() + H.xpack_off, yp.data() + H.ypack_off);
+    }
     std::copy(H.slot0.begin(), H.slot0.end(), c64.begin() + H.com_off);
     BatchJob<T>& J = table[j];
     std::memset(&J, 0, sizeof(J));
@@ -173,6 +226,7 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
     J.rounds = H.rounds;
     J.student = H.family == IS_STUDENT_T;
     J.tkind = H.tkind == IS_TARGET_LOGISTIC ? kTLogistic : kTLinreg;
+    J.tbf16 = H.bf16 ? 1 : 0;
     J.slot_elems = H.SS;
     J.rs = radial_normal_stride(H.family, H.dof);
     J.us = radial_uniform_stride(H.family, H.dof);
@@ -196,6 +250,8 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
     J.lw = b->lw.as<T>() + H.log_off;
     J.xpack = b->xpack.as<T>() + H.xpack_off;
     J.ypack = b->ypack.as<T>() + H.ypack_off;
+    J.xbf = H.bf16 ? reinterpret_cast<const uint32_t*>(b->xbf.as<uint16_t>() + H.xbf_off) : nullptr;
+    J.x0 = H.bf16 ? b->x0d.as<double>() + H.x0_off : nullptr;
     J.slots64 = b->slots64.as<double>() + H.slot_off;
     J.slotsT = b->slotsT.as<T>() + H.slot_off;
     J.committed64 = b->com64.as<double>() + H.com_off;
@@ -212,6 +268,8 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
   }
   HIP_TRY(locked_memcpy(b->xpack.p, xp.data(), nxp * sizeof(T), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(b->ypack.p, yp.data(), nyp * sizeof(T), hipMemcpyHostToDevice));
+  if (nbf) HIP_TRY(locked_memcpy(b->xbf.p, xb.data(), nbf * sizeof(uint16_t), hipMemcpyHostToDevice));
+  if (nx0) HIP_TRY(locked_memcpy(b->x0d.p, x0s.data(), nx0 * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(b->com64.p, c64.data(), ncom * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(b->comT.p, cT.data(), ncom * sizeof(T), hipMemcpyHostToDevice));
   if ((rc = b->jobs_dev.alloc(table.size() * sizeof(BatchJob<T>)))) return rc;
@@ -388,6 +446,11 @@ int is_batch_create(is_ctx* ctx, const is_job_desc* descs, int32_t njobs, int32_
     if (!err.empty()) return fail(IS_EINVAL, "job %d: %s", j, err.c_str());
     H.slot0.assign(H.SS, 0.0);
     pack_bslot(p, H.d, H.P, H.slot0.data());
+    H.bf16 = precision == IS_FP32 && D.target_kind == IS_TARGET_LINREG && H.P >= 8;
+    if (H.bf16) {
+      H.x0.assign(H.P, 0.0);
+      for (int c = 0; c < H.d; ++c) H.x0[c] = p.mu[c];
+    }
     // target: [n, p, prior_var, X, y] (logistic) or [n, p, sigma, tau, X, y] (linear regression)
     const int head = H.tkind == IS_TARGET_LOGISTIC ? 3 : 4;
     const double* tp = D.target_params;

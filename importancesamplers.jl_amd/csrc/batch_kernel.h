@@ -334,6 +334,76 @@ __device__ inline double batch_target(const BatchJob<T>& J, const T (&breg)[4][P
   return logistic ? mine : -J.inv_two_var * mine;
 }
 
+// FP32 linear regression, P >= 8: the log-likelihood of the wave's 64 staged samples
+// about the centre x0, sum_i (r0_i - x_i . delta)^2 with r0 = y - X x0 (FP64, packed) and
+// delta = beta - x0, x_i . delta as three BF16 products on v_mfma_f32_16x16x32_bf16
+// (xh dh + xh dl + xl dh, each operand split into a BF16 value and the BF16 rounding of
+// its remainder), FP32 residual sums folded into FP64 every 8 row tiles.
+template <int P>
+__device__ inline double batch_target_bf16(const BatchJob<float>& J, const float* stage, int lane) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  // B operands: lane l, sample 16 q + l % 16, coordinates 8 (l / 16) + e
+  bf16x8_t bh[4], bl[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float* row = stage + (16 * q + (lane & 15)) * kBatchStage;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int c = 8 * (lane >> 4) + e;
+      float dv = 0.0f;
+      if (c < P) dv = (float)((double)row[c] - J.x0[c]);
+      const __bf16 h = (__bf16)dv;
+      bh[q][e] = h;
+      bl[q][e] = (__bf16)(dv - (float)h);
+    }
+  }
+  const u32x4_t* xp = reinterpret_cast<const u32x4_t*>(J.xbf) + lane;
+  const V4* yp = reinterpret_cast<const V4*>(J.ypack) + lane;
+  double ll[4] = {0.0, 0.0, 0.0, 0.0};
+  float part[4] = {0.f, 0.f, 0.f, 0.f};
+  u32x4_t nh = xp[0], nl = xp[64];
+  V4 yn = yp[0];
+  for (int64_t rt = 0; rt < J.row_tiles; ++rt) {
+    const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, nh), al = __builtin_bit_cast(bf16x8_t, nl);
+    const V4 r0 = yn;
+    if (rt + 1 < J.row_tiles) {
+      nh = xp[(rt + 1) * 128];
+      nl = xp[(rt + 1) * 128 + 64];
+      yn = yp[(rt + 1) * 64];
+    }
+    float4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[q], (float4_t){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[q], acc[q], 0, 0, 0);
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[q], acc[q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float res = r0[r] - acc[q][r];
+        part[q] = fmaf(res, res, part[q]);
+      }
+    if ((rt & 7) == 7) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ll[q] += (double)part[q];
+        part[q] = 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ll[q] += (double)part[q];
+    ll[q] += __shfl_xor(ll[q], 16, 64);
+    ll[q] += __shfl_xor(ll[q], 32, 64);
+  }
+  const int qq = lane >> 4;
+  const double mine = qq == 0 ? ll[0] : qq == 1 ? ll[1] : qq == 2 ? ll[2] : ll[3];
+  return -J.inv_two_var * mine;
+}
+
 // FP32 jobs: r^2 = |M (x - mu)|^2 of the wave's 64 samples under one slot on
 // v_mfma_f32_16x16x4f32 (A = the slot's factor fragments, B = the samples' centred
 // coordinates, four 16-sample tiles q), the squares summed over each lane's tile rows
@@ -739,9 +809,20 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       // target on MFMA for the wave's 64 samples (coordinates staged in LDS)
       wave_sync();
       T bx[4][P / 4];
-      batch_operands<T, P>(stage, lane, bx);
+      double ll;
+      if constexpr (kMfma && P >= 8) {
+        if (J.tbf16) {
+          ll = batch_target_bf16<P>(J, stage, lane);
+          batch_operands<T, P>(stage, lane, bx);
+        } else {
+          batch_operands<T, P>(stage, lane, bx);
+          ll = batch_target<T, P>(J, bx, lane);
+        }
+      } else {
+        batch_operands<T, P>(stage, lane, bx);
+        ll = batch_target<T, P>(J, bx, lane);
+      }
       wave_sync();
-      const double ll = batch_target<T, P>(J, bx, lane);
       double b2 = 0.0;
 #pragma unroll
       for (int c = 0; c < P; ++c) b2 += (double)x[c] * (double)x[c];

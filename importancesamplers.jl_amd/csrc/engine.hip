@@ -1346,21 +1346,6 @@ int alloc_job_buffers(is_job* j) {
 }  // namespace
 
 // ---------------------------------------------------------------- centred BF16 logistic data
-// Round-to-nearest-even FP32 -> BF16 (the hardware conversion's rounding).
-static uint16_t bf16_bits(float f) {
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-static double bf16_value(uint16_t h) {
-  const uint32_t u = (uint32_t)h << 16;
-  float f;
-  std::memcpy(&f, &u, 4);
-  return (double)f;
-}
-
 // The data of the centred BF16 logistic form (k_logistic_bf16), from the FP64 rows in
 // j->tdata: each row split as x = xh + xl (BF16 each, xl the rounding of x - xh) in the
 // MFMA A-operand order, x_i . x0 in FP64 rounded to FP32, the responses, the centre.

@@ -171,6 +171,21 @@ inline std::string make_prop(int family, double dof, int kind, int d, const doub
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Round-to-nearest-even FP32 -> BF16 (the hardware conversion's rounding), and back.
+inline uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+inline double bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return (double)f;
+}
+
 // ---------------------------------------------------------------- capture lock
 // A stream capture on one thread must not overlap a device-wide runtime call on
 // another: hipMalloc, hipFree and the blocking copies and fills may synchronise
