@@ -53,29 +53,27 @@ struct Vec16<double> {
 };
 
 // Student-t radial multiplier of one sample (the 8 buffered gamma attempts of
-// _buffered_unit_gamma, src/native_execution.jl:147-180) with the attempts generated
-// one Philox block (four attempts) at a time, the second block only for lanes that
-// reject all of the first: the values and the result of the buffered form, without
-// holding all 17 values in registers.
-__device__ inline void batch_radial_normals(const RngKey& key, uint64_t sample, uint64_t round, int blk,
-                                            double (&rn)[4]) {
-  const U64x4 w = philox4x64(sample, round, kPurposeRadialNormals, (uint64_t)blk, key.k0, key.k1);
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
-    double sn, cs;
-    sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
-    rn[2 * p] = r * cs;
-    rn[2 * p + 1] = r * sn;
-  }
+// _buffered_unit_gamma, src/native_execution.jl:147-180) with each attempt's values
+// generated when the acceptance loop reaches it: attempts 4 b .. 4 b + 3 take Philox
+// block b of the radial normals (Box-Muller pairs (0, 1) and (2, 3)) and of the radial
+// uniforms; a pair's logarithm and sine/cosine are computed for the lanes still
+// searching.  The values and the result of the buffered form, without the 17 values in
+// registers and without the second pair for the samples accepted at attempt 0 or 1.
+__device__ inline void batch_radial_pair(const U64x4& w, int p, double& n0, double& n1) {
+  const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
+  double sn, cs;
+  sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
+  n0 = r * cs;
+  n1 = r * sn;
 }
 
 __device__ inline bool batch_multiplier(const RngKey& key, uint64_t sample, uint64_t round, int us, double dof,
                                         double* out) {
-  double rn[4];
   if (us == 0 || dof == 1.0) {
-    batch_radial_normals(key, sample, round, 0, rn);
-    const double den = fabs(rn[0]);
+    const U64x4 w = philox4x64(sample, round, kPurposeRadialNormals, 0, key.k0, key.k1);
+    double n0, n1;
+    batch_radial_pair(w, 0, n0, n1);
+    const double den = fabs(n0);
     if (!(isfinite(den) && den > 0.0)) return false;
     *out = 1.0 / den;
     return true;
@@ -83,20 +81,32 @@ __device__ inline bool batch_multiplier(const RngKey& key, uint64_t sample, uint
   const GammaSetup g = gamma_setup(dof);
 #pragma unroll 1
   for (int blk = 0; blk < kGammaAttempts / 4; ++blk) {
-    batch_radial_normals(key, sample, round, blk, rn);
+    const U64x4 wn = philox4x64(sample, round, kPurposeRadialNormals, (uint64_t)blk, key.k0, key.k1);
     const U64x4 wu = philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)blk, key.k0, key.k1);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double root = 1.0 + g.coefficient * rn[q];
-      if (!(root > 0.0)) continue;
-      if (!gamma_accepts(g, rn[q], root, u53(wu.v[q]))) continue;
-      return gamma_multiplier(
-          g, dof, root,
-          [&] {
-            const int b = kGammaAttempts / 4;  // the extra uniform: block 2, word 0
-            return u53(philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)b, key.k0, key.k1).v[0]);
-          },
-          out);
+    for (int p = 0; p < 2; ++p) {
+      double rn[2];
+      batch_radial_pair(wn, p, rn[0], rn[1]);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double root = 1.0 + g.coefficient * rn[h];
+        if (!(root > 0.0)) continue;
+        const double u = u53(wu.v[2 * p + h]);
+        // the squeeze accepts most attempts: the logarithms only when a lane needs them
+        const double n2 = rn[h] * rn[h];
+        bool acc = u < 1.0 - 0.0331 * (n2 * n2);
+        if (__any(!acc)) {
+          if (!acc) acc = gamma_accepts(g, rn[h], root, u);
+        }
+        if (!acc) continue;
+        return gamma_multiplier(
+            g, dof, root,
+            [&] {
+              const int b = kGammaAttempts / 4;  // the extra uniform: block 2, word 0
+              return u53(philox4x64(sample, round, kPurposeRadialUniforms, (uint64_t)b, key.k0, key.k1).v[0]);
+            },
+            out);
+      }
     }
   }
   return false;
@@ -763,11 +773,22 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
           T zq[4];
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
-            const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
-            double sn, cs;
-            sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
-            zq[2 * p] = T(r * cs * mult);
-            zq[2 * p + 1] = T(r * sn * mult);
+            if constexpr (sizeof(T) == 4) {
+              // FP32 jobs: the Box-Muller pair in FP32 from the same 53-bit uniforms
+              // (relative differences from the FP64 pair of order 1e-7)
+              const float r = sqrtf(-2.0f * logf((float)u53_open(w.v[2 * p])));
+              float sn, cs;
+              sincospif(2.0f * (float)u53_open(w.v[2 * p + 1]), &sn, &cs);
+              const float m = (float)mult;
+              zq[2 * p] = r * cs * m;
+              zq[2 * p + 1] = r * sn * m;
+            } else {
+              const double r = sqrt(-2.0 * log(u53_open(w.v[2 * p])));
+              double sn, cs;
+              sincospi(2.0 * u53_open(w.v[2 * p + 1]), &sn, &cs);
+              zq[2 * p] = T(r * cs * mult);
+              zq[2 * p + 1] = T(r * sn * mult);
+            }
           }
           // rows r >= 4 blk: the row's four entries of this column block in one
           // 16-byte-aligned LDS read (entries past the diagonal are not added),
