@@ -63,7 +63,7 @@ struct BatchJob {
   T* lw;                // log weights
   const T* xpack;       // data rows in MFMA A-fragment order (row_tiles x P/4 x 64)
   const T* ypack;       // responses in MFMA output order (row_tiles x 64 x 4); with tbf16 the
-                        // centred residuals y - X x0 (FP64, rounded to T)
+                        // centred residuals y - X x0 (FP64, rounded to T), one per row
   const uint32_t* xbf;  // tbf16: rows split x = xh + xl in BF16 as 16x16x32 A operands,
                         // [row tile][hi, lo][lane][8 BF16]
   const double* x0;     // tbf16: the centre (P entries, zero past d)

@@ -125,7 +125,8 @@ void pack_data(const BatchJobHost& J, const double* X, const double* y, T* xpack
 // The centred BF16 linear-regression target's data: each row split as x = xh + xl
 // (BF16 each, xl the rounding of x - xh) in the 16x16x32 A-operand order (lane l:
 // row 16 rt + l % 16, coordinates 8 (l / 16) .. + 7), and the residuals at the centre,
-// y - X x0 in FP64 rounded to FP32, in the MFMA output order.
+// y - X x0 in FP64 rounded to FP32, one per row (16 per row tile; lane l reads the
+// four of its output rows 4 (l / 16) .. + 3).
 void pack_bf16(const BatchJobHost& J, const double* X, const double* y, uint16_t* xbf, float* ypack) {
   for (int64_t rt = 0; rt < J.row_tiles; ++rt)
     for (int l = 0; l < 64; ++l) {
@@ -142,17 +143,14 @@ void pack_bf16(const BatchJobHost& J, const double* X, const double* y, uint16_t
         xbf[((rt * 2 + 1) * 64 + l) * 8 + e] = lo;
       }
     }
-  for (int64_t rt = 0; rt < J.row_tiles; ++rt)
-    for (int l = 0; l < 64; ++l)
-      for (int q = 0; q < 4; ++q) {
-        const int64_t r = 16 * rt + 4 * (l >> 4) + q;
-        double v = 0.0;
-        if (r < J.n_rows) {
-          v = y[r];
-          for (int c = 0; c < J.d; ++c) v -= X[r * J.d + c] * J.x0[c];
-        }
-        ypack[(rt * 64 + l) * 4 + q] = (float)v;
-      }
+  for (int64_t r = 0; r < 16 * J.row_tiles; ++r) {
+    double v = 0.0;
+    if (r < J.n_rows) {
+      v = y[r];
+      for (int c = 0; c < J.d; ++c) v -= X[r * J.d + c] * J.x0[c];
+    }
+    ypack[r] = (float)v;
+  }
 }
 
 template <class T>
@@ -170,7 +168,7 @@ int batch_upload(is_batch* b, const std::vector<const double*>& Xs, const std::v
     J.x0_off = nx0;
     if (J.bf16) nx0 += (size_t)J.P;
     J.ypack_off = nyp;
-    nyp += (size_t)J.row_tiles * 64 * 4;
+    nyp += (size_t)J.row_tiles * (J.bf16 ? 16 : 64 * 4);
     J.slot_off = nslot;
     nslot += (size_t)(J.rounds + 1) * J.SS;
     J.com_off = ncom;

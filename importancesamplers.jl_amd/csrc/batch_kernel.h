@@ -368,7 +368,8 @@ __device__ inline double batch_target_bf16(const BatchJob<float>& J, const float
     }
   }
   const u32x4_t* xp = reinterpret_cast<const u32x4_t*>(J.xbf) + lane;
-  const V4* yp = reinterpret_cast<const V4*>(J.ypack) + lane;
+  // the residuals at the centre, one per row: this lane's output rows 4 (l / 16) .. + 3
+  const V4* yp = reinterpret_cast<const V4*>(J.ypack) + (lane >> 4);
   double ll[4] = {0.0, 0.0, 0.0, 0.0};
   float part[4] = {0.f, 0.f, 0.f, 0.f};
   u32x4_t nh = xp[0], nl = xp[64];
@@ -379,7 +380,7 @@ __device__ inline double batch_target_bf16(const BatchJob<float>& J, const float
     if (rt + 1 < J.row_tiles) {
       nh = xp[(rt + 1) * 128];
       nl = xp[(rt + 1) * 128 + 64];
-      yn = yp[(rt + 1) * 64];
+      yn = yp[(rt + 1) * 4];
     }
     float4_t acc[4];
 #pragma unroll
