@@ -986,8 +986,14 @@ int run_rounds(is_job* j) {
       reduce(j->partials, nrec, 2, j->reduced, f);
       // clip at the isqrt(n)-th largest raw weight (src/npmc_execution.jl:21-33)
       const int64_t kth = N - isqrt64(N) + 1;
-      hipLaunchKernelGGL((k_select_kth<T>), dim3(1), dim3(1024), 0, s, j->logweights.as<T>(), first, N, kth,
-                         j->clip.as<double>());
+      {
+        const int cus = j->ctx ? j->ctx->cus : 256;
+        const int sb = (int)std::max<int64_t>(1, std::min<int64_t>(4 * cus, cdiv(N, 4 * kSelectBlock)));
+        SelectState* st = reinterpret_cast<SelectState*>(j->clip.as<char>() + 64);
+        for (int pass = 0; pass < 8; ++pass)
+          hipLaunchKernelGGL((k_select_pass<T>), dim3(sb), dim3(kSelectBlock), 0, s, j->logweights.as<T>(), first, N,
+                             pass, kth, st, j->clip.as<double>());
+      }
       SweepArgs<T> w2 = sweep_args<T>(j, t);
       w2.lo = first;
       w2.hi = last;
@@ -1280,7 +1286,9 @@ int alloc_job_buffers(is_job* j) {
   if ((rc = j->reduced.alloc((size_t)(1 + nv_mom) * sizeof(double)))) return rc;
   if ((rc = j->reduced2.alloc((size_t)(1 + nv_res) * sizeof(double)))) return rc;
   if ((rc = j->summary.alloc((size_t)5 * std::max(1, j->T) * sizeof(double)))) return rc;
-  if ((rc = j->clip.alloc(64))) return rc;
+  // the clip value, then the radix select's state (zero histogram and ticket)
+  if ((rc = j->clip.alloc(64 + sizeof(SelectState)))) return rc;
+  HIP_TRY(locked_memset(j->clip.p, 0, 64 + sizeof(SelectState)));
   {
     // control block layout (bytes): run word 64 | counters 64 | failure records 24 T
     // | fit status 4 T | resampling status 4 T (padded to 8) | ESS 8 T | log Z 8 T | adaptation ESS 8 T

@@ -722,7 +722,7 @@ static __global__ __launch_bounds__(256) void k_mis_assign(RngKey key, const uns
 
 // ---------------------------------------------------------------- order statistic (NPMC clip)
 // Exact k-th smallest (1-based) of lw[first, first+n) by an 8-pass radix select
-// on order-preserving 64-bit keys, one workgroup.
+// on order-preserving 64-bit keys, each pass one launch over the whole chip.
 __device__ inline uint64_t order_key(double v) {
   const uint64_t b = __double_as_longlong(v);
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
@@ -732,42 +732,69 @@ __device__ inline double key_value(uint64_t k) {
   return __longlong_as_double((long long)b);
 }
 
+// State of the multi-workgroup radix select, in device memory: the key prefix fixed by
+// the passes so far, the rank left within it, the global histogram of the current
+// pass and the workgroups' ticket.  The histogram and ticket are zero between passes
+// (the last workgroup of every pass clears them; the job's setup zeroes them once).
+struct SelectState {
+  unsigned long long prefix;
+  long long k;
+  unsigned int ticket, pad_;
+  unsigned int hist[256];
+};
+constexpr int kSelectBlock = 256;
+
+// One pass (digit `pass`, bits 56 - 8 pass .. + 7) over lw[first, first + n): every
+// workgroup histograms the digit of the keys that carry the prefix in LDS and adds its
+// counts to the global histogram; the last workgroup to finish (a ticket) picks the
+// bin holding rank k (kth at pass 0), moves the prefix and the rank on, clears the
+// histogram and the ticket, and after the last pass writes the k-th smallest value.
+// The counts are exact, so the result does not depend on the launch shape.
 template <class T>
-__global__ __launch_bounds__(1024) void k_select_kth(const T* __restrict__ lw, int64_t first, int64_t n,
-                                                     int64_t kth, double* __restrict__ out) {
-  __shared__ unsigned int hist[256];
-  __shared__ uint64_t prefix_s;
-  __shared__ int64_t k_s;
+__global__ __launch_bounds__(kSelectBlock) void k_select_pass(const T* __restrict__ lw, int64_t first, int64_t n,
+                                                              int pass, int64_t kth, SelectState* st,
+                                                              double* __restrict__ out) {
+  __shared__ unsigned int h[256];
+  __shared__ int last;
   const int tid = threadIdx.x;
+  h[tid] = 0;
+  __syncthreads();
+  const int shift = 56 - 8 * pass;
+  const uint64_t prefix =
+      pass == 0 ? 0ull : __hip_atomic_load(&st->prefix, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
+  for (int64_t i = (int64_t)blockIdx.x * kSelectBlock + tid; i < n; i += (int64_t)gridDim.x * kSelectBlock) {
+    const uint64_t key = order_key((double)lw[first + i]);
+    if ((key & hmask) == prefix) atomicAdd(&h[(key >> shift) & 0xff], 1u);
+  }
+  __syncthreads();
+  if (h[tid]) atomicAdd(&st->hist[tid], h[tid]);
+  __syncthreads();
   if (tid == 0) {
-    prefix_s = 0;
-    k_s = kth;
+    // release: this workgroup's counts before its ticket; acquire: every earlier one's
+    const unsigned int t = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
   }
-  for (int pass = 0; pass < 8; ++pass) {
-    const int shift = 56 - 8 * pass;
-    for (int b = tid; b < 256; b += blockDim.x) hist[b] = 0;
-    __syncthreads();
-    const uint64_t prefix = prefix_s;
-    const uint64_t hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
-    for (int64_t i = tid; i < n; i += blockDim.x) {
-      const uint64_t key = order_key((double)lw[first + i]);
-      if ((key & hmask) == prefix) atomicAdd(&hist[(key >> shift) & 0xff], 1u);
+  __syncthreads();
+  if (!last) return;
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  h[tid] = __hip_atomic_load(&st->hist[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (tid == 0) {
+    int64_t k = pass == 0 ? kth : (int64_t)__hip_atomic_load(&st->k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int b = 0;
+    for (; b < 256; ++b) {
+      if (k <= (int64_t)h[b]) break;
+      k -= h[b];
     }
-    __syncthreads();
-    if (tid == 0) {
-      int64_t k = k_s;
-      int b = 0;
-      for (; b < 256; ++b) {
-        if (k <= (int64_t)hist[b]) break;
-        k -= hist[b];
-      }
-      if (b > 255) b = 255;
-      k_s = k;
-      prefix_s = prefix | ((uint64_t)b << shift);
-    }
-    __syncthreads();
+    if (b > 255) b = 255;
+    const uint64_t np = prefix | ((uint64_t)b << shift);
+    __hip_atomic_store(&st->k, (long long)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st->prefix, (unsigned long long)np, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (pass == 7) *out = key_value(np);
   }
-  if (tid == 0) *out = key_value(prefix_s);
+  __hip_atomic_store(&st->hist[tid], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- deterministic blocked CDF
