@@ -381,7 +381,8 @@ def test_npmc_recurrence_known_answer(eng, oracle_mod):
 
 
 @pytest.mark.parametrize("d,schedule", [(2, [16, 16]), (5, [1000, 1500, 700]), (10, [4000] * 6),
-                                        (33, [6000] * 3), (48, [7000] * 3), (64, [9000] * 3)])
+                                        (33, [6000] * 3), (48, [7000] * 3), (64, [9000] * 3),
+                                        (4, [150000, 150000])])  # the clip's select over ~150 workgroups
 def test_npmc_parity(eng, oracle_mod, d, schedule):
     _say(f"case: NPMC parity d={d}")
     rng = np.random.default_rng(d)
