@@ -858,7 +858,7 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       }
       // deterministic mixture over slots 0..t (src/mis_execution.jl:30-95); FP32:
       // the whitened coordinates of the whole wave on MFMA, per slot
-      T m = ninf, sum = T(0);
+      T m = ninf, sum = T(0), sumc = T(0);  // FP32: compensated sum
       bool genbad = false;
       if constexpr (kMfma) {
         for (int k = 0; k <= t; ++k) {
@@ -866,14 +866,9 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
           const T r2 = batch_mfma_r2<P>(frags + k * FE, sk, bx, lane);
           const T q = batch_radial_r2(fam, sk[bslot_lognorm(P)], r2);
           if (k == t && (q != q || q == ninf)) genbad = true;
-          const T v = J.logN + q;
-          if (v > m) {
-            sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
-            m = v;
-          } else {
-            sum += LseAcc<T>::fast_exp(v - m);
-          }
+          lse_push(J.logN + q, m, sum, sumc);
         }
+        sum += sumc;
       } else if (ok) {
         T y[P];
         for (int k = 0; k <= t; ++k) {

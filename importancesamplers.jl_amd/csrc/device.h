@@ -223,6 +223,38 @@ __device__ inline float LseAcc<float>::fast_exp(float v) {
   return __expf(v);
 }
 
+// One term v of a running log-sum-exp (maximum m, sum s of exp(v - m)).  FP32: the
+// sum carries a compensation c, Fast2Sum against s >= 1 >= exp(v - m) and, when a new
+// maximum rescales the sum, the exact product error and TwoSum, so K terms cost about
+// one rounding instead of K; the caller adds c to s before the logarithm.  FP64: the
+// plain recurrence (c stays 0).
+template <class T>
+__device__ __forceinline__ void lse_push(T v, T& m, T& s, T& c) {
+  if (v > m) {
+    const T f = LseAcc<T>::fast_exp(m - v);
+    if constexpr (sizeof(T) == 4) {
+      const T a = s * f;
+      const T ae = fmaf(s, f, -a);
+      const T t = a + T(1);
+      const T bv = t - a;
+      c = c * f + (ae + ((a - (t - bv)) + (T(1) - bv)));
+      s = t;
+    } else {
+      s = s * f + T(1);
+    }
+    m = v;
+  } else {
+    const T e = LseAcc<T>::fast_exp(v - m);
+    if constexpr (sizeof(T) == 4) {
+      const T t = s + e;
+      c += (s - t) + e;
+      s = t;
+    } else {
+      s += e;
+    }
+  }
+}
+
 // Deterministic exponential for the resampling CDF: Cody-Waite reduction with
 // a two-part ln 2, degree-13 Taylor polynomial by fused multiply-add Horner,
 // scaling by 2^k.  The CPU oracle evaluates the same operation sequence.

@@ -297,7 +297,7 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
       ok = false;
     }
   }
-  T m = -T(__builtin_huge_val()), sum = T(0);
+  T m = -T(__builtin_huge_val()), sum = T(0), sumc = T(0);
   int genbad = 0;
   if (a.mis_terms != 0) {
     // static MIS: each sample's own terms (the generating slot, or its group;
@@ -314,12 +314,7 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
         const T q = slot_logdensity<T, D, RG>(a.slots + (int64_t)k * a.stride, d, a.fam, x, y);
         if (k == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
         const T v = (a.mis_terms == 2 ? a.mis_gcoef[mi] : T(0)) + q;
-        if (v > m) {
-          sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
-          m = v;
-        } else {
-          sum += LseAcc<T>::fast_exp(v - m);
-        }
+        lse_push(v, m, sum, sumc);
       }
     }
   } else if (a.pre_mix) {
@@ -356,16 +351,12 @@ __device__ __forceinline__ void new_block(const RoundArgs<T>& a, int64_t blk, T&
             q = slot_logdensity<T, D, RG>(sl, d, a.fam, x, y);
           if (a.k0 + kk == gen && (q != q || q == -T(__builtin_huge_val()))) genbad = 1;
           const T v = cmass[kk - base] + q;
-          if (v > m) {
-            sum = sum * LseAcc<T>::fast_exp(m - v) + T(1);
-            m = v;
-          } else {
-            sum += LseAcc<T>::fast_exp(v - m);
-          }
+          lse_push(v, m, sum, sumc);
         }
       }
     }
   }
+  sum += sumc;  // the compensation of the FP32 sum (zero in FP64)
   if (W > 1) {
     part_m[wave * 64 + lane] = m;
     part_s[wave * 64 + lane] = sum;

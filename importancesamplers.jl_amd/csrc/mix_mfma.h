@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_mix_mfma(MixMfmaArgs a) {
     if (tid < KP + 2) mu_s[buf][tid] = premu;
   };
   const float ninf = -__builtin_huge_valf();
-  float m = ninf, sum = 0.0f;
+  float m = ninf, sum = 0.0f, sumc = 0.0f;  // compensated sum
   bool bad = false;
   fetch(0);
   stage(0);
@@ -174,15 +174,10 @@ __global__ __launch_bounds__(256) void k_mix_mfma(MixMfmaArgs a) {
     }
     if (slot == gen && (q != q || q == ninf)) bad = true;
     const float v = mu_s[buf][KP + 1] + q;
-    if (v > m) {
-      sum = sum * LseAcc<float>::fast_exp(m - v) + 1.0f;
-      m = v;
-    } else {
-      sum += LseAcc<float>::fast_exp(v - m);
-    }
+    lse_push(v, m, sum, sumc);
     __syncthreads();  // every wave is done with buffer buf ^ 1 (component k - 1)
     if (k + 1 < a.ncomp) stage(buf ^ 1);
     __syncthreads();
   }
-  if (active && lane < 16) a.out[j] = bad ? __builtin_nanf("") : m + logf(sum);
+  if (active && lane < 16) a.out[j] = bad ? __builtin_nanf("") : m + logf(sum + sumc);
 }
