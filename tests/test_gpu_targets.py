@@ -276,3 +276,38 @@ def test_fp32_bank_mixture_on_matrix_cores(eng, oracle_mod, p, K, kind):
         dens = [oracle_mod.proposal_logdensity(q, xs[i]) for q in props]
         mix = np.logaddexp.reduce(np.array(dens) + logm)
         assert abs(lw[i] - (tv - mix)) < 2e-3, (i, lw[i], tv - mix)
+
+
+def test_config3_fp32_gate_over_30_replayed_rounds(eng):
+    """Config 3 at its full shape (X 100,000 x 51, 64 Laplace copies x 512 samples, 30
+    rounds, systematic resampling): the FP32 form (centred BF16 target, MFMA mixture,
+    compensated sums) against an FP64 run that replays its ancestors, so both weigh
+    samples of the same banks: every round's log Z within 1e-3, and the pooled log Z."""
+    _say("case: config 3, FP32 vs FP64 replay, 30 rounds")
+    rng = np.random.default_rng(51)
+    n, p = 100_000, 51
+    X = np.hstack([rng.standard_normal((n, p - 1)), np.ones((n, 1))])
+    beta = rng.normal(0.0, math.sqrt(1.0 / 50.0), p)
+    y = (rng.random(n) < 1.0 / (1.0 + np.exp(-X @ beta))).astype(float)
+    b = np.zeros(p)
+    for _ in range(25):  # MAP by Newton
+        mu = 1.0 / (1.0 + np.exp(-X @ b))
+        H = (X * (mu * (1 - mu))[:, None]).T @ X + np.eye(p) / 10.0
+        step = np.linalg.solve(H, X.T @ (y - mu) - b / 10.0)
+        b = b + step
+        if np.max(np.abs(step)) < 1e-10:
+            break
+    L = np.linalg.cholesky(np.linalg.inv(H))
+    bank = eng.ProposalBank([eng.FactorGaussian(b, L) for _ in range(64)], np.ones(64))
+    alg = eng.DeterministicMixturePMC(bank, rounds=30, round_size=64 * 512, method=eng.SystematicResampling())
+    tgt = eng.LogisticRegressionTarget(X, y, 10.0)
+    s32 = eng.prepare_sampler(3, tgt, alg, precision="fp32")
+    r32 = eng.importance_sample_(s32)
+    s64 = eng.prepare_sampler(3, tgt, alg, precision="fp64")
+    s64.set_ancestors(s32.ancestors())
+    r64 = eng.importance_sample_(s64)
+    assert np.array_equal(s64.ancestors(), s32.ancestors())
+    dlz = np.abs(r32.diagnostics["round_lognormalizers"] - r64.diagnostics["round_lognormalizers"])
+    assert len(dlz) == 30 and np.all(np.isfinite(dlz))
+    assert np.max(dlz) < 1e-3, np.max(dlz)
+    assert abs(eng.lognormalizer(r32) - eng.lognormalizer(r64)) < 1e-3
