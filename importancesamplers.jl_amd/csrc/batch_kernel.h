@@ -155,11 +155,6 @@ __device__ inline T bslot_logdensity(const T* s, const Family<T>& fam, const T (
 // acc[3..4]: first-order sums sum e (x - c) for coordinates 0..15 and 16..31
 // (every column of these tiles holds the same sums).
 constexpr int kBatchTiles = 5;
-// FP32 jobs accumulate each 64-sample step's moments on FP32 MFMA (FP64 across steps)
-template <class T>
-__host__ __device__ constexpr bool batch_fp32_moments() {
-  return sizeof(T) == 4;
-}
 struct BatchAcc {
   double s0, s2;
   double4_t acc[kBatchTiles];
@@ -187,56 +182,27 @@ __device__ inline void batch_accumulate(T lw, const T (&x)[P], const T* centre, 
   A.s0 += e;
   A.s2 += e * e;
   T* row = stage + lane * kBatchStage;
-  if constexpr (batch_fp32_moments<T>()) {
-    // FP32 jobs: the staged products se (x - c) in FP32 and, when P is not a multiple
-    // of 16, se itself as coordinate P, so the first-order sums are row P of the
-    // second-moment tiles; v_mfma_f32_16x16x4f32 over the 64 samples, then the tiles
-    // added into the FP64 accumulators (FP32 tile layout: register r of lane l holds
-    // row 4 (l / 16) + r, column l % 16)
-    constexpr bool embed = P % 16 != 0;
+  // when P is not a multiple of 16, sqrt(e) itself is staged as coordinate P, so the
+  // first-order sums come out as row P of the second-moment tiles (1 or 3 MFMAs per
+  // 4 samples instead of 2 or 5); the tiles stay Gram matrices of the staged vectors,
+  // products exact in FP64, so they stay positive semidefinite
+  constexpr bool embed = P % 16 != 0;
 #pragma unroll
-    for (int c = 0; c < 16 * NT; ++c) {
+  for (int c = 0; c < 16 * NT; ++c) {
+    if constexpr (sizeof(T) == 4) {
+      // FP32 jobs stage in FP32 arithmetic (the stage holds floats either way)
       T v = T(0);
       if (c < P) v = (se != T(0)) ? se * (x[c] - centre[c]) : T(0);
       if (embed && c == P) v = se;
       row[c] = v;
+    } else {
+      double v = 0.0;
+      if (c < P) v = (sd != 0.0) ? sd * ((double)x[c] - (double)centre[c]) : 0.0;
+      if (embed && c == P) v = sd;
+      row[c] = (T)v;
     }
-    if (!embed) row[kBatchMaxDim] = se;
-    wave_sync();
-    float4_t f[kBatchTiles];
-#pragma unroll
-    for (int p = 0; p < kBatchTiles; ++p) f[p] = (float4_t){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int st = 0; st < 16; ++st) {
-      const T* rs = stage + (4 * st + (lane >> 4)) * kBatchStage;
-      const float a0 = rs[lane & 15];
-      f[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, f[0], 0, 0, 0);
-      if (!embed) f[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, rs[kBatchMaxDim], f[3], 0, 0, 0);
-      if (NT > 1) {
-        const float a1 = rs[16 + (lane & 15)];
-        f[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a0, f[1], 0, 0, 0);
-        f[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a1, f[2], 0, 0, 0);
-        if (!embed) f[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, rs[kBatchMaxDim], f[4], 0, 0, 0);
-      }
-      if ((st & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-    }
-    wave_sync();
-#pragma unroll
-    for (int p = 0; p < kBatchTiles; ++p) {
-      const bool used = (p == 0) || (NT > 1 && p < 3) || (!embed && (p == 3 || (NT > 1 && p == 4)));
-      if (used)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) A.acc[p][i] += (double)f[p][i];
-    }
-    return;
   }
-#pragma unroll
-  for (int c = 0; c < 16 * NT; ++c) {
-    double v = 0.0;
-    if (c < P) v = (sd != 0.0) ? sd * ((double)x[c] - (double)centre[c]) : 0.0;
-    row[c] = (T)v;
-  }
-  row[kBatchMaxDim] = se;
+  if (!embed) row[kBatchMaxDim] = se;
   wave_sync();
   // v_mfma_f64_16x16x4f64 over 4-sample steps: A = the staged column block of
   // coordinates 16 ti + l % 16, B = the same block (second moments) or the
@@ -245,14 +211,13 @@ __device__ inline void batch_accumulate(T lw, const T (&x)[P], const T* centre, 
   for (int st = 0; st < 16; ++st) {
     const T* rs = stage + (4 * st + (lane >> 4)) * kBatchStage;
     const double a0 = (double)rs[lane & 15];
-    const double b1 = (double)rs[kBatchMaxDim];
     A.acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, A.acc[0], 0, 0, 0);
-    A.acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, A.acc[3], 0, 0, 0);
+    if (!embed) A.acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, (double)rs[kBatchMaxDim], A.acc[3], 0, 0, 0);
     if (NT > 1) {
       const double a1 = (double)rs[16 + (lane & 15)];
       A.acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, A.acc[1], 0, 0, 0);
       A.acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, A.acc[2], 0, 0, 0);
-      A.acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, A.acc[4], 0, 0, 0);
+      if (!embed) A.acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, (double)rs[kBatchMaxDim], A.acc[4], 0, 0, 0);
     }
     // four steps' operands in flight at a time, not all sixteen
     if ((st & 3) == 3) __builtin_amdgcn_sched_barrier(0);
@@ -611,9 +576,7 @@ __device__ inline void batch_finish(const BatchJob<T>& J, int t, double* rec0, i
       const int r = lane;
       const int ti = r >> 4, tj = c >> 4, ra = r & 15, cb = c & 15;
       const int p = ti * (ti + 1) / 2 + tj;
-      // FP64 tiles: row 4 r + l / 16 in register r; FP32 tiles: row 4 (l / 16) + r
-      const int ln = batch_fp32_moments<T>() ? cb + 16 * (ra >> 2) : cb + 16 * (ra & 3);
-      const int reg_i = batch_fp32_moments<T>() ? (ra & 3) : (ra >> 2);
+      const int ln = cb + 16 * (ra & 3), reg_i = ra >> 2;
       const double S = rec0[35 + p * 256 + ln * 4 + reg_i];
       double v = S / s0 - mean * (rec0[3 + c] / s0);
       if (c == lane) v += ridge;
@@ -976,31 +939,22 @@ __device__ inline void batch_job(const BatchJob<T>& J, uint64_t run_hi, T* lds_t
       rec[1] = s0;
       rec[2] = s2;
     }
-    if constexpr (batch_fp32_moments<T>()) {
-      // FP32 tile layout: entry (row i, column j) in lane j + 16 (i / 4), register i % 4
-      if constexpr (P % 16 != 0) {
-        // first-order sums: row P % 16 of tile (0, 0) (P < 16) or of tiles (1, 0), (1, 1)
-        constexpr int ra = P & 15;
-        if ((lane >> 4) == (ra >> 2)) {
-          const int cb = lane & 15;
-          if (P < 16) {
-            rec[3 + cb] = cb < P ? A.acc[0][ra & 3] : 0.0;
-            rec[3 + 16 + cb] = 0.0;
-          } else {
-            rec[3 + cb] = A.acc[1][ra & 3];
-            rec[3 + 16 + cb] = 16 + cb < P ? A.acc[2][ra & 3] : 0.0;
-          }
+    // tile entry (row i, column j) sits in lane j + 16 (i & 3), register i >> 2
+    if constexpr (P % 16 != 0) {
+      // first-order sums: row P % 16 of tile (0, 0) (P < 16) or of tiles (1, 0), (1, 1)
+      constexpr int ra = P & 15;
+      if ((lane >> 4) == (ra & 3)) {
+        const int cb = lane & 15;
+        if (P < 16) {
+          rec[3 + cb] = cb < P ? A.acc[0][ra >> 2] : 0.0;
+          rec[3 + 16 + cb] = 0.0;
+        } else {
+          rec[3 + cb] = A.acc[1][ra >> 2];
+          rec[3 + 16 + cb] = 16 + cb < P ? A.acc[2][ra >> 2] : 0.0;
         }
-      } else if ((lane & 15) == 0) {
-        // first-order sums: column 0 of tiles 3 and 4
-        const int g = lane >> 4;
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) rec[3 + 16 * ti + 4 * g + q] = A.acc[3 + ti][q];
       }
     } else if ((lane & 15) == 0) {
-      // first-order sums: tile entry (row i, column 0) sits in lane 16 (i & 3), register i >> 2
+      // first-order sums: column 0 of tiles 3 and 4, entry (row i, column 0) in lane 16 (i & 3)
       const int g = lane >> 4;
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)

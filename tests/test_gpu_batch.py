@@ -286,3 +286,24 @@ def test_batch_config4_512_posteriors_autotuned(eng, oracle_mod):
     assert not b2.run().any()
     for j in picked + [0, 511]:
         assert np.array_equal(b2.result(j).logweights, b.result(j).logweights), j
+
+
+def test_batch_fp32_fits_rank_deficient_rounds(eng):
+    """A wide initial proposal N(0, 9 I) leaves the first rounds with an effective sample
+    size below d, so the weighted covariance is nearly rank-deficient and only its ridge
+    keeps it positive definite.  The moment tiles are Gram matrices of the staged
+    products with exact FP64 products, so FP32 jobs fit every job that FP64 fits."""
+    _say("case: batch FP32 vs FP64 fits from a wide initial proposal, 64 jobs")
+    rng = np.random.default_rng(5)
+    rngs, targets, algs = [], [], []
+    for j in range(64):
+        n = int(rng.integers(50, 1000))
+        d = int(rng.integers(4, 33))
+        X = rng.standard_normal((n, d))
+        y = X @ rng.normal(0, 2, d) + 0.5 * rng.standard_normal(n)
+        rngs.append(100 + j)
+        targets.append(eng.LinearRegressionTarget(X, y, 0.5, 2.0))
+        algs.append(eng.AMIS(eng.FactorGaussian(np.zeros(d), 3.0 * np.eye(d)), rounds=6, round_size=2048))
+    st = {p: eng.prepare_batch(rngs, targets, algs, precision=p).run() for p in ("fp64", "fp32")}
+    only32 = np.nonzero((st["fp32"] != 0) & (st["fp64"] == 0))[0]
+    assert only32.size == 0, only32
