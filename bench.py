@@ -371,6 +371,12 @@ def main_serve(args):
 
     pkg = __graft_entry__.load_package()
     pkg.context(0)
+    print(json.dumps(serve_summary(pkg)))
+
+
+def serve_summary(pkg):
+    """Config 5: the Poisson request stream served with continuous batching in FP32,
+    against the same arrivals run one at a time in FP64."""
     arrivals, reqs = serve_stream(pkg)
     served = serve_run(pkg, arrivals, reqs, "fp32", SERVE_SLOTS)
     plain = serve_run(pkg, arrivals, reqs, "fp64", 1)
@@ -402,7 +408,7 @@ def main_serve(args):
         "p99_speedup_vs_unbatched": plain["p99_ms"] / served["p99_ms"],
         "throughput_ratio_vs_unbatched": served["requests_per_s"] / plain["requests_per_s"],
     }
-    print(json.dumps(out))
+    return out
 
 
 # ---------------------------------------------------------------- plain IS (config 1) and DM-PMC (config 3)
@@ -808,7 +814,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="default workload: skip the config-3/4 figures")
+    ap.add_argument("--no-extras", action="store_true", help="default workload: skip the figures of the other configs")
     ap.add_argument("--workload", default="amis", choices=["amis", "batch", "serve", "plain", "dmpmc", "mis", "copy", "grouped"])
     ap.add_argument("--tuning", default=None, help="batch workload: 'waves,blocks_per_cu' instead of autotuning")
     ap.add_argument("--skip-fp64-check", action="store_true", help="batch workload: skip the FP64 reference solve")
@@ -994,15 +1000,16 @@ def main():
 
 def extra_workloads(pkg, args, threads, fp64_form=None, oracle_ref=None):
     """The grouped forms of configs 1 and 2 (many solves per launch, FP32, against the
-    plain FP64 form), the DM-PMC logistic (config 3) and batched-posterior (config 4)
-    workloads at a few steps each, every one with its own accuracy figures; a failure is
-    reported in place of the figures."""
+    plain FP64 form), the DM-PMC logistic (config 3), batched-posterior (config 4) and
+    serving (config 5) workloads at a few steps each, every one with its own accuracy
+    figures; a failure is reported in place of the figures."""
     out = {}
     for name, fn in (("config1_grouped", lambda: grouped_plain_summary(pkg, 10, 2)),
                      ("config2_grouped", lambda: grouped_amis_summary(pkg, 4, 1, fp64_form, oracle_ref)),
                      ("config3_dmpmc_logistic", lambda: dmpmc_summary(pkg, 3, 1)),
                      ("config4_batched_linreg", lambda: batch_summary(pkg, 3, 1, threads, None, True,
-                                                                      not args.no_cpu))):
+                                                                      not args.no_cpu)),
+                     ("config5_serving", lambda: serve_summary(pkg))):
         t0 = time.perf_counter()
         try:
             out[name] = fn()
