@@ -131,7 +131,7 @@ struct is_job {
   std::vector<double> t_x0;              // the centre: the log-target offset's anchor
   bool t_bf16 = false;
   int t_ks2 = 0;                          // 32-wide K steps
-  DevBuf t_xfrag, t_base, t_y, t_x0d;
+  DevBuf t_xfrag, t_base, t_x0d;
   // FP32 jobs with a two-pass (batch-target) round, 16 <= d <= 64: the mixture pass on
   // the matrix cores (k_mix_prep, k_mix_mfma): inverse-factor fragments, centres and
   // log normalisers of up to nslots components, and the samples' mixture values
@@ -804,7 +804,6 @@ int run_rounds(is_job* j) {
       ba.n = j->tn;
       ba.xfrag = j->t_xfrag.as<u32x4_t>();
       ba.base = j->t_base.as<float>();
-      ba.yv = j->t_y.as<float>();
       ba.x0 = j->t_x0d.as<double>();
       ba.beta = (j->has_tf ? j->XL : j->X).as<float>();
       ba.cap = j->total;
@@ -1363,6 +1362,11 @@ int upload_logistic_bf16(is_job* j, int p) {
   const int64_t ntiles = cdiv(n, 16);
   const double* X = j->tdata.data();
   const double* y = X + (size_t)n * p;
+  // rows with y = 1 are stored negated (s = 1 - 2 y), so each row's term is
+  // -max(s eta, 0) - log1p(e^-|eta|) without the response; other responses keep
+  // the FP32 GEMM form
+  for (int64_t i = 0; i < n; ++i)
+    if (!(y[i] == 0.0 || y[i] == 1.0)) return IS_OK;
   std::vector<uint16_t> frag((size_t)ntiles * ks2 * 2 * 64 * 8, 0);
   for (int64_t t = 0; t < ntiles; ++t)
     for (int ks = 0; ks < ks2; ++ks)
@@ -1371,27 +1375,25 @@ int upload_logistic_bf16(is_job* j, int p) {
         for (int e = 0; e < 8; ++e) {
           const int c = 32 * ks + 8 * (l >> 4) + e;
           if (row >= n || c >= p) continue;
-          const double x = X[(size_t)row * p + c];
+          const double x = y[row] == 1.0 ? -X[(size_t)row * p + c] : X[(size_t)row * p + c];
           const uint16_t h = bf16_bits((float)x);
           const uint16_t lo = bf16_bits((float)(x - bf16_value(h)));
           frag[((((size_t)t * ks2 + ks) * 2 + 0) * 64 + l) * 8 + e] = h;
           frag[((((size_t)t * ks2 + ks) * 2 + 1) * 64 + l) * 8 + e] = lo;
         }
       }
-  std::vector<float> base((size_t)ntiles * 16, 0.0f), yv((size_t)ntiles * 16, 0.0f);
+  std::vector<float> base((size_t)ntiles * 16, 0.0f);
   for (int64_t i = 0; i < n; ++i) {
     double v = 0.0;
     for (int c = 0; c < p; ++c) v += X[(size_t)i * p + c] * j->t_x0[c];
-    base[i] = (float)v;
-    yv[i] = (float)y[i];
+    base[i] = (float)(y[i] == 1.0 ? -v : v);
   }
   int rc;
   if ((rc = j->t_xfrag.alloc(frag.size() * sizeof(uint16_t))) || (rc = j->t_base.alloc(base.size() * sizeof(float))) ||
-      (rc = j->t_y.alloc(yv.size() * sizeof(float))) || (rc = j->t_x0d.alloc((size_t)p * sizeof(double))))
+      (rc = j->t_x0d.alloc((size_t)p * sizeof(double))))
     return rc;
   HIP_TRY(locked_memcpy(j->t_xfrag.p, frag.data(), frag.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(j->t_base.p, base.data(), base.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(locked_memcpy(j->t_y.p, yv.data(), yv.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(locked_memcpy(j->t_x0d.p, j->t_x0.data(), (size_t)p * sizeof(double), hipMemcpyHostToDevice));
   j->t_bf16 = true;
   j->t_ks2 = ks2;

@@ -293,6 +293,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // draw's distance from the centre, not the coefficient itself.  Each row's term
 // y eta - max(eta, 0) - log1p(e^-|eta|) uses the one eta, so an error in eta enters
 // the sum weighted by the gradient y - sigmoid(eta), small near the posterior mode.
+// Responses are 0 or 1 (other data keep the FP32 GEMM form), and rows with y = 1
+// are stored negated (rows and centre term), so with eta_s = (1 - 2 y) eta the term is
+// -max(eta_s, 0) - log1p(e^-|eta_s|), exactly, with no response read per row.
 // The log1p terms of a lane's rows are summed as one logarithm of the product of
 // u = 1 + t over eight rows, t = e^-|eta|, with each u's rounding t - (u - 1) added
 // exactly (a t below half an FP32 ulp of 1 is kept that way), and the logarithm by an
@@ -305,7 +308,6 @@ struct LogisticBf16Args {
   int64_t n;                // data rows
   const u32x4_t* xfrag;     // [row tile][K step][Xh, Xl][lane]: 8 BF16 of row l % 16, columns 8 (l / 16) + e
   const float* base;        // ntiles * 16: x_i . x0 (zero past n)
-  const float* yv;          // ntiles * 16: responses (zero past n)
   const double* x0;         // p
   const float* beta;        // samples, SoA: beta[c * cap + first + s]
   int64_t cap, first, n_new;
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(256) void k_logistic_bf16(LogisticBf16Args a) {
   // this lane's output rows of a tile: 4 (l / 16) + r
   const int rbase = 4 * (lane >> 4);
   u32x4_t ah[KS2], al[KS2];
-  float4_t bs, ys;
+  float4_t bs;
   auto fetch = [&](int64_t t) {
     const int64_t tc = t < t_hi ? t : t_hi - 1;
 #pragma unroll
@@ -383,7 +385,6 @@ __global__ __launch_bounds__(256) void k_logistic_bf16(LogisticBf16Args a) {
       al[ks] = a.xfrag[((tc * KS2 + ks) * 2 + 1) * 64 + lane];
     }
     bs = *reinterpret_cast<const float4_t*>(a.base + tc * 16 + rbase);
-    ys = *reinterpret_cast<const float4_t*>(a.yv + tc * 16 + rbase);
   };
   auto flush = [&]() {
 #pragma unroll
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void k_logistic_bf16(LogisticBf16Args a) {
   if (t < t_hi) fetch(t);
   for (; t < t_hi; t += 4) {
     u32x4_t ch[KS2], cl[KS2];
-    float4_t cb = bs, cy = ys;
+    float4_t cb = bs;
 #pragma unroll
     for (int ks = 0; ks < KS2; ++ks) {
       ch[ks] = ah[ks];
@@ -437,8 +438,8 @@ __global__ __launch_bounds__(256) void k_logistic_bf16(LogisticBf16Args a) {
         const float t0 = __builtin_amdgcn_exp2f(e_hi);
         const float tv = fmaf(t0 * 0.6931471805599453f, e_lo, t0);
         const float u = 1.0f + tv;
-        // y eta - max(eta, 0) is exact for y in {0, 1}
-        pmax[q] += fmaf(cy[r], eta, -fmaxf(eta, 0.0f));
+        // rows with y = 1 are stored negated, so y eta - max(eta, 0) = -max(eta_s, 0)
+        pmax[q] -= fmaxf(eta, 0.0f);
         perr[q] += tv - (u - 1.0f);
         pm[q] *= u;
       }
