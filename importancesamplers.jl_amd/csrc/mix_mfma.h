@@ -7,7 +7,7 @@
 // the inverse factor M = L^{-1} (lower triangular, KP x KP with KP = 16 KT) and the
 // 64 samples' centred coordinates:
 //   k_mix_prep   one wave per component: M by forward substitution of L M = I in FP64
-//                (each lane one column), stored in FP32 as v_mfma_f32_16x16x4f32
+//                (each lane one column, L's rows read across lanes), stored in FP32 as v_mfma_f32_16x16x4f32
 //                A-operand fragments of the lower-triangular 16-row tiles only;
 //   k_mix_mfma   one block per 64 samples, wave w the 16 samples 16 w .. 16 w + 15:
 //                per component, the B operands x - mu (four dimensions per K step,
@@ -19,6 +19,8 @@
 // density gets a NaN mixture value (the new-sample kernel fails it as invalid, as its
 // own loop does).
 #pragma once
+
+__device__ inline double readlane_f64(double v, int lane);  // kernels.h
 
 // Fragments of one component: lower-triangular (row tile i, K step kk) pairs, kk < 4 (i + 1),
 // each 64 lanes x 1 float; pairs in (i, kk) order.
@@ -35,30 +37,41 @@ struct MixPrepArgs {
 };
 
 // One wave per component.  Lane c < d solves L m = e_c for column c of M:
-// m_r = (delta_rc - sum_{c<=k<r} L_rk m_k) / L_rr for r >= c (m_r = 0 for r < c), in
-// FP64, with L staged in LDS (each L_rk read by every lane at once) and the columns in
-// LDS (lane c's own column).
+// m_r = (delta_rc - sum_{k<r} L_rk m_k) / L_rr (m_r = 0 for r < c), in FP64, the column
+// in registers and L_rk read from lane r (which holds row r of L) with v_readlane, so
+// the solve has no memory operations; the columns are transposed through LDS.
 template <int KT>
 __global__ __launch_bounds__(64) void k_mix_prep(MixPrepArgs a) {
   constexpr int KP = 16 * KT;
-  __shared__ double L_s[KP * (KP + 1) / 2 + KP];  // L packed by rows, then 1 / L_rr
-  __shared__ double col_s[KP][64];                 // col_s[r][c] = M[r][c]
+  __shared__ float Ms[KP][KP + 1];  // Ms[r][c] = M[r][c]
   const int comp = blockIdx.x;
   const int lane = threadIdx.x;
   const double* s = a.slots64 + (int64_t)(a.k0 + comp) * a.stride;
   const int d = a.d;
-  const int nl = d * (d + 1) / 2;
-  for (int q = lane; q < nl; q += 64) L_s[q] = s[2 * d + q];
-  for (int q = lane; q < d; q += 64) L_s[nl + q] = s[d + q];
-  for (int r = 0; r < KP; ++r) col_s[r][lane] = 0.0;
-  __syncthreads();
-  const int c = lane;
-  for (int r = 0; r < d; ++r) {
-    const int row = r * (r + 1) / 2;
-    double v = (r == c) ? 1.0 : 0.0;
-    for (int k = 0; k < r; ++k)
-      if (k >= c) v -= L_s[row + k] * col_s[k][c];
-    if (r >= c && c < d) col_s[r][c] = v * L_s[nl + r];  // the slot holds 1 / L_rr
+  double Lrow[KP], inv = 1.0;
+#pragma unroll
+  for (int c = 0; c < KP; ++c) Lrow[c] = 0.0;
+  if (lane < d) {
+    const int row = 2 * d + lane * (lane + 1) / 2;
+#pragma unroll
+    for (int c = 0; c < KP; ++c)
+      if (c <= lane) Lrow[c] = s[row + c];
+    inv = s[d + lane];  // the slot holds 1 / L_rr
+  }
+  double m[KP];
+#pragma unroll
+  for (int r = 0; r < KP; ++r) {
+    m[r] = 0.0;
+    if (r < d) {
+      double v = (r == lane) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < r; ++k) v -= readlane_f64(Lrow[k], r) * m[k];
+      m[r] = v * readlane_f64(inv, r);
+    }
+  }
+  if (lane < KP) {
+#pragma unroll
+    for (int r = 0; r < KP; ++r) Ms[r][lane] = (float)m[r];
   }
   __syncthreads();
   // fragments: pair (i, kk), lane l: M[16 i + l % 16][4 kk + l / 16]
@@ -68,7 +81,7 @@ __global__ __launch_bounds__(64) void k_mix_prep(MixPrepArgs a) {
   for (int i = 0; i < KT; ++i)
 #pragma unroll
     for (int kk = 0; kk < 4 * (i + 1); ++kk) {
-      out[pidx * 64 + lane] = (float)col_s[16 * i + (lane & 15)][4 * kk + (lane >> 4)];
+      out[pidx * 64 + lane] = Ms[16 * i + (lane & 15)][4 * kk + (lane >> 4)];
       ++pidx;
     }
   for (int q = lane; q < KP; q += 64) a.mu[(int64_t)comp * KP + q] = q < d ? (float)s[q] : 0.0f;
